@@ -1,0 +1,120 @@
+/*
+ * nbx_oracle.h -- CPU oracle for the Barnes-Hut step (TEST INFRASTRUCTURE).
+ *
+ * This is a plain restatement of the reference CPU algorithm
+ * (tyreecepaul/n-body-simulation, src/ C++ files), used only as the checker:
+ * by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
+ * The product (the HIP library) never links, loads or calls it.
+ *
+ * All arrays are structure-of-arrays, caller-owned, length n unless stated.
+ * Every function is single-threaded and deterministic. Build flags are fixed
+ * by oracle/Makefile (-O2 -ffp-contract=off, baseline x86-64: no FMA), which
+ * is the configuration whose bits the HIP kernels reproduce.
+ *
+ * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ *   or_uniform_disc    reference build (oracle/_ref) + golden frame-0 fixture
+ *   or_iterate         closed form + reference build
+ *   or_root_quad       exact min/max + reference tree root
+ *   or_keys            reference QuadTree node ranges (every leaf), numpy
+ *                      restatement of the descent
+ *   or_sort_perm       stable-sort semantics (numpy lexsort)
+ *   or_build_tree      reference QuadTree::build node arrays, bit-exact, on
+ *                      key-presorted input (no reordering by the reference)
+ *   or_accelerations   reference QuadTree::acceleration, bit-exact; theta->0
+ *                      limit vs the FP64 direct sum; fastInvSqrt exhaustive
+ *   or_collide         reference collide() output bit-exact; pair set vs an
+ *                      O(N^2) brute-force sweep in numpy
+ *   or_step            reference Simulation::step (per phase), invariants
+ *   or_exact_forces    closed-form two/three-body cases
+ */
+#ifndef NBX_ORACLE_H
+#define NBX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Force law / summation selectors shared with the HIP library. */
+enum { OR_LAW_REFERENCE = 0, OR_LAW_ACCURATE = 1 };
+
+/* Utils.cpp:6-54 uniformDisc(n, seed): bodies in generator order (sorted by
+ * |x|^2), ids assigned before the sort. */
+void or_uniform_disc(uint64_t n, uint32_t seed, float *px, float *py,
+                     float *vx, float *vy, float *m, float *r, uint32_t *id);
+
+/* Body.h:20-23 Body::update / Simulation.cpp:44-50: v += a*dt; x += v*dt. */
+void or_iterate(uint64_t n, float dt, float *px, float *py, float *vx,
+                float *vy, const float *ax, const float *ay);
+
+/* QuadTree.cpp:30-51 Quad::newContaining -> quad = {cx, cy, size}. */
+void or_root_quad(uint64_t n, const float *px, const float *py, float quad[3]);
+
+/* Float-descent quadrant key (QuadTree.h:14-25 intoQuadrant + the partition
+ * predicates of QuadTree.cpp:60-78): 2 bits per level, digit = 2*(y>=cy) +
+ * (x>=cx); depth 0 digit in bits 31:30 of hi, depths 16..31 in lo. */
+void or_keys(uint64_t n, const float *px, const float *py,
+             const float quad[3], uint32_t *hi, uint32_t *lo);
+
+/* Stable sort of indices by (hi, lo). */
+void or_sort_perm(uint64_t n, const uint32_t *hi, const uint32_t *lo,
+                  uint32_t *perm);
+
+/* QuadTree.cpp:154-262 build + accumulateLeaf + propagate, on bodies already
+ * in (hi, lo) order. Node arrays need `capacity` entries. Returns the node
+ * count, or -1 if capacity would be exceeded. levels_out gets the number of
+ * levels. Nodes at depth 32 stay leaves (keys hold 32 levels). */
+int64_t or_build_tree(uint64_t n, const float *px, const float *py,
+                      const float *m, const uint32_t *hi, const uint32_t *lo,
+                      const float quad[3], int64_t capacity, float *comX,
+                      float *comY, float *mass, float *centerX,
+                      float *centerY, float *size, int32_t *children,
+                      int32_t *next, int32_t *start, int32_t *end,
+                      int32_t *levels_out);
+
+/* QuadTree.cpp:264-389 acceleration() for every body (positions px,py),
+ * tree from or_build_tree. law = OR_LAW_REFERENCE uses fastInvSqrt
+ * (QuadTree.cpp:11-26); OR_LAW_ACCURATE uses 1/sqrtf. Also returns per-body
+ * interaction counts (far nodes, body pairs) when the pointers are non-null. */
+void or_accelerations(uint64_t n, const float *px, const float *py,
+                      const float *m, int64_t nodes, const float *comX,
+                      const float *comY, const float *mass,
+                      const float *size, const int32_t *children,
+                      const int32_t *next, const int32_t *start,
+                      const int32_t *end, float theta, float epsilon,
+                      int law, float *ax, float *ay, uint32_t *far_count,
+                      uint32_t *pair_count);
+
+/* Collision.cpp:135-349 collide(): detects the pair list, orders it by
+ * pairLess and resolves it sequentially (Collision.cpp:16-84). pos/vel are
+ * updated in place. The ordered pairs are written to later[]/earlier[] (up to
+ * pair_cap); returns the pair count (which may exceed pair_cap). cell_out
+ * (may be null) receives the grid cell size. */
+int64_t or_collide(uint64_t n, float *px, float *py, float *vx, float *vy,
+                   const float *m, const float *r, int32_t *later,
+                   int32_t *earlier, int64_t pair_cap, float *cell_out);
+
+/* Simulation.cpp:11-16 step(): iterate -> collide (if enabled) -> build ->
+ * accelerations. Bodies are permuted into key order by the build (all nine
+ * arrays). Returns 0, or -1 on node-capacity overflow. */
+int or_step(uint64_t n, float dt, float theta, float epsilon, int collisions,
+            int law, float *px, float *py, float *vx, float *vy, float *ax,
+            float *ay, float *m, float *r, uint32_t *id);
+
+/* FP64 direct sum with the tree's force law m*d/(|d|(|d|^2+eps^2))
+ * (CheckCuda.cu:456-475) for the listed target bodies; self term skipped. */
+void or_exact_forces(uint64_t n, const float *px, const float *py,
+                     const float *m, uint64_t n_targets,
+                     const uint32_t *targets, double eps2, double *ax,
+                     double *ay);
+
+/* QuadTree.cpp:11-26 fastInvSqrt, exposed for the exhaustive known-answer
+ * test: out[i] = fisr(bits-reinterpreted in[i]) for i in [0, count). */
+void or_fast_inv_sqrt_bits(uint64_t count, const uint32_t *in, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

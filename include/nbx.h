@@ -1,0 +1,206 @@
+/*
+ * nbx.h -- C ABI of libnbx, the MI355X (gfx950) Barnes-Hut step.
+ *
+ * The reference (tyreecepaul/n-body-simulation) has no plugin or FFI layer:
+ * its boundary is the C++ class API of Simulation / CudaSimulation, the
+ * `<n_bodies> <total_steps> <output_every> [output_path]` CLI and the NBS2
+ * file. Each entry point below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns NBX_OK (0) or an NBX_ERR_* code; the message of
+ *     the last failure on the calling thread is nbx_last_error(). A wrapper
+ *     that wants the reference's abort-on-error behaviour (CUDA_CHECK,
+ *     src/cuda/DeviceBodies.cuh:18-31) prints it and exits(1).
+ *   - A handle owns all device memory and one HIP stream; the caller owns
+ *     host arrays, which are copied in and out. A handle is not thread-safe;
+ *     separate handles are independent.
+ *   - Arrays are structure-of-arrays, one entry per body: positions px/py,
+ *     velocities vx/vy, accelerations ax/ay, mass m, radius r (float32) and
+ *     the stable identity id (uint32) of src/Body.h:5-24.
+ *   - The build reorders bodies every step (as QuadTree::build and
+ *     CudaQuadTree::build do); "device order" is that order, ids identify
+ *     bodies.
+ */
+#ifndef NBX_H
+#define NBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBX_OK 0
+#define NBX_ERR_ARG 1      /* bad argument (null pointer, n above capacity) */
+#define NBX_ERR_HIP 2      /* a HIP runtime call failed */
+#define NBX_ERR_CAPACITY 3 /* node, pair or sort-segment capacity exceeded */
+#define NBX_ERR_STATE 4    /* call not valid in the handle's current state */
+
+/* Force law: the reference's (fastInvSqrt, src/QuadTree.cpp:11-26) or an
+ * accurate inverse square root. */
+#define NBX_LAW_REFERENCE 0
+#define NBX_LAW_ACCURATE 1
+
+/* Per-body force summation: bit-exact replay of the reference's FP32 order
+ * (src/QuadTree.cpp:264-389, four interleaved leaf sums), or FP32 with
+ * compensated (Kahan) accumulation and a hardware reciprocal. */
+#define NBX_SUM_REFERENCE 0
+#define NBX_SUM_COMPENSATED 1
+
+typedef struct nbx_sim nbx_sim;
+
+/* Physics and execution parameters. Defaults (nbx_default_params) are the
+ * reference's: dt 0.05, theta 1, epsilon 1, collisions on
+ * (src/Simulation.cpp:7-9; src/cuda/CudaSimulation.cuh:39-43). */
+typedef struct nbx_params {
+  float dt;
+  float theta;
+  float epsilon;
+  int32_t collisions; /* CudaSimulation::collisions */
+  int32_t law;        /* NBX_LAW_* */
+  int32_t summation;  /* NBX_SUM_* */
+  int32_t use_graph;  /* replay captured HIP graphs in nbx_step (1) or launch
+                         eagerly (0) */
+  int32_t reserved;
+} nbx_params;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *nbx_last_error(void);
+
+/* ABI version of this header (NBX_ABI_VERSION). */
+#define NBX_ABI_VERSION 1
+int nbx_abi_version(void);
+
+/* Fills *p with the reference defaults. */
+void nbx_default_params(nbx_params *p);
+
+/* CudaSimulation(n) storage / CudaQuadTree(n) (src/cuda/CudaQuadTree.cu:
+ * 269-334): a handle able to hold up to `capacity` bodies, with node storage
+ * for 8*capacity+8 tree nodes. Allocates everything up front; steps never
+ * allocate. */
+int nbx_create(uint64_t capacity, nbx_sim **out);
+
+/* Releases all device memory and the stream. Null is accepted. */
+int nbx_destroy(nbx_sim *sim);
+
+/* Simulation(n) / CudaSimulation(n) (src/Simulation.cpp:7-9,
+ * src/cuda/CudaSimulation.cu:34-38): loads uniformDisc(n, seed)
+ * (src/Utils.cpp:6-54), in generator order, and resets the frame counter. */
+int nbx_init_disc(nbx_sim *sim, uint64_t n, uint32_t seed);
+
+/* CudaSimulation(const std::vector<Body>&) and CudaSimulation::reset
+ * (src/cuda/CudaSimulation.cu:34-48), DeviceBodies::upload
+ * (src/cuda/DeviceBodies.cu:25-50). ax/ay may be null (zero). */
+int nbx_upload(nbx_sim *sim, uint64_t n, const float *px, const float *py,
+               const float *vx, const float *vy, const float *ax,
+               const float *ay, const float *m, const float *r,
+               const uint32_t *id);
+
+/* DeviceBodies::download (src/cuda/DeviceBodies.cu:52-72): state in device
+ * order. Any pointer may be null to skip that field. */
+int nbx_download(nbx_sim *sim, float *px, float *py, float *vx, float *vy,
+                 float *ax, float *ay, float *m, float *r, uint32_t *id);
+
+/* The output-frame path of sim_cuda (src/cuda/MainCuda.cu:90-105): state
+ * scattered into id order (index k holds the body with id k; ids must be a
+ * permutation of 0..n-1). Any pointer may be null. */
+int nbx_download_by_id(nbx_sim *sim, float *px, float *py, float *vx,
+                       float *vy, float *m, float *r);
+
+/* Body count currently loaded. */
+int nbx_count(nbx_sim *sim, uint64_t *n);
+
+/* CudaSimulation's public dt/theta/epsilon/collisions members. */
+int nbx_set_params(nbx_sim *sim, const nbx_params *p);
+int nbx_get_params(nbx_sim *sim, nbx_params *p);
+
+/* Simulation::step / CudaSimulation::step (src/Simulation.cpp:11-16,
+ * src/cuda/CudaSimulation.cu:51-58), `steps` times: iterate -> collide (if
+ * enabled) -> build -> forces. Returns after the work is queued and any
+ * device-side error of these steps has been checked (one host sync). */
+int nbx_step(nbx_sim *sim, uint32_t steps);
+
+/* The individual phases (src/cuda/CudaSimulation.cu:60-74). */
+int nbx_iterate(nbx_sim *sim); /* DeviceBodies.cuh:111-113 iterate() */
+int nbx_collide(nbx_sim *sim); /* CudaCollider::collide */
+int nbx_attract(nbx_sim *sim); /* build + forces */
+int nbx_build(nbx_sim *sim);   /* CudaQuadTree::build (reorders bodies) */
+int nbx_forces(nbx_sim *sim);  /* CudaQuadTree::computeForces */
+
+/* Steps completed since the last init/upload (CudaSimulation::frame). */
+int nbx_frame(nbx_sim *sim, uint64_t *frame);
+
+/* Waits for queued work and reports any device-side error. */
+int nbx_sync(nbx_sim *sim);
+
+/* ---- Inspection, for checks (CudaQuadTree::download / nodeCount /
+ *      levelCount, src/cuda/CudaQuadTree.cuh:42-63). ---- */
+
+/* Node count and level count of the last build. */
+int nbx_tree_info(nbx_sim *sim, int64_t *nodes, int32_t *levels);
+
+/* Node arrays of the last build (QuadTree.h:35-70 Node fields): centre of
+ * mass, mass, quad centre/size, children, next, body range. Copies
+ * min(nodes, cap) entries; any pointer may be null. */
+int nbx_download_tree(nbx_sim *sim, int64_t cap, float *comX, float *comY,
+                      float *mass, float *centerX, float *centerY,
+                      float *size, int32_t *children, int32_t *next,
+                      int32_t *start, int32_t *end);
+
+/* Root quad {cx, cy, size} of the last build (Quad::newContaining). */
+int nbx_root_quad(nbx_sim *sim, float quad[3]);
+
+/* Keys of the last build in sorted order (hi = depths 0-15, lo = depths
+ * 16-31, 2 bits per level) and the permutation it applied: perm[k] = the
+ * pre-build device index of the body now at k. Any pointer may be null. */
+int nbx_download_keys(nbx_sim *sim, uint32_t *hi, uint32_t *lo,
+                      uint32_t *perm);
+
+/* Last collide (Collision.cpp:135-349): grid cell size, number of pairs,
+ * number of connected contact components and the largest one's pair count. */
+int nbx_collide_info(nbx_sim *sim, float *cell, int64_t *pairs,
+                     int64_t *components, int64_t *largest);
+
+/* The last collide's pairs (device indices, later/earlier as in
+ * Collision.cpp:120-127) in the order they were resolved, grouped by
+ * component: comp[k] is the component's smallest body index. Pairs of one
+ * component appear in the reference's pairLess order. */
+int nbx_download_pairs(nbx_sim *sim, int64_t cap, int32_t *later,
+                       int32_t *earlier, int32_t *comp, int64_t *count);
+
+/* Runs the force walk once more in counting mode over the current tree and
+ * returns far-node (mass > 0) and body-pair interaction totals. Does not
+ * change the state. */
+int nbx_count_interactions(nbx_sim *sim, uint64_t *far_nodes,
+                           uint64_t *body_pairs);
+
+/* ---- Timing (the per-phase event bench of src/cuda/CheckCuda.cu:677-712).
+ * With profiling on, nbx_step / phase calls launch eagerly and record HIP
+ * events on the handle's stream around each phase. ---- */
+#define NBX_PHASE_ITERATE 0
+#define NBX_PHASE_COLLIDE 1
+#define NBX_PHASE_BUILD 2
+#define NBX_PHASE_WALK 3
+#define NBX_PHASES 4
+int nbx_set_profiling(nbx_sim *sim, int on);
+/* Accumulated milliseconds per phase and the number of steps they cover;
+ * resets the accumulators. */
+int nbx_phase_times(nbx_sim *sim, double ms[NBX_PHASES], uint64_t *steps);
+
+/* The handle's HIP stream (a hipStream_t), for callers that order their own
+ * work after it. */
+int nbx_stream(nbx_sim *sim, void **stream);
+
+/* ---- FP64 direct sum (CheckCuda.cu:456-475 exactTreeLawForces) on the
+ *      GPU, the accuracy judge: accelerations of the listed target bodies
+ *      (device indices) from all bodies, law m*d/(|d|(|d|^2+eps^2)). ---- */
+int nbx_exact_forces(nbx_sim *sim, uint64_t n_targets,
+                     const uint32_t *targets, double eps2, double *ax,
+                     double *ay);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,496 @@
+// k_collide.h -- uniform-grid collision broadphase and deterministic pair
+// resolution (Collision.cpp:135-349).
+//
+// Detection finds exactly the reference's pair set: small boxes (no wider
+// than a grid cell) pair with small boxes in the 3x3 block of cells around
+// them, large boxes pair with every box, and a pair is kept when its boxes
+// overlap (Collision.cpp:95-98) -- each unordered pair is emitted once, from
+// the side of its `later` box (Collision.cpp:88-90). The reference then
+// resolves all pairs sequentially in pairLess order. resolve() touches only
+// its two bodies, so pairs in different connected components of the contact
+// graph commute: components are found with a lock-free union-find, and each
+// component replays its own pairs, in pairLess order, on one lane (or one
+// workgroup for large components). The result is bit-identical to the
+// sequential replay.
+#pragma once
+
+#include "nbx_device.h"
+
+namespace nbx {
+
+struct Box {
+  float minX, maxX, minY, maxY;
+};
+
+// Vec2 pos -/+ Vec2::one() * radius (Collision.cpp:156-163).
+__device__ inline Box bodyBox(float x, float y, float r) {
+  return Box{x - r, x + r, y - r, y + r};
+}
+
+// Collision.cpp:88-90 before().
+__device__ inline bool beforeBox(float aMinX, int a, float bMinX, int b) {
+  return aMinX < bMinX || (aMinX == bMinX && a < b);
+}
+
+// Collision.cpp:95-98 overlaps(earlier, later).
+__device__ inline bool overlapsBox(const Box &e, const Box &l) {
+  return e.maxX >= l.minX && l.minY <= e.maxY && e.minY <= l.maxY;
+}
+
+// Collision.cpp:100-104 cellCoord: floor in double, clamped to +-2^29.
+__device__ inline int32_t cellCoord(float v, float invCell) {
+  const double limit = 536870912.0;
+  double c = floor(static_cast<double>(v) * static_cast<double>(invCell));
+  c = c < -limit ? -limit : (limit < c ? limit : c);
+  return static_cast<int32_t>(c);
+}
+
+// Cell -> bucket: horizontally adjacent cells land in adjacent buckets.
+__device__ inline uint32_t cellBucket(int32_t cx, int32_t cy, uint32_t mask) {
+  return (static_cast<uint32_t>(cy) * 0x9E3779B1u + static_cast<uint32_t>(cx)) &
+         mask;
+}
+
+// Collision.cpp:333-340 pairLess on (laterMinX, later, earlierMinX, earlier);
+// pairs are int4 {later, earlier, bits(laterMinX), bits(earlierMinX)}.
+__device__ inline bool pairLess(const int4 &p, const int4 &q) {
+  if (p.x != q.x) {
+    const float a = __int_as_float(p.z), b = __int_as_float(q.z);
+    return a < b || (a == b && p.x < q.x);
+  }
+  const float a = __int_as_float(p.w), b = __int_as_float(q.w);
+  return a < b || (a == b && p.y < q.y);
+}
+
+// Collision.cpp:16-84 resolve(bodies, i, j), on the SoA arrays.
+__device__ inline void resolvePair(float *__restrict__ px,
+                                   float *__restrict__ py,
+                                   float *__restrict__ vx,
+                                   float *__restrict__ vy,
+                                   const float *__restrict__ m,
+                                   const float *__restrict__ r, int i, int j) {
+  float p1x = px[i], p1y = py[i];
+  float p2x = px[j], p2y = py[j];
+  const float r1 = r[i], r2 = r[j];
+  float dx = p2x - p1x, dy = p2y - p1y;
+  const float rr = r1 + r2;
+  if (dx * dx + dy * dy > rr * rr) {
+    return;
+  }
+  const float v1x = vx[i], v1y = vy[i];
+  const float v2x = vx[j], v2y = vy[j];
+  const float wx = v2x - v1x, wy = v2y - v1y;
+  float dDotV = dx * wx + dy * wy;
+  const float m1 = m[i], m2 = m[j];
+  const float weight1 = m2 / (m1 + m2);
+  const float weight2 = m1 / (m1 + m2);
+  const bool dNonZero = !(dx == 0.0f && dy == 0.0f);
+  if (dDotV >= 0.0f && dNonZero) {
+    const float mag = sqrtf(dx * dx + dy * dy);
+    const float s = rr / mag - 1.0f;
+    const float tx = dx * s, ty = dy * s;
+    px[i] = p1x - weight1 * tx;
+    py[i] = p1y - weight1 * ty;
+    px[j] = p2x + weight2 * tx;
+    py[j] = p2y + weight2 * ty;
+    return;
+  }
+  const float vSqEarly = wx * wx + wy * wy;
+  if (vSqEarly < 1e-12f) {
+    float dirx = 1.0f, diry = 0.0f;
+    if (dNonZero) {
+      const float inv = 1.0f / sqrtf(dx * dx + dy * dy);
+      dirx = dx * inv;
+      diry = dy * inv;
+    }
+    const float tx = dirx * rr, ty = diry * rr;
+    px[i] = p1x - weight1 * tx;
+    py[i] = p1y - weight1 * ty;
+    px[j] = p2x + weight2 * tx;
+    py[j] = p2y + weight2 * ty;
+    return;
+  }
+  const float vSq = wx * wx + wy * wy;
+  float dSq = dx * dx + dy * dy;
+  const float rSq = rr * rr;
+  const float disc = dDotV * dDotV - vSq * (dSq - rSq);
+  const float t = (dDotV + sqrtf(disc < 0.0f ? 0.0f : disc)) / vSq;
+  p1x = p1x - v1x * t;
+  p1y = p1y - v1y * t;
+  p2x = p2x - v2x * t;
+  p2y = p2y - v2y * t;
+  dx = p2x - p1x;
+  dy = p2y - p1y;
+  dDotV = dx * wx + dy * wy;
+  dSq = dx * dx + dy * dy;
+  const float s = 1.5f * dDotV / dSq;
+  const float tx = dx * s, ty = dy * s;
+  const float n1x = v1x + tx * weight1, n1y = v1y + ty * weight1;
+  const float n2x = v2x - tx * weight2, n2y = v2y - ty * weight2;
+  vx[i] = n1x;
+  vy[i] = n1y;
+  vx[j] = n2x;
+  vy[j] = n2y;
+  px[i] = p1x + n1x * t;
+  py[i] = p1y + n1y * t;
+  px[j] = p2x + n2x * t;
+  py[j] = p2y + n2y * t;
+}
+
+__global__ void k_collide_reset(uint32_t n, Ctl *__restrict__ ctl) {
+  ctl->selPrefix = 0;
+  ctl->selRank = n / 2;
+  ctl->largeCount = 0;
+  ctl->pairCount = 0;
+  ctl->compCount = 0;
+  ctl->compPairs = 0;
+  ctl->bigCount = 0;
+  ctl->largest = 0;
+  ctl->status &= ~(kStatusPairOverflow | kStatusComponentTooLarge);
+}
+
+// Box widths (Collision.cpp:163-164), input of the median cell size.
+__global__ __launch_bounds__(kBlock) void k_widths(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ r, float *__restrict__ widths) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) {
+    return;
+  }
+  const Box b = bodyBox(px[i], py[i], r[i]);
+  const float wx = b.maxX - b.minX, wy = b.maxY - b.minY;
+  widths[i] = (wx < wy) ? wy : wx;
+}
+
+// Large/small split, grid cells, bucket counts; also seeds the union-find.
+__global__ __launch_bounds__(kBlock) void k_cells(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
+    uint32_t *__restrict__ bucketOf, uint32_t *__restrict__ bucketCount,
+    int32_t *__restrict__ largeList, int32_t *__restrict__ ufParent) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) {
+    return;
+  }
+  ufParent[i] = static_cast<int32_t>(i);
+  const float cell = ctl->cell, invCell = ctl->invCell;
+  const Box b = bodyBox(px[i], py[i], r[i]);
+  if (b.maxX - b.minX > cell || b.maxY - b.minY > cell) {
+    const uint32_t slot = atomicAdd(&ctl->largeCount, 1u);
+    largeList[slot] = static_cast<int32_t>(i);
+    bucketOf[i] = 0xffffffffu;
+    return;
+  }
+  const int32_t cx = cellCoord(b.minX, invCell);
+  const int32_t cy = cellCoord(b.minY, invCell);
+  const uint32_t h = cellBucket(cx, cy, mask);
+  bucketOf[i] = h;
+  atomicAdd(&bucketCount[h], 1u);
+}
+
+// Counting-sort scatter of small boxes into their buckets. Consumes the
+// bucket counts back to zero, ready for the next step.
+__global__ __launch_bounds__(kBlock) void k_grid_scatter(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ r, const Ctl *__restrict__ ctl,
+    const uint32_t *__restrict__ bucketOf,
+    uint32_t *__restrict__ bucketCount,
+    const uint32_t *__restrict__ bucketStart, float4 *__restrict__ gridBox,
+    int4 *__restrict__ gridCell) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) {
+    return;
+  }
+  const uint32_t h = bucketOf[i];
+  if (h == 0xffffffffu) {
+    return;
+  }
+  const uint32_t slot = bucketStart[h] + atomicSub(&bucketCount[h], 1u) - 1u;
+  const float invCell = ctl->invCell;
+  const Box b = bodyBox(px[i], py[i], r[i]);
+  gridBox[slot] = make_float4(b.minX, b.maxX, b.minY, b.maxY);
+  gridCell[slot] = make_int4(cellCoord(b.minX, invCell),
+                             cellCoord(b.minY, invCell),
+                             static_cast<int32_t>(i), 0);
+}
+
+__device__ inline void emitPair(Ctl *__restrict__ ctl, int4 *__restrict__ pairs,
+                                uint32_t cap, int later, int earlier,
+                                float laterMinX, float earlierMinX) {
+  const uint32_t slot = atomicAdd(&ctl->pairCount, 1u);
+  if (slot < cap) {
+    pairs[slot] = make_int4(later, earlier, __float_as_int(laterMinX),
+                            __float_as_int(earlierMinX));
+  } else {
+    atomicOr(&ctl->status, kStatusPairOverflow);
+  }
+}
+
+// One thread per small box (grid slot): its earlier partners among small
+// boxes of the 3x3 neighbouring cells and among all large boxes.
+__global__ __launch_bounds__(kBlock) void k_find_small(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
+    const uint32_t *__restrict__ bucketStart,
+    const float4 *__restrict__ gridBox, const int4 *__restrict__ gridCell,
+    const int32_t *__restrict__ largeList, int4 *__restrict__ pairs,
+    uint32_t cap) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t large = ctl->largeCount;
+  if (s >= n - large) {
+    return;
+  }
+  const float4 eb = gridBox[s];
+  const int4 ec = gridCell[s];
+  const Box me{eb.x, eb.y, eb.z, eb.w};
+  const int self = ec.z;
+  for (int dy = -1; dy <= 1; ++dy) {
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int32_t cx = ec.x + dx, cy = ec.y + dy;
+      const uint32_t h = cellBucket(cx, cy, mask);
+      const uint32_t end = bucketStart[h + 1];
+      for (uint32_t o = bucketStart[h]; o < end; ++o) {
+        const int4 oc = gridCell[o];
+        if (oc.x != cx || oc.y != cy || oc.z == self) {
+          continue;
+        }
+        const float4 ob = gridBox[o];
+        if (beforeBox(ob.x, oc.z, me.minX, self) &&
+            overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
+          emitPair(ctl, pairs, cap, self, oc.z, me.minX, ob.x);
+        }
+      }
+    }
+  }
+  for (uint32_t k = 0; k < large; ++k) {
+    const int l = largeList[k];
+    const Box lb = bodyBox(px[l], py[l], r[l]);
+    if (beforeBox(lb.minX, l, me.minX, self) && overlapsBox(lb, me)) {
+      emitPair(ctl, pairs, cap, self, l, me.minX, lb.minX);
+    }
+  }
+}
+
+// Every body against every large box that comes later (Collision.cpp:
+// 319-330, oriented): covers large-small pairs with the large box later and
+// large-large pairs once.
+__global__ __launch_bounds__(kBlock) void k_find_large(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ r, Ctl *__restrict__ ctl,
+    const int32_t *__restrict__ largeList, int4 *__restrict__ pairs,
+    uint32_t cap) {
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t large = ctl->largeCount;
+  if (x >= n || large == 0) {
+    return;
+  }
+  const Box xb = bodyBox(px[x], py[x], r[x]);
+  for (uint32_t k = 0; k < large; ++k) {
+    const int l = largeList[k];
+    if (l == static_cast<int>(x)) {
+      continue;
+    }
+    const Box lb = bodyBox(px[l], py[l], r[l]);
+    if (beforeBox(xb.minX, static_cast<int>(x), lb.minX, l) &&
+        overlapsBox(xb, lb)) {
+      emitPair(ctl, pairs, cap, l, static_cast<int>(x), lb.minX, xb.minX);
+    }
+  }
+}
+
+// ---- connected components of the contact graph ----
+
+__device__ inline int ufLoad(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of x with path halving. Parent links only ever move to a smaller
+// index, so the walk terminates and the final root is the component's
+// smallest body index.
+__device__ inline int ufFind(int32_t *parent, int x) {
+  while (true) {
+    const int p = ufLoad(&parent[x]);
+    if (p == x) {
+      return x;
+    }
+    const int gp = ufLoad(&parent[p]);
+    if (gp == p) {
+      return p;
+    }
+    __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    x = gp;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_union(
+    const Ctl *__restrict__ ctl, uint32_t cap, const int4 *__restrict__ pairs,
+    int32_t *__restrict__ parent) {
+  const uint32_t count = min(ctl->pairCount, cap);
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;
+       k += gridDim.x * kBlock) {
+    const int4 p = pairs[k];
+    int a = ufFind(parent, p.x);
+    int b = ufFind(parent, p.y);
+    while (a != b) {
+      const int hi = a > b ? a : b;
+      const int lo = a > b ? b : a;
+      const int old = atomicCAS(&parent[hi], hi, lo);
+      if (old == hi) {
+        break;
+      }
+      a = ufFind(parent, old);
+      b = ufFind(parent, lo);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_comp_count(
+    const Ctl *__restrict__ ctl, uint32_t cap, const int4 *__restrict__ pairs,
+    int32_t *__restrict__ parent, int32_t *__restrict__ pairComp,
+    uint32_t *__restrict__ compCount) {
+  const uint32_t count = min(ctl->pairCount, cap);
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;
+       k += gridDim.x * kBlock) {
+    const int root = ufFind(parent, pairs[k].x);
+    pairComp[k] = root;
+    atomicAdd(&compCount[root], 1u);
+  }
+}
+
+// Allocates each component's contiguous pair segment.
+__global__ __launch_bounds__(kBlock) void k_comp_alloc(
+    uint32_t n, Ctl *__restrict__ ctl, const uint32_t *__restrict__ compCount,
+    uint32_t *__restrict__ compStart, int4 *__restrict__ compInfo) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) {
+    return;
+  }
+  const uint32_t c = compCount[i];
+  if (c == 0) {
+    return;
+  }
+  const uint32_t start = atomicAdd(&ctl->compPairs, c);
+  const uint32_t j = atomicAdd(&ctl->compCount, 1u);
+  compStart[i] = start;
+  compInfo[j] = make_int4(static_cast<int>(i), static_cast<int>(start),
+                          static_cast<int>(c), 0);
+  atomicMax(&ctl->largest, c);
+}
+
+// Moves every pair into its component's segment; consumes the counts back
+// to zero.
+__global__ __launch_bounds__(kBlock) void k_comp_scatter(
+    const Ctl *__restrict__ ctl, uint32_t cap, const int4 *__restrict__ pairs,
+    const int32_t *__restrict__ pairComp, uint32_t *__restrict__ compCount,
+    const uint32_t *__restrict__ compStart, int4 *__restrict__ segPairs) {
+  const uint32_t count = min(ctl->pairCount, cap);
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;
+       k += gridDim.x * kBlock) {
+    const int root = pairComp[k];
+    const uint32_t slot =
+        compStart[root] + atomicSub(&compCount[root], 1u) - 1u;
+    segPairs[slot] = pairs[k];
+  }
+}
+
+// One lane per component of up to kSmallComp pairs: insertion sort into
+// pairLess order, then the sequential replay. Larger components are queued
+// for k_resolve_big.
+__global__ __launch_bounds__(kBlock) void k_resolve_small(
+    Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
+    int4 *__restrict__ segPairs, int32_t *__restrict__ bigList,
+    float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
+    float *__restrict__ vy, const float *__restrict__ m,
+    const float *__restrict__ r) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= ctl->compCount || (ctl->status & kStatusPairOverflow)) {
+    return;
+  }
+  const int4 info = compInfo[j];
+  const int start = info.y, size = info.z;
+  if (size > kSmallComp) {
+    bigList[atomicAdd(&ctl->bigCount, 1u)] = static_cast<int32_t>(j);
+    return;
+  }
+  int4 p[kSmallComp];
+  for (int k = 0; k < size; ++k) {
+    const int4 v = segPairs[start + k];
+    int q = k;
+    while (q > 0 && pairLess(v, p[q - 1])) {
+      p[q] = p[q - 1];
+      --q;
+    }
+    p[q] = v;
+  }
+  for (int k = 0; k < size; ++k) {
+    segPairs[start + k] = p[k];
+    resolvePair(px, py, vx, vy, m, r, p[k].x, p[k].y);
+  }
+}
+
+// One workgroup per large component: bitonic sort of its pairs in LDS, then
+// lane 0 replays them in order.
+__global__ __launch_bounds__(kBlock) void k_resolve_big(
+    Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
+    const int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
+    float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
+    float *__restrict__ vy, const float *__restrict__ m,
+    const float *__restrict__ r) {
+  __shared__ int4 sp[kBigCompMax];
+  __shared__ int16_t order[kBigCompMax];
+  const uint32_t big = ctl->bigCount;
+  for (uint32_t b = blockIdx.x; b < big; b += gridDim.x) {
+    const int4 info = compInfo[bigList[b]];
+    const int start = info.y, size = info.z;
+    if (size > kBigCompMax) {
+      if (threadIdx.x == 0) {
+        atomicOr(&ctl->status, kStatusComponentTooLarge);
+      }
+      continue;
+    }
+    int p2 = 1;
+    while (p2 < size) {
+      p2 <<= 1;
+    }
+    for (int k = threadIdx.x; k < p2; k += kBlock) {
+      if (k < size) {
+        sp[k] = segPairs[start + k];
+      }
+      order[k] = static_cast<int16_t>(k);
+    }
+    __syncthreads();
+    for (int len = 2; len <= p2; len <<= 1) {
+      for (int stride = len >> 1; stride > 0; stride >>= 1) {
+        for (int k = threadIdx.x; k < p2; k += kBlock) {
+          const int partner = k ^ stride;
+          if (partner > k) {
+            const bool up = (k & len) == 0;
+            const int a = order[k], c = order[partner];
+            // Padding (index >= size) sorts after every real pair.
+            const bool aGreater =
+                a >= size ? (c < size || a > c)
+                          : (c < size && pairLess(sp[c], sp[a]));
+            if (aGreater == up) {
+              order[k] = static_cast<int16_t>(c);
+              order[partner] = static_cast<int16_t>(a);
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int k = threadIdx.x; k < size; k += kBlock) {
+      segPairs[start + k] = sp[order[k]];
+    }
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < size; ++k) {
+        const int4 q = sp[order[k]];
+        resolvePair(px, py, vx, vy, m, r, q.x, q.y);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+} // namespace nbx

@@ -1,0 +1,322 @@
+// k_sort.h -- device-wide exclusive scan, stable LSD radix sort (8-bit
+// digits, LDS-staged scatter) and exact radix selection.
+#pragma once
+
+#include "nbx_device.h"
+
+namespace nbx {
+
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems; // 4096 values per tile
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems; // 4096 keys per tile
+
+// Exclusive scan of one value per thread across the block; returns the
+// prefix and writes the block total to *total. Uses `scratch` (>= 4 words).
+__device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t *scratch,
+                                              uint32_t *total) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, kWave);
+    if (lane >= off) {
+      x += y;
+    }
+  }
+  if (lane == kWave - 1) {
+    scratch[wave] = x;
+  }
+  __syncthreads();
+  uint32_t waveOff = 0, sum = 0;
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    const uint32_t s = scratch[w];
+    waveOff += (w < wave) ? s : 0u;
+    sum += s;
+  }
+  __syncthreads();
+  *total = sum;
+  return waveOff + x - v;
+}
+
+// ---- exclusive scan over `count` uint32 (in place allowed) ----
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(
+    uint32_t count, const uint32_t *__restrict__ in,
+    uint32_t *__restrict__ tileSums) {
+  __shared__ uint32_t scratch[kBlock / kWave];
+  const uint32_t base = blockIdx.x * kScanTile;
+  uint32_t s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t idx = base + k * kBlock + threadIdx.x;
+    s += idx < count ? in[idx] : 0u;
+  }
+  uint32_t total;
+  blockExclusiveScan(s, scratch, &total);
+  if (threadIdx.x == 0) {
+    tileSums[blockIdx.x] = total;
+  }
+}
+
+// One workgroup: exclusive scan of `tiles` tile sums, in place.
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(
+    uint32_t tiles, uint32_t *__restrict__ tileSums) {
+  __shared__ uint32_t scratch[kBlock / kWave];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < tiles; base += kBlock) {
+    const uint32_t idx = base + threadIdx.x;
+    const uint32_t v = idx < tiles ? tileSums[idx] : 0u;
+    uint32_t total;
+    const uint32_t pre = blockExclusiveScan(v, scratch, &total);
+    if (idx < tiles) {
+      tileSums[idx] = carry + pre;
+    }
+    carry += total;
+  }
+}
+
+// Blocked per-thread scan of a tile: thread t owns values
+// [base + t*kScanItems, ...), so the output order is the input order.
+__global__ __launch_bounds__(kBlock) void k_scan_down(
+    uint32_t count, const uint32_t *__restrict__ in,
+    const uint32_t *__restrict__ tileSums, uint32_t *__restrict__ out) {
+  __shared__ uint32_t scratch[kBlock / kWave];
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t idx = base + k;
+    v[k] = idx < count ? in[idx] : 0u;
+    s += v[k];
+  }
+  uint32_t total;
+  uint32_t run = blockExclusiveScan(s, scratch, &total) + tileSums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t idx = base + k;
+    if (idx < count) {
+      out[idx] = run;
+    }
+    run += v[k];
+  }
+}
+
+// ---- stable LSD radix sort, one 8-bit digit per pass ----
+
+// Per-tile digit histogram -> hist[digit * tiles + tile].
+__global__ __launch_bounds__(kBlock) void k_rs_hist(
+    uint32_t n, const uint32_t *__restrict__ keys, int shift,
+    uint32_t *__restrict__ hist, uint32_t tiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t idx = base + k * kBlock + threadIdx.x;
+    const bool valid = idx < n;
+    const uint32_t d = valid ? (keys[idx] >> shift) & 255u : 0u;
+    const uint64_t peers = matchDigit8(d, valid);
+    if (valid && (peers & laneMaskLt()) == 0) {
+      atomicAdd(&h[d], static_cast<uint32_t>(__popcll(peers)));
+    }
+  }
+  __syncthreads();
+  hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// One workgroup: exclusive scan of the digit-major histogram table (in
+// place), giving every (digit, tile) its global output offset.
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t entries,
+                                                  uint32_t *__restrict__ t) {
+  __shared__ uint32_t sums[1024];
+  const uint32_t per = (entries + 1023) / 1024;
+  const uint32_t lo = threadIdx.x * per;
+  const uint32_t hi = min(entries, lo + per);
+  uint32_t s = 0;
+  for (uint32_t i = lo; i < hi; ++i) {
+    s += t[i];
+  }
+  sums[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = threadIdx.x >= static_cast<uint32_t>(off)
+                           ? sums[threadIdx.x - off]
+                           : 0u;
+    __syncthreads();
+    sums[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = sums[threadIdx.x] - s;
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t v = t[i];
+    t[i] = run;
+    run += v;
+  }
+}
+
+// Stable scatter of one tile: ranks each key among equal digits in tile
+// order, stages the tile digit-sorted in LDS, then writes runs coalesced.
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(
+    uint32_t n, const uint32_t *__restrict__ keysIn,
+    const uint32_t *__restrict__ valsIn, int shift,
+    const uint32_t *__restrict__ offsets, uint32_t tiles,
+    uint32_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut) {
+  __shared__ uint32_t sk[kSortTile];
+  __shared__ uint32_t sv[kSortTile];
+  __shared__ uint32_t digitStart[256];
+  __shared__ uint32_t goff[256];
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t waveCnt[kBlock / kWave][256];
+  __shared__ uint32_t scratch[kBlock / kWave];
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const uint32_t base = blockIdx.x * kSortTile;
+  const uint32_t tileCount = min<uint32_t>(kSortTile, n - base);
+
+  // Tile histogram -> tile-local digit starts.
+  run[tid] = 0;
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    waveCnt[w][tid] = 0;
+  }
+  __syncthreads();
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t idx = base + k * kBlock + tid;
+    const bool valid = idx < n;
+    const uint32_t d = valid ? (keysIn[idx] >> shift) & 255u : 0u;
+    const uint64_t peers = matchDigit8(d, valid);
+    if (valid && (peers & laneMaskLt()) == 0) {
+      atomicAdd(&run[d], static_cast<uint32_t>(__popcll(peers)));
+    }
+  }
+  __syncthreads();
+  uint32_t total;
+  digitStart[tid] = blockExclusiveScan(run[tid], scratch, &total);
+  goff[tid] = offsets[tid * tiles + blockIdx.x];
+  __syncthreads();
+  run[tid] = 0;
+  __syncthreads();
+
+  // Rank in tile order (k, tid) and stage.
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t idx = base + k * kBlock + tid;
+    const bool valid = idx < n;
+    const uint32_t key = valid ? keysIn[idx] : 0u;
+    const uint32_t val = valid ? valsIn[idx] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    const uint64_t peers = matchDigit8(d, valid);
+    const uint32_t wrank = static_cast<uint32_t>(__popcll(peers & laneMaskLt()));
+    if (valid && wrank == 0) {
+      waveCnt[wave][d] = static_cast<uint32_t>(__popcll(peers));
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t pre = run[d];
+      for (int w = 0; w < wave; ++w) {
+        pre += waveCnt[w][d];
+      }
+      const uint32_t pos = digitStart[d] + pre + wrank;
+      sk[pos] = key;
+      sv[pos] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      add += waveCnt[w][tid];
+      waveCnt[w][tid] = 0;
+    }
+    run[tid] += add;
+    __syncthreads();
+  }
+
+  for (uint32_t p = tid; p < tileCount; p += kBlock) {
+    const uint32_t key = sk[p];
+    const uint32_t d = (key >> shift) & 255u;
+    const uint32_t dst = goff[d] + (p - digitStart[d]);
+    keysOut[dst] = key;
+    valsOut[dst] = sv[p];
+  }
+}
+
+// ---- exact k-th smallest of non-negative floats (radix select) ----
+// Three passes over 11, 11 and 10 bits of the float's bit pattern; the state
+// (decided prefix, remaining rank) lives in Ctl.
+
+__global__ __launch_bounds__(kBlock) void k_select_hist(
+    uint32_t n, const float *__restrict__ v, int pass,
+    const Ctl *__restrict__ ctl, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t h[kSelBins];
+  for (int b = threadIdx.x; b < kSelBins; b += kBlock) {
+    h[b] = 0;
+  }
+  __syncthreads();
+  const int shift = pass == 0 ? 21 : (pass == 1 ? 10 : 0);
+  const uint32_t mask = pass == 2 ? 1023u : 2047u;
+  const int keep = 32 - (pass == 0 ? 32 : (pass == 1 ? 21 : 10)); // decided bits
+  const uint32_t prefix = ctl->selPrefix;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += gridDim.x * kBlock) {
+    const uint32_t u = __float_as_uint(v[i]);
+    const bool match = keep == 0 || (u >> (32 - keep)) == (prefix >> (32 - keep));
+    if (match) {
+      atomicAdd(&h[(u >> shift) & mask], 1u);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kSelBins; b += kBlock) {
+    if (h[b]) {
+      atomicAdd(&hist[b], h[b]);
+    }
+  }
+}
+
+// One workgroup: picks the bin holding the remaining rank, clears the
+// histogram; after the last pass turns the selected width into the
+// collision grid cell (Collision.cpp:182-189).
+__global__ __launch_bounds__(kBlock) void k_select_pick(
+    int pass, Ctl *__restrict__ ctl, uint32_t *__restrict__ hist) {
+  __shared__ uint32_t chunk[kBlock];
+  __shared__ uint32_t scratch[kBlock / kWave];
+  const int per = kSelBins / kBlock; // 8 bins per thread
+  uint32_t local[kSelBins / kBlock];
+  uint32_t s = 0;
+  for (int k = 0; k < per; ++k) {
+    local[k] = hist[threadIdx.x * per + k];
+    s += local[k];
+  }
+  uint32_t total;
+  const uint32_t pre = blockExclusiveScan(s, scratch, &total);
+  const uint32_t rank = ctl->selRank;
+  chunk[threadIdx.x] = 0;
+  __syncthreads();
+  if (rank >= pre && rank < pre + s) {
+    uint32_t c = pre;
+    for (int k = 0; k < per; ++k) {
+      if (rank < c + local[k]) {
+        const int shift = pass == 0 ? 21 : (pass == 1 ? 10 : 0);
+        const uint32_t bin = threadIdx.x * per + k;
+        const uint32_t prefix = ctl->selPrefix | (bin << shift);
+        ctl->selPrefix = prefix;
+        ctl->selRank = rank - c;
+        if (pass == 2) {
+          const float w = __uint_as_float(prefix);
+          float cell = w * 1.01f;
+          if (!(cell > 0.0f)) {
+            cell = 1.0f;
+          }
+          ctl->cell = cell;
+          ctl->invCell = 1.0f / cell;
+        }
+        break;
+      }
+      c += local[k];
+    }
+  }
+  for (int k = 0; k < per; ++k) {
+    hist[threadIdx.x * per + k] = 0;
+  }
+}
+
+} // namespace nbx

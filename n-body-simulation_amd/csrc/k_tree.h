@@ -1,0 +1,611 @@
+// k_tree.h -- quadtree build on the GPU, reproducing QuadTree::build
+// (QuadTree.cpp:154-262) node for node.
+//
+// Keys. Each body gets a 32-level quadrant key by replaying the reference's
+// float descent: at every level the digit is 2*(y >= cy) + (x >= cx) against
+// the node centre, and the centre moves by Quad::intoQuadrant
+// (QuadTree.h:14-25). The reference partitions bodies with exactly these
+// comparisons, so sorting by the key groups bodies exactly as its
+// partitions do. Depths 0-15 live in the high word (radix-sorted); depths
+// 16-31 in the low word, which matters only inside runs of more than 16
+// bodies with equal high words (sorted separately, rarely non-empty).
+//
+// Topology without a per-level loop. A node at depth d is a run of sorted
+// bodies sharing d digits; it splits when it holds more than 16 bodies. With
+// lcp(a, b) = number of digits shared by sorted bodies a and b, body j
+// starts a splitting run at depth d iff
+//     (j == 0 or lcp(j-1, j) < d)  and  lcp(j, j+16) >= d  and  d <= 31,
+// a contiguous range of depths per body. The reference places the children
+// of level d's k-th splitting node at levelEnd(d) + 4k (QuadTree.cpp:
+// 176-204), and nodes within a level are in key order, so ranking the run
+// starts per depth gives every node index directly: one counting pass, one
+// scan per depth, one emit pass, one node-creation pass.
+#pragma once
+
+#include "k_sort.h"
+#include "nbx_device.h"
+
+namespace nbx {
+
+// Exclusive scan across a block of BS threads (BS a multiple of 64).
+template <int BS>
+__device__ inline uint32_t blockScanT(uint32_t v, uint32_t *scratch,
+                                      uint32_t *total) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, kWave);
+    if (lane >= off) {
+      x += y;
+    }
+  }
+  if (lane == kWave - 1) {
+    scratch[wave] = x;
+  }
+  __syncthreads();
+  uint32_t waveOff = 0, sum = 0;
+  for (int w = 0; w < BS / kWave; ++w) {
+    const uint32_t s = scratch[w];
+    waveOff += (w < wave) ? s : 0u;
+    sum += s;
+  }
+  __syncthreads();
+  *total = sum;
+  return waveOff + x - v;
+}
+
+__global__ void k_build_reset(Ctl *__restrict__ ctl) {
+  ctl->bbox[0] = 0xffffffffu;
+  ctl->bbox[1] = 0u;
+  ctl->bbox[2] = 0xffffffffu;
+  ctl->bbox[3] = 0u;
+  ctl->deepCount = 0;
+  ctl->status &= ~(kStatusNodeOverflow | kStatusDeepTooLarge);
+}
+
+// Quad::newContaining (QuadTree.cpp:30-51): exact min/max via order-
+// preserving integer atomics (NaN coordinates are ignored, as std::min/max
+// ignore them there).
+__global__ __launch_bounds__(kBlock) void k_bbox(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    Ctl *__restrict__ ctl) {
+  uint32_t mnx = 0xffffffffu, mxx = 0u, mny = 0xffffffffu, mxy = 0u;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += gridDim.x * kBlock) {
+    const float x = px[i], y = py[i];
+    if (x == x) {
+      const uint32_t u = ordBits(x);
+      mnx = min(mnx, u);
+      mxx = max(mxx, u);
+    }
+    if (y == y) {
+      const uint32_t u = ordBits(y);
+      mny = min(mny, u);
+      mxy = max(mxy, u);
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    mnx = min(mnx, static_cast<uint32_t>(__shfl_xor(mnx, off, kWave)));
+    mxx = max(mxx, static_cast<uint32_t>(__shfl_xor(mxx, off, kWave)));
+    mny = min(mny, static_cast<uint32_t>(__shfl_xor(mny, off, kWave)));
+    mxy = max(mxy, static_cast<uint32_t>(__shfl_xor(mxy, off, kWave)));
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicMin(&ctl->bbox[0], mnx);
+    atomicMax(&ctl->bbox[1], mxx);
+    atomicMin(&ctl->bbox[2], mny);
+    atomicMax(&ctl->bbox[3], mxy);
+  }
+}
+
+// Root quad (QuadTree.cpp:46-50) and the float-descent key of every body.
+__global__ __launch_bounds__(kBlock) void k_keys(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    Ctl *__restrict__ ctl, uint32_t *__restrict__ keyHi,
+    uint32_t *__restrict__ keyLo, uint32_t *__restrict__ vals) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const float minX = ordFloat(ctl->bbox[0]), maxX = ordFloat(ctl->bbox[1]);
+  const float minY = ordFloat(ctl->bbox[2]), maxY = ordFloat(ctl->bbox[3]);
+  const float qcx = (minX + maxX) * 0.5f;
+  const float qcy = (minY + maxY) * 0.5f;
+  const float wx = maxX - minX, wy = maxY - minY;
+  const float qs = (wx < wy) ? wy : wx;
+  if (i == 0) {
+    ctl->quad[0] = qcx;
+    ctl->quad[1] = qcy;
+    ctl->quad[2] = qs;
+  }
+  if (i >= n) {
+    return;
+  }
+  const float x = px[i], y = py[i];
+  float cx = qcx, cy = qcy, s = qs;
+  uint32_t hi = 0, lo = 0;
+#pragma unroll 4
+  for (int d = 0; d < 16; ++d) {
+    const int bx = (x < cx) ? 0 : 1;
+    const int by = (y < cy) ? 0 : 1;
+    hi |= static_cast<uint32_t>(2 * by + bx) << (30 - 2 * d);
+    s *= 0.5f;
+    cx += (static_cast<float>(bx) - 0.5f) * s;
+    cy += (static_cast<float>(by) - 0.5f) * s;
+  }
+#pragma unroll 4
+  for (int d = 0; d < 16; ++d) {
+    const int bx = (x < cx) ? 0 : 1;
+    const int by = (y < cy) ? 0 : 1;
+    lo |= static_cast<uint32_t>(2 * by + bx) << (30 - 2 * d);
+    s *= 0.5f;
+    cx += (static_cast<float>(bx) - 0.5f) * s;
+    cy += (static_cast<float>(by) - 0.5f) * s;
+  }
+  keyHi[i] = hi;
+  keyLo[i] = lo;
+  vals[i] = i;
+}
+
+// Runs of more than 16 bodies with equal high key words: their order by the
+// low word decides nodes below depth 16.
+__global__ __launch_bounds__(kBlock) void k_deep_detect(
+    uint32_t n, const uint32_t *__restrict__ hi, Ctl *__restrict__ ctl,
+    int2 *__restrict__ deepList) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j + kLeafCapacity >= n) {
+    return;
+  }
+  const uint32_t h = hi[j];
+  if ((j > 0 && hi[j - 1] == h) || hi[j + kLeafCapacity] != h) {
+    return;
+  }
+  uint32_t a = j + kLeafCapacity + 1, b = n;
+  while (a < b) {
+    const uint32_t mid = (a + b) >> 1;
+    if (hi[mid] == h) {
+      a = mid + 1;
+    } else {
+      b = mid;
+    }
+  }
+  const int slot = atomicAdd(&ctl->deepCount, 1);
+  deepList[slot] = make_int2(static_cast<int>(j), static_cast<int>(a - j));
+}
+
+// One workgroup per deep run: stable bitonic sort of its permutation
+// entries by low key word, in LDS.
+__global__ __launch_bounds__(kBlock) void k_deep_sort(
+    Ctl *__restrict__ ctl, const int2 *__restrict__ deepList,
+    const uint32_t *__restrict__ keyLo, uint32_t *__restrict__ perm) {
+  __shared__ uint32_t sk[kDeepMax];
+  __shared__ uint32_t sv[kDeepMax];
+  __shared__ int16_t order[kDeepMax];
+  const int count = ctl->deepCount;
+  for (int g = blockIdx.x; g < count; g += gridDim.x) {
+    const int2 seg = deepList[g];
+    const int start = seg.x, len = seg.y;
+    if (len > kDeepMax) {
+      if (threadIdx.x == 0) {
+        atomicOr(&ctl->status, kStatusDeepTooLarge);
+      }
+      continue;
+    }
+    int p2 = 1;
+    while (p2 < len) {
+      p2 <<= 1;
+    }
+    for (int k = threadIdx.x; k < p2; k += kBlock) {
+      if (k < len) {
+        sv[k] = perm[start + k];
+        sk[k] = keyLo[sv[k]];
+      }
+      order[k] = static_cast<int16_t>(k);
+    }
+    __syncthreads();
+    for (int size = 2; size <= p2; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int k = threadIdx.x; k < p2; k += kBlock) {
+          const int partner = k ^ stride;
+          if (partner > k) {
+            const bool up = (k & size) == 0;
+            const int a = order[k], c = order[partner];
+            // (key, position) order; padding sorts last.
+            bool aGreater;
+            if (a >= len || c >= len) {
+              aGreater = a > c;
+            } else {
+              aGreater = sk[a] > sk[c] || (sk[a] == sk[c] && a > c);
+            }
+            if (aGreater == up) {
+              order[k] = static_cast<int16_t>(c);
+              order[partner] = static_cast<int16_t>(a);
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int k = threadIdx.x; k < len; k += kBlock) {
+      perm[start + k] = sv[order[k]];
+    }
+    __syncthreads();
+  }
+}
+
+// Applies the sort permutation to every body array (CudaQuadTree.cu:
+// 408-414 does this with seven launches; here it is one pass).
+__global__ __launch_bounds__(kBlock) void k_permute(
+    uint32_t n, const uint32_t *__restrict__ perm,
+    const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ vx, const float *__restrict__ vy,
+    const float *__restrict__ m, const float *__restrict__ r,
+    const uint32_t *__restrict__ id, float *__restrict__ opx,
+    float *__restrict__ opy, float *__restrict__ ovx, float *__restrict__ ovy,
+    float *__restrict__ om, float *__restrict__ orr,
+    uint32_t *__restrict__ oid) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) {
+    return;
+  }
+  const uint32_t s = perm[j];
+  opx[j] = px[s];
+  opy[j] = py[s];
+  ovx[j] = vx[s];
+  ovy[j] = vy[s];
+  om[j] = m[s];
+  orr[j] = r[s];
+  oid[j] = id[s];
+}
+
+// Depth range [A, B] over which sorted body j starts a splitting run.
+__device__ inline void splitRange(uint32_t n, const uint32_t *__restrict__ hi,
+                                  const uint32_t *__restrict__ lo,
+                                  const uint32_t *__restrict__ perm,
+                                  uint32_t j, int &A, int &B) {
+  if (j >= n) {
+    A = 1;
+    B = 0;
+    return;
+  }
+  A = j == 0 ? 0 : commonDigits(hi, lo, perm, j - 1, j) + 1;
+  if (j + kLeafCapacity < n) {
+    const int c = commonDigits(hi, lo, perm, j, j + kLeafCapacity);
+    B = c < kMaxDepth - 1 ? c : kMaxDepth - 1;
+  } else {
+    B = -1;
+  }
+}
+
+// Per 256-body tile: number of splitting runs starting in it, per depth ->
+// tileCount[d * tiles + tile].
+__global__ __launch_bounds__(kBlock) void k_lcp_count(
+    uint32_t n, const uint32_t *__restrict__ hi,
+    const uint32_t *__restrict__ lo, const uint32_t *__restrict__ perm,
+    uint32_t tiles, uint32_t *__restrict__ tileCount) {
+  __shared__ uint32_t waveCnt[kBlock / kWave][kMaxDepth];
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+  int A, B;
+  splitRange(n, hi, lo, perm, j, A, B);
+  for (int d = 0; d < kMaxDepth; ++d) {
+    const uint64_t mask = __ballot(A <= d && d <= B);
+    if (lane == 0) {
+      waveCnt[wave][d] = static_cast<uint32_t>(__popcll(mask));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxDepth) {
+    uint32_t s = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      s += waveCnt[w][threadIdx.x];
+    }
+    tileCount[threadIdx.x * tiles + blockIdx.x] = s;
+  }
+}
+
+// Block d: exclusive scan over tiles of depth d's counts (in place) and the
+// depth total.
+__global__ __launch_bounds__(1024) void k_level_scan(
+    uint32_t tiles, uint32_t *__restrict__ tileCount, Ctl *__restrict__ ctl) {
+  __shared__ uint32_t scratch[1024 / kWave];
+  const int d = blockIdx.x;
+  uint32_t *row = tileCount + static_cast<size_t>(d) * tiles;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < tiles; base += 1024) {
+    const uint32_t t = base + threadIdx.x;
+    const uint32_t v = t < tiles ? row[t] : 0u;
+    uint32_t total;
+    const uint32_t pre = blockScanT<1024>(v, scratch, &total);
+    if (t < tiles) {
+      row[t] = carry + pre;
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) {
+    ctl->splits[d] = static_cast<int32_t>(carry);
+  }
+}
+
+// Level bookkeeping (QuadTree.cpp:188-204) for all depths at once, the
+// capacity check and the root node.
+__global__ void k_level_finalize(uint32_t n, int64_t capacity,
+                                 Ctl *__restrict__ ctl,
+                                 float4 *__restrict__ nodeF,
+                                 int4 *__restrict__ nodeI,
+                                 float *__restrict__ centerX,
+                                 float *__restrict__ centerY,
+                                 float *__restrict__ size,
+                                 int32_t *__restrict__ nodeOfSplit) {
+  int32_t base = 0;
+  for (int d = 0; d < kMaxDepth; ++d) {
+    ctl->splitBase[d] = base;
+    base += ctl->splits[d];
+  }
+  ctl->splitBase[kMaxDepth] = base;
+  ctl->totalSplits = base;
+  int64_t start = 1;
+  ctl->levelStart[0] = 0;
+  ctl->levelStart[1] = 1;
+  int levels = 1;
+  for (int d = 1; d <= kMaxDepth; ++d) {
+    const int64_t cnt = 4ll * ctl->splits[d - 1];
+    if (cnt > 0) {
+      levels = d + 1;
+    }
+    start += cnt;
+    ctl->levelStart[d + 1] = static_cast<int32_t>(start < 0x7fffffff ? start : 0x7fffffff);
+  }
+  ctl->levels = levels;
+  if (start > capacity) {
+    ctl->nodeCount = 0;
+    ctl->status |= kStatusNodeOverflow;
+    return;
+  }
+  ctl->nodeCount = static_cast<int32_t>(start);
+  const float s = ctl->quad[2];
+  nodeI[0] = make_int4(ctl->splits[0] > 0 ? 1 : 0, 0, 0, static_cast<int>(n));
+  nodeF[0] = make_float4(0.0f, 0.0f, 0.0f, s * s);
+  centerX[0] = ctl->quad[0];
+  centerY[0] = ctl->quad[1];
+  size[0] = s;
+  nodeOfSplit[0] = 0;
+}
+
+// Writes every splitting run start, ranked within its depth, into the split
+// list: splitList[splitBase[d] + rank] = j.
+__global__ __launch_bounds__(kBlock) void k_emit(
+    uint32_t n, const uint32_t *__restrict__ hi,
+    const uint32_t *__restrict__ lo, const uint32_t *__restrict__ perm,
+    uint32_t tiles, const uint32_t *__restrict__ tileOffset,
+    const Ctl *__restrict__ ctl, int32_t *__restrict__ splitList) {
+  __shared__ uint32_t waveCnt[kBlock / kWave][kMaxDepth];
+  if (ctl->status & kStatusNodeOverflow) {
+    return;
+  }
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+  int A, B;
+  splitRange(n, hi, lo, perm, j, A, B);
+  for (int d = 0; d < kMaxDepth; ++d) {
+    const uint64_t mask = __ballot(A <= d && d <= B);
+    if (lane == 0) {
+      waveCnt[wave][d] = static_cast<uint32_t>(__popcll(mask));
+    }
+  }
+  __syncthreads();
+  const uint64_t lt = laneMaskLt();
+  for (int d = 0; d < kMaxDepth; ++d) {
+    const bool flag = A <= d && d <= B;
+    const uint64_t mask = __ballot(flag);
+    if (flag) {
+      uint32_t rank = tileOffset[static_cast<size_t>(d) * tiles + blockIdx.x] +
+                      static_cast<uint32_t>(__popcll(mask & lt));
+      for (int w = 0; w < wave; ++w) {
+        rank += waveCnt[w][d];
+      }
+      splitList[ctl->splitBase[d] + rank] = static_cast<int32_t>(j);
+    }
+  }
+}
+
+// Creates the 4 children of every splitting node (QuadTree::subdivide,
+// QuadTree.cpp:55-91): body ranges split at the digit boundaries, child quads
+// by intoQuadrant, and each child's own `children` slot if it splits.
+__global__ __launch_bounds__(kBlock) void k_nodes(
+    uint32_t n, const uint32_t *__restrict__ hi,
+    const uint32_t *__restrict__ lo, const uint32_t *__restrict__ perm,
+    const Ctl *__restrict__ ctl, const int32_t *__restrict__ splitList,
+    float4 *__restrict__ nodeF, int4 *__restrict__ nodeI,
+    float *__restrict__ centerX, float *__restrict__ centerY,
+    float *__restrict__ size, int32_t *__restrict__ nodeOfSplit) {
+  if (ctl->status & kStatusNodeOverflow) {
+    return;
+  }
+  const int total = ctl->totalSplits;
+  for (int s = blockIdx.x * kBlock + threadIdx.x; s < total;
+       s += gridDim.x * kBlock) {
+    int d = 0;
+    while (s >= ctl->splitBase[d + 1]) {
+      ++d;
+    }
+    const int rank = s - ctl->splitBase[d];
+    const int j = splitList[s];
+    // Run end: first body not sharing d digits with j.
+    int e = static_cast<int>(n);
+    if (d > 0) {
+      int a = j + kLeafCapacity + 1, b = static_cast<int>(n);
+      while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (commonDigits(hi, lo, perm, j, mid) >= d) {
+          a = mid + 1;
+        } else {
+          b = mid;
+        }
+      }
+      e = a;
+    }
+    // This node's quad: replay intoQuadrant from the root.
+    float cx = ctl->quad[0], cy = ctl->quad[1], sz = ctl->quad[2];
+    for (int t = 0; t < d; ++t) {
+      const int q = digitAt(hi, lo, perm, j, t);
+      sz *= 0.5f;
+      cx += (static_cast<float>(q & 1) - 0.5f) * sz;
+      cy += (static_cast<float>(q >> 1) - 0.5f) * sz;
+    }
+    int bound[5];
+    bound[0] = j;
+    bound[4] = e;
+    for (int q = 1; q < 4; ++q) {
+      int a = bound[q - 1], b = e;
+      while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (digitAt(hi, lo, perm, mid, d) < q) {
+          a = mid + 1;
+        } else {
+          b = mid;
+        }
+      }
+      bound[q] = a;
+    }
+    const int c = ctl->levelStart[d + 1] + 4 * rank;
+    const float childSize = sz * 0.5f;
+    for (int q = 0; q < 4; ++q) {
+      const int ch = c + q;
+      const int cs = bound[q], ce = bound[q + 1];
+      int grand = 0;
+      if (ce - cs > kLeafCapacity && d + 1 < kMaxDepth) {
+        // This child splits too: find its rank among depth d+1 run starts.
+        const int base = ctl->splitBase[d + 1];
+        int a = base, b = ctl->splitBase[d + 2];
+        while (a < b) {
+          const int mid = (a + b) >> 1;
+          if (splitList[mid] < cs) {
+            a = mid + 1;
+          } else {
+            b = mid;
+          }
+        }
+        if (a < ctl->splitBase[d + 2] && splitList[a] == cs) {
+          grand = ctl->levelStart[d + 2] + 4 * (a - base);
+          nodeOfSplit[a] = ch;
+        } else {
+          atomicOr(&const_cast<Ctl *>(ctl)->status, kStatusTreeInconsistent);
+        }
+      }
+      const float ccx = cx + (static_cast<float>(q & 1) - 0.5f) * childSize;
+      const float ccy = cy + (static_cast<float>(q >> 1) - 0.5f) * childSize;
+      centerX[ch] = ccx;
+      centerY[ch] = ccy;
+      size[ch] = childSize;
+      nodeF[ch] = make_float4(0.0f, 0.0f, 0.0f, childSize * childSize);
+      nodeI[ch] = make_int4(grand, 0, cs, ce);
+    }
+  }
+}
+
+// Depth of node x (x >= 1): largest D with levelStart[D] <= x.
+__device__ inline int nodeDepth(const Ctl *__restrict__ ctl, int x) {
+  int d = 1;
+  while (d < kMaxDepth && ctl->levelStart[d + 1] <= x) {
+    ++d;
+  }
+  return d;
+}
+
+// Per node: the `next` skip pointer (QuadTree.cpp:87-90: child q < 3 points
+// at q + 1, the last child inherits its parent's), and for leaves the mass
+// and weighted-position sums (accumulateLeaf, QuadTree.cpp:93-142, four
+// interleaved accumulators) and centre of mass (QuadTree.cpp:256-261).
+__global__ __launch_bounds__(kBlock) void k_node_finish(
+    const Ctl *__restrict__ ctl, const float *__restrict__ px,
+    const float *__restrict__ py, const float *__restrict__ m,
+    const int32_t *__restrict__ nodeOfSplit, float4 *__restrict__ nodeF,
+    int4 *__restrict__ nodeI, float2 *__restrict__ wsum) {
+  if (ctl->status & kStatusNodeOverflow) {
+    return;
+  }
+  const int count = ctl->nodeCount;
+  for (int x = blockIdx.x * kBlock + threadIdx.x; x < count;
+       x += gridDim.x * kBlock) {
+    int y = x;
+    for (int guard = 0; y != 0 && ((y - 1) & 3) == 3; ++guard) {
+      const int D = nodeDepth(ctl, y);
+      y = nodeOfSplit[ctl->splitBase[D - 1] + (y - ctl->levelStart[D]) / 4];
+      if (guard > kMaxDepth || y < 0 || y >= count) {
+        atomicOr(&const_cast<Ctl *>(ctl)->status, kStatusTreeInconsistent);
+        y = 0;
+        break;
+      }
+    }
+    const int4 ni = nodeI[x];
+    nodeI[x].y = y == 0 ? 0 : y + 1;
+    if (ni.x != 0) {
+      continue;
+    }
+    float p0x = 0, p0y = 0, p1x = 0, p1y = 0, p2x = 0, p2y = 0, p3x = 0,
+          p3y = 0;
+    float m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    int i = ni.z;
+    const int e = ni.w;
+    for (; i + 3 < e; i += 4) {
+      p0x += px[i] * m[i];
+      p0y += py[i] * m[i];
+      m0 += m[i];
+      p1x += px[i + 1] * m[i + 1];
+      p1y += py[i + 1] * m[i + 1];
+      m1 += m[i + 1];
+      p2x += px[i + 2] * m[i + 2];
+      p2y += py[i + 2] * m[i + 2];
+      m2 += m[i + 2];
+      p3x += px[i + 3] * m[i + 3];
+      p3y += py[i + 3] * m[i + 3];
+      m3 += m[i + 3];
+    }
+    for (; i < e; ++i) {
+      p0x += px[i] * m[i];
+      p0y += py[i] * m[i];
+      m0 += m[i];
+    }
+    const float wx = p0x + p1x + p2x + p3x;
+    const float wy = p0y + p1y + p2y + p3y;
+    const float ms = m0 + m1 + m2 + m3;
+    const float dv = (ms < FLT_MIN) ? FLT_MIN : ms;
+    wsum[x] = make_float2(wx, wy);
+    nodeF[x].x = wx / dv;
+    nodeF[x].y = wy / dv;
+    nodeF[x].z = ms;
+  }
+}
+
+// Branch sums for depth d from its children (QuadTree.cpp:239-254), then the
+// centre of mass. Launched deepest level first.
+__global__ __launch_bounds__(kBlock) void k_upward_level(
+    int d, const Ctl *__restrict__ ctl, float4 *__restrict__ nodeF,
+    const int4 *__restrict__ nodeI, float2 *__restrict__ wsum) {
+  if (ctl->status & kStatusNodeOverflow) {
+    return;
+  }
+  const int lo = ctl->levelStart[d];
+  const int hi = min(ctl->levelStart[d + 1], ctl->nodeCount);
+  for (int x = lo + blockIdx.x * kBlock + threadIdx.x; x < hi;
+       x += gridDim.x * kBlock) {
+    const int c = nodeI[x].x;
+    if (c == 0) {
+      continue;
+    }
+    const float2 w0 = wsum[c], w1 = wsum[c + 1], w2 = wsum[c + 2],
+                 w3 = wsum[c + 3];
+    const float wx = w0.x + w1.x + w2.x + w3.x;
+    const float wy = w0.y + w1.y + w2.y + w3.y;
+    const float ms = nodeF[c].z + nodeF[c + 1].z + nodeF[c + 2].z +
+                     nodeF[c + 3].z;
+    const float dv = (ms < FLT_MIN) ? FLT_MIN : ms;
+    wsum[x] = make_float2(wx, wy);
+    nodeF[x].x = wx / dv;
+    nodeF[x].y = wy / dv;
+    nodeF[x].z = ms;
+  }
+}
+
+} // namespace nbx

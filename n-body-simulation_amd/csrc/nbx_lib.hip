@@ -1,0 +1,1050 @@
+// nbx_lib.hip -- libnbx runtime: device buffers, launch sequences, HIP graphs
+// and the C ABI declared in include/nbx.h.
+//
+// One handle = one simulation resident in HBM: body arrays (double-buffered
+// so the build's permutation is a single gather pass), collision scratch,
+// sort scratch and tree storage, all allocated once at nbx_create. A step
+// is a fixed sequence of kernel launches on the handle's stream whose sizes
+// live on the device (Ctl), so it needs no host round trip and can be
+// captured into a HIP graph (one per buffer parity).
+#include "nbx.h"
+
+#include "k_collide.h"
+#include "k_sort.h"
+#include "k_tree.h"
+#include "k_walk.h"
+#include "nbx_device.h"
+#include "nbx_host.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace nbx;
+
+namespace {
+
+thread_local std::string g_error;
+
+int setError(int code, const std::string &msg) {
+  g_error = msg;
+  return code;
+}
+
+#define NBX_HIP(call)                                                         \
+  do {                                                                        \
+    hipError_t nbxErr_ = (call);                                              \
+    if (nbxErr_ != hipSuccess) {                                              \
+      return setError(NBX_ERR_HIP, std::string(#call) + " failed: " +         \
+                                       hipGetErrorString(nbxErr_));           \
+    }                                                                         \
+  } while (0)
+
+#define NBX_TRY(expr)                                                         \
+  do {                                                                        \
+    int nbxRc_ = (expr);                                                      \
+    if (nbxRc_ != NBX_OK) {                                                   \
+      return nbxRc_;                                                          \
+    }                                                                         \
+  } while (0)
+
+uint32_t blocksFor(uint64_t n, uint64_t per = kBlock) {
+  return static_cast<uint32_t>(std::max<uint64_t>(1, (n + per - 1) / per));
+}
+
+uint64_t pow2AtLeast(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) {
+    p <<= 1;
+  }
+  return p;
+}
+
+constexpr int kUpwardDepths = kMaxDepth; // branches exist at depths 0..31
+
+} // namespace
+
+struct nbx_sim {
+  uint64_t cap = 0;
+  uint64_t n = 0;
+  int cur = 0; // body buffer holding the state
+  uint64_t frame = 0;
+  nbx_params params{};
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+
+  // bodies
+  float *px[2]{}, *py[2]{}, *vx[2]{}, *vy[2]{}, *m[2]{}, *r[2]{};
+  uint32_t *id[2]{};
+  float *ax = nullptr, *ay = nullptr;
+
+  // collide
+  float *widths = nullptr;
+  uint32_t *selHist = nullptr;
+  uint64_t buckets = 0; // power of two >= 2 * capacity
+  uint32_t *bucketOf = nullptr, *bucketCount = nullptr, *bucketStart = nullptr;
+  uint32_t *scanTiles = nullptr;
+  float4 *gridBox = nullptr;
+  int4 *gridCell = nullptr;
+  int32_t *largeList = nullptr, *ufParent = nullptr;
+  uint32_t pairCap = 0;
+  int4 *pairs = nullptr, *segPairs = nullptr;
+  int32_t *pairComp = nullptr;
+  uint32_t *compCount = nullptr, *compStart = nullptr;
+  int4 *compInfo = nullptr;
+  int32_t *bigList = nullptr;
+
+  // tree
+  uint32_t *keyHi[2]{}, *vals[2]{};
+  uint32_t *keyLo = nullptr, *sortHist = nullptr;
+  int2 *deepList = nullptr;
+  uint32_t *tileCount = nullptr;
+  int64_t nodeCap = 0;
+  int32_t *splitList = nullptr, *nodeOfSplit = nullptr;
+  float4 *nodeF = nullptr;
+  int4 *nodeI = nullptr;
+  float *centerX = nullptr, *centerY = nullptr, *size = nullptr;
+  float2 *wsum = nullptr;
+  bool treeBuilt = false;
+  bool collided = false;
+
+  Ctl *ctl = nullptr;
+
+  // profiling
+  bool profiling = false;
+  hipEvent_t ev[NBX_PHASES + 1]{};
+  double phaseMs[NBX_PHASES]{};
+  uint64_t profSteps = 0;
+  bool pendingEvents = false;
+
+  // captured steps, one per starting buffer
+  hipGraphExec_t graph[2]{};
+};
+
+namespace {
+
+template <class T>
+int devAlloc(nbx_sim *s, T **p, uint64_t count, bool zero = false) {
+  void *q = nullptr;
+  const size_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+  NBX_HIP(hipMalloc(&q, bytes));
+  s->allocs.push_back(q);
+  if (zero) {
+    NBX_HIP(hipMemset(q, 0, bytes));
+  }
+  *p = static_cast<T *>(q);
+  return NBX_OK;
+}
+
+void dropGraphs(nbx_sim *s) {
+  for (auto &g : s->graph) {
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  }
+}
+
+int checkLaunch() {
+  NBX_HIP(hipGetLastError());
+  return NBX_OK;
+}
+
+#define LAUNCH(kernel, grid, block, ...)                                      \
+  do {                                                                        \
+    hipLaunchKernelGGL((kernel), dim3(grid), dim3(block), 0, s->stream,       \
+                       __VA_ARGS__);                                          \
+    NBX_TRY(checkLaunch());                                                   \
+  } while (0)
+
+int launchIterate(nbx_sim *s);
+int launchCollide(nbx_sim *s);
+int launchBuild(nbx_sim *s);
+int launchForces(nbx_sim *s, bool count);
+
+__global__ __launch_bounds__(kBlock) void k_iterate(
+    uint32_t n, float dt, float *__restrict__ px, float *__restrict__ py,
+    float *__restrict__ vx, float *__restrict__ vy,
+    const float *__restrict__ ax, const float *__restrict__ ay) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) {
+    return;
+  }
+  // Body::update (Body.h:20-23): kick with last step's acceleration, drift.
+  const float nvx = vx[i] + ax[i] * dt;
+  const float nvy = vy[i] + ay[i] * dt;
+  vx[i] = nvx;
+  vy[i] = nvy;
+  px[i] = px[i] + nvx * dt;
+  py[i] = py[i] + nvy * dt;
+}
+
+__global__ void k_zero_counts(Ctl *__restrict__ ctl) {
+  ctl->farCount = 0;
+  ctl->bodyPairCount = 0;
+}
+
+int launchIterate(nbx_sim *s) {
+  const uint32_t n = static_cast<uint32_t>(s->n);
+  if (n == 0) {
+    return NBX_OK;
+  }
+  const int c = s->cur;
+  LAUNCH(k_iterate, blocksFor(n), kBlock, n, s->params.dt, s->px[c], s->py[c],
+         s->vx[c], s->vy[c], s->ax, s->ay);
+  return NBX_OK;
+}
+
+int launchScan(nbx_sim *s, uint32_t count, const uint32_t *in, uint32_t *out) {
+  const uint32_t tiles = blocksFor(count, kScanTile);
+  LAUNCH(k_scan_reduce, tiles, kBlock, count, in, s->scanTiles);
+  LAUNCH(k_scan_tiles, 1, kBlock, tiles, s->scanTiles);
+  LAUNCH(k_scan_down, tiles, kBlock, count, in, s->scanTiles, out);
+  return NBX_OK;
+}
+
+int launchCollide(nbx_sim *s) {
+  const uint32_t n = static_cast<uint32_t>(s->n);
+  if (n < 2) {
+    return NBX_OK;
+  }
+  const int c = s->cur;
+  float *px = s->px[c], *py = s->py[c], *vx = s->vx[c], *vy = s->vy[c];
+  const float *m = s->m[c], *r = s->r[c];
+  const uint32_t nb = blocksFor(n);
+  const uint32_t buckets =
+      static_cast<uint32_t>(std::max<uint64_t>(1024, pow2AtLeast(2ull * n)));
+  const uint32_t mask = buckets - 1;
+  const uint32_t pairGrid = std::min<uint32_t>(blocksFor(s->pairCap), 2048);
+
+  LAUNCH(k_collide_reset, 1, 1, n, s->ctl);
+  LAUNCH(k_widths, nb, kBlock, n, px, py, r, s->widths);
+  const uint32_t selGrid = std::min<uint32_t>(nb, 512);
+  for (int pass = 0; pass < 3; ++pass) {
+    LAUNCH(k_select_hist, selGrid, kBlock, n, s->widths, pass, s->ctl,
+           s->selHist);
+    LAUNCH(k_select_pick, 1, kBlock, pass, s->ctl, s->selHist);
+  }
+  LAUNCH(k_cells, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
+         s->bucketCount, s->largeList, s->ufParent);
+  NBX_TRY(launchScan(s, buckets + 1, s->bucketCount, s->bucketStart));
+  LAUNCH(k_grid_scatter, nb, kBlock, n, px, py, r, s->ctl, s->bucketOf,
+         s->bucketCount, s->bucketStart, s->gridBox, s->gridCell);
+  LAUNCH(k_find_small, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketStart,
+         s->gridBox, s->gridCell, s->largeList, s->pairs, s->pairCap);
+  LAUNCH(k_find_large, nb, kBlock, n, px, py, r, s->ctl, s->largeList,
+         s->pairs, s->pairCap);
+  LAUNCH(k_union, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs, s->ufParent);
+  LAUNCH(k_comp_count, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
+         s->ufParent, s->pairComp, s->compCount);
+  LAUNCH(k_comp_alloc, nb, kBlock, n, s->ctl, s->compCount, s->compStart,
+         s->compInfo);
+  LAUNCH(k_comp_scatter, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
+         s->pairComp, s->compCount, s->compStart, s->segPairs);
+  LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->segPairs,
+         s->bigList, px, py, vx, vy, m, r);
+  LAUNCH(k_resolve_big, 256, kBlock, s->ctl, s->compInfo, s->bigList,
+         s->segPairs, px, py, vx, vy, m, r);
+  s->collided = true;
+  return NBX_OK;
+}
+
+int launchBuild(nbx_sim *s) {
+  const uint32_t n = static_cast<uint32_t>(s->n);
+  if (n == 0) {
+    s->treeBuilt = false;
+    return NBX_OK;
+  }
+  const int c = s->cur, o = c ^ 1;
+  const uint32_t nb = blocksFor(n);
+  LAUNCH(k_build_reset, 1, 1, s->ctl);
+  LAUNCH(k_bbox, std::min<uint32_t>(nb, 1024), kBlock, n, s->px[c], s->py[c],
+         s->ctl);
+  LAUNCH(k_keys, nb, kBlock, n, s->px[c], s->py[c], s->ctl, s->keyHi[0],
+         s->keyLo, s->vals[0]);
+  // Stable LSD radix sort of the high key word, 4 x 8 bits; ends in buffer 0.
+  const uint32_t tiles = blocksFor(n, kSortTile);
+  for (int pass = 0; pass < 4; ++pass) {
+    const int src = pass & 1, dst = src ^ 1;
+    LAUNCH(k_rs_hist, tiles, kBlock, n, s->keyHi[src], 8 * pass, s->sortHist,
+           tiles);
+    LAUNCH(k_rs_scan, 1, 1024, 256u * tiles, s->sortHist);
+    LAUNCH(k_rs_scatter, tiles, kBlock, n, s->keyHi[src], s->vals[src],
+           8 * pass, s->sortHist, tiles, s->keyHi[dst], s->vals[dst]);
+  }
+  const uint32_t *hi = s->keyHi[0];
+  uint32_t *perm = s->vals[0];
+  LAUNCH(k_deep_detect, nb, kBlock, n, hi, s->ctl, s->deepList);
+  LAUNCH(k_deep_sort, 64, kBlock, s->ctl, s->deepList, s->keyLo, perm);
+  LAUNCH(k_permute, nb, kBlock, n, perm, s->px[c], s->py[c], s->vx[c],
+         s->vy[c], s->m[c], s->r[c], s->id[c], s->px[o], s->py[o], s->vx[o],
+         s->vy[o], s->m[o], s->r[o], s->id[o]);
+  s->cur = o;
+  const uint32_t tiles256 = nb;
+  LAUNCH(k_lcp_count, tiles256, kBlock, n, hi, s->keyLo, perm, tiles256,
+         s->tileCount);
+  LAUNCH(k_level_scan, kMaxDepth, 1024, tiles256, s->tileCount, s->ctl);
+  LAUNCH(k_level_finalize, 1, 1, n, s->nodeCap, s->ctl, s->nodeF, s->nodeI,
+         s->centerX, s->centerY, s->size, s->nodeOfSplit);
+  LAUNCH(k_emit, tiles256, kBlock, n, hi, s->keyLo, perm, tiles256,
+         s->tileCount, s->ctl, s->splitList);
+  const uint32_t nodeGrid = std::max<uint32_t>(64, std::min<uint32_t>(nb, 2048));
+  LAUNCH(k_nodes, nodeGrid, kBlock, n, hi, s->keyLo, perm, s->ctl,
+         s->splitList, s->nodeF, s->nodeI, s->centerX, s->centerY, s->size,
+         s->nodeOfSplit);
+  LAUNCH(k_node_finish, nodeGrid, kBlock, s->ctl, s->px[o], s->py[o], s->m[o],
+         s->nodeOfSplit, s->nodeF, s->nodeI, s->wsum);
+  const uint32_t upGrid = std::max<uint32_t>(16, std::min<uint32_t>(nb / 2, 1024));
+  for (int d = kUpwardDepths - 1; d >= 0; --d) {
+    LAUNCH(k_upward_level, upGrid, kBlock, d, s->ctl, s->nodeF, s->nodeI,
+           s->wsum);
+  }
+  s->treeBuilt = true;
+  return NBX_OK;
+}
+
+int launchForces(nbx_sim *s, bool count) {
+  const uint32_t n = static_cast<uint32_t>(s->n);
+  if (n == 0) {
+    return NBX_OK;
+  }
+  if (!s->treeBuilt) {
+    return setError(NBX_ERR_STATE, "forces requested before any tree build");
+  }
+  const int c = s->cur;
+  const float tSq = s->params.theta * s->params.theta;
+  const float eSq = s->params.epsilon * s->params.epsilon;
+  const uint32_t grid = blocksFor(n);
+  const bool ref = s->params.summation == NBX_SUM_REFERENCE;
+  const bool acc = s->params.law == NBX_LAW_ACCURATE;
+#define WALK(SUMV, LAWV, CNT)                                                 \
+  LAUNCH((k_walk<SUMV, LAWV, CNT>), grid, kBlock, n, s->nodeF, s->nodeI,      \
+         s->px[c], s->py[c], s->m[c], tSq, eSq, s->ax, s->ay, s->ctl)
+  if (count) {
+    if (ref) {
+      if (acc) WALK(kSumReference, kLawAccurate, true);
+      else WALK(kSumReference, kLawReference, true);
+    } else {
+      if (acc) WALK(kSumCompensated, kLawAccurate, true);
+      else WALK(kSumCompensated, kLawReference, true);
+    }
+  } else {
+    if (ref) {
+      if (acc) WALK(kSumReference, kLawAccurate, false);
+      else WALK(kSumReference, kLawReference, false);
+    } else {
+      if (acc) WALK(kSumCompensated, kLawAccurate, false);
+      else WALK(kSumCompensated, kLawReference, false);
+    }
+  }
+#undef WALK
+  return NBX_OK;
+}
+
+int record(nbx_sim *s, int k) {
+  if (s->profiling) {
+    NBX_HIP(hipEventRecord(s->ev[k], s->stream));
+  }
+  return NBX_OK;
+}
+
+int collectEvents(nbx_sim *s) {
+  if (!s->pendingEvents) {
+    return NBX_OK;
+  }
+  NBX_HIP(hipEventSynchronize(s->ev[NBX_PHASES]));
+  for (int p = 0; p < NBX_PHASES; ++p) {
+    float ms = 0.0f;
+    NBX_HIP(hipEventElapsedTime(&ms, s->ev[p], s->ev[p + 1]));
+    s->phaseMs[p] += ms;
+  }
+  s->profSteps += 1;
+  s->pendingEvents = false;
+  return NBX_OK;
+}
+
+// One simulation step (Simulation.cpp:11-16), eager.
+int launchStep(nbx_sim *s) {
+  NBX_TRY(record(s, NBX_PHASE_ITERATE));
+  NBX_TRY(launchIterate(s));
+  NBX_TRY(record(s, NBX_PHASE_COLLIDE));
+  if (s->params.collisions) {
+    NBX_TRY(launchCollide(s));
+  }
+  NBX_TRY(record(s, NBX_PHASE_BUILD));
+  NBX_TRY(launchBuild(s));
+  NBX_TRY(record(s, NBX_PHASE_WALK));
+  NBX_TRY(launchForces(s, false));
+  NBX_TRY(record(s, NBX_PHASES));
+  if (s->profiling) {
+    s->pendingEvents = true;
+    NBX_TRY(collectEvents(s));
+  }
+  return NBX_OK;
+}
+
+int captureStep(nbx_sim *s, int start) {
+  const int saved = s->cur;
+  const bool savedProf = s->profiling;
+  s->cur = start;
+  s->profiling = false;
+  hipGraph_t g = nullptr;
+  NBX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = launchStep(s);
+  hipError_t e = hipStreamEndCapture(s->stream, &g);
+  s->cur = saved;
+  s->profiling = savedProf;
+  if (rc != NBX_OK) {
+    if (g) {
+      (void)hipGraphDestroy(g);
+    }
+    return rc;
+  }
+  if (e != hipSuccess) {
+    return setError(NBX_ERR_HIP,
+                    std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  e = hipGraphInstantiate(&s->graph[start], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    s->graph[start] = nullptr;
+    return setError(NBX_ERR_HIP,
+                    std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return NBX_OK;
+}
+
+int readStatus(nbx_sim *s, uint32_t *status) {
+  NBX_HIP(hipMemcpyAsync(status, &s->ctl->status, sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, s->stream));
+  NBX_HIP(hipStreamSynchronize(s->stream));
+  return NBX_OK;
+}
+
+int syncAndCheck(nbx_sim *s) {
+  uint32_t status = 0;
+  NBX_TRY(readStatus(s, &status));
+  NBX_TRY(collectEvents(s));
+  if (status & kStatusNodeOverflow) {
+    return setError(NBX_ERR_CAPACITY, "tree node capacity exceeded");
+  }
+  if (status & kStatusPairOverflow) {
+    return setError(NBX_ERR_CAPACITY, "collision pair capacity exceeded");
+  }
+  if (status & kStatusDeepTooLarge) {
+    return setError(NBX_ERR_CAPACITY,
+                    "more than 4096 bodies share one depth-16 tree cell");
+  }
+  if (status & kStatusTreeInconsistent) {
+    return setError(NBX_ERR_STATE, "internal tree consistency check failed");
+  }
+  if (status & kStatusComponentTooLarge) {
+    return setError(NBX_ERR_CAPACITY,
+                    "a collision contact component exceeds 4096 pairs");
+  }
+  return NBX_OK;
+}
+
+int readCtl(nbx_sim *s, Ctl *out) {
+  NBX_HIP(hipMemcpyAsync(out, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
+                         s->stream));
+  NBX_HIP(hipStreamSynchronize(s->stream));
+  return NBX_OK;
+}
+
+template <class T>
+int copyOut(nbx_sim *s, T *dst, const T *src, uint64_t count) {
+  if (dst && count) {
+    NBX_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost,
+                           s->stream));
+  }
+  return NBX_OK;
+}
+
+template <class T>
+int copyIn(nbx_sim *s, T *dst, const T *src, uint64_t count) {
+  if (count) {
+    NBX_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice,
+                           s->stream));
+  }
+  return NBX_OK;
+}
+
+int zeroDevice(nbx_sim *s, void *dst, uint64_t bytes) {
+  if (bytes) {
+    NBX_HIP(hipMemsetAsync(dst, 0, bytes, s->stream));
+  }
+  return NBX_OK;
+}
+
+int validParams(const nbx_params *p) {
+  if (!(p->dt == p->dt) || !(p->theta >= 0.0f) || !(p->epsilon == p->epsilon)) {
+    return setError(NBX_ERR_ARG, "dt/theta/epsilon must be numbers, theta >= 0");
+  }
+  if (p->law != NBX_LAW_REFERENCE && p->law != NBX_LAW_ACCURATE) {
+    return setError(NBX_ERR_ARG, "unknown force law");
+  }
+  if (p->summation != NBX_SUM_REFERENCE &&
+      p->summation != NBX_SUM_COMPENSATED) {
+    return setError(NBX_ERR_ARG, "unknown summation mode");
+  }
+  return NBX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *nbx_last_error(void) { return g_error.c_str(); }
+
+int nbx_abi_version(void) { return NBX_ABI_VERSION; }
+
+void nbx_default_params(nbx_params *p) {
+  if (!p) {
+    return;
+  }
+  std::memset(p, 0, sizeof *p);
+  p->dt = 0.05f;
+  p->theta = 1.0f;
+  p->epsilon = 1.0f;
+  p->collisions = 1;
+  p->law = NBX_LAW_REFERENCE;
+  p->summation = NBX_SUM_COMPENSATED;
+  p->use_graph = 1;
+}
+
+int nbx_create(uint64_t capacity, nbx_sim **out) {
+  if (!out) {
+    return setError(NBX_ERR_ARG, "out is null");
+  }
+  *out = nullptr;
+  if (capacity == 0 || capacity > (1ull << 27)) {
+    return setError(NBX_ERR_ARG, "capacity must be in [1, 2^27]");
+  }
+  nbx_sim *s = new nbx_sim();
+  s->cap = capacity;
+  nbx_default_params(&s->params);
+  auto fail = [&](int rc) {
+    nbx_destroy(s);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return setError(NBX_ERR_HIP, "hipStreamCreateWithFlags failed");
+  }
+  const uint64_t n = capacity;
+  int rc = NBX_OK;
+#define A(ptr, count, zero)                                                   \
+  if ((rc = devAlloc(s, &(ptr), (count), (zero))) != NBX_OK)                  \
+    return fail(rc);
+  for (int b = 0; b < 2; ++b) {
+    A(s->px[b], n, false) A(s->py[b], n, false) A(s->vx[b], n, false)
+    A(s->vy[b], n, false) A(s->m[b], n, false) A(s->r[b], n, false)
+    A(s->id[b], n, false)
+    A(s->keyHi[b], n, false) A(s->vals[b], n, false)
+  }
+  A(s->ax, n, true) A(s->ay, n, true)
+  A(s->widths, n, false) A(s->selHist, kSelBins, true)
+  s->buckets = std::max<uint64_t>(1024, pow2AtLeast(2 * n));
+  A(s->bucketOf, n, false) A(s->bucketCount, s->buckets + 1, true)
+  A(s->bucketStart, s->buckets + 1, false)
+  A(s->scanTiles, blocksFor(s->buckets + 1, kScanTile) + 1, false)
+  A(s->gridBox, n, false) A(s->gridCell, n, false)
+  A(s->largeList, n, false) A(s->ufParent, n, false)
+  s->pairCap = static_cast<uint32_t>(std::min<uint64_t>(8 * n + 1024, 0xffffff00ull));
+  A(s->pairs, s->pairCap, false) A(s->segPairs, s->pairCap, false)
+  A(s->pairComp, s->pairCap, false)
+  A(s->compCount, n, true) A(s->compStart, n, false) A(s->compInfo, n, false)
+  A(s->bigList, n, false)
+  A(s->keyLo, n, false)
+  A(s->sortHist, 256ull * blocksFor(n, kSortTile), false)
+  A(s->deepList, n / (kLeafCapacity + 1) + 1, false)
+  A(s->tileCount, static_cast<uint64_t>(kMaxDepth) * blocksFor(n), false)
+  s->nodeCap = static_cast<int64_t>(8 * n + 8);
+  A(s->splitList, s->nodeCap / 4 + 1, false)
+  A(s->nodeOfSplit, s->nodeCap / 4 + 1, false)
+  A(s->nodeF, s->nodeCap, false) A(s->nodeI, s->nodeCap, false)
+  A(s->centerX, s->nodeCap, false) A(s->centerY, s->nodeCap, false)
+  A(s->size, s->nodeCap, false) A(s->wsum, s->nodeCap, false)
+  A(s->ctl, 1, true)
+#undef A
+  for (auto &e : s->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      return fail(setError(NBX_ERR_HIP, "hipEventCreate failed"));
+    }
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    return fail(setError(NBX_ERR_HIP, "hipDeviceSynchronize failed"));
+  }
+  *out = s;
+  return NBX_OK;
+}
+
+int nbx_destroy(nbx_sim *s) {
+  if (!s) {
+    return NBX_OK;
+  }
+  if (s->stream) {
+    (void)hipStreamSynchronize(s->stream);
+  }
+  dropGraphs(s);
+  for (auto &e : s->ev) {
+    if (e) {
+      (void)hipEventDestroy(e);
+    }
+  }
+  for (void *p : s->allocs) {
+    (void)hipFree(p);
+  }
+  if (s->stream) {
+    (void)hipStreamDestroy(s->stream);
+  }
+  delete s;
+  return NBX_OK;
+}
+
+int nbx_upload(nbx_sim *s, uint64_t n, const float *px, const float *py,
+               const float *vx, const float *vy, const float *ax,
+               const float *ay, const float *m, const float *r,
+               const uint32_t *id) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  if (n > s->cap) {
+    return setError(NBX_ERR_ARG, "n exceeds the handle's capacity");
+  }
+  if (n && (!px || !py || !vx || !vy || !m || !r || !id)) {
+    return setError(NBX_ERR_ARG, "null body array");
+  }
+  NBX_HIP(hipStreamSynchronize(s->stream));
+  dropGraphs(s);
+  s->cur = 0;
+  s->n = n;
+  NBX_TRY(copyIn(s, s->px[0], px, n));
+  NBX_TRY(copyIn(s, s->py[0], py, n));
+  NBX_TRY(copyIn(s, s->vx[0], vx, n));
+  NBX_TRY(copyIn(s, s->vy[0], vy, n));
+  NBX_TRY(copyIn(s, s->m[0], m, n));
+  NBX_TRY(copyIn(s, s->r[0], r, n));
+  NBX_TRY(copyIn(s, s->id[0], id, n));
+  if (ax) {
+    NBX_TRY(copyIn(s, s->ax, ax, n));
+  } else {
+    NBX_TRY(zeroDevice(s, s->ax, n * sizeof(float)));
+  }
+  if (ay) {
+    NBX_TRY(copyIn(s, s->ay, ay, n));
+  } else {
+    NBX_TRY(zeroDevice(s, s->ay, n * sizeof(float)));
+  }
+  NBX_HIP(hipStreamSynchronize(s->stream));
+  s->frame = 0;
+  s->treeBuilt = false;
+  s->collided = false;
+  return NBX_OK;
+}
+
+int nbx_init_disc(nbx_sim *s, uint64_t n, uint32_t seed) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  if (n > s->cap) {
+    return setError(NBX_ERR_ARG, "n exceeds the handle's capacity");
+  }
+  std::vector<float> px(n), py(n), vx(n), vy(n), m(n), r(n);
+  std::vector<uint32_t> id(n);
+  nbx_host::uniformDisc(n, seed, px.data(), py.data(), vx.data(), vy.data(),
+                        m.data(), r.data(), id.data());
+  return nbx_upload(s, n, px.data(), py.data(), vx.data(), vy.data(), nullptr,
+                    nullptr, m.data(), r.data(), id.data());
+}
+
+int nbx_download(nbx_sim *s, float *px, float *py, float *vx, float *vy,
+                 float *ax, float *ay, float *m, float *r, uint32_t *id) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  const int c = s->cur;
+  const uint64_t n = s->n;
+  NBX_TRY(copyOut(s, px, s->px[c], n));
+  NBX_TRY(copyOut(s, py, s->py[c], n));
+  NBX_TRY(copyOut(s, vx, s->vx[c], n));
+  NBX_TRY(copyOut(s, vy, s->vy[c], n));
+  NBX_TRY(copyOut(s, ax, s->ax, n));
+  NBX_TRY(copyOut(s, ay, s->ay, n));
+  NBX_TRY(copyOut(s, m, s->m[c], n));
+  NBX_TRY(copyOut(s, r, s->r[c], n));
+  NBX_TRY(copyOut(s, id, s->id[c], n));
+  return syncAndCheck(s);
+}
+
+int nbx_download_by_id(nbx_sim *s, float *px, float *py, float *vx, float *vy,
+                       float *m, float *r) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  const uint64_t n = s->n;
+  std::vector<uint32_t> id(n);
+  std::vector<float> tmp(n);
+  NBX_TRY(nbx_download(s, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, id.data()));
+  for (uint64_t i = 0; i < n; ++i) {
+    if (id[i] >= n) {
+      return setError(NBX_ERR_STATE, "ids are not a permutation of 0..n-1");
+    }
+  }
+  float *outs[6] = {px, py, vx, vy, m, r};
+  const int c = s->cur;
+  float *srcs[6] = {s->px[c], s->py[c], s->vx[c], s->vy[c], s->m[c], s->r[c]};
+  for (int f = 0; f < 6; ++f) {
+    if (!outs[f]) {
+      continue;
+    }
+    NBX_TRY(copyOut(s, tmp.data(), srcs[f], n));
+    NBX_HIP(hipStreamSynchronize(s->stream));
+    for (uint64_t i = 0; i < n; ++i) {
+      outs[f][id[i]] = tmp[i];
+    }
+  }
+  return NBX_OK;
+}
+
+int nbx_count(nbx_sim *s, uint64_t *n) {
+  if (!s || !n) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  *n = s->n;
+  return NBX_OK;
+}
+
+int nbx_set_params(nbx_sim *s, const nbx_params *p) {
+  if (!s || !p) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  NBX_TRY(validParams(p));
+  if (std::memcmp(&s->params, p, sizeof *p) != 0) {
+    NBX_HIP(hipStreamSynchronize(s->stream));
+    dropGraphs(s);
+    s->params = *p;
+  }
+  return NBX_OK;
+}
+
+int nbx_get_params(nbx_sim *s, nbx_params *p) {
+  if (!s || !p) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  *p = s->params;
+  return NBX_OK;
+}
+
+int nbx_step(nbx_sim *s, uint32_t steps) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  const bool graphs = s->params.use_graph && !s->profiling && s->n > 0;
+  for (uint32_t k = 0; k < steps; ++k) {
+    if (graphs) {
+      const int c = s->cur;
+      if (!s->graph[c]) {
+        NBX_TRY(captureStep(s, c));
+      }
+      NBX_HIP(hipGraphLaunch(s->graph[c], s->stream));
+      s->cur = c ^ 1;
+      s->treeBuilt = true;
+      s->collided = s->params.collisions != 0;
+    } else {
+      NBX_TRY(launchStep(s));
+    }
+    s->frame += 1;
+  }
+  return syncAndCheck(s);
+}
+
+int nbx_iterate(nbx_sim *s) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(launchIterate(s));
+  return syncAndCheck(s);
+}
+
+int nbx_collide(nbx_sim *s) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(launchCollide(s));
+  return syncAndCheck(s);
+}
+
+int nbx_build(nbx_sim *s) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(launchBuild(s));
+  return syncAndCheck(s);
+}
+
+int nbx_forces(nbx_sim *s) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(launchForces(s, false));
+  return syncAndCheck(s);
+}
+
+int nbx_attract(nbx_sim *s) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(launchBuild(s));
+  NBX_TRY(launchForces(s, false));
+  return syncAndCheck(s);
+}
+
+int nbx_frame(nbx_sim *s, uint64_t *frame) {
+  if (!s || !frame) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  *frame = s->frame;
+  return NBX_OK;
+}
+
+int nbx_sync(nbx_sim *s) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  return syncAndCheck(s);
+}
+
+int nbx_tree_info(nbx_sim *s, int64_t *nodes, int32_t *levels) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(syncAndCheck(s));
+  Ctl c{};
+  NBX_TRY(readCtl(s, &c));
+  if (nodes) {
+    *nodes = s->treeBuilt ? c.nodeCount : 0;
+  }
+  if (levels) {
+    *levels = s->treeBuilt ? c.levels : 0;
+  }
+  return NBX_OK;
+}
+
+int nbx_download_tree(nbx_sim *s, int64_t cap, float *comX, float *comY,
+                      float *mass, float *centerX, float *centerY,
+                      float *size, int32_t *children, int32_t *next,
+                      int32_t *start, int32_t *end) {
+  int64_t nodes = 0;
+  NBX_TRY(nbx_tree_info(s, &nodes, nullptr));
+  const uint64_t k = static_cast<uint64_t>(std::min(nodes, cap));
+  std::vector<float4> f(k);
+  std::vector<int4> iv(k);
+  NBX_TRY(copyOut(s, f.data(), s->nodeF, k));
+  NBX_TRY(copyOut(s, iv.data(), s->nodeI, k));
+  NBX_TRY(copyOut(s, centerX, s->centerX, k));
+  NBX_TRY(copyOut(s, centerY, s->centerY, k));
+  NBX_TRY(copyOut(s, size, s->size, k));
+  NBX_HIP(hipStreamSynchronize(s->stream));
+  for (uint64_t i = 0; i < k; ++i) {
+    if (comX) comX[i] = f[i].x;
+    if (comY) comY[i] = f[i].y;
+    if (mass) mass[i] = f[i].z;
+    if (children) children[i] = iv[i].x;
+    if (next) next[i] = iv[i].y;
+    if (start) start[i] = iv[i].z;
+    if (end) end[i] = iv[i].w;
+  }
+  return NBX_OK;
+}
+
+int nbx_root_quad(nbx_sim *s, float quad[3]) {
+  if (!s || !quad) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  Ctl c{};
+  NBX_TRY(readCtl(s, &c));
+  quad[0] = c.quad[0];
+  quad[1] = c.quad[1];
+  quad[2] = c.quad[2];
+  return NBX_OK;
+}
+
+int nbx_download_keys(nbx_sim *s, uint32_t *hi, uint32_t *lo, uint32_t *perm) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  const uint64_t n = s->n;
+  std::vector<uint32_t> p(n), l(n);
+  NBX_TRY(copyOut(s, hi, s->keyHi[0], n));
+  NBX_TRY(copyOut(s, p.data(), s->vals[0], n));
+  NBX_TRY(copyOut(s, l.data(), s->keyLo, n));
+  NBX_HIP(hipStreamSynchronize(s->stream));
+  for (uint64_t k = 0; k < n; ++k) {
+    if (perm) perm[k] = p[k];
+    if (lo) lo[k] = p[k] < n ? l[p[k]] : 0u;
+  }
+  return NBX_OK;
+}
+
+int nbx_collide_info(nbx_sim *s, float *cell, int64_t *pairs,
+                     int64_t *components, int64_t *largest) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(syncAndCheck(s));
+  Ctl c{};
+  NBX_TRY(readCtl(s, &c));
+  const bool ok = s->collided && s->n >= 2;
+  if (cell) *cell = ok ? c.cell : 0.0f;
+  if (pairs) *pairs = ok ? c.pairCount : 0;
+  if (components) *components = ok ? c.compCount : 0;
+  if (largest) *largest = ok ? c.largest : 0;
+  return NBX_OK;
+}
+
+int nbx_download_pairs(nbx_sim *s, int64_t cap, int32_t *later,
+                       int32_t *earlier, int32_t *comp, int64_t *count) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(syncAndCheck(s));
+  Ctl c{};
+  NBX_TRY(readCtl(s, &c));
+  if (!s->collided || s->n < 2) {
+    if (count) *count = 0;
+    return NBX_OK;
+  }
+  const uint64_t total = c.compPairs;
+  std::vector<int4> info(c.compCount);
+  std::vector<int4> seg(total);
+  NBX_TRY(copyOut(s, info.data(), s->compInfo, info.size()));
+  NBX_TRY(copyOut(s, seg.data(), s->segPairs, seg.size()));
+  NBX_HIP(hipStreamSynchronize(s->stream));
+  int64_t k = 0;
+  for (const int4 &ci : info) {
+    for (int q = 0; q < ci.z; ++q, ++k) {
+      if (k < cap) {
+        const int4 &p = seg[ci.y + q];
+        if (later) later[k] = p.x;
+        if (earlier) earlier[k] = p.y;
+        if (comp) comp[k] = ci.x;
+      }
+    }
+  }
+  if (count) *count = k;
+  return NBX_OK;
+}
+
+int nbx_count_interactions(nbx_sim *s, uint64_t *far_nodes,
+                           uint64_t *body_pairs) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(1), 0, s->stream, s->ctl);
+  NBX_TRY(checkLaunch());
+  // Counting walk writes the same accelerations as the plain walk.
+  NBX_TRY(launchForces(s, true));
+  NBX_TRY(syncAndCheck(s));
+  Ctl c{};
+  NBX_TRY(readCtl(s, &c));
+  if (far_nodes) *far_nodes = c.farCount;
+  if (body_pairs) *body_pairs = c.bodyPairCount;
+  return NBX_OK;
+}
+
+int nbx_set_profiling(nbx_sim *s, int on) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(syncAndCheck(s));
+  s->profiling = on != 0;
+  return NBX_OK;
+}
+
+int nbx_phase_times(nbx_sim *s, double ms[NBX_PHASES], uint64_t *steps) {
+  if (!s || !ms) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  NBX_TRY(syncAndCheck(s));
+  for (int p = 0; p < NBX_PHASES; ++p) {
+    ms[p] = s->phaseMs[p];
+    s->phaseMs[p] = 0.0;
+  }
+  if (steps) *steps = s->profSteps;
+  s->profSteps = 0;
+  return NBX_OK;
+}
+
+int nbx_stream(nbx_sim *s, void **stream) {
+  if (!s || !stream) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  *stream = s->stream;
+  return NBX_OK;
+}
+
+int nbx_exact_forces(nbx_sim *s, uint64_t n_targets, const uint32_t *targets,
+                     double eps2, double *ax, double *ay) {
+  if (!s || (n_targets && (!targets || !ax || !ay))) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (n_targets == 0 || s->n == 0) {
+    return NBX_OK;
+  }
+  for (uint64_t t = 0; t < n_targets; ++t) {
+    if (targets[t] >= s->n) {
+      return setError(NBX_ERR_ARG, "target index out of range");
+    }
+  }
+  NBX_TRY(syncAndCheck(s));
+  uint32_t *dT = nullptr;
+  double2 *part = nullptr;
+  double *dax = nullptr, *day = nullptr;
+  const uint32_t nt = static_cast<uint32_t>(n_targets);
+  int rc = NBX_OK;
+  auto cleanup = [&]() {
+    (void)hipFree(dT);
+    (void)hipFree(part);
+    (void)hipFree(dax);
+    (void)hipFree(day);
+  };
+  if (hipMalloc(&dT, nt * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&part, static_cast<size_t>(kExactChunks) * nt * sizeof(double2)) !=
+          hipSuccess ||
+      hipMalloc(&dax, nt * sizeof(double)) != hipSuccess ||
+      hipMalloc(&day, nt * sizeof(double)) != hipSuccess) {
+    cleanup();
+    return setError(NBX_ERR_HIP, "hipMalloc failed in nbx_exact_forces");
+  }
+  const int c = s->cur;
+  if (hipMemcpyAsync(dT, targets, nt * sizeof(uint32_t), hipMemcpyHostToDevice,
+                     s->stream) != hipSuccess) {
+    rc = setError(NBX_ERR_HIP, "target upload failed");
+  }
+  if (rc == NBX_OK) {
+    hipLaunchKernelGGL(k_exact_partial, dim3(blocksFor(nt), kExactChunks),
+                       dim3(kBlock), 0, s->stream,
+                       static_cast<uint32_t>(s->n), nt, dT, s->px[c], s->py[c],
+                       s->m[c], eps2, part);
+    hipLaunchKernelGGL(k_exact_reduce, dim3(blocksFor(nt)), dim3(kBlock), 0,
+                       s->stream, nt, part, dax, day);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(ax, dax, nt * sizeof(double), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess ||
+        hipMemcpyAsync(ay, day, nt * sizeof(double), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess) {
+      rc = setError(NBX_ERR_HIP, "exact-force kernels failed");
+    }
+  }
+  cleanup();
+  return rc;
+}
+
+} // extern "C"

@@ -1,0 +1,323 @@
+"""Python binding of libnbx over its C ABI (include/nbx.h).
+
+Mirrors the reference's GPU simulation class, CudaSimulation
+(src/cuda/CudaSimulation.cuh:25-51): construct from a body count (rotating
+disc, src/Utils.cpp:6-54) or from body arrays, set dt / theta / epsilon /
+collisions, then step() / iterate() / collide() / attract(). Bodies stay on
+the GPU; download() copies them out in device order, download_by_id() in id
+order.
+
+The library is loaded from build/libnbx.so next to this file; there is no
+fallback: if the library is missing or no GPU is present, construction
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libnbx.so")
+
+NBX_OK = 0
+LAW_REFERENCE, LAW_ACCURATE = 0, 1
+SUM_REFERENCE, SUM_COMPENSATED = 0, 1
+PHASES = ("iterate", "collide", "build", "walk")
+
+# Every symbol include/nbx.h declares.
+EXPORTS = (
+    "nbx_last_error", "nbx_abi_version", "nbx_default_params", "nbx_create",
+    "nbx_destroy", "nbx_init_disc", "nbx_upload", "nbx_download",
+    "nbx_download_by_id", "nbx_count", "nbx_set_params", "nbx_get_params",
+    "nbx_step", "nbx_iterate", "nbx_collide", "nbx_attract", "nbx_build",
+    "nbx_forces", "nbx_frame", "nbx_sync", "nbx_tree_info", "nbx_download_tree",
+    "nbx_root_quad", "nbx_download_keys", "nbx_collide_info", "nbx_download_pairs",
+    "nbx_count_interactions", "nbx_set_profiling", "nbx_phase_times", "nbx_stream",
+    "nbx_exact_forces",
+)
+
+
+class NbxError(RuntimeError):
+    """A libnbx call returned a non-zero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libnbx error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("dt", C.c_float), ("theta", C.c_float), ("epsilon", C.c_float),
+                ("collisions", C.c_int32), ("law", C.c_int32),
+                ("summation", C.c_int32), ("use_graph", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libnbx (once). Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} not built (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(path)
+    L.nbx_last_error.restype = C.c_char_p
+    L.nbx_default_params.argtypes = [C.POINTER(Params)]
+    L.nbx_default_params.restype = None
+    L.nbx_create.argtypes = [C.c_uint64, C.POINTER(_vp)]
+    L.nbx_destroy.argtypes = [_vp]
+    L.nbx_init_disc.argtypes = [_vp, C.c_uint64, C.c_uint32]
+    L.nbx_upload.argtypes = [_vp, C.c_uint64] + [_vp] * 9
+    L.nbx_download.argtypes = [_vp] + [_vp] * 9
+    L.nbx_download_by_id.argtypes = [_vp] + [_vp] * 6
+    L.nbx_count.argtypes = [_vp, C.POINTER(C.c_uint64)]
+    L.nbx_set_params.argtypes = [_vp, C.POINTER(Params)]
+    L.nbx_get_params.argtypes = [_vp, C.POINTER(Params)]
+    L.nbx_step.argtypes = [_vp, C.c_uint32]
+    for f in ("nbx_iterate", "nbx_collide", "nbx_attract", "nbx_build", "nbx_forces",
+              "nbx_sync"):
+        getattr(L, f).argtypes = [_vp]
+    L.nbx_frame.argtypes = [_vp, C.POINTER(C.c_uint64)]
+    L.nbx_tree_info.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    L.nbx_download_tree.argtypes = [_vp, C.c_int64] + [_vp] * 10
+    L.nbx_root_quad.argtypes = [_vp, _f32p]
+    L.nbx_download_keys.argtypes = [_vp, _vp, _vp, _vp]
+    L.nbx_collide_info.argtypes = [_vp, C.POINTER(C.c_float), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.nbx_download_pairs.argtypes = [_vp, C.c_int64, _vp, _vp, _vp, C.POINTER(C.c_int64)]
+    L.nbx_count_interactions.argtypes = [_vp, C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint64)]
+    L.nbx_set_profiling.argtypes = [_vp, C.c_int]
+    L.nbx_phase_times.argtypes = [_vp, _f64p, C.POINTER(C.c_uint64)]
+    L.nbx_stream.argtypes = [_vp, C.POINTER(_vp)]
+    L.nbx_exact_forces.argtypes = [_vp, C.c_uint64, _u32p, C.c_double, _f64p, _f64p]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+@dataclass
+class State:
+    """Body arrays (src/Body.h:5-24), structure of arrays."""
+
+    px: np.ndarray
+    py: np.ndarray
+    vx: np.ndarray
+    vy: np.ndarray
+    ax: np.ndarray
+    ay: np.ndarray
+    m: np.ndarray
+    r: np.ndarray
+    id: np.ndarray
+
+
+class Simulation:
+    """GPU Barnes-Hut simulation (CudaSimulation's interface).
+
+    Simulation(n, seed=0) loads uniformDisc(n, seed); Simulation(state=...)
+    loads given bodies. Attributes dt, theta, epsilon, collisions, law,
+    summation and use_graph are read on every call, like CudaSimulation's
+    public members.
+    """
+
+    def __init__(self, n: int | None = None, seed: int = 0, state=None,
+                 capacity: int | None = None):
+        self.L = load_library()
+        if state is not None:
+            n_bodies = int(np.asarray(state.px).shape[0])
+        elif n is not None:
+            n_bodies = int(n)
+        else:
+            raise ValueError("give n or state")
+        cap = max(1, capacity if capacity is not None else n_bodies)
+        h = _vp()
+        self._check(self.L.nbx_create(cap, C.byref(h)))
+        self.h = h
+        p = Params()
+        self.L.nbx_default_params(C.byref(p))
+        self.dt, self.theta, self.epsilon = p.dt, p.theta, p.epsilon
+        self.collisions = bool(p.collisions)
+        self.law, self.summation, self.use_graph = p.law, p.summation, bool(p.use_graph)
+        if state is not None:
+            self.upload(state)
+        else:
+            self._check(self.L.nbx_init_disc(self.h, n_bodies, seed))
+
+    # -- plumbing --
+    def _check(self, rc: int) -> None:
+        if rc != NBX_OK:
+            raise NbxError(rc, self.L.nbx_last_error().decode())
+
+    def _push_params(self) -> None:
+        p = Params(self.dt, self.theta, self.epsilon, int(bool(self.collisions)),
+                   int(self.law), int(self.summation), int(bool(self.use_graph)), 0)
+        self._check(self.L.nbx_set_params(self.h, C.byref(p)))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.nbx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- state --
+    @property
+    def n(self) -> int:
+        v = C.c_uint64()
+        self._check(self.L.nbx_count(self.h, C.byref(v)))
+        return int(v.value)
+
+    @property
+    def frame(self) -> int:
+        v = C.c_uint64()
+        self._check(self.L.nbx_frame(self.h, C.byref(v)))
+        return int(v.value)
+
+    def upload(self, s) -> None:
+        arr = lambda a, t: np.ascontiguousarray(a, t)
+        n = int(np.asarray(s.px).shape[0])
+        f = [arr(getattr(s, k), np.float32) for k in ("px", "py", "vx", "vy")]
+        ax = getattr(s, "ax", None)
+        ay = getattr(s, "ay", None)
+        ax = None if ax is None else arr(ax, np.float32)
+        ay = None if ay is None else arr(ay, np.float32)
+        m, r = arr(s.m, np.float32), arr(s.r, np.float32)
+        ids = arr(s.id, np.uint32)
+        self._check(self.L.nbx_upload(self.h, n, *(_ptr(a) for a in f), _ptr(ax),
+                                      _ptr(ay), _ptr(m), _ptr(r), _ptr(ids)))
+
+    def download(self) -> State:
+        n = self.n
+        z = lambda t=np.float32: np.zeros(n, t)
+        st = State(z(), z(), z(), z(), z(), z(), z(), z(), z(np.uint32))
+        self._check(self.L.nbx_download(self.h, *(_ptr(getattr(st, k)) for k in
+                                                  ("px", "py", "vx", "vy", "ax", "ay",
+                                                   "m", "r", "id"))))
+        return st
+
+    def download_by_id(self):
+        n = self.n
+        out = [np.zeros(n, np.float32) for _ in range(6)]
+        self._check(self.L.nbx_download_by_id(self.h, *(_ptr(a) for a in out)))
+        return dict(zip(("px", "py", "vx", "vy", "m", "r"), out))
+
+    # -- the step (Simulation.cpp:11-16) --
+    def step(self, steps: int = 1) -> None:
+        self._push_params()
+        self._check(self.L.nbx_step(self.h, steps))
+
+    def iterate(self) -> None:
+        self._push_params()
+        self._check(self.L.nbx_iterate(self.h))
+
+    def collide(self) -> None:
+        self._push_params()
+        self._check(self.L.nbx_collide(self.h))
+
+    def attract(self) -> None:
+        self._push_params()
+        self._check(self.L.nbx_attract(self.h))
+
+    def build(self) -> None:
+        self._push_params()
+        self._check(self.L.nbx_build(self.h))
+
+    def forces(self) -> None:
+        self._push_params()
+        self._check(self.L.nbx_forces(self.h))
+
+    def sync(self) -> None:
+        self._check(self.L.nbx_sync(self.h))
+
+    # -- inspection --
+    def tree_info(self):
+        nodes, levels = C.c_int64(), C.c_int32()
+        self._check(self.L.nbx_tree_info(self.h, C.byref(nodes), C.byref(levels)))
+        return int(nodes.value), int(levels.value)
+
+    def tree(self) -> dict:
+        nodes, levels = self.tree_info()
+        f = {k: np.zeros(nodes, np.float32) for k in
+             ("comX", "comY", "mass", "centerX", "centerY", "size")}
+        i = {k: np.zeros(nodes, np.int32) for k in ("children", "next", "start", "end")}
+        self._check(self.L.nbx_download_tree(
+            self.h, nodes, *(_ptr(f[k]) for k in ("comX", "comY", "mass", "centerX",
+                                                  "centerY", "size")),
+            *(_ptr(i[k]) for k in ("children", "next", "start", "end"))))
+        out = {**f, **i}
+        out["levels"] = levels
+        return out
+
+    def root_quad(self) -> np.ndarray:
+        q = np.zeros(3, np.float32)
+        self._check(self.L.nbx_root_quad(self.h, q))
+        return q
+
+    def keys(self):
+        n = self.n
+        hi, lo, perm = (np.zeros(n, np.uint32) for _ in range(3))
+        self._check(self.L.nbx_download_keys(self.h, _ptr(hi), _ptr(lo), _ptr(perm)))
+        return hi, lo, perm
+
+    def collide_info(self) -> dict:
+        cell, pairs, comps, largest = C.c_float(), C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.L.nbx_collide_info(self.h, C.byref(cell), C.byref(pairs),
+                                            C.byref(comps), C.byref(largest)))
+        return dict(cell=cell.value, pairs=pairs.value, components=comps.value,
+                    largest=largest.value)
+
+    def pairs(self):
+        info = self.collide_info()
+        cap = max(1, info["pairs"])
+        later, earlier, comp = (np.zeros(cap, np.int32) for _ in range(3))
+        cnt = C.c_int64()
+        self._check(self.L.nbx_download_pairs(self.h, cap, _ptr(later), _ptr(earlier),
+                                              _ptr(comp), C.byref(cnt)))
+        k = int(cnt.value)
+        return later[:k], earlier[:k], comp[:k]
+
+    def count_interactions(self):
+        self._push_params()
+        far, pairs = C.c_uint64(), C.c_uint64()
+        self._check(self.L.nbx_count_interactions(self.h, C.byref(far), C.byref(pairs)))
+        return int(far.value), int(pairs.value)
+
+    def set_profiling(self, on: bool) -> None:
+        self._check(self.L.nbx_set_profiling(self.h, int(bool(on))))
+
+    def phase_times(self):
+        ms = np.zeros(4, np.float64)
+        steps = C.c_uint64()
+        self._check(self.L.nbx_phase_times(self.h, ms, C.byref(steps)))
+        return dict(zip(PHASES, ms.tolist())), int(steps.value)
+
+    def exact_forces(self, targets, eps2: float = 1.0):
+        t = np.ascontiguousarray(targets, np.uint32)
+        ax = np.zeros(t.shape[0], np.float64)
+        ay = np.zeros(t.shape[0], np.float64)
+        self._check(self.L.nbx_exact_forces(self.h, t.shape[0], t, eps2, ax, ay))
+        return ax, ay

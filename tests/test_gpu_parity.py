@@ -1,0 +1,297 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Integer and index results (keys, permutation, node topology, collision pairs)
+must be bit-exact; with summation=SUM_REFERENCE every float result must be
+bit-exact too (the kernels replay the reference's FP32 operation order). The
+compensated summation mode is held to a per-step RMS relative force error
+of <= 1e-5 against the oracle at the same theta.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RMS_TOL = 1e-5  # compensated walk vs reference summation, same theta
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def assert_bits_equal(got, want, what):
+    g, w = bits(got), bits(want)
+    assert g.shape == w.shape, f"{what}: shape {g.shape} vs {w.shape}"
+    bad = np.flatnonzero(g != w)
+    assert bad.size == 0, (f"{what}: {bad.size} of {g.size} differ, first at {bad[0]}: "
+                           f"{got[bad[0]]!r} vs {want[bad[0]]!r}")
+
+
+def rms_rel(gx, gy, wx, wy):
+    num = np.sum((gx.astype(np.float64) - wx) ** 2 + (gy.astype(np.float64) - wy) ** 2)
+    den = np.sum(wx.astype(np.float64) ** 2 + wy.astype(np.float64) ** 2)
+    return float(np.sqrt(num / den))
+
+
+@pytest.fixture(scope="module")
+def states(oracle):
+    """Named body states built by the oracle (seconds of CPU work)."""
+    out = {}
+    out["disc10k"] = oracle.uniform_disc(10000)
+    s = oracle.uniform_disc(20000)
+    for _ in range(60):
+        oracle.step(s)
+    out["disc20k_60"] = s
+    # Before the next collide, as the reference's cuda_check builds its
+    # collision states (CheckCuda.cu:189-196).
+    c = s.copy()
+    oracle.iterate(c)
+    out["collide20k_60"] = c
+    c0 = oracle.uniform_disc(10000)
+    oracle.iterate(c0)
+    out["collide10k_0"] = c0
+    return out
+
+
+def new_sim(nbx, state, summation=None, theta=1.0, law=None):
+    sim = nbx.Simulation(state=state)
+    sim.theta = theta
+    if summation is not None:
+        sim.summation = summation
+    if law is not None:
+        sim.law = law
+    return sim
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 1000, 10000])
+def test_disc_initial_conditions(nbx, oracle, n):
+    want = oracle.uniform_disc(n)
+    with nbx.Simulation(n) as sim:
+        got = sim.download()
+    for f in ("px", "py", "vx", "vy", "m", "r"):
+        assert_bits_equal(getattr(got, f), getattr(want, f), f)
+    assert np.array_equal(got.id, want.id)
+
+
+def test_iterate(nbx, oracle, states):
+    s = states["disc10k"].copy()
+    rng = np.random.default_rng(1)
+    s.ax[:] = rng.standard_normal(s.n).astype(np.float32)
+    s.ay[:] = rng.standard_normal(s.n).astype(np.float32)
+    with new_sim(nbx, s) as sim:
+        sim.iterate()
+        got = sim.download()
+    oracle.iterate(s)
+    for f in ("px", "py", "vx", "vy"):
+        assert_bits_equal(getattr(got, f), getattr(s, f), f)
+
+
+@pytest.mark.parametrize("name", ["disc10k", "disc20k_60"])
+def test_build_bit_exact(nbx, oracle, states, name):
+    s = states[name]
+    srt, hi, lo, perm, quad = oracle.sorted_state(s)
+    tree = oracle.build_tree(srt, hi, lo, quad)
+    with new_sim(nbx, s) as sim:
+        sim.build()
+        ghi, glo, gperm = sim.keys()
+        gq = sim.root_quad()
+        gt = sim.tree()
+        got = sim.download()
+    assert_bits_equal(gq, quad, "root quad")
+    assert np.array_equal(ghi, hi), "sorted high key words"
+    assert np.array_equal(glo, lo), "low key words in sorted order"
+    assert np.array_equal(gperm, perm), "sort permutation"
+    assert gt["levels"] == tree.levels
+    assert gt["comX"].shape[0] == tree.count, "node count"
+    for k in ("children", "next", "start", "end"):
+        assert np.array_equal(gt[k], getattr(tree, k)), k
+    for k in ("centerX", "centerY", "size", "mass", "comX", "comY"):
+        assert_bits_equal(gt[k], getattr(tree, k), k)
+    for f in ("px", "py", "vx", "vy", "m", "r"):
+        assert_bits_equal(getattr(got, f), getattr(srt, f), f)
+    assert np.array_equal(got.id, srt.id)
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+@pytest.mark.parametrize("law", [0, 1])
+def test_forces_reference_summation_bit_exact(nbx, oracle, states, theta, law):
+    s = states["disc20k_60"]
+    srt, hi, lo, perm, quad = oracle.sorted_state(s)
+    tree = oracle.build_tree(srt, hi, lo, quad)
+    wx, wy = oracle.accelerations(srt, tree, theta, law=law)
+    with new_sim(nbx, s, summation=nbx.SUM_REFERENCE, theta=theta, law=law) as sim:
+        sim.attract()
+        got = sim.download()
+    assert_bits_equal(got.ax, wx, "ax")
+    assert_bits_equal(got.ay, wy, "ay")
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5])
+def test_forces_compensated_within_tolerance(nbx, oracle, states, theta):
+    s = states["disc20k_60"]
+    srt, hi, lo, perm, quad = oracle.sorted_state(s)
+    tree = oracle.build_tree(srt, hi, lo, quad)
+    wx, wy = oracle.accelerations(srt, tree, theta)
+    with new_sim(nbx, s, summation=nbx.SUM_COMPENSATED, theta=theta) as sim:
+        sim.attract()
+        got = sim.download()
+    err = rms_rel(got.ax, got.ay, wx, wy)
+    assert err <= RMS_TOL, f"RMS relative force error {err:.3g} > {RMS_TOL}"
+
+
+@pytest.mark.parametrize("name", ["collide10k_0", "collide20k_60"])
+def test_collide_bit_exact(nbx, oracle, states, name):
+    s = states[name]
+    want = s.copy()
+    later, earlier, cell = oracle.collide(want)
+    with new_sim(nbx, s) as sim:
+        sim.collide()
+        got = sim.download()
+        info = sim.collide_info()
+        glater, gearlier, gcomp = sim.pairs()
+    assert info["cell"] == np.float32(cell)
+    assert info["pairs"] == later.shape[0]
+    # Same unordered pair set, each pair oriented (later, earlier) the same way.
+    key = lambda a, b: (a.astype(np.int64) << 32) | b.astype(np.int64)
+    assert np.array_equal(np.sort(key(glater, gearlier)), np.sort(key(later, earlier)))
+    # Within each contact component, the reference's resolution order.
+    seq = {int(k): i for i, k in enumerate(key(later, earlier))}
+    gseq = np.array([seq[int(k)] for k in key(glater, gearlier)])
+    for c in np.unique(gcomp):
+        order = gseq[gcomp == c]
+        assert np.all(np.diff(order) > 0), f"component {c} replayed out of order"
+    for f in ("px", "py", "vx", "vy"):
+        assert_bits_equal(getattr(got, f), getattr(want, f), f)
+
+
+def test_step_trajectory_bit_exact(nbx, oracle, states):
+    s = states["disc10k"].copy()
+    with new_sim(nbx, s, summation=nbx.SUM_REFERENCE) as sim:
+        sim.step(5)
+        got = sim.download()
+    for _ in range(5):
+        oracle.step(s)
+    assert np.array_equal(got.id, s.id), "body order"
+    for f in ("px", "py", "vx", "vy", "ax", "ay"):
+        assert_bits_equal(getattr(got, f), getattr(s, f), f)
+
+
+def test_graph_replays_match_eager(nbx, states):
+    s = states["disc20k_60"]
+    runs = []
+    for use_graph in (False, True):
+        with new_sim(nbx, s) as sim:
+            sim.use_graph = use_graph
+            out = []
+            for _ in range(3):  # graph replays with host syncs in between
+                sim.step(2)
+                out.append(sim.download())
+            runs.append(out)
+    for a, b in zip(*runs):
+        for f in ("px", "py", "vx", "vy", "ax", "ay"):
+            assert_bits_equal(getattr(b, f), getattr(a, f), f)
+        assert np.array_equal(a.id, b.id)
+
+
+def test_deterministic(nbx, states):
+    s = states["disc20k_60"]
+    outs = []
+    for _ in range(2):
+        with new_sim(nbx, s) as sim:
+            sim.step(10)
+            outs.append(sim.download())
+    for f in ("px", "py", "vx", "vy", "ax", "ay"):
+        assert_bits_equal(getattr(outs[0], f), getattr(outs[1], f), f)
+
+
+@pytest.mark.parametrize("n", [1, 2, 16, 17, 33, 300])
+def test_small_systems(nbx, oracle, n):
+    s = oracle.uniform_disc(n)
+    want = s.copy()
+    with new_sim(nbx, s, summation=nbx.SUM_REFERENCE) as sim:
+        sim.step(3)
+        got = sim.download()
+    for _ in range(3):
+        oracle.step(want)
+    for f in ("px", "py", "vx", "vy", "ax", "ay"):
+        assert_bits_equal(getattr(got, f), getattr(want, f), f)
+
+
+def test_deep_cluster_below_depth_16(nbx, oracle):
+    """More than 16 bodies inside one depth-16 cell: the low key word orders them."""
+    from oracle_lib import Bodies
+    rng = np.random.default_rng(7)
+    n = 3000
+    b = Bodies.empty(n)
+    b.px[:] = rng.uniform(-1e4, 1e4, n).astype(np.float32)
+    b.py[:] = rng.uniform(-1e4, 1e4, n).astype(np.float32)
+    b.px[:60] = (1234.5 + rng.uniform(0, 1e-3, 60)).astype(np.float32)
+    b.py[:60] = (-321.25 + rng.uniform(0, 1e-3, 60)).astype(np.float32)
+    b.m[:] = 1.0
+    b.r[:] = 1e-4
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    srt, hi, lo, perm, quad = oracle.sorted_state(b)
+    tree = oracle.build_tree(srt, hi, lo, quad)
+    assert tree.levels > 17, "state must reach below depth 16"
+    wx, wy = oracle.accelerations(srt, tree, 0.5)
+    with new_sim(nbx, b, summation=nbx.SUM_REFERENCE, theta=0.5) as sim:
+        sim.attract()
+        gt = sim.tree()
+        _, _, gperm = sim.keys()
+        got = sim.download()
+    assert np.array_equal(gperm, perm)
+    for k in ("children", "next", "start", "end"):
+        assert np.array_equal(gt[k], getattr(tree, k)), k
+    for k in ("comX", "comY", "mass", "size"):
+        assert_bits_equal(gt[k], getattr(tree, k), k)
+    assert_bits_equal(got.ax, wx, "ax")
+    assert_bits_equal(got.ay, wy, "ay")
+
+
+def test_large_contact_component(nbx, oracle):
+    """A packed clump: one contact component far above 16 pairs (workgroup path)."""
+    from oracle_lib import Bodies
+    rng = np.random.default_rng(3)
+    side = 30
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)
+    n = g.shape[0] + 500
+    b = Bodies.empty(n)
+    b.px[: g.shape[0]] = (g[:, 0] * 1.9 + rng.uniform(-0.05, 0.05, g.shape[0])).astype(np.float32)
+    b.py[: g.shape[0]] = (g[:, 1] * 1.9 + rng.uniform(-0.05, 0.05, g.shape[0])).astype(np.float32)
+    b.px[g.shape[0]:] = rng.uniform(-2000, 2000, 500).astype(np.float32)
+    b.py[g.shape[0]:] = rng.uniform(-2000, 2000, 500).astype(np.float32)
+    b.vx[:] = rng.standard_normal(n).astype(np.float32)
+    b.vy[:] = rng.standard_normal(n).astype(np.float32)
+    b.m[:] = 1.0
+    b.r[:] = 1.0
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    want = b.copy()
+    oracle.collide(want)
+    with new_sim(nbx, b) as sim:
+        sim.collide()
+        info = sim.collide_info()
+        got = sim.download()
+    assert info["largest"] > 16
+    for f in ("px", "py", "vx", "vy"):
+        assert_bits_equal(getattr(got, f), getattr(want, f), f)
+
+
+def test_exact_forces_fp64(nbx, oracle, states):
+    s = states["disc10k"]
+    targets = np.arange(0, s.n, 37, dtype=np.uint32)
+    wx, wy = oracle.exact_forces(s, targets)
+    with new_sim(nbx, s) as sim:
+        gx, gy = sim.exact_forces(targets)
+    den = np.sqrt(wx ** 2 + wy ** 2)
+    rel = np.sqrt((gx - wx) ** 2 + (gy - wy) ** 2) / den
+    assert np.max(rel) < 1e-12
+
+
+def test_interaction_counts(nbx, oracle, states):
+    s = states["disc20k_60"]
+    srt, hi, lo, perm, quad = oracle.sorted_state(s)
+    tree = oracle.build_tree(srt, hi, lo, quad)
+    _, _, far, pairs = oracle.accelerations(srt, tree, 0.5, counts=True)
+    with new_sim(nbx, s, theta=0.5) as sim:
+        sim.attract()
+        gf, gp = sim.count_interactions()
+    assert gf == int(far.sum()) and gp == int(pairs.sum())

@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Throughput of the Barnes-Hut step on one MI355X.
+
+Workload: the reference's rotating disc (uniformDisc, seed 0) with 1,000,000
+bodies, theta = 0.5, epsilon = 1, dt = 0.05, collisions on -- the GPU timing
+configuration of the reference (plain Barnes-Hut, N in {320k, 1M}). A step is
+Simulation::step: iterate -> collide -> build the quadtree -> force walk.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--theta T]
+
+Prints one JSON line. `value` is steps/s over K steps with the state resident
+in HBM (no host transfer in the timed region). The timed region runs the
+captured per-phase HIP graphs with HIP events between phases, on the
+library's stream; the walk kernel's average duration comes from those events
+and prices the roofline (20 FP32 operations per body-node interaction,
+against the 157.3 TFLOP/s FP32 peak). The CPU baseline is the reference's own
+code (oracle/_ref/nbx_ref_bench, OpenMP) on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import importlib.util
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+FLOP_PER_INTERACTION = 20      # far-field monopole or body pair, FP32
+PEAK_FP32_TFLOPS = 157.3       # MI355X published FP32 peak
+REF_BENCH = os.path.join(ROOT, "oracle", "_ref", "nbx_ref_bench")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_1m_theta05_traffic.txt")
+
+
+def load_nbx():
+    path = os.path.join(ROOT, "n-body-simulation_amd", "nbx.py")
+    spec = importlib.util.spec_from_file_location("nbx", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["nbx"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def device_sync() -> None:
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
+
+def walk_traffic():
+    """HBM bytes per walk launch from the committed rocprofv3 PMC summary."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n: int, theta: float, steps: int):
+    """Reference OpenMP build on this host's cores, bounded sample."""
+    if not os.path.exists(REF_BENCH):
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(cores, 16))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores",
+               OMP_PROC_BIND="close")
+    t0 = time.time()
+    out = subprocess.run([REF_BENCH, str(n), str(steps), str(theta)], env=env,
+                         capture_output=True, text=True, timeout=900, check=True)
+    m = re.search(r"avg_compute_step=([0-9.]+) ms", out.stdout)
+    ms = float(m.group(1))
+    print(f"[bench] cpu baseline: {out.stdout.strip()} (wall {time.time() - t0:.1f} s)",
+          file=sys.stderr, flush=True)
+    return {"value": round(1000.0 / ms, 4), "unit": "steps/s", "cores": threads,
+            "kind": "reference",
+            "sample": (f"reference Simulation (QuadTree/Collision/Utils.cpp, OpenMP, "
+                       f"-Ofast -march=x86-64-v3), same {n}-body disc at theta={theta}, "
+                       f"steps 1..{steps} from the initial state; {ms:.1f} ms/step")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; a multi-GPU run is not part of it "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    nbx = load_nbx()
+    print(f"[bench] n={args.n} theta={args.theta} warmup={args.warmup} "
+          f"steps={args.steps}", file=sys.stderr, flush=True)
+    sim = nbx.Simulation(args.n)
+    sim.theta = args.theta
+    sim.collisions = True
+    sim.law = nbx.LAW_REFERENCE
+    sim.summation = nbx.SUM_COMPENSATED
+    sim.use_graph = True
+    sim.step(args.warmup)
+    far0, pair0 = sim.count_interactions()
+
+    # Timed region: K steps, per-phase graphs with events between phases.
+    sim.set_profiling(True)
+    device_sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    device_sync()
+    t1 = time.perf_counter()
+    phases, psteps = sim.phase_times()
+    sim.set_profiling(False)
+    elapsed = t1 - t0
+    far1, pair1, visits1, terms1 = sim.count_interactions(detail=True)
+    lane_util = (far1 + pair1) / (64.0 * (visits1 + terms1))
+    print(f"[bench] timed {args.steps} steps in {elapsed:.4f} s", file=sys.stderr,
+          flush=True)
+
+    # Same work as one graph per step (no events), for reference.
+    device_sync()
+    t2 = time.perf_counter()
+    sim.step(args.steps)
+    device_sync()
+    single_graph_sps = args.steps / (time.perf_counter() - t2)
+
+    info = sim.collide_info()
+    nodes, levels = sim.tree_info()
+    sim.close()
+
+    phase_ms = {k: v / max(psteps, 1) for k, v in phases.items()}
+    walk_s = phase_ms["walk"] / 1e3
+    interactions = 0.5 * ((far0 + pair0) + (far1 + pair1))
+    achieved = interactions * FLOP_PER_INTERACTION / walk_s / 1e12
+    value = args.steps / elapsed
+    roofline = {
+        "bound": "valu-fp32",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        "traffic": walk_traffic(),
+        "kernel": "k_walk<compensated, reference law>",
+        "flop_per_launch": interactions * FLOP_PER_INTERACTION,
+        "walk_ms": round(phase_ms["walk"], 4),
+        "walk_lane_utilisation": round(lane_util, 4),
+    }
+    cpu = None if args.no_cpu else cpu_baseline(args.n, args.theta, args.cpu_steps)
+    line = {
+        "metric": "steps/s at 1M bodies (plain Barnes-Hut, theta=0.5, collisions on)",
+        "value": round(value, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (uniformDisc seed 0, generated on the host, resident in HBM)",
+        "config": {
+            "workload": (f"rotating disc, N={args.n}, theta={args.theta}, eps=1, "
+                         f"dt=0.05, collisions on, steps {args.warmup + 1}.."
+                         f"{args.warmup + args.steps}"),
+            "n_bodies": args.n, "theta": args.theta, "epsilon": 1.0, "dt": 0.05,
+            "collisions": True, "force_law": "reference (fastInvSqrt)",
+            "summation": "compensated", "parallelism": "single GPU",
+        },
+        "interactions_per_step": interactions,
+        "interactions_per_s": interactions * value,
+        "phase_ms": {k: round(v, 4) for k, v in phase_ms.items()},
+        "single_graph_steps_per_s": round(single_graph_sps, 3),
+        "tree": {"nodes": nodes, "levels": levels},
+        "collisions": info,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

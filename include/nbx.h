@@ -170,10 +170,14 @@ int nbx_download_pairs(nbx_sim *sim, int64_t cap, int32_t *later,
                        int32_t *earlier, int32_t *comp, int64_t *count);
 
 /* Runs the force walk once more in counting mode over the current tree and
- * returns far-node (mass > 0) and body-pair interaction totals. Does not
- * change the state. */
+ * returns far-node (mass > 0) and body-pair interaction totals, plus the
+ * walk's wavefront work: node visits summed over waves and leaf-body terms
+ * evaluated per wave (each costs all 64 lanes), so lane utilisation is
+ * (far + pairs) / (64 * (visits + leaf terms)). Any pointer may be null.
+ * Does not change the state. (No reference counterpart: instrumentation.) */
 int nbx_count_interactions(nbx_sim *sim, uint64_t *far_nodes,
-                           uint64_t *body_pairs);
+                           uint64_t *body_pairs, uint64_t *wave_visits,
+                           uint64_t *wave_leaf_terms);
 
 /* ---- Timing (the per-phase event bench of src/cuda/CheckCuda.cu:677-712).
  * With profiling on, nbx_step / phase calls launch eagerly and record HIP

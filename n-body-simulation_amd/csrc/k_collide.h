@@ -14,6 +14,7 @@
 // sequential replay.
 #pragma once
 
+#include "k_sort.h"
 #include "nbx_device.h"
 
 namespace nbx {
@@ -45,9 +46,22 @@ __device__ inline int32_t cellCoord(float v, float invCell) {
   return static_cast<int32_t>(c);
 }
 
-// Cell -> bucket: horizontally adjacent cells land in adjacent buckets.
+// Spreads the low 16 bits of v over the even bit positions.
+__device__ inline uint32_t spread16(uint32_t v) {
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+// Cell -> bucket: Morton interleave of the cell coordinates (low 16 bits
+// each), so cells close in space land in nearby buckets; cells that collide
+// in a bucket are told apart by their exact coordinates.
 __device__ inline uint32_t cellBucket(int32_t cx, int32_t cy, uint32_t mask) {
-  return (static_cast<uint32_t>(cy) * 0x9E3779B1u + static_cast<uint32_t>(cx)) &
+  return (spread16(static_cast<uint32_t>(cx)) |
+          (spread16(static_cast<uint32_t>(cy)) << 1)) &
          mask;
 }
 
@@ -137,23 +151,25 @@ __device__ inline void resolvePair(float *__restrict__ px,
   py[j] = p2y + n2y * t;
 }
 
-__global__ void k_collide_reset(uint32_t n, Ctl *__restrict__ ctl) {
-  ctl->selPrefix = 0;
-  ctl->selRank = n / 2;
-  ctl->largeCount = 0;
-  ctl->pairCount = 0;
-  ctl->compCount = 0;
-  ctl->compPairs = 0;
-  ctl->bigCount = 0;
-  ctl->largest = 0;
-  ctl->status &= ~(kStatusPairOverflow | kStatusComponentTooLarge);
-}
-
-// Box widths (Collision.cpp:163-164), input of the median cell size.
+// Box widths (Collision.cpp:163-164), input of the median cell size. The
+// first thread also resets the collide counters (nothing else in this kernel
+// reads them).
 __global__ __launch_bounds__(kBlock) void k_widths(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ r, float *__restrict__ widths) {
+    const float *__restrict__ r, float *__restrict__ widths,
+    Ctl *__restrict__ ctl) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i == 0) {
+    ctl->selPrefix = 0;
+    ctl->selRank = n / 2;
+    ctl->largeCount = 0;
+    ctl->pairCount = 0;
+    ctl->compCount = 0;
+    ctl->compPairs = 0;
+    ctl->bigCount = 0;
+    ctl->largest = 0;
+    ctl->status &= ~(kStatusPairOverflow | kStatusComponentTooLarge);
+  }
   if (i >= n) {
     return;
   }
@@ -214,61 +230,110 @@ __global__ __launch_bounds__(kBlock) void k_grid_scatter(
                              static_cast<int32_t>(i), 0);
 }
 
-__device__ inline void emitPair(Ctl *__restrict__ ctl, int4 *__restrict__ pairs,
-                                uint32_t cap, int later, int earlier,
-                                float laterMinX, float earlierMinX) {
-  const uint32_t slot = atomicAdd(&ctl->pairCount, 1u);
-  if (slot < cap) {
-    pairs[slot] = make_int4(later, earlier, __float_as_int(laterMinX),
-                            __float_as_int(earlierMinX));
+// Pair output of one workgroup, staged in LDS: lanes append with LDS
+// atomics and the workgroup reserves its global range with a single atomic
+// (a global counter hit once per pair serialises across the whole chip).
+constexpr int kPairStage = 1024;
+
+struct PairStage {
+  int4 buf[kPairStage];
+  uint32_t count;
+  uint32_t base;
+};
+
+__device__ inline void stageInit(PairStage &st) {
+  if (threadIdx.x == 0) {
+    st.count = 0;
+  }
+  __syncthreads();
+}
+
+__device__ inline void stagePair(PairStage &st, Ctl *__restrict__ ctl,
+                                 int4 *__restrict__ pairs, uint32_t cap,
+                                 int later, int earlier, float laterMinX,
+                                 float earlierMinX) {
+  const int4 p = make_int4(later, earlier, __float_as_int(laterMinX),
+                           __float_as_int(earlierMinX));
+  const uint32_t slot = atomicAdd(&st.count, 1u);
+  if (slot < kPairStage) {
+    st.buf[slot] = p;
+    return;
+  }
+  // Staging full (very dense clumps): direct append.
+  const uint32_t g = atomicAdd(&ctl->pairCount, 1u);
+  if (g < cap) {
+    pairs[g] = p;
   } else {
     atomicOr(&ctl->status, kStatusPairOverflow);
   }
 }
 
-// One thread per small box (grid slot): its earlier partners among small
-// boxes of the 3x3 neighbouring cells and among all large boxes.
+__device__ inline void stageFlush(PairStage &st, Ctl *__restrict__ ctl,
+                                  int4 *__restrict__ pairs, uint32_t cap) {
+  __syncthreads();
+  const uint32_t cnt = min<uint32_t>(st.count, kPairStage);
+  if (threadIdx.x == 0) {
+    st.base = cnt ? atomicAdd(&ctl->pairCount, cnt) : 0u;
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+    const uint32_t g = st.base + k;
+    if (g < cap) {
+      pairs[g] = st.buf[k];
+    } else {
+      atomicOr(&ctl->status, kStatusPairOverflow);
+    }
+  }
+}
+
+// One thread per small body, in device order (the previous build's key
+// order, so a wave covers a compact patch of space): its earlier partners
+// among small boxes of the 3x3 neighbouring cells and among all large boxes.
 __global__ __launch_bounds__(kBlock) void k_find_small(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
     const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
+    const uint32_t *__restrict__ bucketOf,
     const uint32_t *__restrict__ bucketStart,
     const float4 *__restrict__ gridBox, const int4 *__restrict__ gridCell,
     const int32_t *__restrict__ largeList, int4 *__restrict__ pairs,
     uint32_t cap) {
-  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t large = ctl->largeCount;
-  if (s >= n - large) {
-    return;
-  }
-  const float4 eb = gridBox[s];
-  const int4 ec = gridCell[s];
-  const Box me{eb.x, eb.y, eb.z, eb.w};
-  const int self = ec.z;
-  for (int dy = -1; dy <= 1; ++dy) {
-    for (int dx = -1; dx <= 1; ++dx) {
-      const int32_t cx = ec.x + dx, cy = ec.y + dy;
-      const uint32_t h = cellBucket(cx, cy, mask);
-      const uint32_t end = bucketStart[h + 1];
-      for (uint32_t o = bucketStart[h]; o < end; ++o) {
-        const int4 oc = gridCell[o];
-        if (oc.x != cx || oc.y != cy || oc.z == self) {
-          continue;
-        }
-        const float4 ob = gridBox[o];
-        if (beforeBox(ob.x, oc.z, me.minX, self) &&
-            overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
-          emitPair(ctl, pairs, cap, self, oc.z, me.minX, ob.x);
+  __shared__ PairStage st;
+  stageInit(st);
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n && bucketOf[i] != 0xffffffffu) {
+    const uint32_t large = ctl->largeCount;
+    const float invCell = ctl->invCell;
+    const Box me = bodyBox(px[i], py[i], r[i]);
+    const int self = static_cast<int>(i);
+    const int32_t mx = cellCoord(me.minX, invCell);
+    const int32_t my = cellCoord(me.minY, invCell);
+    for (int dy = -1; dy <= 1; ++dy) {
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int32_t cx = mx + dx, cy = my + dy;
+        const uint32_t h = cellBucket(cx, cy, mask);
+        const uint32_t end = bucketStart[h + 1];
+        for (uint32_t o = bucketStart[h]; o < end; ++o) {
+          const int4 oc = gridCell[o];
+          if (oc.x != cx || oc.y != cy || oc.z == self) {
+            continue;
+          }
+          const float4 ob = gridBox[o];
+          if (beforeBox(ob.x, oc.z, me.minX, self) &&
+              overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
+            stagePair(st, ctl, pairs, cap, self, oc.z, me.minX, ob.x);
+          }
         }
       }
     }
-  }
-  for (uint32_t k = 0; k < large; ++k) {
-    const int l = largeList[k];
-    const Box lb = bodyBox(px[l], py[l], r[l]);
-    if (beforeBox(lb.minX, l, me.minX, self) && overlapsBox(lb, me)) {
-      emitPair(ctl, pairs, cap, self, l, me.minX, lb.minX);
+    for (uint32_t k = 0; k < large; ++k) {
+      const int l = largeList[k];
+      const Box lb = bodyBox(px[l], py[l], r[l]);
+      if (beforeBox(lb.minX, l, me.minX, self) && overlapsBox(lb, me)) {
+        stagePair(st, ctl, pairs, cap, self, l, me.minX, lb.minX);
+      }
     }
   }
+  stageFlush(st, ctl, pairs, cap);
 }
 
 // Every body against every large box that comes later (Collision.cpp:
@@ -279,23 +344,29 @@ __global__ __launch_bounds__(kBlock) void k_find_large(
     const float *__restrict__ r, Ctl *__restrict__ ctl,
     const int32_t *__restrict__ largeList, int4 *__restrict__ pairs,
     uint32_t cap) {
-  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t large = ctl->largeCount;
-  if (x >= n || large == 0) {
-    return;
+  if (large == 0) {
+    return; // uniform across the grid
   }
-  const Box xb = bodyBox(px[x], py[x], r[x]);
-  for (uint32_t k = 0; k < large; ++k) {
-    const int l = largeList[k];
-    if (l == static_cast<int>(x)) {
-      continue;
-    }
-    const Box lb = bodyBox(px[l], py[l], r[l]);
-    if (beforeBox(xb.minX, static_cast<int>(x), lb.minX, l) &&
-        overlapsBox(xb, lb)) {
-      emitPair(ctl, pairs, cap, l, static_cast<int>(x), lb.minX, xb.minX);
+  __shared__ PairStage st;
+  stageInit(st);
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  if (x < n) {
+    const Box xb = bodyBox(px[x], py[x], r[x]);
+    for (uint32_t k = 0; k < large; ++k) {
+      const int l = largeList[k];
+      if (l == static_cast<int>(x)) {
+        continue;
+      }
+      const Box lb = bodyBox(px[l], py[l], r[l]);
+      if (beforeBox(xb.minX, static_cast<int>(x), lb.minX, l) &&
+          overlapsBox(xb, lb)) {
+        stagePair(st, ctl, pairs, cap, l, static_cast<int>(x), lb.minX,
+                  xb.minX);
+      }
     }
   }
+  stageFlush(st, ctl, pairs, cap);
 }
 
 // ---- connected components of the contact graph ----
@@ -358,24 +429,62 @@ __global__ __launch_bounds__(kBlock) void k_comp_count(
   }
 }
 
-// Allocates each component's contiguous pair segment.
+constexpr int kAllocItems = 16; // bodies per thread in k_comp_alloc
+
+// Allocates each component's contiguous pair segment. A workgroup covers
+// 4096 bodies (blocked per thread, so offsets follow body order) and
+// reserves its ranges with one atomic per counter.
 __global__ __launch_bounds__(kBlock) void k_comp_alloc(
     uint32_t n, Ctl *__restrict__ ctl, const uint32_t *__restrict__ compCount,
     uint32_t *__restrict__ compStart, int4 *__restrict__ compInfo) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) {
-    return;
+  __shared__ uint32_t scratch[kBlock / kWave];
+  __shared__ uint32_t wmax[kBlock / kWave];
+  __shared__ uint32_t base[2];
+  const uint32_t first =
+      (blockIdx.x * kBlock + threadIdx.x) * static_cast<uint32_t>(kAllocItems);
+  uint32_t c[kAllocItems];
+  uint32_t pairs = 0, roots = 0, mx = 0;
+#pragma unroll
+  for (int k = 0; k < kAllocItems; ++k) {
+    c[k] = first + k < n ? compCount[first + k] : 0u;
+    pairs += c[k];
+    roots += c[k] > 0 ? 1u : 0u;
+    mx = max(mx, c[k]);
   }
-  const uint32_t c = compCount[i];
-  if (c == 0) {
-    return;
+  uint32_t totalPairs, totalRoots;
+  uint32_t prePairs = blockExclusiveScan(pairs, scratch, &totalPairs);
+  uint32_t preRoots = blockExclusiveScan(roots, scratch, &totalRoots);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    mx = max(mx, static_cast<uint32_t>(__shfl_xor(mx, off, kWave)));
   }
-  const uint32_t start = atomicAdd(&ctl->compPairs, c);
-  const uint32_t j = atomicAdd(&ctl->compCount, 1u);
-  compStart[i] = start;
-  compInfo[j] = make_int4(static_cast<int>(i), static_cast<int>(start),
-                          static_cast<int>(c), 0);
-  atomicMax(&ctl->largest, c);
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    wmax[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    base[0] = totalPairs ? atomicAdd(&ctl->compPairs, totalPairs) : 0u;
+    base[1] = totalRoots ? atomicAdd(&ctl->compCount, totalRoots) : 0u;
+    uint32_t bm = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      bm = max(bm, wmax[w]);
+    }
+    if (bm) {
+      atomicMax(&ctl->largest, bm);
+    }
+  }
+  __syncthreads();
+  uint32_t p = base[0] + prePairs, q = base[1] + preRoots;
+#pragma unroll
+  for (int k = 0; k < kAllocItems; ++k) {
+    if (c[k] > 0) {
+      compStart[first + k] = p;
+      compInfo[q] = make_int4(static_cast<int>(first + k),
+                              static_cast<int>(p), static_cast<int>(c[k]), 0);
+      p += c[k];
+      q += 1;
+    }
+  }
 }
 
 // Moves every pair into its component's segment; consumes the counts back

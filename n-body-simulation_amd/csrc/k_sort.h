@@ -126,33 +126,27 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(
   hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// One workgroup: exclusive scan of the digit-major histogram table (in
-// place), giving every (digit, tile) its global output offset.
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t entries,
-                                                  uint32_t *__restrict__ t) {
-  __shared__ uint32_t sums[1024];
-  const uint32_t per = (entries + 1023) / 1024;
-  const uint32_t lo = threadIdx.x * per;
-  const uint32_t hi = min(entries, lo + per);
-  uint32_t s = 0;
-  for (uint32_t i = lo; i < hi; ++i) {
-    s += t[i];
+// Block d: exclusive scan, in place, of digit d's per-tile counts (one row
+// of the digit-major table) and the row total. The scatter adds the digit's
+// base (scan of the 256 row totals) itself.
+__global__ __launch_bounds__(kBlock) void k_rs_scan(
+    uint32_t tiles, uint32_t *__restrict__ hist,
+    uint32_t *__restrict__ rowTotal) {
+  __shared__ uint32_t scratch[kBlock / kWave];
+  uint32_t *row = hist + static_cast<size_t>(blockIdx.x) * tiles;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < tiles; base += kBlock) {
+    const uint32_t t = base + threadIdx.x;
+    const uint32_t v = t < tiles ? row[t] : 0u;
+    uint32_t total;
+    const uint32_t pre = blockExclusiveScan(v, scratch, &total);
+    if (t < tiles) {
+      row[t] = carry + pre;
+    }
+    carry += total;
   }
-  sums[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint32_t v = threadIdx.x >= static_cast<uint32_t>(off)
-                           ? sums[threadIdx.x - off]
-                           : 0u;
-    __syncthreads();
-    sums[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t run = sums[threadIdx.x] - s;
-  for (uint32_t i = lo; i < hi; ++i) {
-    const uint32_t v = t[i];
-    t[i] = run;
-    run += v;
+  if (threadIdx.x == 0) {
+    rowTotal[blockIdx.x] = carry;
   }
 }
 
@@ -161,7 +155,8 @@ __global__ __launch_bounds__(1024) void k_rs_scan(uint32_t entries,
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(
     uint32_t n, const uint32_t *__restrict__ keysIn,
     const uint32_t *__restrict__ valsIn, int shift,
-    const uint32_t *__restrict__ offsets, uint32_t tiles,
+    const uint32_t *__restrict__ offsets,
+    const uint32_t *__restrict__ rowTotal, uint32_t tiles,
     uint32_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut) {
   __shared__ uint32_t sk[kSortTile];
   __shared__ uint32_t sv[kSortTile];
@@ -194,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
   __syncthreads();
   uint32_t total;
   digitStart[tid] = blockExclusiveScan(run[tid], scratch, &total);
-  goff[tid] = offsets[tid * tiles + blockIdx.x];
+  const uint32_t digitBase = blockExclusiveScan(rowTotal[tid], scratch, &total);
+  goff[tid] = digitBase + offsets[tid * tiles + blockIdx.x];
   __syncthreads();
   run[tid] = 0;
   __syncthreads();

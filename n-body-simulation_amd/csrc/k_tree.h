@@ -56,21 +56,15 @@ __device__ inline uint32_t blockScanT(uint32_t v, uint32_t *scratch,
   return waveOff + x - v;
 }
 
-__global__ void k_build_reset(Ctl *__restrict__ ctl) {
-  ctl->bbox[0] = 0xffffffffu;
-  ctl->bbox[1] = 0u;
-  ctl->bbox[2] = 0xffffffffu;
-  ctl->bbox[3] = 0u;
-  ctl->deepCount = 0;
-  ctl->status &= ~(kStatusNodeOverflow | kStatusDeepTooLarge);
-}
+constexpr int kBboxParts = 256; // workgroups of the bounding-box pass
 
-// Quad::newContaining (QuadTree.cpp:30-51): exact min/max via order-
-// preserving integer atomics (NaN coordinates are ignored, as std::min/max
-// ignore them there).
+// Quad::newContaining (QuadTree.cpp:30-51), stage 1: per-workgroup exact
+// min/max as order-preserving bits (NaN coordinates are ignored, as
+// std::min/max ignore them there).
 __global__ __launch_bounds__(kBlock) void k_bbox(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    Ctl *__restrict__ ctl) {
+    uint4 *__restrict__ parts) {
+  __shared__ uint4 red[kBlock / kWave];
   uint32_t mnx = 0xffffffffu, mxx = 0u, mny = 0xffffffffu, mxy = 0u;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n;
        i += gridDim.x * kBlock) {
@@ -94,29 +88,69 @@ __global__ __launch_bounds__(kBlock) void k_bbox(
     mxy = max(mxy, static_cast<uint32_t>(__shfl_xor(mxy, off, kWave)));
   }
   if ((threadIdx.x & (kWave - 1)) == 0) {
-    atomicMin(&ctl->bbox[0], mnx);
-    atomicMax(&ctl->bbox[1], mxx);
-    atomicMin(&ctl->bbox[2], mny);
-    atomicMax(&ctl->bbox[3], mxy);
+    red[threadIdx.x >> 6] = make_uint4(mnx, mxx, mny, mxy);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint4 b = red[0];
+    for (int w = 1; w < kBlock / kWave; ++w) {
+      b.x = min(b.x, red[w].x);
+      b.y = max(b.y, red[w].y);
+      b.z = min(b.z, red[w].z);
+      b.w = max(b.w, red[w].w);
+    }
+    parts[blockIdx.x] = b;
   }
 }
 
 // Root quad (QuadTree.cpp:46-50) and the float-descent key of every body.
+// Stage 2 of the bounding box runs in every workgroup here (the partials
+// are a few KB in L2), so no atomics and no reset are needed. Thread 0 also
+// publishes the root quad and clears the build's counters.
 __global__ __launch_bounds__(kBlock) void k_keys(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    Ctl *__restrict__ ctl, uint32_t *__restrict__ keyHi,
-    uint32_t *__restrict__ keyLo, uint32_t *__restrict__ vals) {
+    const uint4 *__restrict__ parts, uint32_t nparts, Ctl *__restrict__ ctl,
+    uint32_t *__restrict__ keyHi, uint32_t *__restrict__ keyLo,
+    uint32_t *__restrict__ vals) {
+  __shared__ uint4 red[kBlock / kWave];
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const float minX = ordFloat(ctl->bbox[0]), maxX = ordFloat(ctl->bbox[1]);
-  const float minY = ordFloat(ctl->bbox[2]), maxY = ordFloat(ctl->bbox[3]);
+  uint4 b = threadIdx.x < nparts ? parts[threadIdx.x]
+                                 : make_uint4(0xffffffffu, 0u, 0xffffffffu, 0u);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    b.x = min(b.x, static_cast<uint32_t>(__shfl_xor(b.x, off, kWave)));
+    b.y = max(b.y, static_cast<uint32_t>(__shfl_xor(b.y, off, kWave)));
+    b.z = min(b.z, static_cast<uint32_t>(__shfl_xor(b.z, off, kWave)));
+    b.w = max(b.w, static_cast<uint32_t>(__shfl_xor(b.w, off, kWave)));
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    red[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  b = red[0];
+  for (int w = 1; w < kBlock / kWave; ++w) {
+    b.x = min(b.x, red[w].x);
+    b.y = max(b.y, red[w].y);
+    b.z = min(b.z, red[w].z);
+    b.w = max(b.w, red[w].w);
+  }
+  const float minX = ordFloat(b.x), maxX = ordFloat(b.y);
+  const float minY = ordFloat(b.z), maxY = ordFloat(b.w);
   const float qcx = (minX + maxX) * 0.5f;
   const float qcy = (minY + maxY) * 0.5f;
   const float wx = maxX - minX, wy = maxY - minY;
   const float qs = (wx < wy) ? wy : wx;
   if (i == 0) {
+    ctl->bbox[0] = b.x;
+    ctl->bbox[1] = b.y;
+    ctl->bbox[2] = b.z;
+    ctl->bbox[3] = b.w;
     ctl->quad[0] = qcx;
     ctl->quad[1] = qcy;
     ctl->quad[2] = qs;
+    ctl->deepCount = 0;
+    ctl->status &= ~(kStatusNodeOverflow | kStatusDeepTooLarge |
+                     kStatusTreeInconsistent);
   }
   if (i >= n) {
     return;
@@ -513,10 +547,63 @@ __device__ inline int nodeDepth(const Ctl *__restrict__ ctl, int x) {
   return d;
 }
 
+constexpr int kUpBodies = 256; // subtrees up to this many bodies: one lane
+
+struct Sum3 {
+  float x, y, m;
+};
+
+// accumulateLeaf (QuadTree.cpp:93-142): four interleaved accumulators,
+// combined as ((s0 + s1) + s2) + s3.
+__device__ inline Sum3 leafSum(const float *__restrict__ px,
+                               const float *__restrict__ py,
+                               const float *__restrict__ m, int i, int e) {
+  float p0x = 0, p0y = 0, p1x = 0, p1y = 0, p2x = 0, p2y = 0, p3x = 0,
+        p3y = 0;
+  float m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  for (; i + 3 < e; i += 4) {
+    p0x += px[i] * m[i];
+    p0y += py[i] * m[i];
+    m0 += m[i];
+    p1x += px[i + 1] * m[i + 1];
+    p1y += py[i + 1] * m[i + 1];
+    m1 += m[i + 1];
+    p2x += px[i + 2] * m[i + 2];
+    p2y += py[i + 2] * m[i + 2];
+    m2 += m[i + 2];
+    p3x += px[i + 3] * m[i + 3];
+    p3y += py[i + 3] * m[i + 3];
+    m3 += m[i + 3];
+  }
+  for (; i < e; ++i) {
+    p0x += px[i] * m[i];
+    p0y += py[i] * m[i];
+    m0 += m[i];
+  }
+  return Sum3{p0x + p1x + p2x + p3x, p0y + p1y + p2y + p3y, m0 + m1 + m2 + m3};
+}
+
+// Branch value from its children, left to right (QuadTree.cpp:243-253).
+__device__ inline Sum3 addSum(const Sum3 &a, const Sum3 &b) {
+  return Sum3{a.x + b.x, a.y + b.y, a.m + b.m};
+}
+
+__device__ inline void writeSum(int x, const Sum3 &v, float4 *__restrict__ nodeF,
+                                float2 *__restrict__ wsum) {
+  // propagate's final pass (QuadTree.cpp:256-261): pos /= max(mass, FLT_MIN).
+  const float dv = (v.m < FLT_MIN) ? FLT_MIN : v.m;
+  wsum[x] = make_float2(v.x, v.y);
+  nodeF[x].x = v.x / dv;
+  nodeF[x].y = v.y / dv;
+  nodeF[x].z = v.m;
+}
+
 // Per node: the `next` skip pointer (QuadTree.cpp:87-90: child q < 3 points
-// at q + 1, the last child inherits its parent's), and for leaves the mass
-// and weighted-position sums (accumulateLeaf, QuadTree.cpp:93-142, four
-// interleaved accumulators) and centre of mass (QuadTree.cpp:256-261).
+// at q + 1, the last child inherits its parent's), and the mass sums of
+// every leaf and of every branch holding at most kUpBodies bodies. A branch
+// is evaluated bottom-up over its own subtree with an explicit stack, in the
+// reference's association, so no level-by-level launches are needed below
+// the top of the tree.
 __global__ __launch_bounds__(kBlock) void k_node_finish(
     const Ctl *__restrict__ ctl, const float *__restrict__ px,
     const float *__restrict__ py, const float *__restrict__ m,
@@ -540,71 +627,113 @@ __global__ __launch_bounds__(kBlock) void k_node_finish(
     }
     const int4 ni = nodeI[x];
     nodeI[x].y = y == 0 ? 0 : y + 1;
-    if (ni.x != 0) {
+    if (ni.x == 0) {
+      writeSum(x, leafSum(px, py, m, ni.z, ni.w), nodeF, wsum);
       continue;
     }
-    float p0x = 0, p0y = 0, p1x = 0, p1y = 0, p2x = 0, p2y = 0, p3x = 0,
-          p3y = 0;
-    float m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-    int i = ni.z;
-    const int e = ni.w;
-    for (; i + 3 < e; i += 4) {
-      p0x += px[i] * m[i];
-      p0y += py[i] * m[i];
-      m0 += m[i];
-      p1x += px[i + 1] * m[i + 1];
-      p1y += py[i + 1] * m[i + 1];
-      m1 += m[i + 1];
-      p2x += px[i + 2] * m[i + 2];
-      p2y += py[i + 2] * m[i + 2];
-      m2 += m[i + 2];
-      p3x += px[i + 3] * m[i + 3];
-      p3y += py[i + 3] * m[i + 3];
-      m3 += m[i + 3];
+    if (ni.w - ni.z > kUpBodies) {
+      continue; // k_upward_top
     }
-    for (; i < e; ++i) {
-      p0x += px[i] * m[i];
-      p0y += py[i] * m[i];
-      m0 += m[i];
+    int stNode[kMaxDepth + 1];
+    int stQ[kMaxDepth + 1];
+    Sum3 stAcc[kMaxDepth + 1];
+    int sp = 0;
+    stNode[0] = x;
+    stQ[0] = 0;
+    Sum3 result{0.0f, 0.0f, 0.0f};
+    while (true) {
+      const int q = stQ[sp];
+      if (q == 4) {
+        const Sum3 v = stAcc[sp];
+        if (sp == 0) {
+          result = v;
+          break;
+        }
+        --sp;
+        stAcc[sp] = stQ[sp] == 0 ? v : addSum(stAcc[sp], v);
+        stQ[sp] += 1;
+        continue;
+      }
+      const int c = nodeI[stNode[sp]].x + q;
+      const int4 ci = nodeI[c];
+      if (ci.x == 0) {
+        const Sum3 v = leafSum(px, py, m, ci.z, ci.w);
+        stAcc[sp] = q == 0 ? v : addSum(stAcc[sp], v);
+        stQ[sp] = q + 1;
+      } else if (sp < kMaxDepth) {
+        ++sp;
+        stNode[sp] = c;
+        stQ[sp] = 0;
+      } else {
+        atomicOr(&const_cast<Ctl *>(ctl)->status, kStatusTreeInconsistent);
+        break;
+      }
     }
-    const float wx = p0x + p1x + p2x + p3x;
-    const float wy = p0y + p1y + p2y + p3y;
-    const float ms = m0 + m1 + m2 + m3;
-    const float dv = (ms < FLT_MIN) ? FLT_MIN : ms;
-    wsum[x] = make_float2(wx, wy);
-    nodeF[x].x = wx / dv;
-    nodeF[x].y = wy / dv;
-    nodeF[x].z = ms;
+    writeSum(x, result, nodeF, wsum);
   }
 }
 
-// Branch sums for depth d from its children (QuadTree.cpp:239-254), then the
-// centre of mass. Launched deepest level first.
-__global__ __launch_bounds__(kBlock) void k_upward_level(
-    int d, const Ctl *__restrict__ ctl, float4 *__restrict__ nodeF,
-    const int4 *__restrict__ nodeI, float2 *__restrict__ wsum) {
-  if (ctl->status & kStatusNodeOverflow) {
+// One workgroup: the branches holding more than kUpBodies bodies. They form
+// a connected top of the tree (a big node's parent is big), found top-down
+// level by level, then summed bottom-up from their already-final children.
+__global__ __launch_bounds__(1024) void k_upward_top(
+    const Ctl *__restrict__ ctl, float4 *__restrict__ nodeF,
+    const int4 *__restrict__ nodeI, float2 *__restrict__ wsum,
+    int32_t *__restrict__ big) {
+  __shared__ int levelOff[kMaxDepth + 3];
+  __shared__ int appended;
+  if (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent)) {
     return;
   }
-  const int lo = ctl->levelStart[d];
-  const int hi = min(ctl->levelStart[d + 1], ctl->nodeCount);
-  for (int x = lo + blockIdx.x * kBlock + threadIdx.x; x < hi;
-       x += gridDim.x * kBlock) {
-    const int c = nodeI[x].x;
-    if (c == 0) {
-      continue;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    const int4 root = nodeI[0];
+    const bool rootBig = ctl->nodeCount > 0 && root.x != 0 &&
+                         root.w - root.z > kUpBodies;
+    if (rootBig) {
+      big[0] = 0;
     }
-    const float2 w0 = wsum[c], w1 = wsum[c + 1], w2 = wsum[c + 2],
-                 w3 = wsum[c + 3];
-    const float wx = w0.x + w1.x + w2.x + w3.x;
-    const float wy = w0.y + w1.y + w2.y + w3.y;
-    const float ms = nodeF[c].z + nodeF[c + 1].z + nodeF[c + 2].z +
-                     nodeF[c + 3].z;
-    const float dv = (ms < FLT_MIN) ? FLT_MIN : ms;
-    wsum[x] = make_float2(wx, wy);
-    nodeF[x].x = wx / dv;
-    nodeF[x].y = wy / dv;
-    nodeF[x].z = ms;
+    levelOff[0] = 0;
+    levelOff[1] = rootBig ? 1 : 0;
+  }
+  __syncthreads();
+  int levels = 0;
+  for (int d = 0; d <= kMaxDepth; ++d) {
+    const int lo = levelOff[d], hi = levelOff[d + 1];
+    if (lo == hi) {
+      break;
+    }
+    levels = d + 1;
+    if (tid == 0) {
+      appended = 0;
+    }
+    __syncthreads();
+    for (int k = lo + tid; k < hi; k += blockDim.x) {
+      const int c = nodeI[big[k]].x;
+      for (int q = 0; q < 4; ++q) {
+        const int4 ci = nodeI[c + q];
+        if (ci.x != 0 && ci.w - ci.z > kUpBodies) {
+          big[hi + atomicAdd(&appended, 1)] = c + q;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      levelOff[d + 2] = hi + appended;
+    }
+    __syncthreads();
+  }
+  for (int d = levels - 1; d >= 0; --d) {
+    for (int k = levelOff[d] + tid; k < levelOff[d + 1]; k += blockDim.x) {
+      const int x = big[k];
+      const int c = nodeI[x].x;
+      Sum3 v{wsum[c].x, wsum[c].y, nodeF[c].z};
+      for (int q = 1; q < 4; ++q) {
+        v = addSum(v, Sum3{wsum[c + q].x, wsum[c + q].y, nodeF[c + q].z});
+      }
+      writeSum(x, v, nodeF, wsum);
+    }
+    __syncthreads();
   }
 }
 

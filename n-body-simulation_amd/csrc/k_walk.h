@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(
   const float y = valid ? py[i] : 0.0f;
   int wait = valid ? -1 : static_cast<int>(kDone);
   float sx = 0.0f, sy = 0.0f, cx = 0.0f, cy = 0.0f;
-  uint32_t nFar = 0, nPair = 0;
+  uint32_t nFar = 0, nPair = 0, nVisit = 0, nLeafTerm = 0;
   int node = 0;
   // A preorder walk visits each node at most once: more iterations than
   // nodes, or an index out of range, means a corrupt tree -- stop and flag.
@@ -79,6 +79,9 @@ __global__ __launch_bounds__(kBlock) void k_walk(
     }
     if (wait == node) {
       wait = -1;
+    }
+    if (COUNT) {
+      ++nVisit;
     }
     const bool active = wait == -1;
     const float4 f = nodeF[node];
@@ -108,6 +111,9 @@ __global__ __launch_bounds__(kBlock) void k_walk(
     const bool needLeaf = active && !far && leaf;
     if (__ballot(needLeaf)) {
       const int s = c.z, e = c.w;
+      if (COUNT) {
+        nLeafTerm += static_cast<uint32_t>(e - s);
+      }
       if (SUM == kSumReference) {
         float a0x = 0, a0y = 0, a1x = 0, a1y = 0, a2x = 0, a2y = 0, a3x = 0,
               a3y = 0;
@@ -174,6 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(
     }
   }
   if (COUNT) {
+    // Wave totals; whole waves may have left early, so no block barrier.
     unsigned long long f64 = nFar, p64 = nPair;
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -183,6 +190,8 @@ __global__ __launch_bounds__(kBlock) void k_walk(
     if (lane == 0) {
       atomicAdd(&ctl->farCount, f64);
       atomicAdd(&ctl->bodyPairCount, p64);
+      atomicAdd(&ctl->waveVisits, static_cast<unsigned long long>(nVisit));
+      atomicAdd(&ctl->waveLeafTerms, static_cast<unsigned long long>(nLeafTerm));
     }
   }
 }

@@ -52,6 +52,7 @@ struct Ctl {
   uint32_t status;
   uint32_t pad[3];
   unsigned long long farCount, bodyPairCount;
+  unsigned long long waveVisits, waveLeafTerms; // counting walk only
 };
 
 // Order-preserving float <-> uint32 map (so exact float min/max can use

@@ -63,8 +63,6 @@ uint64_t pow2AtLeast(uint64_t v) {
   return p;
 }
 
-constexpr int kUpwardDepths = kMaxDepth; // branches exist at depths 0..31
-
 } // namespace
 
 struct nbx_sim {
@@ -99,7 +97,7 @@ struct nbx_sim {
 
   // tree
   uint32_t *keyHi[2]{}, *vals[2]{};
-  uint32_t *keyLo = nullptr, *sortHist = nullptr;
+  uint32_t *keyLo = nullptr, *sortHist = nullptr, *sortRowTotal = nullptr;
   int2 *deepList = nullptr;
   uint32_t *tileCount = nullptr;
   int64_t nodeCap = 0;
@@ -108,20 +106,26 @@ struct nbx_sim {
   int4 *nodeI = nullptr;
   float *centerX = nullptr, *centerY = nullptr, *size = nullptr;
   float2 *wsum = nullptr;
+  int32_t *bigNodes = nullptr;
+  uint4 *bboxParts = nullptr;
   bool treeBuilt = false;
   bool collided = false;
 
   Ctl *ctl = nullptr;
 
-  // profiling
+  // profiling: NBX_PHASES + 1 events per step, recorded on the stream and
+  // read back once per API call (no per-step host sync)
   bool profiling = false;
-  hipEvent_t ev[NBX_PHASES + 1]{};
+  std::vector<hipEvent_t> evPool;
+  size_t evUsed = 0;
   double phaseMs[NBX_PHASES]{};
   uint64_t profSteps = 0;
-  bool pendingEvents = false;
 
-  // captured steps, one per starting buffer
+  // captured work, by the buffer holding the state when the step starts:
+  // the whole step, and each phase on its own (used when profiling, so
+  // events can sit between the phases)
   hipGraphExec_t graph[2]{};
+  hipGraphExec_t phaseGraph[NBX_PHASES][2]{};
 };
 
 namespace {
@@ -144,6 +148,14 @@ void dropGraphs(nbx_sim *s) {
     if (g) {
       (void)hipGraphExecDestroy(g);
       g = nullptr;
+    }
+  }
+  for (auto &row : s->phaseGraph) {
+    for (auto &g : row) {
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
     }
   }
 }
@@ -185,6 +197,8 @@ __global__ __launch_bounds__(kBlock) void k_iterate(
 __global__ void k_zero_counts(Ctl *__restrict__ ctl) {
   ctl->farCount = 0;
   ctl->bodyPairCount = 0;
+  ctl->waveVisits = 0;
+  ctl->waveLeafTerms = 0;
 }
 
 int launchIterate(nbx_sim *s) {
@@ -220,8 +234,7 @@ int launchCollide(nbx_sim *s) {
   const uint32_t mask = buckets - 1;
   const uint32_t pairGrid = std::min<uint32_t>(blocksFor(s->pairCap), 2048);
 
-  LAUNCH(k_collide_reset, 1, 1, n, s->ctl);
-  LAUNCH(k_widths, nb, kBlock, n, px, py, r, s->widths);
+  LAUNCH(k_widths, nb, kBlock, n, px, py, r, s->widths, s->ctl);
   const uint32_t selGrid = std::min<uint32_t>(nb, 512);
   for (int pass = 0; pass < 3; ++pass) {
     LAUNCH(k_select_hist, selGrid, kBlock, n, s->widths, pass, s->ctl,
@@ -233,15 +246,16 @@ int launchCollide(nbx_sim *s) {
   NBX_TRY(launchScan(s, buckets + 1, s->bucketCount, s->bucketStart));
   LAUNCH(k_grid_scatter, nb, kBlock, n, px, py, r, s->ctl, s->bucketOf,
          s->bucketCount, s->bucketStart, s->gridBox, s->gridCell);
-  LAUNCH(k_find_small, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketStart,
-         s->gridBox, s->gridCell, s->largeList, s->pairs, s->pairCap);
+  LAUNCH(k_find_small, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
+         s->bucketStart, s->gridBox, s->gridCell, s->largeList, s->pairs,
+         s->pairCap);
   LAUNCH(k_find_large, nb, kBlock, n, px, py, r, s->ctl, s->largeList,
          s->pairs, s->pairCap);
   LAUNCH(k_union, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs, s->ufParent);
   LAUNCH(k_comp_count, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
          s->ufParent, s->pairComp, s->compCount);
-  LAUNCH(k_comp_alloc, nb, kBlock, n, s->ctl, s->compCount, s->compStart,
-         s->compInfo);
+  LAUNCH(k_comp_alloc, blocksFor(n, kBlock * kAllocItems), kBlock, n, s->ctl,
+         s->compCount, s->compStart, s->compInfo);
   LAUNCH(k_comp_scatter, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
          s->pairComp, s->compCount, s->compStart, s->segPairs);
   LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->segPairs,
@@ -260,20 +274,20 @@ int launchBuild(nbx_sim *s) {
   }
   const int c = s->cur, o = c ^ 1;
   const uint32_t nb = blocksFor(n);
-  LAUNCH(k_build_reset, 1, 1, s->ctl);
-  LAUNCH(k_bbox, std::min<uint32_t>(nb, 1024), kBlock, n, s->px[c], s->py[c],
-         s->ctl);
-  LAUNCH(k_keys, nb, kBlock, n, s->px[c], s->py[c], s->ctl, s->keyHi[0],
-         s->keyLo, s->vals[0]);
+  const uint32_t parts = std::min<uint32_t>(nb, kBboxParts);
+  LAUNCH(k_bbox, parts, kBlock, n, s->px[c], s->py[c], s->bboxParts);
+  LAUNCH(k_keys, nb, kBlock, n, s->px[c], s->py[c], s->bboxParts, parts, s->ctl,
+         s->keyHi[0], s->keyLo, s->vals[0]);
   // Stable LSD radix sort of the high key word, 4 x 8 bits; ends in buffer 0.
   const uint32_t tiles = blocksFor(n, kSortTile);
   for (int pass = 0; pass < 4; ++pass) {
     const int src = pass & 1, dst = src ^ 1;
     LAUNCH(k_rs_hist, tiles, kBlock, n, s->keyHi[src], 8 * pass, s->sortHist,
            tiles);
-    LAUNCH(k_rs_scan, 1, 1024, 256u * tiles, s->sortHist);
+    LAUNCH(k_rs_scan, 256, kBlock, tiles, s->sortHist, s->sortRowTotal);
     LAUNCH(k_rs_scatter, tiles, kBlock, n, s->keyHi[src], s->vals[src],
-           8 * pass, s->sortHist, tiles, s->keyHi[dst], s->vals[dst]);
+           8 * pass, s->sortHist, s->sortRowTotal, tiles, s->keyHi[dst],
+           s->vals[dst]);
   }
   const uint32_t *hi = s->keyHi[0];
   uint32_t *perm = s->vals[0];
@@ -297,11 +311,8 @@ int launchBuild(nbx_sim *s) {
          s->nodeOfSplit);
   LAUNCH(k_node_finish, nodeGrid, kBlock, s->ctl, s->px[o], s->py[o], s->m[o],
          s->nodeOfSplit, s->nodeF, s->nodeI, s->wsum);
-  const uint32_t upGrid = std::max<uint32_t>(16, std::min<uint32_t>(nb / 2, 1024));
-  for (int d = kUpwardDepths - 1; d >= 0; --d) {
-    LAUNCH(k_upward_level, upGrid, kBlock, d, s->ctl, s->nodeF, s->nodeI,
-           s->wsum);
-  }
+  LAUNCH(k_upward_top, 1, 1024, s->ctl, s->nodeF, s->nodeI, s->wsum,
+         s->bigNodes);
   s->treeBuilt = true;
   return NBX_OK;
 }
@@ -344,56 +355,73 @@ int launchForces(nbx_sim *s, bool count) {
   return NBX_OK;
 }
 
-int record(nbx_sim *s, int k) {
-  if (s->profiling) {
-    NBX_HIP(hipEventRecord(s->ev[k], s->stream));
-  }
-  return NBX_OK;
-}
-
-int collectEvents(nbx_sim *s) {
-  if (!s->pendingEvents) {
+int record(nbx_sim *s) {
+  if (!s->profiling) {
     return NBX_OK;
   }
-  NBX_HIP(hipEventSynchronize(s->ev[NBX_PHASES]));
-  for (int p = 0; p < NBX_PHASES; ++p) {
-    float ms = 0.0f;
-    NBX_HIP(hipEventElapsedTime(&ms, s->ev[p], s->ev[p + 1]));
-    s->phaseMs[p] += ms;
+  if (s->evUsed == s->evPool.size()) {
+    hipEvent_t e = nullptr;
+    NBX_HIP(hipEventCreate(&e));
+    s->evPool.push_back(e);
   }
-  s->profSteps += 1;
-  s->pendingEvents = false;
+  NBX_HIP(hipEventRecord(s->evPool[s->evUsed++], s->stream));
   return NBX_OK;
 }
 
-// One simulation step (Simulation.cpp:11-16), eager.
+// Folds the recorded per-step phase events into the phase totals.
+int collectEvents(nbx_sim *s) {
+  const size_t per = NBX_PHASES + 1;
+  if (s->evUsed < per) {
+    s->evUsed = 0;
+    return NBX_OK;
+  }
+  NBX_HIP(hipEventSynchronize(s->evPool[s->evUsed - 1]));
+  const size_t steps = s->evUsed / per;
+  for (size_t k = 0; k < steps; ++k) {
+    for (int p = 0; p < NBX_PHASES; ++p) {
+      float ms = 0.0f;
+      NBX_HIP(hipEventElapsedTime(&ms, s->evPool[k * per + p],
+                                  s->evPool[k * per + p + 1]));
+      s->phaseMs[p] += ms;
+    }
+  }
+  s->profSteps += steps;
+  s->evUsed = 0;
+  return NBX_OK;
+}
+
+int launchPhase(nbx_sim *s, int phase) {
+  switch (phase) {
+  case NBX_PHASE_ITERATE:
+    return launchIterate(s);
+  case NBX_PHASE_COLLIDE:
+    return s->params.collisions ? launchCollide(s) : NBX_OK;
+  case NBX_PHASE_BUILD:
+    return launchBuild(s);
+  default:
+    return launchForces(s, false);
+  }
+}
+
+// One simulation step (Simulation.cpp:11-16), launched eagerly.
 int launchStep(nbx_sim *s) {
-  NBX_TRY(record(s, NBX_PHASE_ITERATE));
-  NBX_TRY(launchIterate(s));
-  NBX_TRY(record(s, NBX_PHASE_COLLIDE));
-  if (s->params.collisions) {
-    NBX_TRY(launchCollide(s));
+  for (int p = 0; p < NBX_PHASES; ++p) {
+    NBX_TRY(record(s));
+    NBX_TRY(launchPhase(s, p));
   }
-  NBX_TRY(record(s, NBX_PHASE_BUILD));
-  NBX_TRY(launchBuild(s));
-  NBX_TRY(record(s, NBX_PHASE_WALK));
-  NBX_TRY(launchForces(s, false));
-  NBX_TRY(record(s, NBX_PHASES));
-  if (s->profiling) {
-    s->pendingEvents = true;
-    NBX_TRY(collectEvents(s));
-  }
-  return NBX_OK;
+  return record(s);
 }
 
-int captureStep(nbx_sim *s, int start) {
+// Captures `phase` (or the whole step when phase < 0) of a step that starts
+// with the state in buffer `start`.
+int capture(nbx_sim *s, int phase, int start, hipGraphExec_t *out) {
   const int saved = s->cur;
   const bool savedProf = s->profiling;
-  s->cur = start;
+  s->cur = (phase == NBX_PHASE_WALK) ? (start ^ 1) : start;
   s->profiling = false;
   hipGraph_t g = nullptr;
   NBX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  int rc = launchStep(s);
+  const int rc = phase < 0 ? launchStep(s) : launchPhase(s, phase);
   hipError_t e = hipStreamEndCapture(s->stream, &g);
   s->cur = saved;
   s->profiling = savedProf;
@@ -407,13 +435,41 @@ int captureStep(nbx_sim *s, int start) {
     return setError(NBX_ERR_HIP,
                     std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   }
-  e = hipGraphInstantiate(&s->graph[start], g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    s->graph[start] = nullptr;
+    *out = nullptr;
     return setError(NBX_ERR_HIP,
                     std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
+  return NBX_OK;
+}
+
+// One step from captured graphs: a single graph, or (profiling) one graph
+// per phase with events recorded between them.
+int graphStep(nbx_sim *s) {
+  const int c = s->cur;
+  if (!s->profiling) {
+    if (!s->graph[c]) {
+      NBX_TRY(capture(s, -1, c, &s->graph[c]));
+    }
+    NBX_HIP(hipGraphLaunch(s->graph[c], s->stream));
+  } else {
+    for (int p = 0; p < NBX_PHASES; ++p) {
+      NBX_TRY(record(s));
+      if (p == NBX_PHASE_COLLIDE && !s->params.collisions) {
+        continue;
+      }
+      if (!s->phaseGraph[p][c]) {
+        NBX_TRY(capture(s, p, c, &s->phaseGraph[p][c]));
+      }
+      NBX_HIP(hipGraphLaunch(s->phaseGraph[p][c], s->stream));
+    }
+    NBX_TRY(record(s));
+  }
+  s->cur = c ^ 1;
+  s->treeBuilt = true;
+  s->collided = s->params.collisions != 0 && s->n >= 2;
   return NBX_OK;
 }
 
@@ -561,6 +617,7 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->bigList, n, false)
   A(s->keyLo, n, false)
   A(s->sortHist, 256ull * blocksFor(n, kSortTile), false)
+  A(s->sortRowTotal, 256, false)
   A(s->deepList, n / (kLeafCapacity + 1) + 1, false)
   A(s->tileCount, static_cast<uint64_t>(kMaxDepth) * blocksFor(n), false)
   s->nodeCap = static_cast<int64_t>(8 * n + 8);
@@ -569,13 +626,10 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->nodeF, s->nodeCap, false) A(s->nodeI, s->nodeCap, false)
   A(s->centerX, s->nodeCap, false) A(s->centerY, s->nodeCap, false)
   A(s->size, s->nodeCap, false) A(s->wsum, s->nodeCap, false)
+  A(s->bigNodes, s->nodeCap / 4 + 1, false)
+  A(s->bboxParts, kBboxParts, false)
   A(s->ctl, 1, true)
 #undef A
-  for (auto &e : s->ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
-      return fail(setError(NBX_ERR_HIP, "hipEventCreate failed"));
-    }
-  }
   if (hipDeviceSynchronize() != hipSuccess) {
     return fail(setError(NBX_ERR_HIP, "hipDeviceSynchronize failed"));
   }
@@ -591,10 +645,8 @@ int nbx_destroy(nbx_sim *s) {
     (void)hipStreamSynchronize(s->stream);
   }
   dropGraphs(s);
-  for (auto &e : s->ev) {
-    if (e) {
-      (void)hipEventDestroy(e);
-    }
+  for (auto &e : s->evPool) {
+    (void)hipEventDestroy(e);
   }
   for (void *p : s->allocs) {
     (void)hipFree(p);
@@ -745,17 +797,10 @@ int nbx_step(nbx_sim *s, uint32_t steps) {
   if (!s) {
     return setError(NBX_ERR_ARG, "null handle");
   }
-  const bool graphs = s->params.use_graph && !s->profiling && s->n > 0;
+  const bool graphs = s->params.use_graph && s->n > 0;
   for (uint32_t k = 0; k < steps; ++k) {
     if (graphs) {
-      const int c = s->cur;
-      if (!s->graph[c]) {
-        NBX_TRY(captureStep(s, c));
-      }
-      NBX_HIP(hipGraphLaunch(s->graph[c], s->stream));
-      s->cur = c ^ 1;
-      s->treeBuilt = true;
-      s->collided = s->params.collisions != 0;
+      NBX_TRY(graphStep(s));
     } else {
       NBX_TRY(launchStep(s));
     }
@@ -942,7 +987,8 @@ int nbx_download_pairs(nbx_sim *s, int64_t cap, int32_t *later,
 }
 
 int nbx_count_interactions(nbx_sim *s, uint64_t *far_nodes,
-                           uint64_t *body_pairs) {
+                           uint64_t *body_pairs, uint64_t *wave_visits,
+                           uint64_t *wave_leaf_terms) {
   if (!s) {
     return setError(NBX_ERR_ARG, "null handle");
   }
@@ -955,6 +1001,8 @@ int nbx_count_interactions(nbx_sim *s, uint64_t *far_nodes,
   NBX_TRY(readCtl(s, &c));
   if (far_nodes) *far_nodes = c.farCount;
   if (body_pairs) *body_pairs = c.bodyPairCount;
+  if (wave_visits) *wave_visits = c.waveVisits;
+  if (wave_leaf_terms) *wave_leaf_terms = c.waveLeafTerms;
   return NBX_OK;
 }
 

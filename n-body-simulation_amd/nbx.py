@@ -96,8 +96,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_collide_info.argtypes = [_vp, C.POINTER(C.c_float), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.nbx_download_pairs.argtypes = [_vp, C.c_int64, _vp, _vp, _vp, C.POINTER(C.c_int64)]
-    L.nbx_count_interactions.argtypes = [_vp, C.POINTER(C.c_uint64),
-                                         C.POINTER(C.c_uint64)]
+    L.nbx_count_interactions.argtypes = [_vp] + [C.POINTER(C.c_uint64)] * 4
     L.nbx_set_profiling.argtypes = [_vp, C.c_int]
     L.nbx_phase_times.argtypes = [_vp, _f64p, C.POINTER(C.c_uint64)]
     L.nbx_stream.argtypes = [_vp, C.POINTER(_vp)]
@@ -300,11 +299,14 @@ class Simulation:
         k = int(cnt.value)
         return later[:k], earlier[:k], comp[:k]
 
-    def count_interactions(self):
+    def count_interactions(self, detail: bool = False):
+        """(far-node, body-pair) interaction totals of a walk over the current
+        tree; with detail=True also (wave node visits, wave leaf terms)."""
         self._push_params()
-        far, pairs = C.c_uint64(), C.c_uint64()
-        self._check(self.L.nbx_count_interactions(self.h, C.byref(far), C.byref(pairs)))
-        return int(far.value), int(pairs.value)
+        v = [C.c_uint64() for _ in range(4)]
+        self._check(self.L.nbx_count_interactions(self.h, *(C.byref(x) for x in v)))
+        out = tuple(int(x.value) for x in v)
+        return out if detail else out[:2]
 
     def set_profiling(self, on: bool) -> None:
         self._check(self.L.nbx_set_profiling(self.h, int(bool(on))))

@@ -155,6 +155,9 @@ def main() -> int:
         "flop_per_launch": interactions * FLOP_PER_INTERACTION,
         "walk_ms": round(phase_ms["walk"], 4),
         "walk_lane_utilisation": round(lane_util, 4),
+        "walk_far_util": round(far1 / (64.0 * visits1), 4),
+        "walk_leaf_util": round(pair1 / (64.0 * max(terms1, 1)), 4),
+        "walk_wave_slots": {"visits": visits1, "leaf_terms": terms1},
     }
     cpu = None if args.no_cpu else cpu_baseline(args.n, args.theta, args.cpu_steps)
     line = {

@@ -149,6 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(
     ctl->quad[1] = qcy;
     ctl->quad[2] = qs;
     ctl->deepCount = 0;
+    ctl->subtreeRoots = 0;
     ctl->status &= ~(kStatusNodeOverflow | kStatusDeepTooLarge |
                      kStatusTreeInconsistent);
   }
@@ -268,7 +269,8 @@ __global__ __launch_bounds__(kBlock) void k_deep_sort(
 }
 
 // Applies the sort permutation to every body array (CudaQuadTree.cu:
-// 408-414 does this with seven launches; here it is one pass).
+// 408-414 does this with seven launches; here it is one pass) and writes the
+// packed {x, y, m} records the force walk reads.
 __global__ __launch_bounds__(kBlock) void k_permute(
     uint32_t n, const uint32_t *__restrict__ perm,
     const float *__restrict__ px, const float *__restrict__ py,
@@ -277,19 +279,21 @@ __global__ __launch_bounds__(kBlock) void k_permute(
     const uint32_t *__restrict__ id, float *__restrict__ opx,
     float *__restrict__ opy, float *__restrict__ ovx, float *__restrict__ ovy,
     float *__restrict__ om, float *__restrict__ orr,
-    uint32_t *__restrict__ oid) {
+    uint32_t *__restrict__ oid, float4 *__restrict__ pack) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) {
     return;
   }
   const uint32_t s = perm[j];
-  opx[j] = px[s];
-  opy[j] = py[s];
+  const float x = px[s], y = py[s], mass = m[s];
+  opx[j] = x;
+  opy[j] = y;
   ovx[j] = vx[s];
   ovy[j] = vy[s];
-  om[j] = m[s];
+  om[j] = mass;
   orr[j] = r[s];
   oid[j] = id[s];
+  pack[j] = make_float4(x, y, mass, 0.0f);
 }
 
 // Depth range [A, B] over which sorted body j starts a splitting run.
@@ -547,7 +551,18 @@ __device__ inline int nodeDepth(const Ctl *__restrict__ ctl, int x) {
   return d;
 }
 
-constexpr int kUpBodies = 256; // subtrees up to this many bodies: one lane
+// Upward pass (QuadTree::propagate, QuadTree.cpp:237-262) in three
+// launches instead of one per level:
+//   k_node_finish  every node: `next` pointer; leaves: their sums; lists the
+//                  subtree roots (branches of <= kSubtreeBodies bodies whose
+//                  parent holds more);
+//   k_subtree      one workgroup per subtree root: its branches, deepest
+//                  level first, synchronised in LDS;
+//   k_upward_top   one workgroup: the remaining branches near the root.
+// Every branch value is ((c0 + c1) + c2) + c3 of its children's values, the
+// reference's association, whichever kernel computes it.
+constexpr int kSubtreeBodies = 4096;
+constexpr int kSubtreeList = 2048; // branch nodes per subtree held in LDS
 
 struct Sum3 {
   float x, y, m;
@@ -583,11 +598,6 @@ __device__ inline Sum3 leafSum(const float *__restrict__ px,
   return Sum3{p0x + p1x + p2x + p3x, p0y + p1y + p2y + p3y, m0 + m1 + m2 + m3};
 }
 
-// Branch value from its children, left to right (QuadTree.cpp:243-253).
-__device__ inline Sum3 addSum(const Sum3 &a, const Sum3 &b) {
-  return Sum3{a.x + b.x, a.y + b.y, a.m + b.m};
-}
-
 __device__ inline void writeSum(int x, const Sum3 &v, float4 *__restrict__ nodeF,
                                 float2 *__restrict__ wsum) {
   // propagate's final pass (QuadTree.cpp:256-261): pos /= max(mass, FLT_MIN).
@@ -598,29 +608,44 @@ __device__ inline void writeSum(int x, const Sum3 &v, float4 *__restrict__ nodeF
   nodeF[x].z = v.m;
 }
 
-// Per node: the `next` skip pointer (QuadTree.cpp:87-90: child q < 3 points
-// at q + 1, the last child inherits its parent's), and the mass sums of
-// every leaf and of every branch holding at most kUpBodies bodies. A branch
-// is evaluated bottom-up over its own subtree with an explicit stack, in the
-// reference's association, so no level-by-level launches are needed below
-// the top of the tree.
+// Branch x from its four children (QuadTree.cpp:243-253).
+__device__ inline void sumBranch(int x, int c, float4 *__restrict__ nodeF,
+                                 float2 *__restrict__ wsum) {
+  const float2 w0 = wsum[c], w1 = wsum[c + 1], w2 = wsum[c + 2],
+               w3 = wsum[c + 3];
+  const Sum3 v{w0.x + w1.x + w2.x + w3.x, w0.y + w1.y + w2.y + w3.y,
+               nodeF[c].z + nodeF[c + 1].z + nodeF[c + 2].z + nodeF[c + 3].z};
+  writeSum(x, v, nodeF, wsum);
+}
+
+// Parent of node x >= 1: the splitting node of depth D-1 whose children
+// block holds x.
+__device__ inline int parentOf(const Ctl *__restrict__ ctl,
+                               const int32_t *__restrict__ nodeOfSplit,
+                               int x) {
+  const int D = nodeDepth(ctl, x);
+  return nodeOfSplit[ctl->splitBase[D - 1] + (x - ctl->levelStart[D]) / 4];
+}
+
 __global__ __launch_bounds__(kBlock) void k_node_finish(
-    const Ctl *__restrict__ ctl, const float *__restrict__ px,
+    Ctl *__restrict__ ctl, const float *__restrict__ px,
     const float *__restrict__ py, const float *__restrict__ m,
     const int32_t *__restrict__ nodeOfSplit, float4 *__restrict__ nodeF,
-    int4 *__restrict__ nodeI, float2 *__restrict__ wsum) {
+    int4 *__restrict__ nodeI, float2 *__restrict__ wsum,
+    uint8_t *__restrict__ done, int32_t *__restrict__ roots) {
   if (ctl->status & kStatusNodeOverflow) {
     return;
   }
   const int count = ctl->nodeCount;
   for (int x = blockIdx.x * kBlock + threadIdx.x; x < count;
        x += gridDim.x * kBlock) {
+    // `next` (QuadTree.cpp:87-90): child q < 3 points at q + 1; the last
+    // child inherits its parent's next.
     int y = x;
     for (int guard = 0; y != 0 && ((y - 1) & 3) == 3; ++guard) {
-      const int D = nodeDepth(ctl, y);
-      y = nodeOfSplit[ctl->splitBase[D - 1] + (y - ctl->levelStart[D]) / 4];
+      y = parentOf(ctl, nodeOfSplit, y);
       if (guard > kMaxDepth || y < 0 || y >= count) {
-        atomicOr(&const_cast<Ctl *>(ctl)->status, kStatusTreeInconsistent);
+        atomicOr(&ctl->status, kStatusTreeInconsistent);
         y = 0;
         break;
       }
@@ -629,57 +654,99 @@ __global__ __launch_bounds__(kBlock) void k_node_finish(
     nodeI[x].y = y == 0 ? 0 : y + 1;
     if (ni.x == 0) {
       writeSum(x, leafSum(px, py, m, ni.z, ni.w), nodeF, wsum);
+      done[x] = 1;
       continue;
     }
-    if (ni.w - ni.z > kUpBodies) {
-      continue; // k_upward_top
-    }
-    int stNode[kMaxDepth + 1];
-    int stQ[kMaxDepth + 1];
-    Sum3 stAcc[kMaxDepth + 1];
-    int sp = 0;
-    stNode[0] = x;
-    stQ[0] = 0;
-    Sum3 result{0.0f, 0.0f, 0.0f};
-    while (true) {
-      const int q = stQ[sp];
-      if (q == 4) {
-        const Sum3 v = stAcc[sp];
-        if (sp == 0) {
-          result = v;
-          break;
-        }
-        --sp;
-        stAcc[sp] = stQ[sp] == 0 ? v : addSum(stAcc[sp], v);
-        stQ[sp] += 1;
-        continue;
+    done[x] = 0;
+    if (ni.w - ni.z <= kSubtreeBodies) {
+      bool root = x == 0;
+      if (!root) {
+        const int4 pi = nodeI[parentOf(ctl, nodeOfSplit, x)];
+        root = pi.w - pi.z > kSubtreeBodies;
       }
-      const int c = nodeI[stNode[sp]].x + q;
-      const int4 ci = nodeI[c];
-      if (ci.x == 0) {
-        const Sum3 v = leafSum(px, py, m, ci.z, ci.w);
-        stAcc[sp] = q == 0 ? v : addSum(stAcc[sp], v);
-        stQ[sp] = q + 1;
-      } else if (sp < kMaxDepth) {
-        ++sp;
-        stNode[sp] = c;
-        stQ[sp] = 0;
-      } else {
-        atomicOr(&const_cast<Ctl *>(ctl)->status, kStatusTreeInconsistent);
-        break;
+      if (root) {
+        roots[atomicAdd(&ctl->subtreeRoots, 1)] = x;
       }
     }
-    writeSum(x, result, nodeF, wsum);
   }
 }
 
-// One workgroup: the branches holding more than kUpBodies bodies. They form
-// a connected top of the tree (a big node's parent is big), found top-down
-// level by level, then summed bottom-up from their already-final children.
+// One workgroup per subtree root: its branch nodes found top-down into LDS,
+// level by level, then summed bottom-up (children first). A subtree with
+// more branches than the LDS list holds is left to k_upward_top.
+__global__ __launch_bounds__(kBlock) void k_subtree(
+    const Ctl *__restrict__ ctl, const int32_t *__restrict__ roots,
+    float4 *__restrict__ nodeF, const int4 *__restrict__ nodeI,
+    float2 *__restrict__ wsum, uint8_t *__restrict__ done) {
+  __shared__ int list[kSubtreeList];
+  __shared__ int levelOff[kMaxDepth + 3];
+  __shared__ int appended;
+  __shared__ int overflow;
+  if (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent)) {
+    return;
+  }
+  const int nroots = ctl->subtreeRoots;
+  for (int g = blockIdx.x; g < nroots; g += gridDim.x) {
+    if (threadIdx.x == 0) {
+      list[0] = roots[g];
+      levelOff[0] = 0;
+      levelOff[1] = 1;
+      overflow = 0;
+    }
+    __syncthreads();
+    int levels = 0;
+    for (int d = 0; d <= kMaxDepth; ++d) {
+      const int lo = levelOff[d], hi = levelOff[d + 1];
+      if (lo == hi) {
+        break;
+      }
+      levels = d + 1;
+      if (threadIdx.x == 0) {
+        appended = 0;
+      }
+      __syncthreads();
+      for (int k = lo + threadIdx.x; k < hi; k += kBlock) {
+        const int c = nodeI[list[k]].x;
+        for (int q = 0; q < 4; ++q) {
+          if (nodeI[c + q].x != 0) {
+            const int slot = hi + atomicAdd(&appended, 1);
+            if (slot < kSubtreeList) {
+              list[slot] = c + q;
+            } else {
+              overflow = 1;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        levelOff[d + 2] = min(hi + appended, kSubtreeList);
+      }
+      __syncthreads();
+    }
+    if (!overflow) {
+      for (int d = levels - 1; d >= 0; --d) {
+        for (int k = levelOff[d] + threadIdx.x; k < levelOff[d + 1];
+             k += kBlock) {
+          const int x = list[k];
+          sumBranch(x, nodeI[x].x, nodeF, wsum);
+          done[x] = 1;
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One workgroup: every branch not yet summed (the nodes above the subtree
+// roots, plus any subtree too large for k_subtree's list). That set is
+// closed upward, so it is found top-down from the root, then summed
+// bottom-up.
 __global__ __launch_bounds__(1024) void k_upward_top(
     const Ctl *__restrict__ ctl, float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, float2 *__restrict__ wsum,
-    int32_t *__restrict__ big) {
+    const uint8_t *__restrict__ done, int32_t *__restrict__ big) {
   __shared__ int levelOff[kMaxDepth + 3];
   __shared__ int appended;
   if (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent)) {
@@ -687,14 +754,12 @@ __global__ __launch_bounds__(1024) void k_upward_top(
   }
   const int tid = threadIdx.x;
   if (tid == 0) {
-    const int4 root = nodeI[0];
-    const bool rootBig = ctl->nodeCount > 0 && root.x != 0 &&
-                         root.w - root.z > kUpBodies;
-    if (rootBig) {
+    const bool rootOpen = ctl->nodeCount > 0 && nodeI[0].x != 0 && !done[0];
+    if (rootOpen) {
       big[0] = 0;
     }
     levelOff[0] = 0;
-    levelOff[1] = rootBig ? 1 : 0;
+    levelOff[1] = rootOpen ? 1 : 0;
   }
   __syncthreads();
   int levels = 0;
@@ -711,8 +776,7 @@ __global__ __launch_bounds__(1024) void k_upward_top(
     for (int k = lo + tid; k < hi; k += blockDim.x) {
       const int c = nodeI[big[k]].x;
       for (int q = 0; q < 4; ++q) {
-        const int4 ci = nodeI[c + q];
-        if (ci.x != 0 && ci.w - ci.z > kUpBodies) {
+        if (nodeI[c + q].x != 0 && !done[c + q]) {
           big[hi + atomicAdd(&appended, 1)] = c + q;
         }
       }
@@ -726,12 +790,7 @@ __global__ __launch_bounds__(1024) void k_upward_top(
   for (int d = levels - 1; d >= 0; --d) {
     for (int k = levelOff[d] + tid; k < levelOff[d + 1]; k += blockDim.x) {
       const int x = big[k];
-      const int c = nodeI[x].x;
-      Sum3 v{wsum[c].x, wsum[c].y, nodeF[c].z};
-      for (int q = 1; q < 4; ++q) {
-        v = addSum(v, Sum3{wsum[c + q].x, wsum[c + q].y, nodeF[c + q].z});
-      }
-      writeSum(x, v, nodeF, wsum);
+      sumBranch(x, nodeI[x].x, nodeF, wsum);
     }
     __syncthreads();
   }

@@ -196,6 +196,174 @@ __global__ __launch_bounds__(kBlock) void k_walk(
   }
 }
 
+// Default walk (SUM_COMPENSATED): same traversal and per-lane decisions as
+// k_walk, restructured for throughput --
+//  * as soon as the next node is chosen, the records of both nodes that can
+//    follow it (first child, `next`) and, for a leaf, its first four packed
+//    {x, y, m} bodies are requested, so a full iteration of work covers the
+//    pointer chase;
+//  * leaf bodies are processed four at a time from unconditional loads
+//    (the packed array is padded by kBodyPad records), with out-of-range
+//    slots masked arithmetically instead of branched around;
+//  * terms are accumulated with FMA into a per-lane partial sum that is
+//    added to a Kahan-compensated total every 8 node visits and at the end.
+constexpr int kFlushEvery = 8;
+constexpr int kBodyPad = 4;
+
+template <int LAW>
+__device__ inline void leafTerm(const float4 &b, float x, float y, float eSq,
+                                float w, float &qx, float &qy) {
+  const float bx = b.x - x;
+  const float by = b.y - y;
+  const float bSq = bx * bx + by * by;
+  const float fac = lawFactor<kSumCompensated, LAW>(b.z, bSq, eSq) * w;
+  qx = fmaf(bx, fac, qx);
+  qy = fmaf(by, fac, qy);
+}
+
+template <int LAW, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_walk_fast(
+    uint32_t n, const float4 *__restrict__ nodeF,
+    const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
+    float tSq, float eSq, float *__restrict__ ax, float *__restrict__ ay,
+    Ctl *__restrict__ ctl) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1));
+  if (base >= n ||
+      (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent))) {
+    return;
+  }
+  const uint32_t i = base + lane;
+  const bool valid = i < n;
+  const float4 me = body[valid ? i : base];
+  const float x = me.x, y = me.y;
+  int wait = valid ? -1 : static_cast<int>(kDone);
+  float sx = 0.0f, sy = 0.0f, cx = 0.0f, cy = 0.0f; // compensated total
+  float qx = 0.0f, qy = 0.0f;                       // partial since flush
+  uint32_t nFar = 0, nPair = 0, nLeafTerm = 0;
+  const int nodeCount = ctl->nodeCount;
+  int node = 0;
+  float4 f = nodeF[0];
+  int4 c = nodeI[0];
+  float4 fk = nodeF[c.x];
+  int4 ck = nodeI[c.x];
+  float4 fn = nodeF[c.y];
+  int4 cn = nodeI[c.y];
+  float4 b0 = body[c.z], b1 = body[c.z + 1], b2 = body[c.z + 2],
+         b3 = body[c.z + 3];
+  int visits = 0;
+  while (true) {
+    if (node < 0 || node >= nodeCount || ++visits > nodeCount) {
+      if (lane == 0) {
+        atomicOr(&ctl->status, kStatusTreeInconsistent);
+      }
+      break;
+    }
+    const int kid = c.x, nxt = c.y;
+    if (wait == node) {
+      wait = -1;
+    }
+    const bool active = wait == -1;
+    const int nextTarget = nxt == 0 ? static_cast<int>(kDone) : nxt;
+    const float dx = f.x - x;
+    const float dy = f.y - y;
+    const float dSq = dx * dx + dy * dy;
+    const bool far = f.w < dSq * tSq;
+    const bool doFar = active && far && f.z > 0.0f;
+    if (__ballot(doFar)) {
+      const float fac = lawFactor<kSumCompensated, LAW>(f.z, dSq, eSq);
+      if (doFar) {
+        qx = fmaf(dx, fac, qx);
+        qy = fmaf(dy, fac, qy);
+        if (COUNT) {
+          ++nFar;
+        }
+      }
+    }
+    if (active && far) {
+      wait = nextTarget;
+    }
+    const bool leaf = kid == 0;
+    const bool needLeaf = active && !far && leaf;
+    if (__ballot(needLeaf)) {
+      const int s = c.z, cnt = c.w - c.z;
+      if (COUNT) {
+        nLeafTerm += static_cast<uint32_t>(cnt);
+        if (needLeaf) {
+          nPair += static_cast<uint32_t>(cnt);
+        }
+      }
+      const float use = needLeaf ? 1.0f : 0.0f;
+      if (cnt > 0) {
+        leafTerm<LAW>(b0, x, y, eSq, use, qx, qy);
+        leafTerm<LAW>(b1, x, y, eSq, cnt > 1 ? use : 0.0f, qx, qy);
+        leafTerm<LAW>(b2, x, y, eSq, cnt > 2 ? use : 0.0f, qx, qy);
+        leafTerm<LAW>(b3, x, y, eSq, cnt > 3 ? use : 0.0f, qx, qy);
+      }
+      for (int k = 4; k < cnt; k += 4) {
+        const float4 e0 = body[s + k], e1 = body[s + k + 1],
+                     e2 = body[s + k + 2], e3 = body[s + k + 3];
+        leafTerm<LAW>(e0, x, y, eSq, use, qx, qy);
+        leafTerm<LAW>(e1, x, y, eSq, cnt > k + 1 ? use : 0.0f, qx, qy);
+        leafTerm<LAW>(e2, x, y, eSq, cnt > k + 2 ? use : 0.0f, qx, qy);
+        leafTerm<LAW>(e3, x, y, eSq, cnt > k + 3 ? use : 0.0f, qx, qy);
+      }
+      if (needLeaf) {
+        wait = nextTarget;
+      }
+    }
+    if ((visits & (kFlushEvery - 1)) == 0) {
+      kahanAdd(sx, cx, qx);
+      kahanAdd(sy, cy, qy);
+      qx = 0.0f;
+      qy = 0.0f;
+    }
+    const bool needOpen = active && !far && !leaf;
+    if (__ballot(needOpen)) {
+      node = kid;
+      f = fk;
+      c = ck;
+    } else {
+      if (nxt == 0) {
+        break;
+      }
+      node = nxt;
+      f = fn;
+      c = cn;
+    }
+    node = __builtin_amdgcn_readfirstlane(node);
+    // Requests for the iteration after next (see above).
+    fk = nodeF[c.x];
+    ck = nodeI[c.x];
+    fn = nodeF[c.y];
+    cn = nodeI[c.y];
+    b0 = body[c.z];
+    b1 = body[c.z + 1];
+    b2 = body[c.z + 2];
+    b3 = body[c.z + 3];
+  }
+  kahanAdd(sx, cx, qx);
+  kahanAdd(sy, cy, qy);
+  if (valid) {
+    ax[i] = sx - cx;
+    ay[i] = sy - cy;
+  }
+  if (COUNT) {
+    unsigned long long f64 = nFar, p64 = nPair;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      f64 += __shfl_xor(f64, off, kWave);
+      p64 += __shfl_xor(p64, off, kWave);
+    }
+    if (lane == 0) {
+      atomicAdd(&ctl->farCount, f64);
+      atomicAdd(&ctl->bodyPairCount, p64);
+      atomicAdd(&ctl->waveVisits, static_cast<unsigned long long>(visits));
+      atomicAdd(&ctl->waveLeafTerms, static_cast<unsigned long long>(nLeafTerm));
+    }
+  }
+}
+
 // ---- FP64 direct sum, CheckCuda.cu:456-475 (tree law, self excluded,
 //      coincident bodies skipped) ----
 constexpr int kExactChunks = 64;

@@ -49,8 +49,9 @@ struct Ctl {
   int32_t levelStart[kMaxDepth + 2]; // first node index at each depth
   int32_t splitBase[kMaxDepth + 1];  // offset of each depth in the split list
   int32_t totalSplits, nodeCount, levels, deepCount;
+  int32_t subtreeRoots;
   uint32_t status;
-  uint32_t pad[3];
+  uint32_t pad[2];
   unsigned long long farCount, bodyPairCount;
   unsigned long long waveVisits, waveLeafTerms; // counting walk only
 };

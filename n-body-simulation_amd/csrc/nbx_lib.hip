@@ -107,6 +107,9 @@ struct nbx_sim {
   float *centerX = nullptr, *centerY = nullptr, *size = nullptr;
   float2 *wsum = nullptr;
   int32_t *bigNodes = nullptr;
+  int32_t *subtreeRoots = nullptr;
+  uint8_t *nodeDone = nullptr;
+  float4 *bodyPack = nullptr; // {x, y, m, 0} in tree order, for the walk
   uint4 *bboxParts = nullptr;
   bool treeBuilt = false;
   bool collided = false;
@@ -295,7 +298,7 @@ int launchBuild(nbx_sim *s) {
   LAUNCH(k_deep_sort, 64, kBlock, s->ctl, s->deepList, s->keyLo, perm);
   LAUNCH(k_permute, nb, kBlock, n, perm, s->px[c], s->py[c], s->vx[c],
          s->vy[c], s->m[c], s->r[c], s->id[c], s->px[o], s->py[o], s->vx[o],
-         s->vy[o], s->m[o], s->r[o], s->id[o]);
+         s->vy[o], s->m[o], s->r[o], s->id[o], s->bodyPack);
   s->cur = o;
   const uint32_t tiles256 = nb;
   LAUNCH(k_lcp_count, tiles256, kBlock, n, hi, s->keyLo, perm, tiles256,
@@ -310,9 +313,13 @@ int launchBuild(nbx_sim *s) {
          s->splitList, s->nodeF, s->nodeI, s->centerX, s->centerY, s->size,
          s->nodeOfSplit);
   LAUNCH(k_node_finish, nodeGrid, kBlock, s->ctl, s->px[o], s->py[o], s->m[o],
-         s->nodeOfSplit, s->nodeF, s->nodeI, s->wsum);
+         s->nodeOfSplit, s->nodeF, s->nodeI, s->wsum, s->nodeDone,
+         s->subtreeRoots);
+  LAUNCH(k_subtree, std::min<uint32_t>(1024, std::max<uint32_t>(1, n / 1024)),
+         kBlock, s->ctl, s->subtreeRoots, s->nodeF, s->nodeI, s->wsum,
+         s->nodeDone);
   LAUNCH(k_upward_top, 1, 1024, s->ctl, s->nodeF, s->nodeI, s->wsum,
-         s->bigNodes);
+         s->nodeDone, s->bigNodes);
   s->treeBuilt = true;
   return NBX_OK;
 }
@@ -334,23 +341,27 @@ int launchForces(nbx_sim *s, bool count) {
 #define WALK(SUMV, LAWV, CNT)                                                 \
   LAUNCH((k_walk<SUMV, LAWV, CNT>), grid, kBlock, n, s->nodeF, s->nodeI,      \
          s->px[c], s->py[c], s->m[c], tSq, eSq, s->ax, s->ay, s->ctl)
-  if (count) {
-    if (ref) {
+#define FAST(LAWV, CNT)                                                       \
+  LAUNCH((k_walk_fast<LAWV, CNT>), grid, kBlock, n, s->nodeF, s->nodeI,       \
+         s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl)
+  if (ref) {
+    if (count) {
       if (acc) WALK(kSumReference, kLawAccurate, true);
       else WALK(kSumReference, kLawReference, true);
     } else {
-      if (acc) WALK(kSumCompensated, kLawAccurate, true);
-      else WALK(kSumCompensated, kLawReference, true);
-    }
-  } else {
-    if (ref) {
       if (acc) WALK(kSumReference, kLawAccurate, false);
       else WALK(kSumReference, kLawReference, false);
+    }
+  } else {
+    if (count) {
+      if (acc) FAST(kLawAccurate, true);
+      else FAST(kLawReference, true);
     } else {
-      if (acc) WALK(kSumCompensated, kLawAccurate, false);
-      else WALK(kSumCompensated, kLawReference, false);
+      if (acc) FAST(kLawAccurate, false);
+      else FAST(kLawReference, false);
     }
   }
+#undef FAST
 #undef WALK
   return NBX_OK;
 }
@@ -627,6 +638,9 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->centerX, s->nodeCap, false) A(s->centerY, s->nodeCap, false)
   A(s->size, s->nodeCap, false) A(s->wsum, s->nodeCap, false)
   A(s->bigNodes, s->nodeCap / 4 + 1, false)
+  A(s->subtreeRoots, s->nodeCap, false)
+  A(s->nodeDone, s->nodeCap, false)
+  A(s->bodyPack, n + kBodyPad, true)
   A(s->bboxParts, kBboxParts, false)
   A(s->ctl, 1, true)
 #undef A

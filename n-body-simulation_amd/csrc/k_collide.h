@@ -307,21 +307,27 @@ __global__ __launch_bounds__(kBlock) void k_find_small(
     const int self = static_cast<int>(i);
     const int32_t mx = cellCoord(me.minX, invCell);
     const int32_t my = cellCoord(me.minY, invCell);
-    for (int dy = -1; dy <= 1; ++dy) {
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int32_t cx = mx + dx, cy = my + dy;
-        const uint32_t h = cellBucket(cx, cy, mask);
-        const uint32_t end = bucketStart[h + 1];
-        for (uint32_t o = bucketStart[h]; o < end; ++o) {
-          const int4 oc = gridCell[o];
-          if (oc.x != cx || oc.y != cy || oc.z == self) {
-            continue;
-          }
-          const float4 ob = gridBox[o];
-          if (beforeBox(ob.x, oc.z, me.minX, self) &&
-              overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
-            stagePair(st, ctl, pairs, cap, self, oc.z, me.minX, ob.x);
-          }
+    // All nine bucket ranges are requested up front (independent loads),
+    // then only the non-empty ones are scanned.
+    uint32_t lo[9], hi[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const uint32_t h = cellBucket(mx + (k % 3) - 1, my + (k / 3) - 1, mask);
+      lo[k] = bucketStart[h];
+      hi[k] = bucketStart[h + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int32_t cx = mx + (k % 3) - 1, cy = my + (k / 3) - 1;
+      for (uint32_t o = lo[k]; o < hi[k]; ++o) {
+        const int4 oc = gridCell[o];
+        if (oc.x != cx || oc.y != cy || oc.z == self) {
+          continue;
+        }
+        const float4 ob = gridBox[o];
+        if (beforeBox(ob.x, oc.z, me.minX, self) &&
+            overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
+          stagePair(st, ctl, pairs, cap, self, oc.z, me.minX, ob.x);
         }
       }
     }

@@ -196,19 +196,23 @@ __global__ __launch_bounds__(kBlock) void k_walk(
   }
 }
 
-// Default walk (SUM_COMPENSATED): same traversal and per-lane decisions as
-// k_walk, restructured for throughput --
-//  * as soon as the next node is chosen, the records of both nodes that can
-//    follow it (first child, `next`) and, for a leaf, its first four packed
-//    {x, y, m} bodies are requested, so a full iteration of work covers the
-//    pointer chase;
-//  * leaf bodies are processed four at a time from unconditional loads
-//    (the packed array is padded by kBodyPad records), with out-of-range
-//    slots masked arithmetically instead of branched around;
+// Default walk (SUM_COMPENSATED). Only the *set* of interactions of each
+// body must match the reference (its summation order is free in this mode),
+// so the wave walks with an explicit stack instead of `next` pointers: an
+// entry is a block of four siblings plus the 64-bit mask of lanes that
+// opened their parent. Popping an entry evaluates the four siblings for
+// exactly those lanes -- far field, leaf bodies, or open -- and pushes each
+// opened sibling's children with the mask of lanes that opened it. Every
+// lane therefore makes the reference's decisions at the reference's nodes
+// (QuadTree.cpp:264-389), visiting each at most once, with the per-node
+// bookkeeping of the stackless walk gone.
+//  * sibling records are contiguous: one block = two 64-byte scalar loads;
+//  * leaf bodies are packed {x, y, m} records read four at a time (the
+//    array is padded by kBodyPad records; out-of-range slots are masked);
 //  * terms are accumulated with FMA into a per-lane partial sum that is
-//    added to a Kahan-compensated total every 8 node visits and at the end.
-constexpr int kFlushEvery = 8;
+//    added to a Kahan-compensated total after every other block.
 constexpr int kBodyPad = 4;
+constexpr int kStackDepth = 128; // >= 3 * kMaxDepth + 4 entries per wave
 
 template <int LAW>
 __device__ inline void leafTerm(const float4 &b, float x, float y, float eSq,
@@ -222,12 +226,60 @@ __device__ inline void leafTerm(const float4 &b, float x, float y, float eSq,
 }
 
 template <int LAW, bool COUNT>
+__device__ inline void visitNode(const float4 &f, const int4 &c, bool in,
+                                 float x, float y, float tSq, float eSq,
+                                 const float4 *__restrict__ body, float &qx,
+                                 float &qy, uint64_t &openMask,
+                                 uint32_t &nFar, uint32_t &nPair,
+                                 uint32_t &nLeafTerm) {
+  const float dx = f.x - x;
+  const float dy = f.y - y;
+  const float dSq = dx * dx + dy * dy;
+  const bool far = f.w < dSq * tSq;
+  const bool doFar = in && far && f.z > 0.0f;
+  if (__ballot(doFar)) {
+    const float fac = lawFactor<kSumCompensated, LAW>(f.z, dSq, eSq);
+    if (doFar) {
+      qx = fmaf(dx, fac, qx);
+      qy = fmaf(dy, fac, qy);
+      if (COUNT) {
+        ++nFar;
+      }
+    }
+  }
+  const bool leaf = c.x == 0;
+  const bool needLeaf = in && !far && leaf;
+  if (__ballot(needLeaf)) {
+    const int s = c.z, cnt = c.w - c.z;
+    if (COUNT) {
+      nLeafTerm += static_cast<uint32_t>(cnt);
+      if (needLeaf) {
+        nPair += static_cast<uint32_t>(cnt);
+      }
+    }
+    const float use = needLeaf ? 1.0f : 0.0f;
+    for (int k = 0; k < cnt; k += 4) {
+      const float4 e0 = body[s + k], e1 = body[s + k + 1],
+                   e2 = body[s + k + 2], e3 = body[s + k + 3];
+      leafTerm<LAW>(e0, x, y, eSq, use, qx, qy);
+      leafTerm<LAW>(e1, x, y, eSq, cnt > k + 1 ? use : 0.0f, qx, qy);
+      leafTerm<LAW>(e2, x, y, eSq, cnt > k + 2 ? use : 0.0f, qx, qy);
+      leafTerm<LAW>(e3, x, y, eSq, cnt > k + 3 ? use : 0.0f, qx, qy);
+    }
+  }
+  openMask = __ballot(in && !far && !leaf);
+}
+
+template <int LAW, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_walk_fast(
     uint32_t n, const float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
     float tSq, float eSq, float *__restrict__ ax, float *__restrict__ ay,
     Ctl *__restrict__ ctl) {
+  __shared__ int stNode[kBlock / kWave][kStackDepth];
+  __shared__ uint64_t stMask[kBlock / kWave][kStackDepth];
   const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t w = threadIdx.x >> 6;
   const uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1));
   if (base >= n ||
       (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent))) {
@@ -237,110 +289,72 @@ __global__ __launch_bounds__(kBlock) void k_walk_fast(
   const bool valid = i < n;
   const float4 me = body[valid ? i : base];
   const float x = me.x, y = me.y;
-  int wait = valid ? -1 : static_cast<int>(kDone);
+  const uint64_t bit = 1ull << lane;
   float sx = 0.0f, sy = 0.0f, cx = 0.0f, cy = 0.0f; // compensated total
   float qx = 0.0f, qy = 0.0f;                       // partial since flush
-  uint32_t nFar = 0, nPair = 0, nLeafTerm = 0;
+  uint32_t nFar = 0, nPair = 0, nLeafTerm = 0, nVisit = 0;
   const int nodeCount = ctl->nodeCount;
-  int node = 0;
-  float4 f = nodeF[0];
-  int4 c = nodeI[0];
-  float4 fk = nodeF[c.x];
-  int4 ck = nodeI[c.x];
-  float4 fn = nodeF[c.y];
-  int4 cn = nodeI[c.y];
-  float4 b0 = body[c.z], b1 = body[c.z + 1], b2 = body[c.z + 2],
-         b3 = body[c.z + 3];
-  int visits = 0;
-  while (true) {
-    if (node < 0 || node >= nodeCount || ++visits > nodeCount) {
+
+  // The root on its own, then blocks of four siblings from the stack.
+  int sp = 0;
+  {
+    uint64_t open = 0;
+    visitNode<LAW, COUNT>(nodeF[0], nodeI[0], valid, x, y, tSq, eSq, body, qx,
+                          qy, open, nFar, nPair, nLeafTerm);
+    nVisit += 1;
+    if (open) {
+      if (lane == 0) {
+        stNode[w][0] = nodeI[0].x;
+        stMask[w][0] = open;
+      }
+      sp = 1;
+    }
+  }
+  int blocks = 0;
+  while (sp > 0) {
+    --sp;
+    const int cb = __builtin_amdgcn_readfirstlane(stNode[w][sp]);
+    const uint64_t mask = stMask[w][sp];
+    if (cb <= 0 || cb + 3 >= nodeCount || ++blocks > nodeCount) {
       if (lane == 0) {
         atomicOr(&ctl->status, kStatusTreeInconsistent);
       }
       break;
     }
-    const int kid = c.x, nxt = c.y;
-    if (wait == node) {
-      wait = -1;
+    const bool in = (mask & bit) != 0;
+    uint64_t open[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      visitNode<LAW, COUNT>(nodeF[cb + q], nodeI[cb + q], in, x, y, tSq, eSq,
+                            body, qx, qy, open[q], nFar, nPair, nLeafTerm);
     }
-    const bool active = wait == -1;
-    const int nextTarget = nxt == 0 ? static_cast<int>(kDone) : nxt;
-    const float dx = f.x - x;
-    const float dy = f.y - y;
-    const float dSq = dx * dx + dy * dy;
-    const bool far = f.w < dSq * tSq;
-    const bool doFar = active && far && f.z > 0.0f;
-    if (__ballot(doFar)) {
-      const float fac = lawFactor<kSumCompensated, LAW>(f.z, dSq, eSq);
-      if (doFar) {
-        qx = fmaf(dx, fac, qx);
-        qy = fmaf(dy, fac, qy);
-        if (COUNT) {
-          ++nFar;
+    if (COUNT) {
+      nVisit += 4;
+    }
+    // Push in reverse so the first child's subtree is walked next.
+#pragma unroll
+    for (int q = 3; q >= 0; --q) {
+      if (open[q]) {
+        if (sp >= kStackDepth) {
+          if (lane == 0) {
+            atomicOr(&ctl->status, kStatusTreeInconsistent);
+          }
+          sp = 0;
+          break;
         }
-      }
-    }
-    if (active && far) {
-      wait = nextTarget;
-    }
-    const bool leaf = kid == 0;
-    const bool needLeaf = active && !far && leaf;
-    if (__ballot(needLeaf)) {
-      const int s = c.z, cnt = c.w - c.z;
-      if (COUNT) {
-        nLeafTerm += static_cast<uint32_t>(cnt);
-        if (needLeaf) {
-          nPair += static_cast<uint32_t>(cnt);
+        if (lane == 0) {
+          stNode[w][sp] = nodeI[cb + q].x;
+          stMask[w][sp] = open[q];
         }
-      }
-      const float use = needLeaf ? 1.0f : 0.0f;
-      if (cnt > 0) {
-        leafTerm<LAW>(b0, x, y, eSq, use, qx, qy);
-        leafTerm<LAW>(b1, x, y, eSq, cnt > 1 ? use : 0.0f, qx, qy);
-        leafTerm<LAW>(b2, x, y, eSq, cnt > 2 ? use : 0.0f, qx, qy);
-        leafTerm<LAW>(b3, x, y, eSq, cnt > 3 ? use : 0.0f, qx, qy);
-      }
-      for (int k = 4; k < cnt; k += 4) {
-        const float4 e0 = body[s + k], e1 = body[s + k + 1],
-                     e2 = body[s + k + 2], e3 = body[s + k + 3];
-        leafTerm<LAW>(e0, x, y, eSq, use, qx, qy);
-        leafTerm<LAW>(e1, x, y, eSq, cnt > k + 1 ? use : 0.0f, qx, qy);
-        leafTerm<LAW>(e2, x, y, eSq, cnt > k + 2 ? use : 0.0f, qx, qy);
-        leafTerm<LAW>(e3, x, y, eSq, cnt > k + 3 ? use : 0.0f, qx, qy);
-      }
-      if (needLeaf) {
-        wait = nextTarget;
+        ++sp;
       }
     }
-    if ((visits & (kFlushEvery - 1)) == 0) {
+    if ((blocks & 1) == 0) {
       kahanAdd(sx, cx, qx);
       kahanAdd(sy, cy, qy);
       qx = 0.0f;
       qy = 0.0f;
     }
-    const bool needOpen = active && !far && !leaf;
-    if (__ballot(needOpen)) {
-      node = kid;
-      f = fk;
-      c = ck;
-    } else {
-      if (nxt == 0) {
-        break;
-      }
-      node = nxt;
-      f = fn;
-      c = cn;
-    }
-    node = __builtin_amdgcn_readfirstlane(node);
-    // Requests for the iteration after next (see above).
-    fk = nodeF[c.x];
-    ck = nodeI[c.x];
-    fn = nodeF[c.y];
-    cn = nodeI[c.y];
-    b0 = body[c.z];
-    b1 = body[c.z + 1];
-    b2 = body[c.z + 2];
-    b3 = body[c.z + 3];
   }
   kahanAdd(sx, cx, qx);
   kahanAdd(sy, cy, qy);
@@ -358,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_fast(
     if (lane == 0) {
       atomicAdd(&ctl->farCount, f64);
       atomicAdd(&ctl->bodyPairCount, p64);
-      atomicAdd(&ctl->waveVisits, static_cast<unsigned long long>(visits));
+      atomicAdd(&ctl->waveVisits, static_cast<unsigned long long>(nVisit));
       atomicAdd(&ctl->waveLeafTerms, static_cast<unsigned long long>(nLeafTerm));
     }
   }

@@ -52,6 +52,24 @@ def device_sync() -> None:
         raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
 
 
+def copy_calibration(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB (the runtime runs it as its
+    blit kernel): the known-size reference for scaling PMC byte counters."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    size = ctypes.c_size_t(mib << 20)
+    if hip.hipMalloc(ctypes.byref(a), size) or hip.hipMalloc(ctypes.byref(b), size):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, size)
+    if hip.hipMemcpy(b, a, size, 3):  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    device_sync()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] calibration copy: {mib} MiB device to device", file=sys.stderr,
+          flush=True)
+
+
 def walk_traffic():
     """HBM bytes per walk launch from the committed rocprofv3 PMC summary."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -95,6 +113,8 @@ def main() -> int:
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one device copy of MIB MiB (PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; a multi-GPU run is not part of it "
@@ -102,6 +122,8 @@ def main() -> int:
         return 2
 
     nbx = load_nbx()
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
     print(f"[bench] n={args.n} theta={args.theta} warmup={args.warmup} "
           f"steps={args.steps}", file=sys.stderr, flush=True)
     sim = nbx.Simulation(args.n)

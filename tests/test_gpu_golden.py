@@ -1,0 +1,149 @@
+"""The HIP path against the reference's own outputs (golden fixtures written
+by tests/golden/make_golden.py from the reference CPU build), the CLI's NBS2
+output, and size-independent properties at 1M bodies."""
+import importlib.util
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle_lib import Bodies
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+CLI = os.path.join(ROOT, "n-body-simulation_amd", "build", "nbx_sim")
+FIELDS = ("px", "py", "vx", "vy", "ax", "ay", "m", "r", "id")
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def fixture_bodies(z, prefix=""):
+    return Bodies(*(np.ascontiguousarray(z[prefix + f]) for f in FIELDS))
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def nbs2():
+    path = os.path.join(ROOT, "n-body-simulation_amd", "nbs2.py")
+    spec = importlib.util.spec_from_file_location("nbs2", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["nbs2"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_disc_equals_reference(nbx):
+    z = load("disc_n2000_s0.npz")
+    with nbx.Simulation(2000, seed=0) as sim:
+        got = sim.download()
+    for f in ("px", "py", "vx", "vy", "m", "r"):
+        assert np.array_equal(bits(getattr(got, f)), bits(z[f])), f
+
+
+@pytest.mark.parametrize("theta,tag", [(1.0, "t1"), (0.5, "t05")])
+def test_tree_and_forces_equal_reference(nbx, theta, tag):
+    z = load("tree_n4000_k30.npz")
+    b = fixture_bodies(z, "in_")
+    with nbx.Simulation(state=b) as sim:
+        sim.summation = nbx.SUM_REFERENCE
+        sim.theta = theta
+        sim.attract()
+        t = sim.tree()
+        got = sim.download()
+    assert np.array_equal(got.id, b.id), "reference order is the key order"
+    for k in ("comX", "comY", "mass", "centerX", "centerY", "size"):
+        assert np.array_equal(bits(t[k]), bits(z["node_" + k])), k
+    for k in ("children", "next", "start", "end"):
+        assert np.array_equal(t[k], z["node_" + k]), k
+    assert np.array_equal(bits(got.ax), bits(z["ax_" + tag]))
+    assert np.array_equal(bits(got.ay), bits(z["ay_" + tag]))
+
+
+def test_collide_equals_reference(nbx):
+    z = load("collide_n4000_k150.npz")
+    with nbx.Simulation(state=fixture_bodies(z, "in_")) as sim:
+        sim.collide()
+        got = sim.download()
+    for f in ("px", "py", "vx", "vy"):
+        assert np.array_equal(bits(getattr(got, f)), bits(z["out_" + f])), f
+
+
+@pytest.mark.parametrize("summation", [0, 1])
+def test_steps_close_to_reference_simulation(nbx, summation):
+    z = load("sim_n3000_k12.npz")
+    with nbx.Simulation(3000) as sim:
+        sim.summation = summation
+        sim.step(int(z["steps"]))
+        got = sim.download_by_id()
+    d = np.hypot(got["px"] - z["px"], got["py"] - z["py"]) / \
+        np.maximum(np.hypot(z["px"], z["py"]), 1)
+    assert np.median(d) < 1e-5 and d.max() < 1e-2
+
+
+def test_cli_writes_nbs2(tmp_path, oracle):
+    out = tmp_path / "run.bin"
+    r = subprocess.run([CLI, "2000", "6", "2", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    line = [l for l in r.stderr.splitlines() if l.startswith("BH: ")]
+    assert line and "total_time=" in line[0] and "avg_compute_step=" in line[0]
+    t = nbs2().read(str(out))
+    assert (t.n_bodies, t.total_steps, t.output_every) == (2000, 6, 2)
+    assert t.num_frames == 4 and t.completed_frames == 4 and t.pos.shape == (4, 2000, 2)
+    assert t.dt == np.float32(0.05)
+    disc = oracle.uniform_disc(2000).by_id()
+    assert np.array_equal(bits(t.mass), bits(disc.m))
+    assert np.array_equal(bits(t.radius), bits(disc.r))
+    assert np.array_equal(bits(t.pos[0, :, 0]), bits(disc.px))
+    assert np.array_equal(bits(t.vel[0, :, 1]), bits(disc.vy))
+    # The CLI, like Main.cpp:117-122, starts from id order; so does the oracle.
+    o = disc
+    for _ in range(6):
+        oracle.step(o)
+    o = o.by_id()
+    d = np.hypot(t.pos[3, :, 0] - o.px, t.pos[3, :, 1] - o.py)
+    assert np.median(d / np.maximum(np.hypot(o.px, o.py), 1)) < 1e-5
+
+
+def test_million_body_invariants(nbx):
+    """1M-body disc: build invariants, valid permutation, leaf tiling, exact
+    root mass, and force accuracy against the FP64 direct sum on a sample."""
+    n = 1_000_000
+    with nbx.Simulation(n) as sim:
+        sim.theta = 0.5
+        sim.step(3)
+        st = sim.download()
+        t = sim.tree()
+        assert np.array_equal(np.sort(st.id), np.arange(n, dtype=np.uint32))
+        leaves = np.flatnonzero(t["children"] == 0)
+        order = np.lexsort((t["end"][leaves], t["start"][leaves]))
+        s, e = t["start"][leaves][order], t["end"][leaves][order]
+        assert s[0] == 0 and e[-1] == n and np.array_equal(s[1:], e[:-1])
+        assert (e - s).max() <= 16
+        assert t["mass"][0] == np.float32(1e6 + n - 1)
+        assert np.all(np.isfinite(st.ax)) and np.all(np.isfinite(st.ay))
+        rng = np.random.default_rng(0)
+        targets = rng.choice(n, 2048, replace=False).astype(np.uint32)
+        ex, ey = sim.exact_forces(targets)
+        gx, gy = st.ax[targets], st.ay[targets]
+        err = np.sqrt(np.sum((gx - ex) ** 2 + (gy - ey) ** 2) / np.sum(ex ** 2 + ey ** 2))
+        # fastInvSqrt sets a ~1e-3 floor (reference law); theta adds little here.
+        assert err < 2e-3, err
+        sim.law = nbx.LAW_ACCURATE
+        sim.forces()
+        st2 = sim.download()
+        gx, gy = st2.ax[targets], st2.ay[targets]
+        err2 = np.sqrt(np.sum((gx - ex) ** 2 + (gy - ey) ** 2) / np.sum(ex ** 2 + ey ** 2))
+        # At 1M bodies FP32 position differences (ulp(5000) ~ 5e-4 against
+        # neighbour distances ~1) bound the accuracy near 1e-3 (the oracle
+        # shows the same growth with N); the accurate law must still beat
+        # fastInvSqrt's floor.
+        assert err2 < err and err2 < 1e-3, (err2, err)

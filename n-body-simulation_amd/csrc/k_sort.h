@@ -8,7 +8,10 @@ namespace nbx {
 
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kBlock * kScanItems; // 4096 values per tile
-constexpr int kSortItems = 16;
+#ifndef NBX_SORT_ITEMS
+#define NBX_SORT_ITEMS 16
+#endif
+constexpr int kSortItems = NBX_SORT_ITEMS;
 constexpr int kSortTile = kBlock * kSortItems; // 4096 keys per tile
 
 // Exclusive scan of one value per thread across the block; returns the

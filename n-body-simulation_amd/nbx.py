@@ -7,9 +7,9 @@ collisions, then step() / iterate() / collide() / attract(). Bodies stay on
 the GPU; download() copies them out in device order, download_by_id() in id
 order.
 
-The library is loaded from build/libnbx.so next to this file; there is no
-fallback: if the library is missing or no GPU is present, construction
-raises.
+The library is loaded from build/libnbx.so next to this file (or the path in
+NBX_LIBRARY); there is no fallback: if the library is missing or no GPU is
+present, construction raises.
 """
 from __future__ import annotations
 
@@ -20,7 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "build", "libnbx.so")
+LIB_PATH = os.environ.get("NBX_LIBRARY") or os.path.join(HERE, "build", "libnbx.so")
 
 NBX_OK = 0
 LAW_REFERENCE, LAW_ACCURATE = 0, 1

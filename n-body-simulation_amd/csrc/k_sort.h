@@ -9,7 +9,7 @@ namespace nbx {
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kBlock * kScanItems; // 4096 values per tile
 #ifndef NBX_SORT_ITEMS
-#define NBX_SORT_ITEMS 16
+#define NBX_SORT_ITEMS 4
 #endif
 constexpr int kSortItems = NBX_SORT_ITEMS;
 constexpr int kSortTile = kBlock * kSortItems; // 4096 keys per tile

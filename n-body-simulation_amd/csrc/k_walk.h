@@ -322,11 +322,28 @@ __global__ __launch_bounds__(kBlock) void k_walk_fast(
       break;
     }
     const bool in = (mask & bit) != 0;
+    // The four sibling records (two 64-byte blocks) are fetched together and
+    // made available at once, rather than field by field where each is used.
+    float4 bf[4];
+    int4 bi[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      bf[q] = nodeF[cb + q];
+      bi[q] = nodeI[cb + q];
+    }
+    asm volatile("" ::"s"(bf[0].x), "s"(bf[0].y), "s"(bf[0].z), "s"(bf[0].w),
+                 "s"(bf[1].x), "s"(bf[1].y), "s"(bf[1].z), "s"(bf[1].w),
+                 "s"(bf[2].x), "s"(bf[2].y), "s"(bf[2].z), "s"(bf[2].w),
+                 "s"(bf[3].x), "s"(bf[3].y), "s"(bf[3].z), "s"(bf[3].w));
+    asm volatile("" ::"s"(bi[0].x), "s"(bi[0].y), "s"(bi[0].z), "s"(bi[0].w),
+                 "s"(bi[1].x), "s"(bi[1].y), "s"(bi[1].z), "s"(bi[1].w),
+                 "s"(bi[2].x), "s"(bi[2].y), "s"(bi[2].z), "s"(bi[2].w),
+                 "s"(bi[3].x), "s"(bi[3].y), "s"(bi[3].z), "s"(bi[3].w));
     uint64_t open[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      visitNode<LAW, COUNT>(nodeF[cb + q], nodeI[cb + q], in, x, y, tSq, eSq,
-                            body, qx, qy, open[q], nFar, nPair, nLeafTerm);
+      visitNode<LAW, COUNT>(bf[q], bi[q], in, x, y, tSq, eSq, body, qx, qy,
+                            open[q], nFar, nPair, nLeafTerm);
     }
     if (COUNT) {
       nVisit += 4;
@@ -343,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_fast(
           break;
         }
         if (lane == 0) {
-          stNode[w][sp] = nodeI[cb + q].x;
+          stNode[w][sp] = bi[q].x;
           stMask[w][sp] = open[q];
         }
         ++sp;

@@ -214,63 +214,110 @@ __global__ __launch_bounds__(kBlock) void k_walk(
 constexpr int kBodyPad = 4;
 constexpr int kStackDepth = 128; // >= 3 * kMaxDepth + 4 entries per wave
 
-template <int LAW>
+// fastInvSqrt (QuadTree.cpp:11-26) with its Newton step as one fused
+// multiply-add: the same approximation, rounded once less.
+__device__ inline float fisrFused(float x) {
+  const uint32_t i = 0x5f3759dfu - (__float_as_uint(x) >> 1);
+  const float y = __uint_as_float(i);
+  return y * fmaf(-(0.5f * x) * y, y, 1.5f);
+}
+
+// A 64-bit value every lane holds (read from one LDS address), moved to
+// scalar registers. readfirstlane works on int: the halves go through
+// uint32_t so the low word is not sign-extended.
+__device__ inline uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = static_cast<uint32_t>(
+      __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(v))));
+  const uint32_t hi = static_cast<uint32_t>(
+      __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(v >> 32))));
+  return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
+}
+
+// CAP = false drops the min(., FLT_MAX) guard; the host selects it only when
+// no factor can overflow (see capFree in the launch code).
+template <int LAW, bool CAP>
+__device__ inline float fastFactor(float mass, float dSq, float eSq) {
+  const float inv =
+      LAW == kLawReference ? fisrFused(dSq) : __builtin_amdgcn_rsqf(dSq);
+  const float f = mass * inv * __builtin_amdgcn_rcpf(dSq + eSq);
+  return CAP ? capMax(f) : f;
+}
+
+template <int LAW, bool CAP>
 __device__ inline void leafTerm(const float4 &b, float x, float y, float eSq,
-                                float w, float &qx, float &qy) {
+                                float &qx, float &qy) {
   const float bx = b.x - x;
   const float by = b.y - y;
-  const float bSq = bx * bx + by * by;
-  const float fac = lawFactor<kSumCompensated, LAW>(b.z, bSq, eSq) * w;
+  const float fac = fastFactor<LAW, CAP>(b.z, fmaf(bx, bx, by * by), eSq);
   qx = fmaf(bx, fac, qx);
   qy = fmaf(by, fac, qy);
 }
 
-template <int LAW, bool COUNT>
-__device__ inline void visitNode(const float4 &f, const int4 &c, bool in,
-                                 float x, float y, float tSq, float eSq,
-                                 const float4 *__restrict__ body, float &qx,
-                                 float &qy, uint64_t &openMask,
-                                 uint32_t &nFar, uint32_t &nPair,
-                                 uint32_t &nLeafTerm) {
+// One node for the lanes of `mask`. Lane membership is carried as wave
+// masks (inverse ballots), so the branches below cost scalar instructions
+// only. The opening test is rounded exactly as the reference rounds it
+// (no contraction): the decisions, and so the interaction set, are the
+// reference's.
+template <int LAW, bool COUNT, bool CAP>
+__device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
+                                     uint64_t mask, float x, float y,
+                                     float tSq, float eSq,
+                                     const float4 *__restrict__ body,
+                                     float &qx, float &qy, uint32_t &nFar,
+                                     uint32_t &nPair, uint32_t &nLeafTerm) {
   const float dx = f.x - x;
   const float dy = f.y - y;
   const float dSq = dx * dx + dy * dy;
-  const bool far = f.w < dSq * tSq;
-  const bool doFar = in && far && f.z > 0.0f;
-  if (__ballot(doFar)) {
-    const float fac = lawFactor<kSumCompensated, LAW>(f.z, dSq, eSq);
-    if (doFar) {
-      qx = fmaf(dx, fac, qx);
-      qy = fmaf(dy, fac, qy);
-      if (COUNT) {
-        ++nFar;
-      }
+  const uint64_t near = mask & ~__builtin_amdgcn_ballot_w64(f.w < dSq * tSq);
+  // f.z > 0 on the scalar unit: positive floats, +inf included, are exactly
+  // the bit patterns in (0, 0x7f800000].
+  const int32_t mb = __float_as_int(f.z);
+  const uint64_t farMask = (mb > 0 && mb <= 0x7f800000) ? (mask & ~near) : 0ull;
+  if (farMask != 0 && __builtin_amdgcn_inverse_ballot_w64(farMask)) {
+    const float fac = fastFactor<LAW, CAP>(f.z, dSq, eSq);
+    qx = fmaf(dx, fac, qx);
+    qy = fmaf(dy, fac, qy);
+    if (COUNT) {
+      ++nFar;
     }
   }
-  const bool leaf = c.x == 0;
-  const bool needLeaf = in && !far && leaf;
-  if (__ballot(needLeaf)) {
+  if (c.x != 0) {
+    return near;
+  }
+  if (near != 0) {
     const int s = c.z, cnt = c.w - c.z;
     if (COUNT) {
       nLeafTerm += static_cast<uint32_t>(cnt);
-      if (needLeaf) {
+    }
+    if (__builtin_amdgcn_inverse_ballot_w64(near)) {
+      if (COUNT) {
         nPair += static_cast<uint32_t>(cnt);
       }
-    }
-    const float use = needLeaf ? 1.0f : 0.0f;
-    for (int k = 0; k < cnt; k += 4) {
-      const float4 e0 = body[s + k], e1 = body[s + k + 1],
-                   e2 = body[s + k + 2], e3 = body[s + k + 3];
-      leafTerm<LAW>(e0, x, y, eSq, use, qx, qy);
-      leafTerm<LAW>(e1, x, y, eSq, cnt > k + 1 ? use : 0.0f, qx, qy);
-      leafTerm<LAW>(e2, x, y, eSq, cnt > k + 2 ? use : 0.0f, qx, qy);
-      leafTerm<LAW>(e3, x, y, eSq, cnt > k + 3 ? use : 0.0f, qx, qy);
+      // Four records per 64-byte load (the array is padded), terms only
+      // for the records inside the leaf.
+      for (int k = 0; k < cnt; k += 4) {
+        const float4 e0 = body[s + k], e1 = body[s + k + 1],
+                     e2 = body[s + k + 2], e3 = body[s + k + 3];
+        asm volatile("" ::"s"(e0.x), "s"(e0.y), "s"(e0.z), "s"(e1.x),
+                     "s"(e1.y), "s"(e1.z), "s"(e2.x), "s"(e2.y), "s"(e2.z),
+                     "s"(e3.x), "s"(e3.y), "s"(e3.z));
+        leafTerm<LAW, CAP>(e0, x, y, eSq, qx, qy);
+        if (cnt > k + 1) {
+          leafTerm<LAW, CAP>(e1, x, y, eSq, qx, qy);
+        }
+        if (cnt > k + 2) {
+          leafTerm<LAW, CAP>(e2, x, y, eSq, qx, qy);
+        }
+        if (cnt > k + 3) {
+          leafTerm<LAW, CAP>(e3, x, y, eSq, qx, qy);
+        }
+      }
     }
   }
-  openMask = __ballot(in && !far && !leaf);
+  return 0;
 }
 
-template <int LAW, bool COUNT>
+template <int LAW, bool COUNT, bool CAP>
 __global__ __launch_bounds__(kBlock) void k_walk_fast(
     uint32_t n, const float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
@@ -289,7 +336,6 @@ __global__ __launch_bounds__(kBlock) void k_walk_fast(
   const bool valid = i < n;
   const float4 me = body[valid ? i : base];
   const float x = me.x, y = me.y;
-  const uint64_t bit = 1ull << lane;
   float sx = 0.0f, sy = 0.0f, cx = 0.0f, cy = 0.0f; // compensated total
   float qx = 0.0f, qy = 0.0f;                       // partial since flush
   uint32_t nFar = 0, nPair = 0, nLeafTerm = 0, nVisit = 0;
@@ -298,9 +344,10 @@ __global__ __launch_bounds__(kBlock) void k_walk_fast(
   // The root on its own, then blocks of four siblings from the stack.
   int sp = 0;
   {
-    uint64_t open = 0;
-    visitNode<LAW, COUNT>(nodeF[0], nodeI[0], valid, x, y, tSq, eSq, body, qx,
-                          qy, open, nFar, nPair, nLeafTerm);
+    const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
+    const uint64_t open =
+        visitNode<LAW, COUNT, CAP>(nodeF[0], nodeI[0], all, x, y, tSq, eSq, body,
+                              qx, qy, nFar, nPair, nLeafTerm);
     nVisit += 1;
     if (open) {
       if (lane == 0) {
@@ -314,14 +361,13 @@ __global__ __launch_bounds__(kBlock) void k_walk_fast(
   while (sp > 0) {
     --sp;
     const int cb = __builtin_amdgcn_readfirstlane(stNode[w][sp]);
-    const uint64_t mask = stMask[w][sp];
+    const uint64_t mask = uniform64(stMask[w][sp]);
     if (cb <= 0 || cb + 3 >= nodeCount || ++blocks > nodeCount) {
       if (lane == 0) {
         atomicOr(&ctl->status, kStatusTreeInconsistent);
       }
       break;
     }
-    const bool in = (mask & bit) != 0;
     // The four sibling records (two 64-byte blocks) are fetched together and
     // made available at once, rather than field by field where each is used.
     float4 bf[4];
@@ -342,8 +388,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_fast(
     uint64_t open[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      visitNode<LAW, COUNT>(bf[q], bi[q], in, x, y, tSq, eSq, body, qx, qy,
-                            open[q], nFar, nPair, nLeafTerm);
+      open[q] = visitNode<LAW, COUNT, CAP>(bf[q], bi[q], mask, x, y, tSq, eSq, body,
+                                      qx, qy, nFar, nPair, nLeafTerm);
     }
     if (COUNT) {
       nVisit += 4;

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -71,6 +72,7 @@ struct nbx_sim {
   int cur = 0; // body buffer holding the state
   uint64_t frame = 0;
   nbx_params params{};
+  double massAbsSum = 0; // sum of |m| of the bodies held (masses never change)
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
 
@@ -341,9 +343,23 @@ int launchForces(nbx_sim *s, bool count) {
 #define WALK(SUMV, LAWV, CNT)                                                 \
   LAUNCH((k_walk<SUMV, LAWV, CNT>), grid, kBlock, n, s->nodeF, s->nodeI,      \
          s->px[c], s->py[c], s->m[c], tSq, eSq, s->ax, s->ay, s->ctl)
+  // Reference law: every factor is |m| * fisr(d2) / (d2 + eps2) with
+  // |m| <= sum |m| and fisr(d2) <= fisr(0) < 2e19, so with the bound below
+  // the min(., FLT_MAX) guard cannot change a value. (The accurate law keeps
+  // the guard: its rsqrt(0) is infinite for a body's own term.)
+  const double eps2 = static_cast<double>(eSq);
+  const bool capFree = !acc && eps2 > 0 &&
+                       s->massAbsSum * 2e19 * std::max(1.0, 1.0 / eps2) < 1e37;
 #define FAST(LAWV, CNT)                                                       \
-  LAUNCH((k_walk_fast<LAWV, CNT>), grid, kBlock, n, s->nodeF, s->nodeI,       \
-         s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl)
+  do {                                                                        \
+    if (capFree) {                                                            \
+      LAUNCH((k_walk_fast<LAWV, CNT, false>), grid, kBlock, n, s->nodeF,      \
+             s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);          \
+    } else {                                                                  \
+      LAUNCH((k_walk_fast<LAWV, CNT, true>), grid, kBlock, n, s->nodeF,       \
+             s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);          \
+    }                                                                         \
+  } while (0)
   if (ref) {
     if (count) {
       if (acc) WALK(kSumReference, kLawAccurate, true);
@@ -689,6 +705,10 @@ int nbx_upload(nbx_sim *s, uint64_t n, const float *px, const float *py,
   dropGraphs(s);
   s->cur = 0;
   s->n = n;
+  s->massAbsSum = 0;
+  for (uint64_t k = 0; k < n; ++k) {
+    s->massAbsSum += std::fabs(static_cast<double>(m[k]));
+  }
   NBX_TRY(copyIn(s, s->px[0], px, n));
   NBX_TRY(copyIn(s, s->py[0], py, n));
   NBX_TRY(copyIn(s, s->vx[0], vx, n));

@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_fast(
   __shared__ uint64_t stMask[kBlock / kWave][kStackDepth];
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t w = threadIdx.x >> 6;
-  const uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1));
+  const uint32_t base = xcdBlock() * kBlock + (threadIdx.x & ~(kWave - 1));
   if (base >= n ||
       (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent))) {
     return;

@@ -70,6 +70,18 @@ __host__ __device__ inline float ordFloat(uint32_t u) {
   return f;
 }
 
+// Workgroups are handed to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2. Renumbering them so that every XCD
+// takes one contiguous range of the grid keeps spatially neighbouring work,
+// which shares tree nodes and bodies, in one L2.
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcdBlock() {
+  const uint32_t b = blockIdx.x, g = gridDim.x;
+  const uint32_t x = b % kXcds, k = b / kXcds;
+  const uint32_t q = g / kXcds, r = g % kXcds;
+  return x * q + min(x, r) + k;
+}
+
 __device__ inline uint64_t laneMaskLt() {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));

@@ -158,7 +158,10 @@ int nbx_download_keys(nbx_sim *sim, uint32_t *hi, uint32_t *lo,
                       uint32_t *perm);
 
 /* Last collide (Collision.cpp:135-349): grid cell size, number of pairs,
- * number of connected contact components and the largest one's pair count. */
+ * number of connected contact components and the largest one's pair count.
+ * The cell is the broadphase's own choice (1.01 x the 5/8 quantile of a
+ * sample of box widths, where the reference uses the exact median); the pair
+ * set does not depend on it. */
 int nbx_collide_info(nbx_sim *sim, float *cell, int64_t *pairs,
                      int64_t *components, int64_t *largest);
 

@@ -1,5 +1,5 @@
 // k_sort.h -- device-wide exclusive scan, stable LSD radix sort (8-bit
-// digits, LDS-staged scatter) and exact radix selection.
+// digits, LDS-staged scatter).
 #pragma once
 
 #include "nbx_device.h"
@@ -236,85 +236,6 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
     const uint32_t dst = goff[d] + (p - digitStart[d]);
     keysOut[dst] = key;
     valsOut[dst] = sv[p];
-  }
-}
-
-// ---- exact k-th smallest of non-negative floats (radix select) ----
-// Three passes over 11, 11 and 10 bits of the float's bit pattern; the state
-// (decided prefix, remaining rank) lives in Ctl.
-
-__global__ __launch_bounds__(kBlock) void k_select_hist(
-    uint32_t n, const float *__restrict__ v, int pass,
-    const Ctl *__restrict__ ctl, uint32_t *__restrict__ hist) {
-  __shared__ uint32_t h[kSelBins];
-  for (int b = threadIdx.x; b < kSelBins; b += kBlock) {
-    h[b] = 0;
-  }
-  __syncthreads();
-  const int shift = pass == 0 ? 21 : (pass == 1 ? 10 : 0);
-  const uint32_t mask = pass == 2 ? 1023u : 2047u;
-  const int keep = 32 - (pass == 0 ? 32 : (pass == 1 ? 21 : 10)); // decided bits
-  const uint32_t prefix = ctl->selPrefix;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += gridDim.x * kBlock) {
-    const uint32_t u = __float_as_uint(v[i]);
-    const bool match = keep == 0 || (u >> (32 - keep)) == (prefix >> (32 - keep));
-    if (match) {
-      atomicAdd(&h[(u >> shift) & mask], 1u);
-    }
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < kSelBins; b += kBlock) {
-    if (h[b]) {
-      atomicAdd(&hist[b], h[b]);
-    }
-  }
-}
-
-// One workgroup: picks the bin holding the remaining rank, clears the
-// histogram; after the last pass turns the selected width into the
-// collision grid cell (Collision.cpp:182-189).
-__global__ __launch_bounds__(kBlock) void k_select_pick(
-    int pass, Ctl *__restrict__ ctl, uint32_t *__restrict__ hist) {
-  __shared__ uint32_t chunk[kBlock];
-  __shared__ uint32_t scratch[kBlock / kWave];
-  const int per = kSelBins / kBlock; // 8 bins per thread
-  uint32_t local[kSelBins / kBlock];
-  uint32_t s = 0;
-  for (int k = 0; k < per; ++k) {
-    local[k] = hist[threadIdx.x * per + k];
-    s += local[k];
-  }
-  uint32_t total;
-  const uint32_t pre = blockExclusiveScan(s, scratch, &total);
-  const uint32_t rank = ctl->selRank;
-  chunk[threadIdx.x] = 0;
-  __syncthreads();
-  if (rank >= pre && rank < pre + s) {
-    uint32_t c = pre;
-    for (int k = 0; k < per; ++k) {
-      if (rank < c + local[k]) {
-        const int shift = pass == 0 ? 21 : (pass == 1 ? 10 : 0);
-        const uint32_t bin = threadIdx.x * per + k;
-        const uint32_t prefix = ctl->selPrefix | (bin << shift);
-        ctl->selPrefix = prefix;
-        ctl->selRank = rank - c;
-        if (pass == 2) {
-          const float w = __uint_as_float(prefix);
-          float cell = w * 1.01f;
-          if (!(cell > 0.0f)) {
-            cell = 1.0f;
-          }
-          ctl->cell = cell;
-          ctl->invCell = 1.0f / cell;
-        }
-        break;
-      }
-      c += local[k];
-    }
-  }
-  for (int k = 0; k < per; ++k) {
-    hist[threadIdx.x * per + k] = 0;
   }
 }
 

@@ -17,7 +17,6 @@ constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kLeafCapacity = 16;  // Simulation.cpp:9 leafCapacity
 constexpr int kMaxDepth = 32;      // key depth: 2 words x 16 levels
-constexpr int kSelBins = 2048;     // radix-select digit (11 bits)
 constexpr int kDeepMax = 4096;     // bodies per equal-hi-key run sortable in LDS
 constexpr int kSmallComp = 16;     // contact components replayed by one lane
 constexpr int kBigCompMax = 4096;  // pairs per component sortable in LDS
@@ -41,7 +40,6 @@ struct Ctl {
   float quad[4];     // root quad: cx, cy, size, unused
   // collide
   float cell, invCell;
-  uint32_t selPrefix, selRank;
   uint32_t largeCount, pairCount;
   uint32_t compCount, compPairs, bigCount, largest;
   // tree

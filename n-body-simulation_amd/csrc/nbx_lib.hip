@@ -82,8 +82,6 @@ struct nbx_sim {
   float *ax = nullptr, *ay = nullptr;
 
   // collide
-  float *widths = nullptr;
-  uint32_t *selHist = nullptr;
   uint64_t buckets = 0; // power of two >= 2 * capacity
   uint32_t *bucketOf = nullptr, *bucketCount = nullptr, *bucketStart = nullptr;
   uint32_t *scanTiles = nullptr;
@@ -239,13 +237,7 @@ int launchCollide(nbx_sim *s) {
   const uint32_t mask = buckets - 1;
   const uint32_t pairGrid = std::min<uint32_t>(blocksFor(s->pairCap), 2048);
 
-  LAUNCH(k_widths, nb, kBlock, n, px, py, r, s->widths, s->ctl);
-  const uint32_t selGrid = std::min<uint32_t>(nb, 512);
-  for (int pass = 0; pass < 3; ++pass) {
-    LAUNCH(k_select_hist, selGrid, kBlock, n, s->widths, pass, s->ctl,
-           s->selHist);
-    LAUNCH(k_select_pick, 1, kBlock, pass, s->ctl, s->selHist);
-  }
+  LAUNCH(k_cell_size, 1, kBlock, n, px, py, r, s->ctl);
   LAUNCH(k_cells, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
          s->bucketCount, s->largeList, s->ufParent);
   NBX_TRY(launchScan(s, buckets + 1, s->bucketCount, s->bucketStart));
@@ -630,7 +622,6 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
     A(s->keyHi[b], n, false) A(s->vals[b], n, false)
   }
   A(s->ax, n, true) A(s->ay, n, true)
-  A(s->widths, n, false) A(s->selHist, kSelBins, true)
   s->buckets = std::max<uint64_t>(1024, pow2AtLeast(2 * n));
   A(s->bucketOf, n, false) A(s->bucketCount, s->buckets + 1, true)
   A(s->bucketStart, s->buckets + 1, false)

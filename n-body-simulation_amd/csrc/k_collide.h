@@ -286,8 +286,9 @@ __global__ __launch_bounds__(kBlock) void k_cell_size(
 __global__ __launch_bounds__(kBlock) void k_cells(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
     const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
-    uint32_t *__restrict__ bucketOf, uint32_t *__restrict__ bucketCount,
-    int32_t *__restrict__ largeList, int32_t *__restrict__ ufParent) {
+    uint32_t *__restrict__ bucketOf, uint32_t *__restrict__ bucketRank,
+    uint32_t *__restrict__ bucketCount, int32_t *__restrict__ largeList,
+    int32_t *__restrict__ ufParent) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) {
     return;
@@ -305,15 +306,17 @@ __global__ __launch_bounds__(kBlock) void k_cells(
   const int32_t cy = cellCoord(b.minY, invCell);
   const uint32_t h = cellBucket(cx, cy, mask);
   bucketOf[i] = h;
-  atomicAdd(&bucketCount[h], 1u);
+  bucketRank[i] = atomicAdd(&bucketCount[h], 1u);
 }
 
-// Counting-sort scatter of small boxes into their buckets. Consumes the
-// bucket counts back to zero, ready for the next step.
+// Counting-sort scatter of small boxes into their buckets, at the rank
+// each body drew in k_cells. Clears the bucket counts (all of them were
+// read by the scan) for the next step.
 __global__ __launch_bounds__(kBlock) void k_grid_scatter(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
     const float *__restrict__ r, const Ctl *__restrict__ ctl,
     const uint32_t *__restrict__ bucketOf,
+    const uint32_t *__restrict__ bucketRank,
     uint32_t *__restrict__ bucketCount,
     const uint32_t *__restrict__ bucketStart, float4 *__restrict__ gridBox,
     int4 *__restrict__ gridCell) {
@@ -325,7 +328,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_scatter(
   if (h == 0xffffffffu) {
     return;
   }
-  const uint32_t slot = bucketStart[h] + atomicSub(&bucketCount[h], 1u) - 1u;
+  const uint32_t slot = bucketStart[h] + bucketRank[i];
+  bucketCount[h] = 0;
   const float invCell = ctl->invCell;
   const Box b = bodyBox(px[i], py[i], r[i]);
   gridBox[slot] = make_float4(b.minX, b.maxX, b.minY, b.maxY);

@@ -83,7 +83,8 @@ struct nbx_sim {
 
   // collide
   uint64_t buckets = 0; // power of two >= 2 * capacity
-  uint32_t *bucketOf = nullptr, *bucketCount = nullptr, *bucketStart = nullptr;
+  uint32_t *bucketOf = nullptr, *bucketRank = nullptr, *bucketCount = nullptr,
+           *bucketStart = nullptr;
   uint32_t *scanTiles = nullptr;
   float4 *gridBox = nullptr;
   int4 *gridCell = nullptr;
@@ -239,10 +240,11 @@ int launchCollide(nbx_sim *s) {
 
   LAUNCH(k_cell_size, 1, kBlock, n, px, py, r, s->ctl);
   LAUNCH(k_cells, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
-         s->bucketCount, s->largeList, s->ufParent);
+         s->bucketRank, s->bucketCount, s->largeList, s->ufParent);
   NBX_TRY(launchScan(s, buckets + 1, s->bucketCount, s->bucketStart));
   LAUNCH(k_grid_scatter, nb, kBlock, n, px, py, r, s->ctl, s->bucketOf,
-         s->bucketCount, s->bucketStart, s->gridBox, s->gridCell);
+         s->bucketRank, s->bucketCount, s->bucketStart, s->gridBox,
+         s->gridCell);
   LAUNCH(k_find_small, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
          s->bucketStart, s->gridBox, s->gridCell, s->largeList, s->pairs,
          s->pairCap);
@@ -623,7 +625,8 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   }
   A(s->ax, n, true) A(s->ay, n, true)
   s->buckets = std::max<uint64_t>(1024, pow2AtLeast(2 * n));
-  A(s->bucketOf, n, false) A(s->bucketCount, s->buckets + 1, true)
+  A(s->bucketOf, n, false) A(s->bucketRank, n, false)
+  A(s->bucketCount, s->buckets + 1, true)
   A(s->bucketStart, s->buckets + 1, false)
   A(s->scanTiles, blocksFor(s->buckets + 1, kScanTile) + 1, false)
   A(s->gridBox, n, false) A(s->gridCell, n, false)

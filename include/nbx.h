@@ -89,6 +89,27 @@ int nbx_destroy(nbx_sim *sim);
  * (src/Utils.cpp:6-54), in generator order, and resets the frame counter. */
 int nbx_init_disc(nbx_sim *sim, uint64_t n, uint32_t seed);
 
+/* Initial conditions of the viewer's scenario menu, makeScenario(s, n, seed)
+ * (src/viewer/Scenarios.cpp:29-62), bit for bit:
+ *   NBX_SCENARIO_DISC          uniformDisc, as nbx_init_disc;
+ *   NBX_SCENARIO_TWO_GALAXIES  TwoGalaxies: discs of n/2 (seed) and n - n/2
+ *                              (seed + 1) bodies on a glancing course, the
+ *                              second counter-rotating (Scenarios.cpp:36-49);
+ *   NBX_SCENARIO_COLD_SQUARE   ColdSquare: unit bodies at rest, uniform in
+ *                              [-3 sqrt(n), 3 sqrt(n)]^2 (Scenarios.cpp:50-57);
+ * and one the reference does not have:
+ *   NBX_SCENARIO_MERGER        two discs of n/2 and n - n/2 bodies (seeds
+ *                              seed, seed + 1) rotating the same way, centres
+ *                              4 R apart (R = 5 sqrt(n/2), the disc radius),
+ *                              closing at 0.3 v_c(R) along a direction 15
+ *                              degrees off the line of centres.
+ * Ids are 0..n-1 in array order. The two-disc scenarios need n >= 2. */
+#define NBX_SCENARIO_DISC 0
+#define NBX_SCENARIO_TWO_GALAXIES 1
+#define NBX_SCENARIO_COLD_SQUARE 2
+#define NBX_SCENARIO_MERGER 3
+int nbx_init_scenario(nbx_sim *sim, int scenario, uint64_t n, uint32_t seed);
+
 /* CudaSimulation(const std::vector<Body>&) and CudaSimulation::reset
  * (src/cuda/CudaSimulation.cu:34-48), DeviceBodies::upload
  * (src/cuda/DeviceBodies.cu:25-50). ax/ay may be null (zero). */

@@ -8,7 +8,9 @@
 //
 // The physics parameters the reference fixes at compile time can be set
 // from the environment: NBX_THETA, NBX_EPSILON, NBX_DT, NBX_COLLISIONS (0/1),
-// NBX_LAW (reference|accurate), NBX_SUM (reference|compensated).
+// NBX_LAW (reference|accurate), NBX_SUM (reference|compensated). NBX_SCENARIO
+// (disc|two-galaxies|cold-square|merger) and NBX_SEED pick the initial
+// condition (default: the reference's uniformDisc, seed 0).
 
 #include "nbx.h"
 
@@ -85,8 +87,24 @@ int main(int argc, char **argv) {
   }
 
   // uniformDisc, then into id order before the first step (Main.cpp:117-122).
-  if (nbx_init_disc(sim, nBodies, 0) != NBX_OK) {
-    die("nbx_init_disc");
+  int scenario = NBX_SCENARIO_DISC;
+  if (const char *v = std::getenv("NBX_SCENARIO")) {
+    if (std::strcmp(v, "two-galaxies") == 0) {
+      scenario = NBX_SCENARIO_TWO_GALAXIES;
+    } else if (std::strcmp(v, "cold-square") == 0) {
+      scenario = NBX_SCENARIO_COLD_SQUARE;
+    } else if (std::strcmp(v, "merger") == 0) {
+      scenario = NBX_SCENARIO_MERGER;
+    } else if (std::strcmp(v, "disc") != 0) {
+      std::fprintf(stderr, "unknown NBX_SCENARIO '%s'\n", v);
+      return 1;
+    }
+  }
+  const char *seedText = std::getenv("NBX_SEED");
+  const std::uint32_t seed =
+      seedText ? static_cast<std::uint32_t>(std::strtoul(seedText, nullptr, 10)) : 0u;
+  if (nbx_init_scenario(sim, scenario, nBodies, seed) != NBX_OK) {
+    die("nbx_init_scenario");
   }
   {
     std::vector<float> px(nBodies), py(nBodies), vx(nBodies), vy(nBodies),

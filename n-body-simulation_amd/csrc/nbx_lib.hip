@@ -742,6 +742,44 @@ int nbx_init_disc(nbx_sim *s, uint64_t n, uint32_t seed) {
                     nullptr, m.data(), r.data(), id.data());
 }
 
+int nbx_init_scenario(nbx_sim *s, int scenario, uint64_t n, uint32_t seed) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  if (n > s->cap) {
+    return setError(NBX_ERR_ARG, "n exceeds the handle's capacity");
+  }
+  using Gen = void (*)(uint64_t, uint32_t, float *, float *, float *, float *,
+                       float *, float *, uint32_t *);
+  Gen gen = nullptr;
+  switch (scenario) {
+  case NBX_SCENARIO_DISC:
+    gen = nbx_host::uniformDisc;
+    break;
+  case NBX_SCENARIO_TWO_GALAXIES:
+    gen = nbx_host::twoGalaxies;
+    break;
+  case NBX_SCENARIO_COLD_SQUARE:
+    gen = nbx_host::coldSquare;
+    break;
+  case NBX_SCENARIO_MERGER:
+    gen = nbx_host::galaxyMerger;
+    break;
+  default:
+    return setError(NBX_ERR_ARG, "unknown scenario");
+  }
+  if (n < 2 && (scenario == NBX_SCENARIO_TWO_GALAXIES ||
+                scenario == NBX_SCENARIO_MERGER)) {
+    return setError(NBX_ERR_ARG, "a two-disc scenario needs n >= 2");
+  }
+  std::vector<float> px(n), py(n), vx(n), vy(n), m(n), r(n);
+  std::vector<uint32_t> id(n);
+  gen(n, seed, px.data(), py.data(), vx.data(), vy.data(), m.data(), r.data(),
+      id.data());
+  return nbx_upload(s, n, px.data(), py.data(), vx.data(), vy.data(), nullptr,
+                    nullptr, m.data(), r.data(), id.data());
+}
+
 int nbx_download(nbx_sim *s, float *px, float *py, float *vx, float *vy,
                  float *ax, float *ay, float *m, float *r, uint32_t *id) {
   if (!s) {

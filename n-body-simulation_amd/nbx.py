@@ -36,8 +36,10 @@ EXPORTS = (
     "nbx_forces", "nbx_frame", "nbx_sync", "nbx_tree_info", "nbx_download_tree",
     "nbx_root_quad", "nbx_download_keys", "nbx_collide_info", "nbx_download_pairs",
     "nbx_count_interactions", "nbx_set_profiling", "nbx_phase_times", "nbx_stream",
-    "nbx_exact_forces",
+    "nbx_exact_forces", "nbx_init_scenario",
 )
+
+SCENARIO_DISC, SCENARIO_TWO_GALAXIES, SCENARIO_COLD_SQUARE, SCENARIO_MERGER = 0, 1, 2, 3
 
 
 class NbxError(RuntimeError):
@@ -78,6 +80,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_create.argtypes = [C.c_uint64, C.POINTER(_vp)]
     L.nbx_destroy.argtypes = [_vp]
     L.nbx_init_disc.argtypes = [_vp, C.c_uint64, C.c_uint32]
+    L.nbx_init_scenario.argtypes = [_vp, C.c_int32, C.c_uint64, C.c_uint32]
     L.nbx_upload.argtypes = [_vp, C.c_uint64] + [_vp] * 9
     L.nbx_download.argtypes = [_vp] + [_vp] * 9
     L.nbx_download_by_id.argtypes = [_vp] + [_vp] * 6
@@ -127,14 +130,15 @@ class State:
 class Simulation:
     """GPU Barnes-Hut simulation (CudaSimulation's interface).
 
-    Simulation(n, seed=0) loads uniformDisc(n, seed); Simulation(state=...)
+    Simulation(n, seed=0) loads uniformDisc(n, seed); scenario= picks another
+    of makeScenario's initial conditions (SCENARIO_*); Simulation(state=...)
     loads given bodies. Attributes dt, theta, epsilon, collisions, law,
     summation and use_graph are read on every call, like CudaSimulation's
     public members.
     """
 
     def __init__(self, n: int | None = None, seed: int = 0, state=None,
-                 capacity: int | None = None):
+                 capacity: int | None = None, scenario: int = SCENARIO_DISC):
         self.L = load_library()
         if state is not None:
             n_bodies = int(np.asarray(state.px).shape[0])
@@ -153,8 +157,10 @@ class Simulation:
         self.law, self.summation, self.use_graph = p.law, p.summation, bool(p.use_graph)
         if state is not None:
             self.upload(state)
-        else:
+        elif scenario == SCENARIO_DISC:
             self._check(self.L.nbx_init_disc(self.h, n_bodies, seed))
+        else:
+            self._check(self.L.nbx_init_scenario(self.h, scenario, n_bodies, seed))
 
     # -- plumbing --
     def _check(self, rc: int) -> None:

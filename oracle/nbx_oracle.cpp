@@ -267,6 +267,79 @@ void or_uniform_disc(uint64_t n, uint32_t seed, float *px, float *py,
   }
 }
 
+// appendDisc (Scenarios.cpp:8-20): a uniformDisc moved by (x, y) with bulk
+// velocity (bvx, bvy), rotation flipped first when `reverse`.
+static void or_append_disc(uint64_t count, uint32_t seed, uint64_t at, float x,
+                           float y, float bvx, float bvy, int reverse,
+                           float *px, float *py, float *vx, float *vy,
+                           float *m, float *r) {
+  std::vector<float> bx(count), by(count), bvxs(count), bvys(count);
+  std::vector<uint32_t> ids(count);
+  or_uniform_disc(count, seed, bx.data(), by.data(), bvxs.data(), bvys.data(),
+                  m + at, r + at, ids.data());
+  for (uint64_t i = 0; i < count; ++i) {
+    float vxi = bvxs[i], vyi = bvys[i];
+    if (reverse) {
+      vxi = vxi * -1.0f;
+      vyi = vyi * -1.0f;
+    }
+    px[at + i] = bx[i] + x;
+    py[at + i] = by[i] + y;
+    vx[at + i] = vxi + bvx;
+    vy[at + i] = vyi + bvy;
+  }
+}
+
+int or_scenario(int kind, uint64_t n, uint32_t seed, float *px, float *py,
+                float *vx, float *vy, float *m, float *r, uint32_t *id) {
+  if (kind < 0 || kind > 3 || ((kind == 1 || kind == 3) && n < 2)) {
+    return -1;
+  }
+  if (kind == 0) {
+    or_uniform_disc(n, seed, px, py, vx, vy, m, r, id);
+    return 0;
+  }
+  if (kind == 1) { // Scenarios.cpp:36-49
+    const uint64_t half = n / 2;
+    const float radius = 5.0f * std::sqrt(static_cast<float>(half));
+    const float sep = 2.5f * radius;
+    const float totalMass = 2.0f * (1e6f + static_cast<float>(half));
+    const float v = 0.3f * std::sqrt(totalMass / sep);
+    or_append_disc(half, seed, 0, -sep / 2, -radius / 3, v, 0.0f, 0, px, py,
+                   vx, vy, m, r);
+    or_append_disc(n - half, seed + 1, half, sep / 2, radius / 3, -v, 0.0f, 1,
+                   px, py, vx, vy, m, r);
+  } else if (kind == 2) { // Scenarios.cpp:50-57; y is drawn first
+    std::mt19937 rng(seed);
+    const float half = 3.0f * std::sqrt(static_cast<float>(n));
+    std::uniform_real_distribution<float> coord(-half, half);
+    for (uint64_t i = 0; i < n; ++i) {
+      py[i] = coord(rng);
+      px[i] = coord(rng);
+      vx[i] = 0.0f;
+      vy[i] = 0.0f;
+      m[i] = 1.0f;
+      r[i] = 1.0f;
+    }
+  } else { // the precision-study merger
+    const uint64_t half = n / 2;
+    const float radius = 5.0f * std::sqrt(static_cast<float>(half));
+    const float mass = 1e6f + static_cast<float>(half) - 1.0f;
+    const float vc = std::sqrt(mass / radius);
+    const float angle = 15.0f * static_cast<float>(M_PI) / 180.0f;
+    const float ux = std::cos(angle), uy = std::sin(angle);
+    const float sp = 0.15f * vc;
+    or_append_disc(half, seed, 0, -2.0f * radius, 0.0f, sp * ux, sp * uy, 0,
+                   px, py, vx, vy, m, r);
+    or_append_disc(n - half, seed + 1, half, 2.0f * radius, 0.0f, -sp * ux,
+                   -sp * uy, 0, px, py, vx, vy, m, r);
+  }
+  for (uint64_t i = 0; i < n; ++i) { // renumber (Scenarios.cpp:22-27)
+    id[i] = static_cast<uint32_t>(i);
+  }
+  return 0;
+}
+
 // Body.h:20-23.
 void or_iterate(uint64_t n, float dt, float *px, float *py, float *vx,
                 float *vy, const float *ax, const float *ay) {

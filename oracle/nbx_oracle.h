@@ -13,6 +13,11 @@
  *
  * Parity pins (what checks each function; see tests/test_oracle_*.py):
  *   or_uniform_disc    reference build (oracle/_ref) + golden frame-0 fixture
+ *   or_scenario        TwoGalaxies / ColdSquare: reference build (makeScenario)
+ *                      + golden frame-0 fixtures; the merger, which the
+ *                      reference does not have: closed forms (centres 4 R
+ *                      apart, bulk velocities, zero total momentum for odd
+ *                      and even n) and its two discs = or_uniform_disc
  *   or_iterate         closed form + reference build
  *   or_root_quad       exact min/max + reference tree root
  *   or_keys            reference QuadTree node ranges (every leaf), numpy
@@ -43,6 +48,15 @@ enum { OR_LAW_REFERENCE = 0, OR_LAW_ACCURATE = 1 };
  * |x|^2), ids assigned before the sort. */
 void or_uniform_disc(uint64_t n, uint32_t seed, float *px, float *py,
                      float *vx, float *vy, float *m, float *r, uint32_t *id);
+
+/* src/viewer/Scenarios.cpp:29-62 makeScenario(kind, n, seed), ids 0..n-1:
+ * kind 0 Disc, 1 TwoGalaxies, 2 ColdSquare; kind 3 is the precision-study
+ * merger (two same-sense discs of n/2 and n - n/2 bodies, seeds seed and
+ * seed + 1, centres at (-2R, 0) and (2R, 0), R = 5 sqrt(n/2), bulk velocities
+ * +/-0.15 v_c (cos 15deg, sin 15deg), v_c = sqrt((1e6 + n/2 - 1) / R)).
+ * Returns 0, or -1 for an unknown kind or a two-disc kind with n < 2. */
+int or_scenario(int kind, uint64_t n, uint32_t seed, float *px, float *py,
+                float *vx, float *vy, float *m, float *r, uint32_t *id);
 
 /* Body.h:20-23 Body::update / Simulation.cpp:44-50: v += a*dt; x += v*dt. */
 void or_iterate(uint64_t n, float dt, float *px, float *py, float *vx,

@@ -18,6 +18,7 @@
 #include "Collision.h"
 #include "Simulation.h"
 #include "Utils.h"
+#include "viewer/Scenarios.h"
 
 namespace {
 
@@ -65,6 +66,18 @@ int64_t ref_uniform_disc(uint64_t n, uint32_t seed, float *px, float *py,
                          float *vx, float *vy, float *m, float *r,
                          uint32_t *id) {
   std::vector<Body> b = uniformDisc(n, seed);
+  if (b.size() > n) {
+    return static_cast<int64_t>(b.size());
+  }
+  scatter(b, px, py, vx, vy, nullptr, nullptr, m, r, id);
+  return static_cast<int64_t>(b.size());
+}
+
+// makeScenario(kind, n, seed) (src/viewer/Scenarios.cpp:29-62). Returns the
+// number of bodies it made (written only when it fits in n).
+int64_t ref_scenario(int kind, uint64_t n, uint32_t seed, float *px, float *py,
+                     float *vx, float *vy, float *m, float *r, uint32_t *id) {
+  std::vector<Body> b = makeScenario(static_cast<Scenario>(kind), n, seed);
   if (b.size() > n) {
     return static_cast<int64_t>(b.size());
   }

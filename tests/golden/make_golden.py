@@ -20,6 +20,9 @@ Fixtures (all float32 bit patterns preserved):
       state after the reference collide().
   sim_n3000_k12.npz
       Simulation(3000) after 12 reference steps, by id (whole-step check).
+  scenario_k1_n1501_s2.npz, scenario_k2_n1200_s9.npz
+      makeScenario(TwoGalaxies, 1501, 2) and makeScenario(ColdSquare, 1200,
+      9) (src/viewer/Scenarios.cpp:29-62), frame 0.
 
 usage: python tests/golden/make_golden.py
 """
@@ -96,6 +99,11 @@ def main():
 
     sim = ref.simulation(3000, 12).by_id()
     save("sim_n3000_k12.npz", steps=12, **bodies_dict(sim))
+
+    for kind, n, seed in ((1, 1501, 2), (2, 1200, 9)):
+        sc = ref.scenario(kind, n, seed)
+        save(f"scenario_k{kind}_n{n}_s{seed}.npz", kind=kind, n=n, seed=seed,
+             **bodies_dict(sc))
 
 
 if __name__ == "__main__":

@@ -127,11 +127,19 @@ class Oracle:
         L.or_exact_forces.argtypes = [_u64, _f32p, _f32p, _f32p, _u64, _u32p, _f64,
                                       _f64p, _f64p]
         L.or_fast_inv_sqrt_bits.argtypes = [_u64, _u32p, _u32p]
+        L.or_scenario.argtypes = [_int, _u64, C.c_uint32] + [_f32p] * 6 + [_u32p]
+        L.or_scenario.restype = _int
         self.L = L
 
     def uniform_disc(self, n: int, seed: int = 0) -> Bodies:
         b = Bodies.empty(n)
         self.L.or_uniform_disc(n, seed, b.px, b.py, b.vx, b.vy, b.m, b.r, b.id)
+        return b
+
+    def scenario(self, kind: int, n: int, seed: int = 0) -> Bodies:
+        b = Bodies.empty(n)
+        rc = self.L.or_scenario(kind, n, seed, b.px, b.py, b.vx, b.vy, b.m, b.r, b.id)
+        assert rc == 0, rc
         return b
 
     def iterate(self, b: Bodies, dt: float = 0.05) -> None:
@@ -228,6 +236,8 @@ class Reference:
         L = C.CDLL(path)
         L.ref_uniform_disc.argtypes = [_u64, C.c_uint32] + [_f32p] * 6 + [_u32p]
         L.ref_uniform_disc.restype = _i64
+        L.ref_scenario.argtypes = [_int, _u64, C.c_uint32] + [_f32p] * 6 + [_u32p]
+        L.ref_scenario.restype = _i64
         L.ref_iterate.argtypes = [_u64, _f32] + [_f32p] * 6
         L.ref_build.argtypes = ([_u64, _f32, _f32] + [_f32p] * 6 + [_u32p, _i64]
                                 + [_f32p] * 6 + [_i32p] * 4 + [_f32p, _f32p])
@@ -240,6 +250,12 @@ class Reference:
         b = Bodies.empty(n)
         got = self.L.ref_uniform_disc(n, seed, b.px, b.py, b.vx, b.vy, b.m, b.r, b.id)
         assert got == n
+        return b
+
+    def scenario(self, kind: int, n: int, seed: int = 0) -> Bodies:
+        b = Bodies.empty(n)
+        got = self.L.ref_scenario(kind, n, seed, b.px, b.py, b.vx, b.vy, b.m, b.r, b.id)
+        assert got == n, got
         return b
 
     def iterate(self, b: Bodies, dt: float = 0.05) -> None:

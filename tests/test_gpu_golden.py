@@ -48,6 +48,33 @@ def test_disc_equals_reference(nbx):
         assert np.array_equal(bits(getattr(got, f)), bits(z[f])), f
 
 
+@pytest.mark.parametrize("name", ["scenario_k1_n1501_s2.npz", "scenario_k2_n1200_s9.npz"])
+def test_scenario_equals_reference(nbx, name):
+    z = load(name)
+    with nbx.Simulation(int(z["n"]), seed=int(z["seed"]), scenario=int(z["kind"])) as sim:
+        got = sim.download()
+    for f in ("px", "py", "vx", "vy", "m", "r"):
+        assert np.array_equal(bits(getattr(got, f)), bits(z[f])), f
+    assert np.array_equal(got.id, z["id"])
+
+
+def test_merger_equals_oracle(nbx, oracle):
+    want = oracle.scenario(3, 3001, 4)
+    with nbx.Simulation(3001, seed=4, scenario=nbx.SCENARIO_MERGER) as sim:
+        got = sim.download()
+        sim.step(2)
+    for f in ("px", "py", "vx", "vy", "m", "r"):
+        assert np.array_equal(bits(getattr(got, f)), bits(getattr(want, f))), f
+
+
+def test_scenario_rejects_bad_arguments(nbx):
+    import ctypes as C
+    with nbx.Simulation(10) as sim:
+        assert sim.L.nbx_init_scenario(sim.h, 9, 10, 0) == 1
+        assert sim.L.nbx_init_scenario(sim.h, nbx.SCENARIO_MERGER, 1, 0) == 1
+        assert sim.L.nbx_init_scenario(sim.h, nbx.SCENARIO_COLD_SQUARE, 11, 0) == 1
+
+
 @pytest.mark.parametrize("theta,tag", [(1.0, "t1"), (0.5, "t05")])
 def test_tree_and_forces_equal_reference(nbx, theta, tag):
     z = load("tree_n4000_k30.npz")
@@ -111,6 +138,22 @@ def test_cli_writes_nbs2(tmp_path, oracle):
     o = o.by_id()
     d = np.hypot(t.pos[3, :, 0] - o.px, t.pos[3, :, 1] - o.py)
     assert np.median(d / np.maximum(np.hypot(o.px, o.py), 1)) < 1e-5
+
+
+def test_cli_scenario(tmp_path, oracle):
+    out = tmp_path / "sq.bin"
+    env = dict(os.environ, NBX_SCENARIO="cold-square", NBX_SEED="9")
+    r = subprocess.run([CLI, "1200", "2", "1", str(out)], capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0, r.stderr
+    t = nbs2().read(str(out))
+    want = load("scenario_k2_n1200_s9.npz")
+    assert np.array_equal(bits(t.pos[0, :, 0]), bits(want["px"]))
+    assert np.array_equal(bits(t.pos[0, :, 1]), bits(want["py"]))
+    assert t.num_frames == 3 and t.completed_frames == 3
+    bad = subprocess.run([CLI, "10", "1", "1"], capture_output=True, text=True,
+                         env=dict(os.environ, NBX_SCENARIO="spiral"))
+    assert bad.returncode == 1 and "NBX_SCENARIO" in bad.stderr
 
 
 def test_million_body_invariants(nbx):

@@ -33,6 +33,15 @@ def test_disc_matches_reference(oracle, name):
     assert np.array_equal(b.id, z["id"])
 
 
+@pytest.mark.parametrize("name", ["scenario_k1_n1501_s2.npz", "scenario_k2_n1200_s9.npz"])
+def test_scenario_matches_reference(oracle, name):
+    z = load(name)
+    b = oracle.scenario(int(z["kind"]), int(z["n"]), int(z["seed"]))
+    for f in ("px", "py", "vx", "vy", "m", "r"):
+        assert same(getattr(b, f), z[f]), f
+    assert np.array_equal(b.id, z["id"])
+
+
 def test_tree_and_forces_match_reference(oracle):
     z = load("tree_n4000_k30.npz")
     b = bodies(z, "in_")

@@ -226,3 +226,35 @@ def test_exact_forces_three_body(oracle):
     ax, ay = oracle.exact_forces(b, np.array([0], np.uint32), eps2=1.0)
     # from body 1: 2 * (1,0) / (1 * 2); from body 2: 3 * (0,2) / (2 * 5)
     assert np.isclose(ax[0], 1.0) and np.isclose(ay[0], 0.6)
+
+
+@pytest.mark.parametrize("n", [2000, 2001])
+def test_merger_closed_form(oracle, n):
+    """The precision-study merger (not in the reference): its two halves are
+    uniformDisc(n/2, seed) and uniformDisc(n - n/2, seed + 1) shifted to
+    (-2R, 0) and (2R, 0), R = 5 sqrt(n/2), with bulk velocities
+    +/-0.15 v_c (cos 15deg, sin 15deg), v_c = sqrt((1e6 + n/2 - 1) / R)."""
+    seed = 3
+    g = oracle.scenario(3, n, seed)
+    half = n // 2
+    a, b = oracle.uniform_disc(half, seed), oracle.uniform_disc(n - half, seed + 1)
+    R = np.float32(5) * np.sqrt(np.float32(half))
+    vc = np.sqrt((np.float32(1e6) + np.float32(half) - np.float32(1)) / R)
+    ang = np.float32(15) * np.float32(np.pi) / np.float32(180)
+    ux, uy = np.float32(np.cos(ang)), np.float32(np.sin(ang))
+    s = np.float32(0.15) * vc
+    assert np.allclose(g.px[:half] - a.px, -2 * R, rtol=0, atol=2e-3 * R)
+    assert np.allclose(g.px[half:] - b.px, 2 * R, rtol=0, atol=2e-3 * R)
+    assert np.array_equal(g.py[:half], a.py) and np.array_equal(g.py[half:], b.py)
+    assert np.allclose(g.vx[:half] - a.vx, s * ux, atol=1e-3 * s)
+    assert np.allclose(g.vy[half:] - b.vy, -s * uy, atol=1e-3 * s)
+    assert np.array_equal(g.m[:half], a.m) and np.array_equal(g.r[half:], b.r)
+    assert np.array_equal(g.id, np.arange(n, dtype=np.uint32))
+    # Centres 4R apart; the bulk momenta cancel when the halves are equal.
+    assert g.px[0] == np.float32(-2 * R) and g.px[half] == np.float32(2 * R)
+    if n % 2 == 0:
+        p = np.array([np.sum(g.m.astype(np.float64) * g.vx), np.sum(g.m.astype(np.float64) * g.vy)])
+        pa = np.array([np.sum(a.m.astype(np.float64) * a.vx), np.sum(a.m.astype(np.float64) * a.vy)])
+        pb = np.array([np.sum(b.m.astype(np.float64) * b.vx), np.sum(b.m.astype(np.float64) * b.vy)])
+        # Only the discs' own (rotational) momenta remain.
+        assert np.allclose(p, pa + pb, rtol=1e-6, atol=1e-3 * s * np.sum(g.m))

@@ -34,6 +34,17 @@ def test_uniform_disc(oracle, reference, n, seed):
     assert np.array_equal(a.id, b.id)
 
 
+@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("n,seed", [(2000, 0), (2001, 7), (3, 5)])
+def test_scenarios(oracle, reference, kind, n, seed):
+    """makeScenario TwoGalaxies (1) and ColdSquare (2), Scenarios.cpp:29-62."""
+    a, b = oracle.scenario(kind, n, seed), reference.scenario(kind, n, seed)
+    for f in ("px", "py", "vx", "vy", "m", "r"):
+        assert same(getattr(a, f), getattr(b, f)), f
+    assert np.array_equal(a.id, b.id)
+    assert np.array_equal(a.id, np.arange(n, dtype=np.uint32))
+
+
 def test_iterate(oracle, reference, ref_states):
     s = ref_states["disc10k_40"]
     a, b = s.copy(), s.copy()

@@ -48,6 +48,15 @@ extern "C" {
 #define NBX_SUM_REFERENCE 0
 #define NBX_SUM_COMPENSATED 1
 
+/* Collision resolution: the reference CPU's (src/Collision.cpp:135-349, pairs
+ * one after another in pairLess order, each seeing the previous result) or
+ * the reference GPU's (src/cuda/CudaCollision.cu, CollisionKernels.cuh:
+ * every pair from the start-of-collide state, each body combining its own
+ * side of its contacts; grid cell 2 x median radius x 1.01). Both bit-exact
+ * against their reference. */
+#define NBX_COLLIDE_SEQUENTIAL 0
+#define NBX_COLLIDE_JACOBI 1
+
 typedef struct nbx_sim nbx_sim;
 
 /* Physics and execution parameters. Defaults (nbx_default_params) are the
@@ -62,7 +71,7 @@ typedef struct nbx_params {
   int32_t summation;  /* NBX_SUM_* */
   int32_t use_graph;  /* replay captured HIP graphs in nbx_step (1) or launch
                          eagerly (0) */
-  int32_t reserved;
+  int32_t collide_mode; /* NBX_COLLIDE_* */
 } nbx_params;
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -182,7 +191,10 @@ int nbx_download_keys(nbx_sim *sim, uint32_t *hi, uint32_t *lo,
  * number of connected contact components and the largest one's pair count.
  * The cell is the broadphase's own choice (1.01 x the 5/8 quantile of a
  * sample of box widths, where the reference uses the exact median); the pair
- * set does not depend on it. */
+ * set does not depend on it. After a Jacobi collide (NBX_COLLIDE_JACOBI):
+ * the reference GPU's cell, the number of overlapping box pairs
+ * (CudaCollider::lastAabbPairs), the number of bodies with a contact and the
+ * largest contact count of one body. */
 int nbx_collide_info(nbx_sim *sim, float *cell, int64_t *pairs,
                      int64_t *components, int64_t *largest);
 
@@ -192,6 +204,13 @@ int nbx_collide_info(nbx_sim *sim, float *cell, int64_t *pairs,
  * component appear in the reference's pairLess order. */
 int nbx_download_pairs(nbx_sim *sim, int64_t cap, int32_t *later,
                        int32_t *earlier, int32_t *comp, int64_t *count);
+
+/* After a Jacobi collide (CudaCollider::lastContacts, CudaCollision.cu:
+ * 309-312): per body, its number of contacts and of overlapping boxes in
+ * which it is the later box, indexed as the bodies were at that collide.
+ * NBX_ERR_STATE if the last collide ran in the sequential mode. */
+int nbx_download_contacts(nbx_sim *sim, int32_t *contacts,
+                          int32_t *aabb_as_later);
 
 /* Runs the force walk once more in counting mode over the current tree and
  * returns far-node (mass > 0) and body-pair interaction totals, plus the

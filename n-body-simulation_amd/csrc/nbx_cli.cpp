@@ -8,7 +8,8 @@
 //
 // The physics parameters the reference fixes at compile time can be set
 // from the environment: NBX_THETA, NBX_EPSILON, NBX_DT, NBX_COLLISIONS (0/1),
-// NBX_LAW (reference|accurate), NBX_SUM (reference|compensated). NBX_SCENARIO
+// NBX_LAW (reference|accurate), NBX_SUM (reference|compensated), NBX_COLLIDE
+// (sequential|jacobi: the reference CPU's or GPU's resolution). NBX_SCENARIO
 // (disc|two-galaxies|cold-square|merger) and NBX_SEED pick the initial
 // condition (default: the reference's uniformDisc, seed 0).
 
@@ -77,6 +78,10 @@ int main(int argc, char **argv) {
   }
   if (const char *v = std::getenv("NBX_LAW")) {
     p.law = std::strcmp(v, "accurate") == 0 ? NBX_LAW_ACCURATE : NBX_LAW_REFERENCE;
+  }
+  if (const char *v = std::getenv("NBX_COLLIDE")) {
+    p.collide_mode =
+        std::strcmp(v, "jacobi") == 0 ? NBX_COLLIDE_JACOBI : NBX_COLLIDE_SEQUENTIAL;
   }
   if (const char *v = std::getenv("NBX_SUM")) {
     p.summation =

@@ -30,6 +30,7 @@ enum : uint32_t {
   kStatusDeepTooLarge = 4u,
   kStatusComponentTooLarge = 8u,
   kStatusTreeInconsistent = 16u, // internal check failed (never expected)
+  kStatusJacobiTooManyLarge = 32u,
 };
 
 // Per-handle control block in device memory: everything the host would

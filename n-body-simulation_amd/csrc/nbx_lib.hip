@@ -10,6 +10,7 @@
 #include "nbx.h"
 
 #include "k_collide.h"
+#include "k_jacobi.h"
 #include "k_sort.h"
 #include "k_tree.h"
 #include "k_walk.h"
@@ -73,6 +74,10 @@ struct nbx_sim {
   uint64_t frame = 0;
   nbx_params params{};
   double massAbsSum = 0; // sum of |m| of the bodies held (masses never change)
+  float jacobiCell = 1.0f; // reference GPU collide cell (radii never change)
+  JacAcc *jacPartials = nullptr;
+  uint32_t jacMaxLarge = 0;
+  bool jacobiCollided = false; // last collide ran in the Jacobi mode
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
 
@@ -224,11 +229,72 @@ int launchScan(nbx_sim *s, uint32_t count, const uint32_t *in, uint32_t *out) {
   return NBX_OK;
 }
 
+// Stable LSD radix sort of 32-bit keys with 32-bit payloads, 4 x 8 bits,
+// from buffers 0 to buffers 0 (through 1).
+int launchRadixSort(nbx_sim *s, uint32_t n, uint32_t *const keys[2],
+                    uint32_t *const vals[2]) {
+  const uint32_t tiles = blocksFor(n, kSortTile);
+  for (int pass = 0; pass < 4; ++pass) {
+    const int src = pass & 1, dst = src ^ 1;
+    LAUNCH(k_rs_hist, tiles, kBlock, n, keys[src], 8 * pass, s->sortHist,
+           tiles);
+    LAUNCH(k_rs_scan, 256, kBlock, tiles, s->sortHist, s->sortRowTotal);
+    LAUNCH(k_rs_scatter, tiles, kBlock, n, keys[src], vals[src], 8 * pass,
+           s->sortHist, s->sortRowTotal, tiles, keys[dst], vals[dst]);
+  }
+  return NBX_OK;
+}
+
+// The reference GPU collide (Jacobi resolution, k_jacobi.h). Scratch comes
+// from buffers the sequential collide and the build use at other times:
+// keyHi/vals (the two sorts), keyLo (high key words), gridBox (sorted 64-bit
+// keys), gridCell (staged results), bucketOf/bucketRank (per-body contact
+// counts, kept for nbx_download_contacts).
+int launchCollideJacobi(nbx_sim *s) {
+  const uint32_t n = static_cast<uint32_t>(s->n);
+  const int c = s->cur;
+  float *px = s->px[c], *py = s->py[c], *vx = s->vx[c], *vy = s->vy[c];
+  const float *m = s->m[c], *r = s->r[c];
+  const uint32_t nb = blocksFor(n);
+  // Keys: low words into keyHi[0] (sorted first), high words into keyLo.
+  LAUNCH(k_jac_keys, nb, kBlock, n, px, py, r, s->jacobiCell, s->ctl,
+         s->keyHi[0], s->keyLo, s->vals[0]);
+  NBX_TRY(launchRadixSort(s, n, s->keyHi, s->vals));
+  LAUNCH(k_jac_gather_hi, nb, kBlock, n, s->vals[0], s->keyLo, s->keyHi[0]);
+  NBX_TRY(launchRadixSort(s, n, s->keyHi, s->vals));
+  uint64_t *keysSorted = reinterpret_cast<uint64_t *>(s->gridBox);
+  const uint32_t *indexSorted = s->vals[0];
+  float4 *out = reinterpret_cast<float4 *>(s->gridCell);
+  int32_t *contacts = reinterpret_cast<int32_t *>(s->bucketOf);
+  int32_t *asLater = reinterpret_cast<int32_t *>(s->bucketRank);
+  LAUNCH(k_jac_sorted, nb, kBlock, n, px, py, r, indexSorted, s->ctl,
+         keysSorted);
+  LAUNCH(k_jac_small, nb, kBlock, n, s->ctl, keysSorted, indexSorted, px, py,
+         vx, vy, m, r, out, contacts, asLater);
+  const uint32_t chunks = blocksFor(n, kJacChunk);
+  const uint32_t ys = std::min<uint32_t>(64, std::max<uint32_t>(1, s->jacMaxLarge));
+  hipLaunchKernelGGL(k_jac_large_partial, dim3(chunks, ys), dim3(kJacLanes), 0,
+                     s->stream, n, chunks, s->jacMaxLarge, s->ctl, indexSorted,
+                     px, py, vx, vy, m, r, s->jacPartials);
+  NBX_TRY(checkLaunch());
+  LAUNCH(k_jac_large_combine, 16, kBlock, n, chunks, s->jacMaxLarge, s->ctl,
+         indexSorted, px, py, vx, vy, m, r, s->jacPartials, out, contacts,
+         asLater);
+  LAUNCH(k_jac_commit, nb, kBlock, n, s->ctl, out, px, py, vx, vy);
+  s->jacobiCollided = true;
+  s->collided = true;
+  return NBX_OK;
+}
+
 int launchCollide(nbx_sim *s) {
   const uint32_t n = static_cast<uint32_t>(s->n);
   if (n < 2) {
     return NBX_OK;
   }
+  if (s->params.collide_mode == NBX_COLLIDE_JACOBI) {
+    return launchCollideJacobi(s);
+  }
+  s->jacobiCollided = false;
   const int c = s->cur;
   float *px = s->px[c], *py = s->py[c], *vx = s->vx[c], *vy = s->vy[c];
   const float *m = s->m[c], *r = s->r[c];
@@ -277,17 +343,8 @@ int launchBuild(nbx_sim *s) {
   LAUNCH(k_bbox, parts, kBlock, n, s->px[c], s->py[c], s->bboxParts);
   LAUNCH(k_keys, nb, kBlock, n, s->px[c], s->py[c], s->bboxParts, parts, s->ctl,
          s->keyHi[0], s->keyLo, s->vals[0]);
-  // Stable LSD radix sort of the high key word, 4 x 8 bits; ends in buffer 0.
-  const uint32_t tiles = blocksFor(n, kSortTile);
-  for (int pass = 0; pass < 4; ++pass) {
-    const int src = pass & 1, dst = src ^ 1;
-    LAUNCH(k_rs_hist, tiles, kBlock, n, s->keyHi[src], 8 * pass, s->sortHist,
-           tiles);
-    LAUNCH(k_rs_scan, 256, kBlock, tiles, s->sortHist, s->sortRowTotal);
-    LAUNCH(k_rs_scatter, tiles, kBlock, n, s->keyHi[src], s->vals[src],
-           8 * pass, s->sortHist, s->sortRowTotal, tiles, s->keyHi[dst],
-           s->vals[dst]);
-  }
+  // Stable LSD radix sort of the high key word; ends in buffer 0.
+  NBX_TRY(launchRadixSort(s, n, s->keyHi, s->vals));
   const uint32_t *hi = s->keyHi[0];
   uint32_t *perm = s->vals[0];
   LAUNCH(k_deep_detect, nb, kBlock, n, hi, s->ctl, s->deepList);
@@ -518,6 +575,10 @@ int syncAndCheck(nbx_sim *s) {
   if (status & kStatusTreeInconsistent) {
     return setError(NBX_ERR_STATE, "internal tree consistency check failed");
   }
+  if (status & kStatusJacobiTooManyLarge) {
+    return setError(NBX_ERR_CAPACITY,
+                    "too many large boxes for the Jacobi collide's partial sums");
+  }
   if (status & kStatusComponentTooLarge) {
     return setError(NBX_ERR_CAPACITY,
                     "a collision contact component exceeds 4096 pairs");
@@ -567,6 +628,10 @@ int validParams(const nbx_params *p) {
   if (p->summation != NBX_SUM_REFERENCE &&
       p->summation != NBX_SUM_COMPENSATED) {
     return setError(NBX_ERR_ARG, "unknown summation mode");
+  }
+  if (p->collide_mode != NBX_COLLIDE_SEQUENTIAL &&
+      p->collide_mode != NBX_COLLIDE_JACOBI) {
+    return setError(NBX_ERR_ARG, "unknown collide mode");
   }
   return NBX_OK;
 }
@@ -653,6 +718,12 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->bodyPack, n + kBodyPad, true)
   A(s->bboxParts, kBboxParts, false)
   A(s->ctl, 1, true)
+  {
+    const uint64_t chunks = blocksFor(n, kJacChunk);
+    const uint64_t entries = kJacPartialBytes / sizeof(JacAcc);
+    s->jacMaxLarge = static_cast<uint32_t>(std::max<uint64_t>(1, entries / chunks));
+    A(s->jacPartials, s->jacMaxLarge * chunks, false)
+  }
 #undef A
   if (hipDeviceSynchronize() != hipSuccess) {
     return fail(setError(NBX_ERR_HIP, "hipDeviceSynchronize failed"));
@@ -703,6 +774,17 @@ int nbx_upload(nbx_sim *s, uint64_t n, const float *px, const float *py,
   for (uint64_t k = 0; k < n; ++k) {
     s->massAbsSum += std::fabs(static_cast<double>(m[k]));
   }
+  // CudaCollider's cell (CudaCollision.cu:197-209): 2 x median radius x 1.01.
+  s->jacobiCell = 1.0f;
+  if (n > 0) {
+    std::vector<float> radii(r, r + n);
+    std::nth_element(radii.begin(), radii.begin() + n / 2, radii.end());
+    s->jacobiCell = 2.0f * radii[n / 2] * 1.01f;
+  }
+  if (!(s->jacobiCell > 0.0f)) {
+    s->jacobiCell = 1.0f;
+  }
+  s->jacobiCollided = false;
   NBX_TRY(copyIn(s, s->px[0], px, n));
   NBX_TRY(copyIn(s, s->py[0], py, n));
   NBX_TRY(copyIn(s, s->vx[0], vx, n));
@@ -1012,10 +1094,40 @@ int nbx_collide_info(nbx_sim *s, float *cell, int64_t *pairs,
   Ctl c{};
   NBX_TRY(readCtl(s, &c));
   const bool ok = s->collided && s->n >= 2;
+  if (ok && s->jacobiCollided) {
+    // CudaCollider::lastAabbPairs and its per-body contact counts.
+    std::vector<int32_t> cnt(s->n), later(s->n);
+    NBX_TRY(copyOut(s, cnt.data(), reinterpret_cast<int32_t *>(s->bucketOf), s->n));
+    NBX_TRY(copyOut(s, later.data(), reinterpret_cast<int32_t *>(s->bucketRank), s->n));
+    int64_t p = 0, touched = 0, most = 0;
+    for (uint64_t i = 0; i < s->n; ++i) {
+      p += later[i];
+      touched += cnt[i] > 0;
+      most = std::max<int64_t>(most, cnt[i]);
+    }
+    if (cell) *cell = c.cell;
+    if (pairs) *pairs = p;
+    if (components) *components = touched;
+    if (largest) *largest = most;
+    return NBX_OK;
+  }
   if (cell) *cell = ok ? c.cell : 0.0f;
   if (pairs) *pairs = ok ? c.pairCount : 0;
   if (components) *components = ok ? c.compCount : 0;
   if (largest) *largest = ok ? c.largest : 0;
+  return NBX_OK;
+}
+
+int nbx_download_contacts(nbx_sim *s, int32_t *contacts, int32_t *aabb_as_later) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(syncAndCheck(s));
+  if (!s->jacobiCollided) {
+    return setError(NBX_ERR_STATE, "the last collide did not run in the Jacobi mode");
+  }
+  NBX_TRY(copyOut(s, contacts, reinterpret_cast<int32_t *>(s->bucketOf), s->n));
+  NBX_TRY(copyOut(s, aabb_as_later, reinterpret_cast<int32_t *>(s->bucketRank), s->n));
   return NBX_OK;
 }
 
@@ -1030,6 +1142,10 @@ int nbx_download_pairs(nbx_sim *s, int64_t cap, int32_t *later,
   if (!s->collided || s->n < 2) {
     if (count) *count = 0;
     return NBX_OK;
+  }
+  if (s->jacobiCollided) {
+    return setError(NBX_ERR_STATE,
+                    "the Jacobi collide does not list pairs (see nbx_download_contacts)");
   }
   const uint64_t total = c.compPairs;
   std::vector<int4> info(c.compCount);

@@ -36,8 +36,10 @@ EXPORTS = (
     "nbx_forces", "nbx_frame", "nbx_sync", "nbx_tree_info", "nbx_download_tree",
     "nbx_root_quad", "nbx_download_keys", "nbx_collide_info", "nbx_download_pairs",
     "nbx_count_interactions", "nbx_set_profiling", "nbx_phase_times", "nbx_stream",
-    "nbx_exact_forces", "nbx_init_scenario",
+    "nbx_exact_forces", "nbx_init_scenario", "nbx_download_contacts",
 )
+
+COLLIDE_SEQUENTIAL, COLLIDE_JACOBI = 0, 1
 
 SCENARIO_DISC, SCENARIO_TWO_GALAXIES, SCENARIO_COLD_SQUARE, SCENARIO_MERGER = 0, 1, 2, 3
 
@@ -54,7 +56,7 @@ class Params(C.Structure):
     _fields_ = [("dt", C.c_float), ("theta", C.c_float), ("epsilon", C.c_float),
                 ("collisions", C.c_int32), ("law", C.c_int32),
                 ("summation", C.c_int32), ("use_graph", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("collide_mode", C.c_int32)]
 
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -81,6 +83,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_destroy.argtypes = [_vp]
     L.nbx_init_disc.argtypes = [_vp, C.c_uint64, C.c_uint32]
     L.nbx_init_scenario.argtypes = [_vp, C.c_int32, C.c_uint64, C.c_uint32]
+    L.nbx_download_contacts.argtypes = [_vp, _vp, _vp]
     L.nbx_upload.argtypes = [_vp, C.c_uint64] + [_vp] * 9
     L.nbx_download.argtypes = [_vp] + [_vp] * 9
     L.nbx_download_by_id.argtypes = [_vp] + [_vp] * 6
@@ -155,6 +158,7 @@ class Simulation:
         self.dt, self.theta, self.epsilon = p.dt, p.theta, p.epsilon
         self.collisions = bool(p.collisions)
         self.law, self.summation, self.use_graph = p.law, p.summation, bool(p.use_graph)
+        self.collide_mode = p.collide_mode
         if state is not None:
             self.upload(state)
         elif scenario == SCENARIO_DISC:
@@ -169,7 +173,8 @@ class Simulation:
 
     def _push_params(self) -> None:
         p = Params(self.dt, self.theta, self.epsilon, int(bool(self.collisions)),
-                   int(self.law), int(self.summation), int(bool(self.use_graph)), 0)
+                   int(self.law), int(self.summation), int(bool(self.use_graph)),
+                   int(self.collide_mode))
         self._check(self.L.nbx_set_params(self.h, C.byref(p)))
 
     def close(self) -> None:
@@ -294,6 +299,14 @@ class Simulation:
                                             C.byref(comps), C.byref(largest)))
         return dict(cell=cell.value, pairs=pairs.value, components=comps.value,
                     largest=largest.value)
+
+    def contacts(self):
+        """After a Jacobi collide: per-body contact counts and later-box
+        overlap counts (device order at that collide)."""
+        n = self.n
+        c, l = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self._check(self.L.nbx_download_contacts(self.h, _ptr(c), _ptr(l)))
+        return c, l
 
     def pairs(self):
         info = self.collide_info()

@@ -618,6 +618,281 @@ void or_accelerations(uint64_t n, const float *px, const float *py,
 }
 
 // Collision.cpp:135-349.
+// ---- the reference GPU collide: Jacobi resolution ----
+namespace {
+struct JBody {
+  float px, py, vx, vy, m, r;
+};
+struct JBox {
+  float minX, maxX, minY, maxY;
+  int index;
+};
+struct JAcc {
+  float dpx = 0, dpy = 0, dvx = 0, dvy = 0, npx = 0, npy = 0, nvx = 0,
+        nvy = 0;
+  int contacts = 0, asLater = 0;
+};
+const uint64_t kJLargeKey = ~0ull;
+
+JBox jbox(int i, const float *px, const float *py, const float *r) {
+  return JBox{px[i] - r[i], px[i] + r[i], py[i] - r[i], py[i] + r[i], i};
+}
+bool jbefore(const JBox &a, const JBox &b) {
+  return a.minX < b.minX || (a.minX == b.minX && a.index < b.index);
+}
+bool joverlaps(const JBox &e, const JBox &l) {
+  return e.maxX >= l.minX && l.minY <= e.maxY && e.minY <= l.maxY;
+}
+// CollisionKernels.cuh:50-55: float floor, clamped to +-2^29 (NaN too).
+int jcell(float v, float invCell) {
+  return static_cast<int>(
+      std::fmin(std::fmax(std::floor(v * invCell), -536870912.0f), 536870912.0f));
+}
+uint64_t jkey(int cx, int cy) {
+  const int64_t bias = 1ll << 30;
+  return (static_cast<uint64_t>(cy + bias) << 32) |
+         static_cast<uint64_t>(cx + bias);
+}
+
+// resolvePair (CollisionKernels.cuh:104-187): body 1 = the later box.
+bool jresolve(const JBody &a, const JBody &b, float &p1x, float &p1y,
+              float &v1x, float &v1y, float &p2x, float &p2y, float &v2x,
+              float &v2y) {
+  float dx = b.px - a.px, dy = b.py - a.py;
+  const float rr = a.r + b.r;
+  if (dx * dx + dy * dy > rr * rr) {
+    return false;
+  }
+  const float vx = b.vx - a.vx, vy = b.vy - a.vy;
+  float dv = dx * vx + dy * vy;
+  const float w1 = b.m / (a.m + b.m), w2 = a.m / (a.m + b.m);
+  const bool nz = dx != 0.0f || dy != 0.0f;
+  v1x = a.vx;
+  v1y = a.vy;
+  v2x = b.vx;
+  v2y = b.vy;
+  if (dv >= 0.0f && nz) {
+    const float sc = rr / std::sqrt(dx * dx + dy * dy) - 1.0f;
+    const float tx = dx * sc, ty = dy * sc;
+    p1x = a.px - w1 * tx;
+    p1y = a.py - w1 * ty;
+    p2x = b.px + w2 * tx;
+    p2y = b.py + w2 * ty;
+    return true;
+  }
+  const float vSq = vx * vx + vy * vy;
+  if (vSq < 1e-12f) {
+    float ux = 1.0f, uy = 0.0f;
+    if (nz) {
+      const float inv = 1.0f / std::sqrt(dx * dx + dy * dy);
+      ux = dx * inv;
+      uy = dy * inv;
+    }
+    const float tx = ux * rr, ty = uy * rr;
+    p1x = a.px - w1 * tx;
+    p1y = a.py - w1 * ty;
+    p2x = b.px + w2 * tx;
+    p2y = b.py + w2 * ty;
+    return true;
+  }
+  const float dSq = dx * dx + dy * dy, rSq = rr * rr;
+  const float t =
+      (dv + std::sqrt(std::fmax(dv * dv - vSq * (dSq - rSq), 0.0f))) / vSq;
+  const float q1x = a.px - a.vx * t, q1y = a.py - a.vy * t;
+  const float q2x = b.px - b.vx * t, q2y = b.py - b.vy * t;
+  dx = q2x - q1x;
+  dy = q2y - q1y;
+  dv = dx * vx + dy * vy;
+  const float sc = 1.5f * dv / (dx * dx + dy * dy);
+  const float tx = dx * sc, ty = dy * sc;
+  v1x = a.vx + tx * w1;
+  v1y = a.vy + ty * w1;
+  v2x = b.vx - tx * w2;
+  v2y = b.vy - ty * w2;
+  p1x = q1x + v1x * t;
+  p1y = q1y + v1y * t;
+  p2x = q2x + v2x * t;
+  p2y = q2y + v2y * t;
+  return true;
+}
+
+// accumulatePair (CollisionKernels.cuh:199-236): a's side of pair (a, b).
+void jaccumulate(const JBox &boxA, const JBody &sa, int b, const float *px,
+                 const float *py, const float *vx, const float *vy,
+                 const float *m, const float *r, JAcc &acc) {
+  const JBox boxB = jbox(b, px, py, r);
+  const bool aLater = jbefore(boxB, boxA);
+  if (!(aLater ? joverlaps(boxB, boxA) : joverlaps(boxA, boxB))) {
+    return;
+  }
+  if (aLater) {
+    ++acc.asLater;
+  }
+  const JBody sb{px[b], py[b], vx[b], vy[b], m[b], r[b]};
+  float p1x, p1y, v1x, v1y, p2x, p2y, v2x, v2y;
+  const bool hit = aLater
+                       ? jresolve(sa, sb, p1x, p1y, v1x, v1y, p2x, p2y, v2x, v2y)
+                       : jresolve(sb, sa, p1x, p1y, v1x, v1y, p2x, p2y, v2x, v2y);
+  if (!hit) {
+    return;
+  }
+  const float qx = aLater ? p1x : p2x, qy = aLater ? p1y : p2y;
+  const float ux = aLater ? v1x : v2x, uy = aLater ? v1y : v2y;
+  ++acc.contacts;
+  acc.dpx += qx - sa.px;
+  acc.dpy += qy - sa.py;
+  acc.dvx += ux - sa.vx;
+  acc.dvy += uy - sa.vy;
+  acc.npx += qx;
+  acc.npy += qy;
+  acc.nvx += ux;
+  acc.nvy += uy;
+}
+
+void jadd(JAcc &t, const JAcc &p) {
+  t.dpx += p.dpx;
+  t.dpy += p.dpy;
+  t.dvx += p.dvx;
+  t.dvy += p.dvy;
+  t.npx += p.npx;
+  t.npy += p.npy;
+  t.nvx += p.nvx;
+  t.nvy += p.nvy;
+  t.contacts += p.contacts;
+  t.asLater += p.asLater;
+}
+} // namespace
+
+float or_jacobi_cell(uint64_t n, const float *r) {
+  float cell = 1.0f;
+  if (n > 0) {
+    std::vector<float> radii(r, r + n);
+    std::nth_element(radii.begin(), radii.begin() + n / 2, radii.end());
+    cell = 2.0f * radii[n / 2] * 1.01f;
+  }
+  if (!(cell > 0.0f)) {
+    cell = 1.0f;
+  }
+  return cell;
+}
+
+int64_t or_collide_jacobi(uint64_t n, float cell, float *px, float *py,
+                          float *vx, float *vy, const float *m, const float *r,
+                          int32_t *contacts, int32_t *aabb_as_later) {
+  if (n < 2) {
+    return 0;
+  }
+  const int N = static_cast<int>(n);
+  const float invCell = 1.0f / cell;
+  std::vector<uint64_t> key(n);
+  int64_t large = 0;
+  for (int i = 0; i < N; ++i) {
+    const JBox b = jbox(i, px, py, r);
+    if (b.maxX - b.minX > cell || b.maxY - b.minY > cell) {
+      key[i] = kJLargeKey;
+      ++large;
+    } else {
+      key[i] = jkey(jcell(b.minX, invCell), jcell(b.minY, invCell));
+    }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < N; ++i) {
+    idx[i] = i;
+  }
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](int a, int b) { return key[a] < key[b]; });
+  std::vector<uint64_t> ks(n);
+  for (int i = 0; i < N; ++i) {
+    ks[i] = key[idx[i]];
+  }
+  const int nSmall = N - static_cast<int>(large);
+  std::vector<float> opx(px, px + n), opy(py, py + n), ovx(vx, vx + n),
+      ovy(vy, vy + n);
+  std::vector<int32_t> cnt(n, 0), later(n, 0);
+  auto write = [&](int a, const JAcc &acc) {
+    float qx = px[a], qy = py[a], ux = vx[a], uy = vy[a];
+    if (acc.contacts == 1) {
+      qx = acc.npx;
+      qy = acc.npy;
+      ux = acc.nvx;
+      uy = acc.nvy;
+    } else if (acc.contacts > 1) {
+      qx = px[a] + acc.dpx;
+      qy = py[a] + acc.dpy;
+      ux = vx[a] + acc.dvx;
+      uy = vy[a] + acc.dvy;
+    }
+    opx[a] = qx;
+    opy[a] = qy;
+    ovx[a] = ux;
+    ovy[a] = uy;
+    cnt[a] = acc.contacts;
+    later[a] = acc.asLater;
+  };
+  // Small boxes (smallBodyContacts, CollisionKernels.cuh:241-264).
+  for (int s = 0; s < nSmall; ++s) {
+    const int a = idx[s];
+    const JBox boxA = jbox(a, px, py, r);
+    const JBody sa{px[a], py[a], vx[a], vy[a], m[a], r[a]};
+    const int cx = jcell(boxA.minX, invCell), cy = jcell(boxA.minY, invCell);
+    JAcc acc;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const uint64_t last = jkey(cx + 1, cy + dy);
+      const uint64_t first = jkey(cx - 1, cy + dy);
+      int e = static_cast<int>(
+          std::lower_bound(ks.begin(), ks.begin() + nSmall, first) - ks.begin());
+      for (; e < nSmall && ks[e] <= last; ++e) {
+        if (idx[e] != a) {
+          jaccumulate(boxA, sa, idx[e], px, py, vx, vy, m, r, acc);
+        }
+      }
+    }
+    for (int e = nSmall; e < N; ++e) {
+      jaccumulate(boxA, sa, idx[e], px, py, vx, vy, m, r, acc);
+    }
+    write(a, acc);
+  }
+  // Large boxes: chunks of 4096 bodies, 128 strided lanes, tree-reduced
+  // (largeBodyPartialKernel), chunks summed in order (largeBodyCombineKernel).
+  const int lanes = 128, chunk = 4096;
+  const int chunks = (N + chunk - 1) / chunk;
+  for (int l = 0; l < static_cast<int>(large); ++l) {
+    const int a = idx[nSmall + l];
+    const JBox boxA = jbox(a, px, py, r);
+    const JBody sa{px[a], py[a], vx[a], vy[a], m[a], r[a]};
+    JAcc total;
+    for (int c = 0; c < chunks; ++c) {
+      const int begin = c * chunk, end = std::min(N, begin + chunk);
+      std::vector<JAcc> part(lanes);
+      for (int t = 0; t < lanes; ++t) {
+        for (int b = begin + t; b < end; b += lanes) {
+          if (b != a) {
+            jaccumulate(boxA, sa, b, px, py, vx, vy, m, r, part[t]);
+          }
+        }
+      }
+      for (int stride = lanes / 2; stride > 0; stride >>= 1) {
+        for (int t = 0; t < stride; ++t) {
+          jadd(part[t], part[t + stride]);
+        }
+      }
+      jadd(total, part[0]);
+    }
+    write(a, total);
+  }
+  std::copy(opx.begin(), opx.end(), px);
+  std::copy(opy.begin(), opy.end(), py);
+  std::copy(ovx.begin(), ovx.end(), vx);
+  std::copy(ovy.begin(), ovy.end(), vy);
+  if (contacts) {
+    std::copy(cnt.begin(), cnt.end(), contacts);
+  }
+  if (aabb_as_later) {
+    std::copy(later.begin(), later.end(), aabb_as_later);
+  }
+  return large;
+}
+
 int64_t or_collide(uint64_t n, float *px, float *py, float *vx, float *vy,
                    const float *m, const float *r, int32_t *later,
                    int32_t *earlier, int64_t pair_cap, float *cell_out) {

@@ -29,6 +29,9 @@
  *                      limit vs the FP64 direct sum; fastInvSqrt exhaustive
  *   or_collide         reference collide() output bit-exact; pair set vs an
  *                      O(N^2) brute-force sweep in numpy
+ *   or_collide_jacobi  reference GPU collide semantics: bit-exact against a
+ *                      host build of the reference's CollisionKernels.cuh
+ *                      (oracle/_ref) driven in CudaCollision.cu's order
  *   or_step            reference Simulation::step (per phase), invariants
  *   or_exact_forces    closed-form two/three-body cases
  */
@@ -108,6 +111,26 @@ void or_accelerations(uint64_t n, const float *px, const float *py,
 int64_t or_collide(uint64_t n, float *px, float *py, float *vx, float *vy,
                    const float *m, const float *r, int32_t *later,
                    int32_t *earlier, int64_t pair_cap, float *cell_out);
+
+/* The reference GPU collide (src/cuda/CudaCollision.cu:205-296 with
+ * src/cuda/CollisionKernels.cuh:104-289): every pair resolved from the
+ * start-of-collide state, each body combining its own side of its contacts
+ * (one contact: that pair's result; several: start + sum of changes), in the
+ * kernels' scan and summation order: small boxes over the 3x3 stencil of a
+ * row-major 64-bit cell key sorted stably with index payload, then every
+ * large box; each large box against chunks of 4096 bodies scanned by 128
+ * strided lanes, a fixed tree reduction per chunk, chunks summed in order.
+ * `cell` is the grid cell (the reference: 2 x median radius x 1.01, fixed at
+ * construction). contacts / aabb_as_later (may be null) receive per body the
+ * number of contacts and of overlapping boxes where it is the later box.
+ * Returns the number of large boxes. */
+int64_t or_collide_jacobi(uint64_t n, float cell, float *px, float *py,
+                          float *vx, float *vy, const float *m, const float *r,
+                          int32_t *contacts, int32_t *aabb_as_later);
+
+/* CudaCollider's cell (CudaCollision.cu:197-209): 2 x the median radius
+ * (nth_element at n/2) x 1.01; 1 when that is not positive. */
+float or_jacobi_cell(uint64_t n, const float *r);
 
 /* Simulation.cpp:11-16 step(): iterate -> collide (if enabled) -> build ->
  * accelerations. Bodies are permuted into key order by the build (all nine

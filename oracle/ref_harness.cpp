@@ -8,6 +8,7 @@
 // reads it by compiling the header with `private` mapped to `public`, which
 // changes no layout.
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -19,6 +20,7 @@
 #include "Simulation.h"
 #include "Utils.h"
 #include "viewer/Scenarios.h"
+#include "cuda/CollisionKernels.cuh"
 
 namespace {
 
@@ -83,6 +85,123 @@ int64_t ref_scenario(int kind, uint64_t n, uint32_t seed, float *px, float *py,
   }
   scatter(b, px, py, vx, vy, nullptr, nullptr, m, r, id);
   return static_cast<int64_t>(b.size());
+}
+
+// The reference GPU collide run on the host: the per-body functions of
+// src/cuda/CollisionKernels.cuh (compiled here as plain C++, as that header
+// intends), driven the way CudaCollision.cu's kernels drive them: keyKernel,
+// a stable sort of (key, index), smallBodyKernel for every small box, and the
+// default split large-box path (largeBodyPartialKernel with its 128-lane
+// fixed tree reduction per 4096-body chunk, then largeBodyCombineKernel).
+// The cell is CudaCollider's: 2 x median radius x 1.01.
+int64_t ref_collide_jacobi(uint64_t n, float *px, float *py, float *vx,
+                           float *vy, const float *m, const float *r,
+                           int32_t *contacts, int32_t *aabb, float *cell_out) {
+  using namespace collision_detail;
+  std::vector<float> radii(r, r + n);
+  float cell = 1.0f;
+  if (!radii.empty()) {
+    std::nth_element(radii.begin(), radii.begin() + radii.size() / 2,
+                     radii.end());
+    cell = 2.0f * radii[radii.size() / 2] * 1.01f;
+  }
+  if (!(cell > 0.0f)) {
+    cell = 1.0f;
+  }
+  *cell_out = cell;
+  const int N = static_cast<int>(n);
+  if (N < 2) {
+    return 0;
+  }
+  const float invCell = 1.0f / cell;
+  std::vector<unsigned long long> keys(n);
+  int largeCount = 0;
+  for (int i = 0; i < N; ++i) {
+    keys[i] = boxKey(makeBox(i, px, py, r), cell, invCell);
+    largeCount += keys[i] == kLargeKey;
+  }
+  std::vector<int> order(n);
+  for (int i = 0; i < N; ++i) {
+    order[i] = i;
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return keys[a] < keys[b]; });
+  std::vector<unsigned long long> keysSorted(n);
+  for (int i = 0; i < N; ++i) {
+    keysSorted[i] = keys[order[i]];
+  }
+  std::vector<float> opx(n), opy(n), ovx(n), ovy(n);
+  std::vector<int> cnt(n), later(n);
+  const BodyArrays bodies{px, py, vx, vy, m, r};
+  const int nSmall = N - largeCount;
+  for (int s = 0; s < nSmall; ++s) {
+    const int a = order[s];
+    const Accum acc = smallBodyContacts(s, N, nSmall, invCell,
+                                        keysSorted.data(), order.data(), bodies);
+    writeResult(a, loadState(a, bodies), acc, opx.data(), opy.data(),
+                ovx.data(), ovy.data(), cnt.data(), later.data());
+  }
+  const int lanes = 128, chunk = 4096;
+  const int chunks = (N + chunk - 1) / chunk;
+  for (int l = 0; l < largeCount; ++l) {
+    const int a = order[nSmall + l];
+    const Box boxA = makeBox(a, px, py, r);
+    const BodyState sa = loadState(a, bodies);
+    Accum total;
+    for (int c = 0; c < chunks; ++c) {
+      const int begin = c * chunk, end = std::min(N, begin + chunk);
+      std::vector<Accum> mine(lanes);
+      for (int t = 0; t < lanes; ++t) {
+        for (int b = begin + t; b < end; b += lanes) {
+          if (b != a) {
+            accumulatePair(boxA, sa, b, bodies, mine[t]);
+          }
+        }
+      }
+      for (int stride = lanes / 2; stride > 0; stride >>= 1) {
+        for (int t = 0; t < stride; ++t) {
+          Accum &x = mine[t];
+          const Accum &y = mine[t + stride];
+          x.dpx += y.dpx;
+          x.dpy += y.dpy;
+          x.dvx += y.dvx;
+          x.dvy += y.dvy;
+          x.npx += y.npx;
+          x.npy += y.npy;
+          x.nvx += y.nvx;
+          x.nvy += y.nvy;
+          x.contacts += y.contacts;
+          x.aabbAsLater += y.aabbAsLater;
+        }
+      }
+      const Accum &p = mine[0];
+      total.dpx += p.dpx;
+      total.dpy += p.dpy;
+      total.dvx += p.dvx;
+      total.dvy += p.dvy;
+      total.npx += p.npx;
+      total.npy += p.npy;
+      total.nvx += p.nvx;
+      total.nvy += p.nvy;
+      total.contacts += p.contacts;
+      total.aabbAsLater += p.aabbAsLater;
+    }
+    writeResult(a, sa, total, opx.data(), opy.data(), ovx.data(), ovy.data(),
+                cnt.data(), later.data());
+  }
+  for (int i = 0; i < N; ++i) {
+    px[i] = opx[i];
+    py[i] = opy[i];
+    vx[i] = ovx[i];
+    vy[i] = ovy[i];
+    if (contacts) {
+      contacts[i] = cnt[i];
+    }
+    if (aabb) {
+      aabb[i] = later[i];
+    }
+  }
+  return largeCount;
 }
 
 // Body::update on every body.

@@ -128,6 +128,10 @@ class Oracle:
                                       _f64p, _f64p]
         L.or_fast_inv_sqrt_bits.argtypes = [_u64, _u32p, _u32p]
         L.or_scenario.argtypes = [_int, _u64, C.c_uint32] + [_f32p] * 6 + [_u32p]
+        L.or_collide_jacobi.argtypes = [_u64, _f32] + [_f32p] * 6 + [_i32p, _i32p]
+        L.or_collide_jacobi.restype = _i64
+        L.or_jacobi_cell.argtypes = [_u64, _f32p]
+        L.or_jacobi_cell.restype = _f32
         L.or_scenario.restype = _int
         self.L = L
 
@@ -141,6 +145,18 @@ class Oracle:
         rc = self.L.or_scenario(kind, n, seed, b.px, b.py, b.vx, b.vy, b.m, b.r, b.id)
         assert rc == 0, rc
         return b
+
+    def jacobi_cell(self, b: Bodies) -> float:
+        return float(self.L.or_jacobi_cell(b.n, b.r))
+
+    def collide_jacobi(self, b: Bodies, cell: float | None = None):
+        """In place on b; returns (contacts, aabb_as_later, large count)."""
+        c = self.jacobi_cell(b) if cell is None else cell
+        contacts = np.zeros(b.n, np.int32)
+        later = np.zeros(b.n, np.int32)
+        large = self.L.or_collide_jacobi(b.n, c, b.px, b.py, b.vx, b.vy, b.m, b.r,
+                                         contacts, later)
+        return contacts, later, int(large)
 
     def iterate(self, b: Bodies, dt: float = 0.05) -> None:
         self.L.or_iterate(b.n, dt, b.px, b.py, b.vx, b.vy, b.ax, b.ay)
@@ -205,6 +221,16 @@ class Oracle:
             raise RuntimeError("pair capacity exceeded")
         return later[:cnt].copy(), earlier[:cnt].copy(), float(cell.value)
 
+    def build_and_forces(self, b: Bodies, theta=1.0, eps=1.0,
+                         law: int = LAW_REFERENCE) -> None:
+        """The rest of a step after collide, in place: key order, tree,
+        accelerations (what or_step does after its collide)."""
+        srt, hi, lo, perm, quad = self.sorted_state(b)
+        t = self.build_tree(srt, hi, lo, quad)
+        srt.ax[:], srt.ay[:] = self.accelerations(srt, t, theta, eps, law)
+        for f in ("px", "py", "vx", "vy", "ax", "ay", "m", "r", "id"):
+            getattr(b, f)[:] = getattr(srt, f)
+
     def step(self, b: Bodies, dt=0.05, theta=1.0, eps=1.0, collisions=True,
              law: int = LAW_REFERENCE) -> None:
         rc = self.L.or_step(b.n, dt, theta, eps, int(collisions), law, b.px, b.py, b.vx,
@@ -237,6 +263,9 @@ class Reference:
         L.ref_uniform_disc.argtypes = [_u64, C.c_uint32] + [_f32p] * 6 + [_u32p]
         L.ref_uniform_disc.restype = _i64
         L.ref_scenario.argtypes = [_int, _u64, C.c_uint32] + [_f32p] * 6 + [_u32p]
+        L.ref_collide_jacobi.argtypes = [_u64] + [_f32p] * 6 + [_i32p, _i32p,
+                                                               C.POINTER(C.c_float)]
+        L.ref_collide_jacobi.restype = _i64
         L.ref_scenario.restype = _i64
         L.ref_iterate.argtypes = [_u64, _f32] + [_f32p] * 6
         L.ref_build.argtypes = ([_u64, _f32, _f32] + [_f32p] * 6 + [_u32p, _i64]
@@ -251,6 +280,15 @@ class Reference:
         got = self.L.ref_uniform_disc(n, seed, b.px, b.py, b.vx, b.vy, b.m, b.r, b.id)
         assert got == n
         return b
+
+    def collide_jacobi(self, b: Bodies):
+        """In place on b; returns (contacts, aabb_as_later, large count, cell)."""
+        contacts = np.zeros(b.n, np.int32)
+        later = np.zeros(b.n, np.int32)
+        cell = C.c_float()
+        large = self.L.ref_collide_jacobi(b.n, b.px, b.py, b.vx, b.vy, b.m, b.r,
+                                          contacts, later, C.byref(cell))
+        return contacts, later, int(large), float(cell.value)
 
     def scenario(self, kind: int, n: int, seed: int = 0) -> Bodies:
         b = Bodies.empty(n)

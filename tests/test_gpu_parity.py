@@ -298,3 +298,70 @@ def test_interaction_counts(nbx, oracle, states):
         sim.attract()
         gf, gp = sim.count_interactions()
     assert gf == int(far.sum()) and gp == int(pairs.sum())
+
+
+def mixed_state(n=9000, seed=5):
+    """Several large boxes, coincident bodies (some at rest relative to each
+    other) and a zero radius, spread over three 4096-body chunks."""
+    from oracle_lib import Bodies
+    rng = np.random.default_rng(seed)
+    b = Bodies.empty(n)
+    b.px[:] = rng.uniform(-60, 60, n).astype(np.float32)
+    b.py[:] = rng.uniform(-60, 60, n).astype(np.float32)
+    b.vx[:] = rng.normal(0, 1, n).astype(np.float32)
+    b.vy[:] = rng.normal(0, 1, n).astype(np.float32)
+    b.m[:] = 1.0
+    b.r[:] = 0.5
+    b.r[:40] = rng.uniform(1.5, 6.0, 40).astype(np.float32)
+    b.m[:40] = b.r[:40] ** 3
+    b.px[50:60] = b.px[49]
+    b.py[50:60] = b.py[49]
+    b.vx[55:60] = b.vx[49]
+    b.vy[55:60] = b.vy[49]
+    b.r[100] = 0.0
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    return b
+
+
+@pytest.mark.parametrize("name", ["collide10k_0", "collide20k_60", "mixed9k"])
+def test_collide_jacobi_bit_exact(nbx, oracle, states, name):
+    """The reference GPU collide semantics (CudaCollision.cu), bit for bit."""
+    s = mixed_state() if name == "mixed9k" else states[name]
+    want = s.copy()
+    contacts, later, large = oracle.collide_jacobi(want)
+    with new_sim(nbx, s) as sim:
+        sim.collide_mode = nbx.COLLIDE_JACOBI
+        sim.collide()
+        got = sim.download()
+        gc, gl = sim.contacts()
+        info = sim.collide_info()
+    for f in ("px", "py", "vx", "vy"):
+        assert_bits_equal(getattr(got, f), getattr(want, f), f)
+    assert np.array_equal(gc, contacts) and np.array_equal(gl, later)
+    assert info["cell"] == np.float32(oracle.jacobi_cell(s))
+    assert info["pairs"] == int(later.sum()) and info["largest"] == int(contacts.max())
+    assert contacts.sum() > 0
+
+
+def test_jacobi_steps_then_sequential(nbx, oracle):
+    """Jacobi-mode steps stay close to the oracle's (collide_jacobi + the
+    rest of the step), and switching back restores the sequential mode."""
+    n, steps = 3000, 5
+    o = oracle.uniform_disc(n)
+    with nbx.Simulation(n) as sim:
+        sim.collide_mode = nbx.COLLIDE_JACOBI
+        sim.summation = nbx.SUM_REFERENCE
+        for _ in range(steps):
+            oracle.iterate(o)
+            oracle.collide_jacobi(o)
+            oracle.build_and_forces(o)
+        sim.step(steps)
+        got = sim.download_by_id()
+        with pytest.raises(nbx.NbxError):
+            sim.pairs()
+        sim.collide_mode = nbx.COLLIDE_SEQUENTIAL
+        sim.step(1)
+        sim.pairs()
+    o = o.by_id()
+    d = np.hypot(got["px"] - o.px, got["py"] - o.py) / np.maximum(np.hypot(o.px, o.py), 1)
+    assert np.median(d) < 1e-6 and np.quantile(d, 0.99) < 1e-4

@@ -8,6 +8,8 @@ committed golden fixtures (test_oracle_golden.py) pin the same facts without it.
 import numpy as np
 import pytest
 
+from oracle_lib import Bodies
+
 
 def bits(a):
     return np.ascontiguousarray(a).view(np.uint32)
@@ -119,6 +121,52 @@ def test_collide_bit_exact(oracle, reference, ref_states, name):
     for f in ("px", "py", "vx", "vy"):
         assert same(getattr(a, f), getattr(b, f)), f
     assert later.shape[0] > 0
+
+
+def jacobi_states(ref_states):
+    """States just before collide, plus one with several large boxes, a
+    zero-radius body, coincident bodies and boxes spread over many chunks."""
+    out = {}
+    for name in ("disc10k", "disc20k_300"):
+        out[name] = ref_states[name].copy()
+    rng = np.random.default_rng(5)
+    n = 9000
+    b = Bodies.empty(n)
+    b.px[:] = rng.uniform(-60, 60, n).astype(np.float32)
+    b.py[:] = rng.uniform(-60, 60, n).astype(np.float32)
+    b.vx[:] = rng.normal(0, 1, n).astype(np.float32)
+    b.vy[:] = rng.normal(0, 1, n).astype(np.float32)
+    b.m[:] = 1.0
+    b.r[:] = 0.5
+    b.r[:40] = rng.uniform(1.5, 6.0, 40).astype(np.float32)   # large boxes
+    b.m[:40] = b.r[:40] ** 3
+    b.px[50:60] = b.px[49]                                    # coincident
+    b.py[50:60] = b.py[49]
+    b.vx[55:60] = b.vx[49]                                    # and at rest
+    b.vy[55:60] = b.vy[49]
+    b.r[100] = 0.0
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    out["mixed9k"] = b
+    return out
+
+
+@pytest.mark.parametrize("name", ["disc10k", "disc20k_300", "mixed9k"])
+def test_collide_jacobi_bit_exact(oracle, reference, ref_states, name):
+    """The reference GPU collide semantics (CudaCollision.cu +
+    CollisionKernels.cuh, run on the host) against the oracle."""
+    s = jacobi_states(ref_states)[name]
+    if name != "mixed9k":
+        oracle.iterate(s)
+    a, b = s.copy(), s.copy()
+    ca, la, na = oracle.collide_jacobi(a)
+    cb, lb, nb, cell = reference.collide_jacobi(b)
+    assert oracle.jacobi_cell(s) == np.float32(cell)
+    for f in ("px", "py", "vx", "vy"):
+        assert same(getattr(a, f), getattr(b, f)), f
+    assert np.array_equal(ca, cb) and np.array_equal(la, lb) and na == nb
+    assert ca.sum() > 0
+    if name == "mixed9k":
+        assert na >= 40 and (ca > 1).any()
 
 
 def test_simulation_steps_match_per_phase(oracle, reference):

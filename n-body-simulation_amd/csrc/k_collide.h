@@ -318,8 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_scatter(
     const uint32_t *__restrict__ bucketOf,
     const uint32_t *__restrict__ bucketRank,
     uint32_t *__restrict__ bucketCount,
-    const uint32_t *__restrict__ bucketStart, float4 *__restrict__ gridBox,
-    int4 *__restrict__ gridCell) {
+    const uint32_t *__restrict__ bucketStart, float4 *__restrict__ grid) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) {
     return;
@@ -332,10 +331,11 @@ __global__ __launch_bounds__(kBlock) void k_grid_scatter(
   bucketCount[h] = 0;
   const float invCell = ctl->invCell;
   const Box b = bodyBox(px[i], py[i], r[i]);
-  gridBox[slot] = make_float4(b.minX, b.maxX, b.minY, b.maxY);
-  gridCell[slot] = make_int4(cellCoord(b.minX, invCell),
-                             cellCoord(b.minY, invCell),
-                             static_cast<int32_t>(i), 0);
+  grid[2 * slot] = make_float4(b.minX, b.maxX, b.minY, b.maxY);
+  grid[2 * slot + 1] = make_float4(
+      __int_as_float(cellCoord(b.minX, invCell)),
+      __int_as_float(cellCoord(b.minY, invCell)),
+      __int_as_float(static_cast<int32_t>(i)), 0.0f);
 }
 
 // Pair output of one workgroup, staged in LDS: lanes append with LDS
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_find_small(
     const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
     const uint32_t *__restrict__ bucketOf,
     const uint32_t *__restrict__ bucketStart,
-    const float4 *__restrict__ gridBox, const int4 *__restrict__ gridCell,
+    const float4 *__restrict__ grid,
     const int32_t *__restrict__ largeList, int4 *__restrict__ pairs,
     uint32_t cap) {
   __shared__ PairStage st;
@@ -428,11 +428,17 @@ __global__ __launch_bounds__(kBlock) void k_find_small(
     for (int k = 0; k < 9; ++k) {
       const int32_t cx = mx + (k % 3) - 1, cy = my + (k / 3) - 1;
       for (uint32_t o = lo[k]; o < hi[k]; ++o) {
-        const int4 oc = gridCell[o];
+        // Box and cell of an entry share one 32-byte record: both are
+        // requested before either is tested.
+        const float4 ob = grid[2 * o];
+        const float4 ocf = grid[2 * o + 1];
+        asm volatile("" ::"v"(ob.x), "v"(ob.y), "v"(ob.z), "v"(ob.w),
+                     "v"(ocf.x), "v"(ocf.y), "v"(ocf.z));
+        const int4 oc = make_int4(__float_as_int(ocf.x), __float_as_int(ocf.y),
+                                  __float_as_int(ocf.z), 0);
         if (oc.x != cx || oc.y != cy || oc.z == self) {
           continue;
         }
-        const float4 ob = gridBox[o];
         if (beforeBox(ob.x, oc.z, me.minX, self) &&
             overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
           stagePair(st, ctl, pairs, cap, self, oc.z, me.minX, ob.x);

@@ -91,8 +91,7 @@ struct nbx_sim {
   uint32_t *bucketOf = nullptr, *bucketRank = nullptr, *bucketCount = nullptr,
            *bucketStart = nullptr;
   uint32_t *scanTiles = nullptr;
-  float4 *gridBox = nullptr;
-  int4 *gridCell = nullptr;
+  float4 *grid = nullptr; // per entry: box, then {cx, cy, index} (32 B)
   int32_t *largeList = nullptr, *ufParent = nullptr;
   uint32_t pairCap = 0;
   int4 *pairs = nullptr, *segPairs = nullptr;
@@ -247,8 +246,8 @@ int launchRadixSort(nbx_sim *s, uint32_t n, uint32_t *const keys[2],
 
 // The reference GPU collide (Jacobi resolution, k_jacobi.h). Scratch comes
 // from buffers the sequential collide and the build use at other times:
-// keyHi/vals (the two sorts), keyLo (high key words), gridBox (sorted 64-bit
-// keys), gridCell (staged results), bucketOf/bucketRank (per-body contact
+// keyHi/vals (the two sorts), keyLo (high key words), grid (sorted 64-bit
+// keys, then staged results), bucketOf/bucketRank (per-body contact
 // counts, kept for nbx_download_contacts).
 int launchCollideJacobi(nbx_sim *s) {
   const uint32_t n = static_cast<uint32_t>(s->n);
@@ -262,9 +261,9 @@ int launchCollideJacobi(nbx_sim *s) {
   NBX_TRY(launchRadixSort(s, n, s->keyHi, s->vals));
   LAUNCH(k_jac_gather_hi, nb, kBlock, n, s->vals[0], s->keyLo, s->keyHi[0]);
   NBX_TRY(launchRadixSort(s, n, s->keyHi, s->vals));
-  uint64_t *keysSorted = reinterpret_cast<uint64_t *>(s->gridBox);
+  uint64_t *keysSorted = reinterpret_cast<uint64_t *>(s->grid);
   const uint32_t *indexSorted = s->vals[0];
-  float4 *out = reinterpret_cast<float4 *>(s->gridCell);
+  float4 *out = s->grid + n; // past the 8 B/body of sorted keys
   int32_t *contacts = reinterpret_cast<int32_t *>(s->bucketOf);
   int32_t *asLater = reinterpret_cast<int32_t *>(s->bucketRank);
   LAUNCH(k_jac_sorted, nb, kBlock, n, px, py, r, indexSorted, s->ctl,
@@ -309,10 +308,9 @@ int launchCollide(nbx_sim *s) {
          s->bucketRank, s->bucketCount, s->largeList, s->ufParent);
   NBX_TRY(launchScan(s, buckets + 1, s->bucketCount, s->bucketStart));
   LAUNCH(k_grid_scatter, nb, kBlock, n, px, py, r, s->ctl, s->bucketOf,
-         s->bucketRank, s->bucketCount, s->bucketStart, s->gridBox,
-         s->gridCell);
+         s->bucketRank, s->bucketCount, s->bucketStart, s->grid);
   LAUNCH(k_find_small, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
-         s->bucketStart, s->gridBox, s->gridCell, s->largeList, s->pairs,
+         s->bucketStart, s->grid, s->largeList, s->pairs,
          s->pairCap);
   LAUNCH(k_find_large, nb, kBlock, n, px, py, r, s->ctl, s->largeList,
          s->pairs, s->pairCap);
@@ -694,7 +692,7 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->bucketCount, s->buckets + 1, true)
   A(s->bucketStart, s->buckets + 1, false)
   A(s->scanTiles, blocksFor(s->buckets + 1, kScanTile) + 1, false)
-  A(s->gridBox, n, false) A(s->gridCell, n, false)
+  A(s->grid, 2 * n, false)
   A(s->largeList, n, false) A(s->ufParent, n, false)
   s->pairCap = static_cast<uint32_t>(std::min<uint64_t>(8 * n + 1024, 0xffffff00ull));
   A(s->pairs, s->pairCap, false) A(s->segPairs, s->pairCap, false)

@@ -342,28 +342,33 @@ __global__ __launch_bounds__(kBlock) void k_grid_scatter(
 // atomics and the workgroup reserves its global range with a single atomic
 // (a global counter hit once per pair serialises across the whole chip).
 constexpr int kPairStage = 1024;
+constexpr int kFindBlock = 1024; // stencil-search workgroup (fewer flushes)
+constexpr int kFindStage = 4096;
 
-struct PairStage {
-  int4 buf[kPairStage];
+template <int CAP = kPairStage> struct PairStageT {
+  int4 buf[CAP];
   uint32_t count;
   uint32_t base;
 };
+using PairStage = PairStageT<kPairStage>;
 
-__device__ inline void stageInit(PairStage &st) {
+template <int CAP>
+__device__ inline void stageInit(PairStageT<CAP> &st) {
   if (threadIdx.x == 0) {
     st.count = 0;
   }
   __syncthreads();
 }
 
-__device__ inline void stagePair(PairStage &st, Ctl *__restrict__ ctl,
+template <int CAP>
+__device__ inline void stagePair(PairStageT<CAP> &st, Ctl *__restrict__ ctl,
                                  int4 *__restrict__ pairs, uint32_t cap,
                                  int later, int earlier, float laterMinX,
                                  float earlierMinX) {
   const int4 p = make_int4(later, earlier, __float_as_int(laterMinX),
                            __float_as_int(earlierMinX));
   const uint32_t slot = atomicAdd(&st.count, 1u);
-  if (slot < kPairStage) {
+  if (slot < CAP) {
     st.buf[slot] = p;
     return;
   }
@@ -376,15 +381,16 @@ __device__ inline void stagePair(PairStage &st, Ctl *__restrict__ ctl,
   }
 }
 
-__device__ inline void stageFlush(PairStage &st, Ctl *__restrict__ ctl,
+template <int CAP>
+__device__ inline void stageFlush(PairStageT<CAP> &st, Ctl *__restrict__ ctl,
                                   int4 *__restrict__ pairs, uint32_t cap) {
   __syncthreads();
-  const uint32_t cnt = min<uint32_t>(st.count, kPairStage);
+  const uint32_t cnt = min<uint32_t>(st.count, CAP);
   if (threadIdx.x == 0) {
     st.base = cnt ? atomicAdd(&ctl->pairCount, cnt) : 0u;
   }
   __syncthreads();
-  for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) {
     const uint32_t g = st.base + k;
     if (g < cap) {
       pairs[g] = st.buf[k];
@@ -397,7 +403,7 @@ __device__ inline void stageFlush(PairStage &st, Ctl *__restrict__ ctl,
 // One thread per small body, in device order (the previous build's key
 // order, so a wave covers a compact patch of space): its earlier partners
 // among small boxes of the 3x3 neighbouring cells and among all large boxes.
-__global__ __launch_bounds__(kBlock) void k_find_small(
+__global__ __launch_bounds__(kFindBlock) void k_find_small(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
     const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
     const uint32_t *__restrict__ bucketOf,
@@ -405,9 +411,9 @@ __global__ __launch_bounds__(kBlock) void k_find_small(
     const float4 *__restrict__ grid,
     const int32_t *__restrict__ largeList, int4 *__restrict__ pairs,
     uint32_t cap) {
-  __shared__ PairStage st;
+  __shared__ PairStageT<kFindStage> st;
   stageInit(st);
-  const uint32_t i = xcdBlock() * kBlock + threadIdx.x;
+  const uint32_t i = xcdBlock() * kFindBlock + threadIdx.x;
   if (i < n && bucketOf[i] != 0xffffffffu) {
     const uint32_t large = ctl->largeCount;
     const float invCell = ctl->invCell;

@@ -309,7 +309,8 @@ int launchCollide(nbx_sim *s) {
   NBX_TRY(launchScan(s, buckets + 1, s->bucketCount, s->bucketStart));
   LAUNCH(k_grid_scatter, nb, kBlock, n, px, py, r, s->ctl, s->bucketOf,
          s->bucketRank, s->bucketCount, s->bucketStart, s->grid);
-  LAUNCH(k_find_small, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
+  LAUNCH(k_find_small, blocksFor(n, kFindBlock), kFindBlock, n, px, py, r,
+         s->ctl, mask, s->bucketOf,
          s->bucketStart, s->grid, s->largeList, s->pairs,
          s->pairCap);
   LAUNCH(k_find_large, nb, kBlock, n, px, py, r, s->ctl, s->largeList,

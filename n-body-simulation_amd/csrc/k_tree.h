@@ -469,20 +469,115 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
     }
     const int rank = s - ctl->splitBase[d];
     const int j = splitList[s];
-    // Run end: first body not sharing d digits with j.
-    int e = static_cast<int>(n);
-    if (d > 0) {
-      int a = j + kLeafCapacity + 1, b = static_cast<int>(n);
+    int bound[5];
+    bound[0] = j;
+    // First depth-(d+1) splitting run at or after j: this node's splitting
+    // children are the next entries of that level's (sorted) list.
+    const int lvBase = ctl->splitBase[d + 1];
+    const int lvEnd = d + 1 < kMaxDepth ? ctl->splitBase[d + 2] : lvBase;
+    int firstChildSplit = lvBase;
+    if (d < 16) {
+      // The run is every body whose high key word starts with j's d-digit
+      // prefix, and its children's boundaries are where the next digit
+      // reaches 1, 2, 3 and overflows: five independent lower-bound
+      // searches (four on the key array, one on the split list), stepped
+      // together so their loads overlap.
+      const int sh = 32 - 2 * d;
+      const uint64_t hj = hi[j];
+      const uint64_t base = sh >= 32 ? 0ull : (hj >> sh) << sh;
+      uint64_t key[4];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        key[q] = base + (static_cast<uint64_t>(q + 1) << (sh - 2));
+      }
+      key[3] = base + (1ull << sh);
+      int a[5], b[5];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = j;
+        b[q] = static_cast<int>(n);
+      }
+      a[4] = lvBase;
+      b[4] = lvEnd;
+      while (a[0] < b[0] || a[1] < b[1] || a[2] < b[2] || a[3] < b[3] ||
+             a[4] < b[4]) {
+        // Finished searches probe index 0 (always valid) so that every load
+        // is unconditional and they issue together.
+        int mid[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          mid[q] = a[q] < b[q] ? (a[q] + b[q]) >> 1 : 0;
+        }
+        uint32_t v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          v[q] = hi[mid[q]];
+        }
+        const int32_t w = splitList[mid[4]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (a[q] < b[q]) {
+            if (static_cast<uint64_t>(v[q]) < key[q]) {
+              a[q] = mid[q] + 1;
+            } else {
+              b[q] = mid[q];
+            }
+          }
+        }
+        if (a[4] < b[4]) {
+          if (w < j) {
+            a[4] = mid[4] + 1;
+          } else {
+            b[4] = mid[4];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        bound[q + 1] = a[q];
+      }
+      firstChildSplit = a[4];
+    } else {
+      // Below depth 16 the digits come from the low key word (rare deep
+      // runs): one search at a time.
+      int e = static_cast<int>(n);
+      {
+        int a = j + kLeafCapacity + 1, b = static_cast<int>(n);
+        while (a < b) {
+          const int mid = (a + b) >> 1;
+          if (commonDigits(hi, lo, perm, j, mid) >= d) {
+            a = mid + 1;
+          } else {
+            b = mid;
+          }
+        }
+        e = a;
+      }
+      bound[4] = e;
+      for (int q = 1; q < 4; ++q) {
+        int a = bound[q - 1], b = e;
+        while (a < b) {
+          const int mid = (a + b) >> 1;
+          if (digitAt(hi, lo, perm, mid, d) < q) {
+            a = mid + 1;
+          } else {
+            b = mid;
+          }
+        }
+        bound[q] = a;
+      }
+      int a = lvBase, b = lvEnd;
       while (a < b) {
         const int mid = (a + b) >> 1;
-        if (commonDigits(hi, lo, perm, j, mid) >= d) {
+        if (splitList[mid] < j) {
           a = mid + 1;
         } else {
           b = mid;
         }
       }
-      e = a;
+      firstChildSplit = a;
     }
+    const int e = bound[4];
     // This node's quad: replay intoQuadrant from the root.
     float cx = ctl->quad[0], cy = ctl->quad[1], sz = ctl->quad[2];
     for (int t = 0; t < d; ++t) {
@@ -491,41 +586,20 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
       cx += (static_cast<float>(q & 1) - 0.5f) * sz;
       cy += (static_cast<float>(q >> 1) - 0.5f) * sz;
     }
-    int bound[5];
-    bound[0] = j;
-    bound[4] = e;
-    for (int q = 1; q < 4; ++q) {
-      int a = bound[q - 1], b = e;
-      while (a < b) {
-        const int mid = (a + b) >> 1;
-        if (digitAt(hi, lo, perm, mid, d) < q) {
-          a = mid + 1;
-        } else {
-          b = mid;
-        }
-      }
-      bound[q] = a;
-    }
+    (void)e;
     const int c = ctl->levelStart[d + 1] + 4 * rank;
+    int nextSplit = firstChildSplit;
     const float childSize = sz * 0.5f;
     for (int q = 0; q < 4; ++q) {
       const int ch = c + q;
       const int cs = bound[q], ce = bound[q + 1];
       int grand = 0;
       if (ce - cs > kLeafCapacity && d + 1 < kMaxDepth) {
-        // This child splits too: find its rank among depth d+1 run starts.
-        const int base = ctl->splitBase[d + 1];
-        int a = base, b = ctl->splitBase[d + 2];
-        while (a < b) {
-          const int mid = (a + b) >> 1;
-          if (splitList[mid] < cs) {
-            a = mid + 1;
-          } else {
-            b = mid;
-          }
-        }
-        if (a < ctl->splitBase[d + 2] && splitList[a] == cs) {
-          grand = ctl->levelStart[d + 2] + 4 * (a - base);
+        // This child splits too: it is the next entry of the depth-(d+1)
+        // split list.
+        const int a = nextSplit++;
+        if (a < lvEnd && splitList[a] == cs) {
+          grand = ctl->levelStart[d + 2] + 4 * (a - lvBase);
           nodeOfSplit[a] = ch;
         } else {
           atomicOr(&const_cast<Ctl *>(ctl)->status, kStatusTreeInconsistent);

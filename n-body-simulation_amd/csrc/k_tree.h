@@ -315,6 +315,35 @@ __device__ inline void splitRange(uint32_t n, const uint32_t *__restrict__ hi,
   }
 }
 
+// Depths [first, last] at which some lane of the wave starts a splitting run
+// (first > last when none): the per-depth ballots below only need those.
+__device__ inline void waveDepthRange(int A, int B, int &first, int &last) {
+  int f = A <= B ? A : kMaxDepth, l = A <= B ? B : -1;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    f = min(f, __shfl_xor(f, off, kWave));
+    l = max(l, __shfl_xor(l, off, kWave));
+  }
+  first = __builtin_amdgcn_readfirstlane(f);
+  last = __builtin_amdgcn_readfirstlane(l);
+}
+
+// Per-wave counts of splitting runs per depth into waveCnt[wave][*] (zero
+// outside the wave's depth range).
+__device__ inline void waveDepthCounts(int A, int B, int first, int last,
+                                       uint32_t *waveCntRow) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if (lane < kMaxDepth) {
+    waveCntRow[lane] = 0;
+  }
+  for (int d = first; d <= last; ++d) {
+    const uint64_t mask = __ballot(A <= d && d <= B);
+    if (lane == 0) {
+      waveCntRow[d] = static_cast<uint32_t>(__popcll(mask));
+    }
+  }
+}
+
 // Per 256-body tile: number of splitting runs starting in it, per depth ->
 // tileCount[d * tiles + tile].
 __global__ __launch_bounds__(kBlock) void k_lcp_count(
@@ -323,15 +352,12 @@ __global__ __launch_bounds__(kBlock) void k_lcp_count(
     uint32_t tiles, uint32_t *__restrict__ tileCount) {
   __shared__ uint32_t waveCnt[kBlock / kWave][kMaxDepth];
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+  const int wave = threadIdx.x >> 6;
   int A, B;
   splitRange(n, hi, lo, perm, j, A, B);
-  for (int d = 0; d < kMaxDepth; ++d) {
-    const uint64_t mask = __ballot(A <= d && d <= B);
-    if (lane == 0) {
-      waveCnt[wave][d] = static_cast<uint32_t>(__popcll(mask));
-    }
-  }
+  int first, last;
+  waveDepthRange(A, B, first, last);
+  waveDepthCounts(A, B, first, last, waveCnt[wave]);
   __syncthreads();
   if (threadIdx.x < kMaxDepth) {
     uint32_t s = 0;
@@ -418,31 +444,34 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     uint32_t tiles, const uint32_t *__restrict__ tileOffset,
     const Ctl *__restrict__ ctl, int32_t *__restrict__ splitList) {
   __shared__ uint32_t waveCnt[kBlock / kWave][kMaxDepth];
+  __shared__ uint32_t tileBase[kMaxDepth];
   if (ctl->status & kStatusNodeOverflow) {
     return;
   }
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6;
+  const int wave = threadIdx.x >> 6;
   int A, B;
   splitRange(n, hi, lo, perm, j, A, B);
-  for (int d = 0; d < kMaxDepth; ++d) {
-    const uint64_t mask = __ballot(A <= d && d <= B);
-    if (lane == 0) {
-      waveCnt[wave][d] = static_cast<uint32_t>(__popcll(mask));
-    }
+  int first, last;
+  waveDepthRange(A, B, first, last);
+  waveDepthCounts(A, B, first, last, waveCnt[wave]);
+  // This tile's output offset per depth, staged once per workgroup.
+  if (threadIdx.x < kMaxDepth) {
+    tileBase[threadIdx.x] =
+        ctl->splitBase[threadIdx.x] +
+        tileOffset[static_cast<size_t>(threadIdx.x) * tiles + blockIdx.x];
   }
   __syncthreads();
   const uint64_t lt = laneMaskLt();
-  for (int d = 0; d < kMaxDepth; ++d) {
+  for (int d = first; d <= last; ++d) {
     const bool flag = A <= d && d <= B;
     const uint64_t mask = __ballot(flag);
     if (flag) {
-      uint32_t rank = tileOffset[static_cast<size_t>(d) * tiles + blockIdx.x] +
-                      static_cast<uint32_t>(__popcll(mask & lt));
+      uint32_t rank = tileBase[d] + static_cast<uint32_t>(__popcll(mask & lt));
       for (int w = 0; w < wave; ++w) {
         rank += waveCnt[w][d];
       }
-      splitList[ctl->splitBase[d] + rank] = static_cast<int32_t>(j);
+      splitList[rank] = static_cast<int32_t>(j);
     }
   }
 }

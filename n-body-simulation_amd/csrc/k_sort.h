@@ -49,11 +49,25 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(
     uint32_t count, const uint32_t *__restrict__ in,
     uint32_t *__restrict__ tileSums) {
   __shared__ uint32_t scratch[kBlock / kWave];
-  const uint32_t base = blockIdx.x * kScanTile;
+  // Thread t sums the 16 consecutive values [base + 16 t, ...): four
+  // 16-byte loads, all issued before any is used.
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t s = 0;
-  for (int k = 0; k < kScanItems; ++k) {
-    const uint32_t idx = base + k * kBlock + threadIdx.x;
-    s += idx < count ? in[idx] : 0u;
+  if (base + kScanItems <= count) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+    uint4 q[kScanItems / 4];
+#pragma unroll
+    for (int k = 0; k < kScanItems / 4; ++k) {
+      q[k] = p[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems / 4; ++k) {
+      s += q[k].x + q[k].y + q[k].z + q[k].w;
+    }
+  } else {
+    for (int k = 0; k < kScanItems; ++k) {
+      s += base + k < count ? in[base + k] : 0u;
+    }
   }
   uint32_t total;
   blockExclusiveScan(s, scratch, &total);
@@ -86,23 +100,52 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(
     const uint32_t *__restrict__ tileSums, uint32_t *__restrict__ out) {
   __shared__ uint32_t scratch[kBlock / kWave];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  const bool full = base + kScanItems <= count;
   uint32_t v[kScanItems];
+  if (full) {
+    // Four 16-byte loads (the buffers are 16-byte aligned, base is a
+    // multiple of 16 values).
+    const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+    for (int k = 0; k < kScanItems / 4; ++k) {
+      const uint4 q = p[k];
+      v[4 * k] = q.x;
+      v[4 * k + 1] = q.y;
+      v[4 * k + 2] = q.z;
+      v[4 * k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = base + k < count ? in[base + k] : 0u;
+    }
+  }
   uint32_t s = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    const uint32_t idx = base + k;
-    v[k] = idx < count ? in[idx] : 0u;
     s += v[k];
   }
   uint32_t total;
   uint32_t run = blockExclusiveScan(s, scratch, &total) + tileSums[blockIdx.x];
+  uint32_t o[kScanItems];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    const uint32_t idx = base + k;
-    if (idx < count) {
-      out[idx] = run;
-    }
+    o[k] = run;
     run += v[k];
+  }
+  if (full) {
+    uint4 *p = reinterpret_cast<uint4 *>(out + base);
+#pragma unroll
+    for (int k = 0; k < kScanItems / 4; ++k) {
+      p[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < count) {
+        out[base + k] = o[k];
+      }
+    }
   }
 }
 

@@ -400,9 +400,10 @@ __device__ inline void stageFlush(PairStageT<CAP> &st, Ctl *__restrict__ ctl,
   }
 }
 
-// One thread per small body, in device order (the previous build's key
-// order, so a wave covers a compact patch of space): its earlier partners
-// among small boxes of the 3x3 neighbouring cells and among all large boxes.
+// One thread per body, in device order (the previous build's key order, so a
+// wave covers a compact patch of space): a small body's earlier partners
+// among small boxes of the 3x3 neighbouring cells, and every body's pairs
+// with large boxes.
 __global__ __launch_bounds__(kFindBlock) void k_find_small(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
     const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
@@ -414,81 +415,61 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
   __shared__ PairStageT<kFindStage> st;
   stageInit(st);
   const uint32_t i = xcdBlock() * kFindBlock + threadIdx.x;
-  if (i < n && bucketOf[i] != 0xffffffffu) {
+  if (i < n) {
     const uint32_t large = ctl->largeCount;
+    const bool small = bucketOf[i] != 0xffffffffu;
     const float invCell = ctl->invCell;
     const Box me = bodyBox(px[i], py[i], r[i]);
     const int self = static_cast<int>(i);
-    const int32_t mx = cellCoord(me.minX, invCell);
-    const int32_t my = cellCoord(me.minY, invCell);
-    // All nine bucket ranges are requested up front (independent loads),
-    // then only the non-empty ones are scanned.
-    uint32_t lo[9], hi[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const uint32_t h = cellBucket(mx + (k % 3) - 1, my + (k / 3) - 1, mask);
-      lo[k] = bucketStart[h];
-      hi[k] = bucketStart[h + 1];
-    }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const int32_t cx = mx + (k % 3) - 1, cy = my + (k / 3) - 1;
-      for (uint32_t o = lo[k]; o < hi[k]; ++o) {
-        // Box and cell of an entry share one 32-byte record: both are
-        // requested before either is tested.
-        const float4 ob = grid[2 * o];
-        const float4 ocf = grid[2 * o + 1];
-        asm volatile("" ::"v"(ob.x), "v"(ob.y), "v"(ob.z), "v"(ob.w),
-                     "v"(ocf.x), "v"(ocf.y), "v"(ocf.z));
-        const int4 oc = make_int4(__float_as_int(ocf.x), __float_as_int(ocf.y),
-                                  __float_as_int(ocf.z), 0);
-        if (oc.x != cx || oc.y != cy || oc.z == self) {
-          continue;
-        }
-        if (beforeBox(ob.x, oc.z, me.minX, self) &&
-            overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
-          stagePair(st, ctl, pairs, cap, self, oc.z, me.minX, ob.x);
+    if (small) {
+      const int32_t mx = cellCoord(me.minX, invCell);
+      const int32_t my = cellCoord(me.minY, invCell);
+      // All nine bucket ranges are requested up front (independent loads),
+      // then only the non-empty ones are scanned.
+      uint32_t lo[9], hi[9];
+  #pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const uint32_t h = cellBucket(mx + (k % 3) - 1, my + (k / 3) - 1, mask);
+        lo[k] = bucketStart[h];
+        hi[k] = bucketStart[h + 1];
+      }
+  #pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const int32_t cx = mx + (k % 3) - 1, cy = my + (k / 3) - 1;
+        for (uint32_t o = lo[k]; o < hi[k]; ++o) {
+          // Box and cell of an entry share one 32-byte record: both are
+          // requested before either is tested.
+          const float4 ob = grid[2 * o];
+          const float4 ocf = grid[2 * o + 1];
+          asm volatile("" ::"v"(ob.x), "v"(ob.y), "v"(ob.z), "v"(ob.w),
+                       "v"(ocf.x), "v"(ocf.y), "v"(ocf.z));
+          const int4 oc = make_int4(__float_as_int(ocf.x), __float_as_int(ocf.y),
+                                    __float_as_int(ocf.z), 0);
+          if (oc.x != cx || oc.y != cy || oc.z == self) {
+            continue;
+          }
+          if (beforeBox(ob.x, oc.z, me.minX, self) &&
+              overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
+            stagePair(st, ctl, pairs, cap, self, oc.z, me.minX, ob.x);
+          }
         }
       }
     }
+    // Large boxes (Collision.cpp:319-330): against every body, each pair
+    // once -- a small body emits both orientations, a large one only the
+    // pairs with later large boxes (the earlier one emits large-large pairs).
     for (uint32_t k = 0; k < large; ++k) {
       const int l = largeList[k];
-      const Box lb = bodyBox(px[l], py[l], r[l]);
-      if (beforeBox(lb.minX, l, me.minX, self) && overlapsBox(lb, me)) {
-        stagePair(st, ctl, pairs, cap, self, l, me.minX, lb.minX);
-      }
-    }
-  }
-  stageFlush(st, ctl, pairs, cap);
-}
-
-// Every body against every large box that comes later (Collision.cpp:
-// 319-330, oriented): covers large-small pairs with the large box later and
-// large-large pairs once.
-__global__ __launch_bounds__(kBlock) void k_find_large(
-    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ r, Ctl *__restrict__ ctl,
-    const int32_t *__restrict__ largeList, int4 *__restrict__ pairs,
-    uint32_t cap) {
-  const uint32_t large = ctl->largeCount;
-  if (large == 0) {
-    return; // uniform across the grid
-  }
-  __shared__ PairStage st;
-  stageInit(st);
-  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
-  if (x < n) {
-    const Box xb = bodyBox(px[x], py[x], r[x]);
-    for (uint32_t k = 0; k < large; ++k) {
-      const int l = largeList[k];
-      if (l == static_cast<int>(x)) {
+      if (l == self) {
         continue;
       }
       const Box lb = bodyBox(px[l], py[l], r[l]);
-      if (beforeBox(xb.minX, static_cast<int>(x), lb.minX, l) &&
-          overlapsBox(xb, lb)) {
-        stagePair(st, ctl, pairs, cap, l, static_cast<int>(x), lb.minX,
-                  xb.minX);
+      if (beforeBox(lb.minX, l, me.minX, self)) {
+        if (small && overlapsBox(lb, me)) {
+          stagePair(st, ctl, pairs, cap, self, l, me.minX, lb.minX);
+        }
+      } else if (overlapsBox(me, lb)) {
+        stagePair(st, ctl, pairs, cap, l, self, lb.minX, me.minX);
       }
     }
   }

@@ -313,8 +313,6 @@ int launchCollide(nbx_sim *s) {
          s->ctl, mask, s->bucketOf,
          s->bucketStart, s->grid, s->largeList, s->pairs,
          s->pairCap);
-  LAUNCH(k_find_large, nb, kBlock, n, px, py, r, s->ctl, s->largeList,
-         s->pairs, s->pairCap);
   LAUNCH(k_union, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs, s->ufParent);
   LAUNCH(k_comp_count, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
          s->ufParent, s->pairComp, s->compCount);

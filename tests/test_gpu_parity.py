@@ -323,6 +323,24 @@ def mixed_state(n=9000, seed=5):
     return b
 
 
+def test_collide_many_large_boxes(nbx, oracle):
+    """Sequential collide with 40 large boxes (large-large and large-small
+    pairs in both orientations), coincident bodies and a zero radius."""
+    s = mixed_state()
+    want = s.copy()
+    later, earlier, _ = oracle.collide(want)
+    with new_sim(nbx, s) as sim:
+        sim.collide()
+        got = sim.download()
+        glater, gearlier, _ = sim.pairs()
+    key = lambda a, b: (a.astype(np.int64) << 32) | b.astype(np.int64)
+    assert np.array_equal(np.sort(key(glater, gearlier)), np.sort(key(later, earlier)))
+    for f in ("px", "py", "vx", "vy"):
+        assert_bits_equal(getattr(got, f), getattr(want, f), f)
+    big = set(range(40))
+    assert any(a in big and b in big for a, b in zip(later.tolist(), earlier.tolist()))
+
+
 @pytest.mark.parametrize("name", ["collide10k_0", "collide20k_60", "mixed9k"])
 def test_collide_jacobi_bit_exact(nbx, oracle, states, name):
     """The reference GPU collide semantics (CudaCollision.cu), bit for bit."""

@@ -150,6 +150,19 @@ def jacobi_states(ref_states):
     return out
 
 
+def test_collide_many_large_boxes(oracle, reference, ref_states):
+    """Sequential collide with 40 large boxes, coincident bodies and a zero
+    radius: oracle against the reference's collide()."""
+    s = jacobi_states(ref_states)["mixed9k"]
+    a, b = s.copy(), s.copy()
+    later, earlier, _ = oracle.collide(a)
+    reference.collide(b)
+    for f in ("px", "py", "vx", "vy"):
+        assert same(getattr(a, f), getattr(b, f)), f
+    big = set(range(40))
+    assert any(x in big and y in big for x, y in zip(later.tolist(), earlier.tolist()))
+
+
 @pytest.mark.parametrize("name", ["disc10k", "disc20k_300", "mixed9k"])
 def test_collide_jacobi_bit_exact(oracle, reference, ref_states, name):
     """The reference GPU collide semantics (CudaCollision.cu +

@@ -211,6 +211,10 @@ __global__ __launch_bounds__(kBlock) void k_walk(
 //    array is padded by kBodyPad records; out-of-range slots are masked);
 //  * terms are accumulated with FMA into a per-lane partial sum that is
 //    added to a Kahan-compensated total after every other block.
+#ifndef NBX_WALK_BLOCK
+#define NBX_WALK_BLOCK 128
+#endif
+constexpr int kWalkBlock = NBX_WALK_BLOCK; // lanes per workgroup of k_walk_fast
 constexpr int kBodyPad = 4;
 constexpr int kStackDepth = 128; // >= 3 * kMaxDepth + 4 entries per wave
 
@@ -318,16 +322,17 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
 }
 
 template <int LAW, bool COUNT, bool CAP>
-__global__ __launch_bounds__(kBlock) void k_walk_fast(
+__global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
     uint32_t n, const float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
     float tSq, float eSq, float *__restrict__ ax, float *__restrict__ ay,
     Ctl *__restrict__ ctl) {
-  __shared__ int stNode[kBlock / kWave][kStackDepth];
-  __shared__ uint64_t stMask[kBlock / kWave][kStackDepth];
+  __shared__ int stNode[kWalkBlock / kWave][kStackDepth];
+  __shared__ uint64_t stMask[kWalkBlock / kWave][kStackDepth];
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t w = threadIdx.x >> 6;
-  const uint32_t base = xcdBlock() * kBlock + (threadIdx.x & ~(kWave - 1));
+  const uint32_t base =
+      xcdBlock() * kWalkBlock + (threadIdx.x & ~(kWave - 1));
   if (base >= n ||
       (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent))) {
     return;

@@ -386,6 +386,7 @@ int launchForces(nbx_sim *s, bool count) {
   const float tSq = s->params.theta * s->params.theta;
   const float eSq = s->params.epsilon * s->params.epsilon;
   const uint32_t grid = blocksFor(n);
+  const uint32_t walkGrid = blocksFor(n, kWalkBlock);
   const bool ref = s->params.summation == NBX_SUM_REFERENCE;
   const bool acc = s->params.law == NBX_LAW_ACCURATE;
 #define WALK(SUMV, LAWV, CNT)                                                 \
@@ -401,11 +402,11 @@ int launchForces(nbx_sim *s, bool count) {
 #define FAST(LAWV, CNT)                                                       \
   do {                                                                        \
     if (capFree) {                                                            \
-      LAUNCH((k_walk_fast<LAWV, CNT, false>), grid, kBlock, n, s->nodeF,      \
-             s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);          \
+      LAUNCH((k_walk_fast<LAWV, CNT, false>), walkGrid, kWalkBlock, n,        \
+             s->nodeF, s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);\
     } else {                                                                  \
-      LAUNCH((k_walk_fast<LAWV, CNT, true>), grid, kBlock, n, s->nodeF,       \
-             s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);          \
+      LAUNCH((k_walk_fast<LAWV, CNT, true>), walkGrid, kWalkBlock, n,         \
+             s->nodeF, s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);\
     }                                                                         \
   } while (0)
   if (ref) {

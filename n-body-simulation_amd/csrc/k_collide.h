@@ -46,22 +46,16 @@ __device__ inline int32_t cellCoord(float v, float invCell) {
   return static_cast<int32_t>(c);
 }
 
-// Spreads the low 16 bits of v over the even bit positions.
-__device__ inline uint32_t spread16(uint32_t v) {
-  v &= 0xffffu;
-  v = (v | (v << 8)) & 0x00ff00ffu;
-  v = (v | (v << 4)) & 0x0f0f0f0fu;
-  v = (v | (v << 2)) & 0x33333333u;
-  v = (v | (v << 1)) & 0x55555555u;
-  return v;
-}
-
-// Cell -> bucket: Morton interleave of the cell coordinates (low 16 bits
-// each), so cells close in space land in nearby buckets; cells that collide
-// in a bucket are told apart by their exact coordinates.
-__device__ inline uint32_t cellBucket(int32_t cx, int32_t cy, uint32_t mask) {
-  return (spread16(static_cast<uint32_t>(cx)) |
-          (spread16(static_cast<uint32_t>(cy)) << 1)) &
+// Cell -> bucket, row-major: the low colBits bits of the column, then the
+// row, folded into the power-of-two bucket count. The three cells
+// (cx - 1 .. cx + 1, cy) of a stencil row are consecutive buckets (unless the
+// row wraps), so their entries form one contiguous range of the grid; cells
+// that share a bucket are told apart by their exact coordinates.
+__device__ inline uint32_t rowBucket(int32_t cx, int32_t cy, uint32_t colBits,
+                                     uint32_t mask) {
+  const uint32_t colMask = (1u << colBits) - 1u;
+  return ((static_cast<uint32_t>(cy) << colBits) +
+          (static_cast<uint32_t>(cx) & colMask)) &
          mask;
 }
 
@@ -285,8 +279,9 @@ __global__ __launch_bounds__(kBlock) void k_cell_size(
 // Large/small split, grid cells, bucket counts; also seeds the union-find.
 __global__ __launch_bounds__(kBlock) void k_cells(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
-    uint32_t *__restrict__ bucketOf, uint32_t *__restrict__ bucketRank,
+    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t colBits,
+    uint32_t mask, uint32_t *__restrict__ bucketOf,
+    uint32_t *__restrict__ bucketRank,
     uint32_t *__restrict__ bucketCount, int32_t *__restrict__ largeList,
     int32_t *__restrict__ ufParent) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -304,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_cells(
   }
   const int32_t cx = cellCoord(b.minX, invCell);
   const int32_t cy = cellCoord(b.minY, invCell);
-  const uint32_t h = cellBucket(cx, cy, mask);
+  const uint32_t h = rowBucket(cx, cy, colBits, mask);
   bucketOf[i] = h;
   bucketRank[i] = atomicAdd(&bucketCount[h], 1u);
 }
@@ -406,8 +401,8 @@ __device__ inline void stageFlush(PairStageT<CAP> &st, Ctl *__restrict__ ctl,
 // with large boxes.
 __global__ __launch_bounds__(kFindBlock) void k_find_small(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
-    const uint32_t *__restrict__ bucketOf,
+    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t colBits,
+    uint32_t mask, const uint32_t *__restrict__ bucketOf,
     const uint32_t *__restrict__ bucketStart,
     const float4 *__restrict__ grid,
     const int32_t *__restrict__ largeList, int4 *__restrict__ pairs,
@@ -424,18 +419,37 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
     if (small) {
       const int32_t mx = cellCoord(me.minX, invCell);
       const int32_t my = cellCoord(me.minY, invCell);
-      // All nine bucket ranges are requested up front (independent loads),
-      // then only the non-empty ones are scanned.
+      // Per stencil row, the bucket range of its three cells -- one
+      // contiguous range unless the row wraps around the bucket columns
+      // (then three single-bucket ranges) -- all requested up front.
+      const uint32_t colMask = (1u << colBits) - 1u;
+      const uint32_t col = static_cast<uint32_t>(mx) & colMask;
+      const bool wraps = col == 0 || col == colMask;
       uint32_t lo[9], hi[9];
-  #pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        const uint32_t h = cellBucket(mx + (k % 3) - 1, my + (k / 3) - 1, mask);
-        lo[k] = bucketStart[h];
-        hi[k] = bucketStart[h + 1];
+      int ranges = 0;
+#pragma unroll
+      for (int row = 0; row < 3; ++row) {
+        const int32_t cy = my + row - 1;
+        if (!wraps) {
+          const uint32_t h = rowBucket(mx, cy, colBits, mask);
+          lo[row] = bucketStart[h - 1];
+          hi[row] = bucketStart[h + 2];
+        }
       }
-  #pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        const int32_t cx = mx + (k % 3) - 1, cy = my + (k / 3) - 1;
+      if (!wraps) {
+        ranges = 3;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          const uint32_t h =
+              rowBucket(mx + (k % 3) - 1, my + (k / 3) - 1, colBits, mask);
+          lo[k] = bucketStart[h];
+          hi[k] = bucketStart[h + 1];
+        }
+        ranges = 9;
+      }
+      for (int k = 0; k < ranges; ++k) {
+        const int32_t cy = my + (ranges == 3 ? k : k / 3) - 1;
         for (uint32_t o = lo[k]; o < hi[k]; ++o) {
           // Box and cell of an entry share one 32-byte record: both are
           // requested before either is tested.
@@ -443,14 +457,14 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
           const float4 ocf = grid[2 * o + 1];
           asm volatile("" ::"v"(ob.x), "v"(ob.y), "v"(ob.z), "v"(ob.w),
                        "v"(ocf.x), "v"(ocf.y), "v"(ocf.z));
-          const int4 oc = make_int4(__float_as_int(ocf.x), __float_as_int(ocf.y),
-                                    __float_as_int(ocf.z), 0);
-          if (oc.x != cx || oc.y != cy || oc.z == self) {
+          const int32_t ox = __float_as_int(ocf.x), oy = __float_as_int(ocf.y);
+          const int other = __float_as_int(ocf.z);
+          if (oy != cy || ox < mx - 1 || ox > mx + 1 || other == self) {
             continue;
           }
-          if (beforeBox(ob.x, oc.z, me.minX, self) &&
+          if (beforeBox(ob.x, other, me.minX, self) &&
               overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
-            stagePair(st, ctl, pairs, cap, self, oc.z, me.minX, ob.x);
+            stagePair(st, ctl, pairs, cap, self, other, me.minX, ob.x);
           }
         }
       }

@@ -17,6 +17,8 @@
 #include "k_sort.h"
 #include "nbx_device.h"
 
+#include <climits>
+
 namespace nbx {
 
 struct Box {
@@ -624,9 +626,56 @@ __global__ __launch_bounds__(kBlock) void k_comp_scatter(
   }
 }
 
-// One lane per component of up to kSmallComp pairs: insertion sort into
-// pairLess order, then the sequential replay. Larger components are queued
-// for k_resolve_big.
+// A component of up to N pairs in registers: loaded together, padded with
+// entries that order last, sorted by a fixed odd-even merge network
+// (Batcher) -- static indices only, so nothing spills to scratch -- then
+// written back in order and replayed.
+template <int N>
+__device__ inline void resolveComponent(int start, int size,
+                                        int4 *__restrict__ segPairs,
+                                        float *__restrict__ px,
+                                        float *__restrict__ py,
+                                        float *__restrict__ vx,
+                                        float *__restrict__ vy,
+                                        const float *__restrict__ m,
+                                        const float *__restrict__ r) {
+  const int4 pad = make_int4(INT_MAX, INT_MAX, 0x7f800000, 0x7f800000);
+  int4 p[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    p[k] = k < size ? segPairs[start + k] : pad;
+  }
+#pragma unroll
+  for (int w = 1; w < N; w <<= 1) {
+#pragma unroll
+    for (int k = w; k >= 1; k >>= 1) {
+#pragma unroll
+      for (int j = k % w; j + k < N; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+          if (i + j + k < N && (i + j) / (2 * w) == (i + j + k) / (2 * w)) {
+            const int a = i + j, b = i + j + k;
+            if (pairLess(p[b], p[a])) {
+              const int4 t = p[a];
+              p[a] = p[b];
+              p[b] = t;
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (k < size) {
+      segPairs[start + k] = p[k];
+      resolvePair(px, py, vx, vy, m, r, p[k].x, p[k].y);
+    }
+  }
+}
+
+// One lane per component of up to kSmallComp pairs: pairLess order, then the
+// sequential replay. Larger components are queued for k_resolve_big.
 __global__ __launch_bounds__(kBlock) void k_resolve_small(
     Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
     int4 *__restrict__ segPairs, int32_t *__restrict__ bigList,
@@ -643,19 +692,16 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
     bigList[atomicAdd(&ctl->bigCount, 1u)] = static_cast<int32_t>(j);
     return;
   }
-  int4 p[kSmallComp];
-  for (int k = 0; k < size; ++k) {
-    const int4 v = segPairs[start + k];
-    int q = k;
-    while (q > 0 && pairLess(v, p[q - 1])) {
-      p[q] = p[q - 1];
-      --q;
-    }
-    p[q] = v;
-  }
-  for (int k = 0; k < size; ++k) {
-    segPairs[start + k] = p[k];
-    resolvePair(px, py, vx, vy, m, r, p[k].x, p[k].y);
+  static_assert(kSmallComp == 16, "size classes below cover 16 pairs");
+  if (size == 1) {
+    const int4 q = segPairs[start];
+    resolvePair(px, py, vx, vy, m, r, q.x, q.y);
+  } else if (size <= 4) {
+    resolveComponent<4>(start, size, segPairs, px, py, vx, vy, m, r);
+  } else if (size <= 8) {
+    resolveComponent<8>(start, size, segPairs, px, py, vx, vy, m, r);
+  } else {
+    resolveComponent<16>(start, size, segPairs, px, py, vx, vy, m, r);
   }
 }
 

@@ -14,8 +14,10 @@ constexpr int kScanTile = kBlock * kScanItems; // 4096 values per tile
 constexpr int kSortItems = NBX_SORT_ITEMS;
 constexpr int kSortTile = kBlock * kSortItems; // 4096 keys per tile
 
-// Exclusive scan of one value per thread across the block; returns the
-// prefix and writes the block total to *total. Uses `scratch` (>= 4 words).
+// Exclusive scan of one value per thread across a BLOCK-lane workgroup;
+// returns the prefix and writes the block total to *total. Uses `scratch`
+// (>= BLOCK / 64 words).
+template <int BLOCK = kBlock>
 __device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t *scratch,
                                               uint32_t *total) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -33,7 +35,8 @@ __device__ inline uint32_t blockExclusiveScan(uint32_t v, uint32_t *scratch,
   }
   __syncthreads();
   uint32_t waveOff = 0, sum = 0;
-  for (int w = 0; w < kBlock / kWave; ++w) {
+#pragma unroll
+  for (int w = 0; w < BLOCK / kWave; ++w) {
     const uint32_t s = scratch[w];
     waveOff += (w < wave) ? s : 0u;
     sum += s;
@@ -279,6 +282,250 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
     const uint32_t dst = goff[d] + (p - digitStart[d]);
     keysOut[dst] = key;
     valsOut[dst] = sv[p];
+  }
+}
+
+// ---- stable LSD radix sort, all four passes in one launch ----
+//
+// G workgroups of 1024 lanes, all resident at once (G is at most the CU
+// count and one such workgroup fits a CU), meeting at grid barriers.
+// Workgroup b owns the contiguous chunk [b*C, b*C + C) of every pass's input,
+// C <= ITEMS * 1024. One pass:
+//   A. load the chunk, rank every key among equal digits in input order
+//      (per wave, then across waves), stage the chunk digit-sorted in LDS,
+//      publish its digit counts to table[digit * G + b]              barrier
+//   B. workgroup d (d < 256, stepping by G): exclusive scan of table row d
+//      in place, and the row total                                    barrier
+//   C. write each staged digit run at its global offset                barrier
+// so a pass costs three grid barriers instead of three launches. Data that
+// crosses workgroups inside the launch (the keys and payloads between
+// passes, the table, the row totals) is read and written with agent-scope
+// (sc1) loads and stores, which are coherent across the XCDs' L2s; the
+// barrier then only has to order them, not write back or invalidate L2.
+constexpr int kPsBlock = 1024;
+constexpr int kPsWaves = kPsBlock / kWave;
+constexpr uint32_t kPsMaxGrid = 1024; // phase B scans one row entry per lane
+constexpr uint32_t kBarrierSpins = 1u << 20; // ~1 s of polling
+// Barrier words: arrival counter [0], generation [32], one arrival counter
+// per XCD at [64 + 32 x] (128 bytes apart).
+constexpr int kBarrierWords = 64 + 32 * kXcds;
+
+// s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt fields at their
+// maxima, so only vector memory operations are waited for).
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+__device__ inline uint32_t loadAgent(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void storeAgent(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier over agent-scope data. Every wave waits for its own memory
+// operations to complete, the workgroup meets, and one lane arrives: first
+// at its XCD's counter (workgroups b with b % 8 == x), the last arrival of
+// each XCD then at the top counter -- 32 + 8 serialised atomics instead of
+// 256 on one address. The last top arrival resets the counters and bumps
+// the generation; the others poll it. Counters are zero at rest. The poll is
+// bounded: a barrier that does not complete (workgroups not co-resident)
+// sets kStatusSortStalled and returns, so the kernel always ends; the host
+// reports the error and clears the barrier words.
+__device__ inline void gridBarrier(uint32_t *__restrict__ bar, uint32_t blocks,
+                                   uint32_t *__restrict__ status) {
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t *gen = bar + 32;
+    const uint32_t g = loadAgent(gen);
+    const uint32_t x = blockIdx.x % kXcds;
+    const uint32_t members = blocks / kXcds + (x < blocks % kXcds ? 1u : 0u);
+    const uint32_t groups = min(blocks, kXcds);
+    uint32_t *sub = bar + 64 + 32 * x;
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0); // g read before arriving
+    bool last = false;
+    if (__hip_atomic_fetch_add(sub, 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
+      storeAgent(sub, 0u);
+      last = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) == groups - 1;
+    }
+    if (last) {
+      storeAgent(bar, 0u);
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0); // resets land before release
+      storeAgent(gen, g + 1u);
+    } else {
+      uint32_t spins = 0;
+      while (loadAgent(gen) == g) {
+        if (++spins > kBarrierSpins || (loadAgent(status) & kStatusSortStalled)) {
+          __hip_atomic_fetch_or(status, kStatusSortStalled, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Sorts (keys0, vals0) by key, stably, through (keys1, vals1); the result
+// ends in (keys0, vals0). table holds 256 * gridDim.x words, rowTotal 256;
+// chunk <= ITEMS * 1024. Global writes are bounds-checked, so even a stalled
+// barrier leaves every value in range.
+template <int ITEMS>
+__global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
+    uint32_t n, uint32_t chunk, uint32_t *keys0, uint32_t *vals0,
+    uint32_t *keys1, uint32_t *vals1, uint32_t *table, uint32_t *rowTotal,
+    uint32_t *bar, Ctl *ctl) {
+  constexpr int kTile = ITEMS * kPsBlock;
+  __shared__ uint32_t sk[kTile];
+  __shared__ uint32_t sv[kTile];
+  __shared__ uint32_t wh[kPsWaves][256]; // per wave: digit counts, then offsets
+  __shared__ uint32_t dstart[256], goff[256];
+  __shared__ uint32_t scratch[kPsWaves];
+
+  const uint32_t t = threadIdx.x;
+  const uint32_t b = blockIdx.x, grid = gridDim.x;
+  const uint32_t lane = t & (kWave - 1), wave = t >> 6;
+  const uint32_t lo = min(n, b * chunk);
+  const uint32_t hi = min(n, lo + chunk);
+  const uint32_t cnt = hi - lo;
+  uint32_t *status = &ctl->status;
+  // Wave w owns chunk positions [w * 64 * ITEMS, (w + 1) * 64 * ITEMS):
+  // item j of a lane sits at w * 64 * ITEMS + j * 64 + lane, so (wave, item,
+  // lane) is input order.
+  const uint32_t first = wave * kWave * ITEMS + lane;
+  // Row-scan segment: the smallest power of two >= the grid.
+  uint32_t seg = 1;
+  while (seg < grid) {
+    seg <<= 1;
+  }
+
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 8 * pass;
+    const uint32_t *kin = (pass & 1) ? keys1 : keys0;
+    const uint32_t *vin = (pass & 1) ? vals1 : vals0;
+    uint32_t *kout = (pass & 1) ? keys0 : keys1;
+    uint32_t *vout = (pass & 1) ? vals0 : vals1;
+
+    // A. load, rank, stage, publish.
+    for (uint32_t i = t; i < kPsWaves * 256; i += kPsBlock) {
+      (&wh[0][0])[i] = 0;
+    }
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t p = first + j * kWave;
+      key[j] = p < cnt ? loadAgent(kin + lo + p) : 0u;
+      val[j] = p < cnt ? loadAgent(vin + lo + p) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const bool valid = first + j * kWave < cnt;
+      const uint32_t dig = (key[j] >> shift) & 255u;
+      const uint64_t peers = matchDigit8(dig, valid);
+      const uint32_t below = static_cast<uint32_t>(__popcll(peers & laneMaskLt()));
+      const uint32_t base = wh[wave][dig];
+      rank[j] = base + below;
+      if (valid && below == 0) {
+        wh[wave][dig] = base + static_cast<uint32_t>(__popcll(peers));
+      }
+    }
+    __syncthreads();
+    // Per digit: wave counts -> offsets of each wave's run; chunk count.
+    uint32_t r = 0;
+    if (t < 256) {
+#pragma unroll
+      for (int w = 0; w < kPsWaves; ++w) {
+        const uint32_t c = wh[w][t];
+        wh[w][t] = r;
+        r += c;
+      }
+      storeAgent(table + t * grid + b, r);
+    }
+    {
+      uint32_t total;
+      const uint32_t pre = blockExclusiveScan<kPsBlock>(r, scratch, &total);
+      if (t < 256) {
+        dstart[t] = pre;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      if (first + j * kWave < cnt) {
+        const uint32_t dig = (key[j] >> shift) & 255u;
+        const uint32_t pos = dstart[dig] + wh[wave][dig] + rank[j];
+        sk[pos] = key[j];
+        sv[pos] = val[j];
+      }
+    }
+    gridBarrier(bar, grid, status);
+
+    // B. row scans, all of this workgroup's rows (b, b + G, ...) at once:
+    // lane t takes entry t % S of row b + G * (t / S), S = segment, and a
+    // segmented scan per row (S is a power of two >= G, so S <= 2G - 1 and
+    // 1024 / S >= 256 / G rows fit).
+    {
+      const uint32_t row = b + grid * (t / seg);
+      const uint32_t e = t & (seg - 1);
+      const bool inRow = row < 256 && t / seg < kPsBlock / seg;
+      const uint32_t v =
+          inRow && e < grid ? loadAgent(table + row * grid + e) : 0u;
+      uint32_t x = v;
+#pragma unroll
+      for (uint32_t off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, kWave);
+        if (off < seg && (lane & (seg - 1)) >= off) {
+          x += y;
+        }
+      }
+      if (seg > kWave) {
+        // Segments span whole waves: add the totals of the segment's
+        // earlier waves.
+        if (lane == kWave - 1) {
+          scratch[wave] = x;
+        }
+        __syncthreads();
+        const uint32_t w0 = (t / seg) * (seg / kWave);
+        for (uint32_t w = w0; w < wave; ++w) {
+          x += scratch[w];
+        }
+        __syncthreads();
+      }
+      if (inRow && e < grid) {
+        storeAgent(table + row * grid + e, x - v);
+      }
+      if (inRow && e == seg - 1) {
+        storeAgent(rowTotal + row, x);
+      }
+    }
+    gridBarrier(bar, grid, status);
+
+    // C. write the staged runs.
+    {
+      const uint32_t v = t < 256 ? loadAgent(rowTotal + t) : 0u;
+      const uint32_t own = t < 256 ? loadAgent(table + t * grid + b) : 0u;
+      uint32_t total;
+      const uint32_t base = blockExclusiveScan<kPsBlock>(v, scratch, &total);
+      if (t < 256) {
+        goff[t] = base + own;
+      }
+    }
+    __syncthreads();
+    for (uint32_t p = t; p < cnt; p += kPsBlock) {
+      const uint32_t kk = sk[p];
+      const uint32_t dig = (kk >> shift) & 255u;
+      const uint32_t dst = goff[dig] + (p - dstart[dig]);
+      if (dst < n) {
+        storeAgent(kout + dst, kk);
+        storeAgent(vout + dst, sv[p]);
+      }
+    }
+    if (pass < 3) {
+      gridBarrier(bar, grid, status);
+    }
   }
 }
 

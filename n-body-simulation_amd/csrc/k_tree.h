@@ -182,30 +182,55 @@ __global__ __launch_bounds__(kBlock) void k_keys(
   vals[i] = i;
 }
 
-// Runs of more than 16 bodies with equal high key words: their order by the
-// low word decides nodes below depth 16.
+// Runs of bodies with equal high key words are put in low-word order, so the
+// bodies end in full-key order (equal full keys keep their order, as the
+// oracle's stable sort keeps them): a run of up to kLeafCapacity bodies by
+// the lane at its start (insertion sort), a longer run -- whose order
+// decides nodes below depth 16 -- queued for k_deep_sort.
 __global__ __launch_bounds__(kBlock) void k_deep_detect(
-    uint32_t n, const uint32_t *__restrict__ hi, Ctl *__restrict__ ctl,
-    int2 *__restrict__ deepList) {
+    uint32_t n, const uint32_t *__restrict__ hi,
+    const uint32_t *__restrict__ keyLo, uint32_t *__restrict__ perm,
+    Ctl *__restrict__ ctl, int2 *__restrict__ deepList) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j + kLeafCapacity >= n) {
+  if (j + 1 >= n) {
     return;
   }
   const uint32_t h = hi[j];
-  if ((j > 0 && hi[j - 1] == h) || hi[j + kLeafCapacity] != h) {
+  if ((j > 0 && hi[j - 1] == h) || hi[j + 1] != h) {
     return;
   }
-  uint32_t a = j + kLeafCapacity + 1, b = n;
-  while (a < b) {
-    const uint32_t mid = (a + b) >> 1;
-    if (hi[mid] == h) {
-      a = mid + 1;
-    } else {
-      b = mid;
+  if (j + kLeafCapacity < n && hi[j + kLeafCapacity] == h) {
+    uint32_t a = j + kLeafCapacity + 1, b = n;
+    while (a < b) {
+      const uint32_t mid = (a + b) >> 1;
+      if (hi[mid] == h) {
+        a = mid + 1;
+      } else {
+        b = mid;
+      }
     }
+    const int slot = atomicAdd(&ctl->deepCount, 1);
+    deepList[slot] = make_int2(static_cast<int>(j), static_cast<int>(a - j));
+    return;
   }
-  const int slot = atomicAdd(&ctl->deepCount, 1);
-  deepList[slot] = make_int2(static_cast<int>(j), static_cast<int>(a - j));
+  uint32_t p[kLeafCapacity], l[kLeafCapacity];
+  int len = 0;
+  while (len < kLeafCapacity && j + len < n && hi[j + len] == h) {
+    const uint32_t v = perm[j + len];
+    const uint32_t key = keyLo[v];
+    int q = len;
+    while (q > 0 && key < l[q - 1]) {
+      p[q] = p[q - 1];
+      l[q] = l[q - 1];
+      --q;
+    }
+    p[q] = v;
+    l[q] = key;
+    ++len;
+  }
+  for (int k = 0; k < len; ++k) {
+    perm[j + k] = p[k];
+  }
 }
 
 // One workgroup per deep run: stable bitonic sort of its permutation

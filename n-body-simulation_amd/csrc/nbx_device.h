@@ -31,6 +31,7 @@ enum : uint32_t {
   kStatusComponentTooLarge = 8u,
   kStatusTreeInconsistent = 16u, // internal check failed (never expected)
   kStatusJacobiTooManyLarge = 32u,
+  kStatusSortStalled = 64u, // a grid barrier of the one-launch sort timed out
 };
 
 // Per-handle control block in device memory: everything the host would

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -103,6 +104,8 @@ struct nbx_sim {
   // tree
   uint32_t *keyHi[2]{}, *vals[2]{};
   uint32_t *keyLo = nullptr, *sortHist = nullptr, *sortRowTotal = nullptr;
+  uint32_t *sortBarrier = nullptr; // grid barrier words of k_sort_onelaunch
+  uint32_t sortGrid = 0; // its maximum workgroups; 0: three launches a pass
   int2 *deepList = nullptr;
   uint32_t *tileCount = nullptr;
   int64_t nodeCap = 0;
@@ -229,9 +232,41 @@ int launchScan(nbx_sim *s, uint32_t count, const uint32_t *in, uint32_t *out) {
 }
 
 // Stable LSD radix sort of 32-bit keys with 32-bit payloads, 4 x 8 bits,
-// from buffers 0 to buffers 0 (through 1).
+// from buffers 0 to buffers 0 (through 1): one launch of k_sort_onelaunch
+// (one workgroup per 4096 keys, at most one per CU), or, where that kernel
+// cannot be resident on every CU, three launches per pass.
 int launchRadixSort(nbx_sim *s, uint32_t n, uint32_t *const keys[2],
                     uint32_t *const vals[2]) {
+  if (n == 0) {
+    return NBX_OK;
+  }
+  if (s->sortGrid) {
+    // Smallest tile that lets the grid cover n; above that, three launches.
+    const uint32_t g4 = blocksFor(n, 4 * kPsBlock);
+    const uint32_t g8 = blocksFor(n, 8 * kPsBlock);
+    const uint32_t g16 = blocksFor(n, 16 * kPsBlock);
+    if (g4 <= s->sortGrid) {
+      const uint32_t chunk = (n + g4 - 1) / g4;
+      LAUNCH(k_sort_onelaunch<4>, g4, kPsBlock, n, chunk, keys[0], vals[0],
+             keys[1], vals[1], s->sortHist, s->sortRowTotal, s->sortBarrier,
+             s->ctl);
+      return NBX_OK;
+    }
+    if (g8 <= s->sortGrid) {
+      const uint32_t chunk = (n + g8 - 1) / g8;
+      LAUNCH(k_sort_onelaunch<8>, g8, kPsBlock, n, chunk, keys[0], vals[0],
+             keys[1], vals[1], s->sortHist, s->sortRowTotal, s->sortBarrier,
+             s->ctl);
+      return NBX_OK;
+    }
+    if (g16 <= s->sortGrid) {
+      const uint32_t chunk = (n + g16 - 1) / g16;
+      LAUNCH(k_sort_onelaunch<16>, g16, kPsBlock, n, chunk, keys[0], vals[0],
+             keys[1], vals[1], s->sortHist, s->sortRowTotal, s->sortBarrier,
+             s->ctl);
+      return NBX_OK;
+    }
+  }
   const uint32_t tiles = blocksFor(n, kSortTile);
   for (int pass = 0; pass < 4; ++pass) {
     const int src = pass & 1, dst = src ^ 1;
@@ -346,7 +381,7 @@ int launchBuild(nbx_sim *s) {
   NBX_TRY(launchRadixSort(s, n, s->keyHi, s->vals));
   const uint32_t *hi = s->keyHi[0];
   uint32_t *perm = s->vals[0];
-  LAUNCH(k_deep_detect, nb, kBlock, n, hi, s->ctl, s->deepList);
+  LAUNCH(k_deep_detect, nb, kBlock, n, hi, s->keyLo, perm, s->ctl, s->deepList);
   LAUNCH(k_deep_sort, 64, kBlock, s->ctl, s->deepList, s->keyLo, perm);
   LAUNCH(k_permute, nb, kBlock, n, perm, s->px[c], s->py[c], s->vx[c],
          s->vy[c], s->m[c], s->r[c], s->id[c], s->px[o], s->py[o], s->vx[o],
@@ -579,6 +614,18 @@ int syncAndCheck(nbx_sim *s) {
     return setError(NBX_ERR_CAPACITY,
                     "too many large boxes for the Jacobi collide's partial sums");
   }
+  if (status & kStatusSortStalled) {
+    // Leave the barrier words at rest for the next sort.
+    const uint32_t cleared = status & ~kStatusSortStalled;
+    NBX_HIP(hipMemsetAsync(s->sortBarrier, 0, kBarrierWords * sizeof(uint32_t),
+                           s->stream));
+    NBX_HIP(hipMemcpyAsync(&s->ctl->status, &cleared, sizeof(uint32_t),
+                           hipMemcpyHostToDevice, s->stream));
+    NBX_HIP(hipStreamSynchronize(s->stream));
+    return setError(NBX_ERR_STATE,
+                    "radix sort grid barrier timed out (workgroups were not "
+                    "all resident)");
+  }
   if (status & kStatusComponentTooLarge) {
     return setError(NBX_ERR_CAPACITY,
                     "a collision contact component exceeds 4096 pairs");
@@ -686,7 +733,7 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
     A(s->px[b], n, false) A(s->py[b], n, false) A(s->vx[b], n, false)
     A(s->vy[b], n, false) A(s->m[b], n, false) A(s->r[b], n, false)
     A(s->id[b], n, false)
-    A(s->keyHi[b], n, false) A(s->vals[b], n, false)
+    A(s->keyHi[b], n, true) A(s->vals[b], n, true)
   }
   A(s->ax, n, true) A(s->ay, n, true)
   s->buckets = std::max<uint64_t>(1024, pow2AtLeast(2 * n));
@@ -702,8 +749,27 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->compCount, n, true) A(s->compStart, n, false) A(s->compInfo, n, false)
   A(s->bigList, n, false)
   A(s->keyLo, n, false)
-  A(s->sortHist, 256ull * blocksFor(n, kSortTile), false)
+  {
+    // The one-launch sort needs every one of its workgroups resident at
+    // once: at most one per CU, and only where one fits a CU.
+    int dev = 0, cus = 0, perCu = 0;
+    const char *mode = std::getenv("NBX_SORT");
+    const bool multi = mode && std::strcmp(mode, "multi") == 0;
+    if (!multi && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                              dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &perCu, reinterpret_cast<const void *>(k_sort_onelaunch<16>),
+            kPsBlock, 0) == hipSuccess &&
+        perCu >= 1 && cus >= 1) {
+      s->sortGrid = std::min<uint32_t>(static_cast<uint32_t>(cus), kPsMaxGrid);
+    }
+  }
+  A(s->sortHist,
+    std::max<uint64_t>(256ull * blocksFor(n, kSortTile), 256ull * s->sortGrid),
+    false)
   A(s->sortRowTotal, 256, false)
+  A(s->sortBarrier, kBarrierWords, true)
   A(s->deepList, n / (kLeafCapacity + 1) + 1, false)
   A(s->tileCount, static_cast<uint64_t>(kMaxDepth) * blocksFor(n), false)
   s->nodeCap = static_cast<int64_t>(8 * n + 8);

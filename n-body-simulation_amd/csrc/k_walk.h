@@ -237,6 +237,41 @@ __device__ inline uint64_t uniform64(uint64_t v) {
   return static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32);
 }
 
+// The wave's traversal stack: entries 0..63 live in VGPR lanes (entry e in
+// lane e of three registers: node, mask low and high words), written by a
+// lane-select move and read with v_readlane at the wave-uniform stack
+// pointer -- no LDS traffic and no lane-0 stores under an exec mask. Entries
+// 64..127 (only trees deeper than ~20 levels reach them) spill to this
+// wave's LDS area.
+struct LaneStack {
+  int node = 0, lo = 0, hi = 0;
+  int *spillNode;       // kStackDepth - kWave entries
+  uint64_t *spillMask;  // kStackDepth - kWave entries
+
+  __device__ inline void put(int e, int nd, uint64_t m) {
+    if (e < kWave) {
+      const bool mine = static_cast<int>(threadIdx.x & (kWave - 1)) == e;
+      node = mine ? nd : node;
+      lo = mine ? static_cast<int>(static_cast<uint32_t>(m)) : lo;
+      hi = mine ? static_cast<int>(static_cast<uint32_t>(m >> 32)) : hi;
+    } else if ((threadIdx.x & (kWave - 1)) == 0) {
+      spillNode[e - kWave] = nd;
+      spillMask[e - kWave] = m;
+    }
+  }
+  __device__ inline void get(int e, int &nd, uint64_t &m) const {
+    if (e < kWave) {
+      nd = __builtin_amdgcn_readlane(node, e);
+      const uint32_t mlo = static_cast<uint32_t>(__builtin_amdgcn_readlane(lo, e));
+      const uint32_t mhi = static_cast<uint32_t>(__builtin_amdgcn_readlane(hi, e));
+      m = static_cast<uint64_t>(mlo) | (static_cast<uint64_t>(mhi) << 32);
+    } else {
+      nd = __builtin_amdgcn_readfirstlane(spillNode[e - kWave]);
+      m = uniform64(spillMask[e - kWave]);
+    }
+  }
+};
+
 // CAP = false drops the min(., FLT_MAX) guard; the host selects it only when
 // no factor can overflow (see capFree in the launch code).
 template <int LAW, bool CAP>
@@ -257,6 +292,49 @@ __device__ inline void leafTerm(const float4 &b, float x, float y, float eSq,
   qy = fmaf(by, fac, qy);
 }
 
+// A leaf's bodies for the lanes of `near` when few lanes need them. The
+// (lane, record) pairs are spread over the wave: with k near lanes and K the
+// smallest power of two >= k, lane slot r = lane % K takes the r-th near
+// lane's body and group g = lane / K takes records g, g + 64/K, ... -- so
+// the leaf costs ceil(cnt * K / 64) term evaluations instead of cnt. The
+// groups' partial sums are added across lanes (butterfly over the group
+// bits) and each near lane takes its slot's total. `slot` is this wave's
+// 64-entry LDS area.
+template <int LAW, bool CAP>
+__device__ inline void leafSpread(uint64_t near, uint32_t k, uint32_t logK,
+                                  int s, int cnt, float x, float y, float eSq,
+                                  const float4 *__restrict__ body,
+                                  float2 *__restrict__ slot, float &qx,
+                                  float &qy) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t rank =
+      static_cast<uint32_t>(__popcll(near & laneMaskLt()));
+  const bool isNear = __builtin_amdgcn_inverse_ballot_w64(near);
+  if (isNear) {
+    slot[rank] = make_float2(x, y);
+  }
+  const uint32_t r = lane & ((1u << logK) - 1u);
+  const int g = static_cast<int>(lane >> logK);
+  const int groups = kWave >> logK;
+  float px = 0.0f, py = 0.0f;
+  if (r < k) {
+    const float2 tp = slot[r];
+    for (int j = g; j < cnt; j += groups) {
+      leafTerm<LAW, CAP>(body[s + j], tp.x, tp.y, eSq, px, py);
+    }
+  }
+  for (uint32_t off = 1u << logK; off < kWave; off <<= 1) {
+    px += __shfl_xor(px, off, kWave);
+    py += __shfl_xor(py, off, kWave);
+  }
+  const float ax = __shfl(px, static_cast<int>(rank), kWave);
+  const float ay = __shfl(py, static_cast<int>(rank), kWave);
+  if (isNear) {
+    qx += ax;
+    qy += ay;
+  }
+}
+
 // One node for the lanes of `mask`. Lane membership is carried as wave
 // masks (inverse ballots), so the branches below cost scalar instructions
 // only. The opening test is rounded exactly as the reference rounds it
@@ -267,7 +345,8 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
                                      uint64_t mask, float x, float y,
                                      float tSq, float eSq,
                                      const float4 *__restrict__ body,
-                                     float &qx, float &qy, uint32_t &nFar,
+                                     float2 *__restrict__ slot, float &qx,
+                                     float &qy, uint32_t &nFar,
                                      uint32_t &nPair, uint32_t &nLeafTerm) {
   const float dx = f.x - x;
   const float dy = f.y - y;
@@ -277,7 +356,7 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
   // the bit patterns in (0, 0x7f800000].
   const int32_t mb = __float_as_int(f.z);
   const uint64_t farMask = (mb > 0 && mb <= 0x7f800000) ? (mask & ~near) : 0ull;
-  if (farMask != 0 && __builtin_amdgcn_inverse_ballot_w64(farMask)) {
+  if (__builtin_amdgcn_inverse_ballot_w64(farMask)) {
     const float fac = fastFactor<LAW, CAP>(f.z, dSq, eSq);
     qx = fmaf(dx, fac, qx);
     qy = fmaf(dy, fac, qy);
@@ -290,6 +369,23 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
   }
   if (near != 0) {
     const int s = c.z, cnt = c.w - c.z;
+    // Spread the pairs over the wave when that takes fewer instructions
+    // (about 18 per record in the loop below; 16 + 6 per butterfly step +
+    // 20 per spread evaluation).
+    const uint32_t k = static_cast<uint32_t>(__popcll(near));
+    const uint32_t logK = k <= 1 ? 0u : 32u - __clz(k - 1);
+    const int iters = (cnt + (kWave >> logK) - 1) >> (6 - logK);
+    if (16 + 6 * static_cast<int>(6 - logK) + 20 * iters < 18 * cnt) {
+      if (COUNT) {
+        nLeafTerm += static_cast<uint32_t>(iters);
+        if (__builtin_amdgcn_inverse_ballot_w64(near)) {
+          nPair += static_cast<uint32_t>(cnt);
+        }
+      }
+      leafSpread<LAW, CAP>(near, k, logK, s, cnt, x, y, eSq, body, slot, qx,
+                           qy);
+      return 0;
+    }
     if (COUNT) {
       nLeafTerm += static_cast<uint32_t>(cnt);
     }
@@ -327,8 +423,9 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
     const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
     float tSq, float eSq, float *__restrict__ ax, float *__restrict__ ay,
     Ctl *__restrict__ ctl) {
-  __shared__ int stNode[kWalkBlock / kWave][kStackDepth];
-  __shared__ uint64_t stMask[kWalkBlock / kWave][kStackDepth];
+  __shared__ float2 slots[kWalkBlock / kWave][kWave];
+  __shared__ int spillNode[kWalkBlock / kWave][kStackDepth - kWave];
+  __shared__ uint64_t spillMask[kWalkBlock / kWave][kStackDepth - kWave];
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t base =
@@ -347,26 +444,27 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
   const int nodeCount = ctl->nodeCount;
 
   // The root on its own, then blocks of four siblings from the stack.
+  LaneStack st;
+  st.spillNode = spillNode[w];
+  st.spillMask = spillMask[w];
   int sp = 0;
   {
     const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
     const uint64_t open =
         visitNode<LAW, COUNT, CAP>(nodeF[0], nodeI[0], all, x, y, tSq, eSq, body,
-                              qx, qy, nFar, nPair, nLeafTerm);
+                                   slots[w], qx, qy, nFar, nPair, nLeafTerm);
     nVisit += 1;
     if (open) {
-      if (lane == 0) {
-        stNode[w][0] = nodeI[0].x;
-        stMask[w][0] = open;
-      }
+      st.put(0, nodeI[0].x, open);
       sp = 1;
     }
   }
   int blocks = 0;
   while (sp > 0) {
     --sp;
-    const int cb = __builtin_amdgcn_readfirstlane(stNode[w][sp]);
-    const uint64_t mask = uniform64(stMask[w][sp]);
+    int cb;
+    uint64_t mask;
+    st.get(sp, cb, mask);
     if (cb <= 0 || cb + 3 >= nodeCount || ++blocks > nodeCount) {
       if (lane == 0) {
         atomicOr(&ctl->status, kStatusTreeInconsistent);
@@ -393,8 +491,9 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
     uint64_t open[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      open[q] = visitNode<LAW, COUNT, CAP>(bf[q], bi[q], mask, x, y, tSq, eSq, body,
-                                      qx, qy, nFar, nPair, nLeafTerm);
+      open[q] = visitNode<LAW, COUNT, CAP>(bf[q], bi[q], mask, x, y, tSq, eSq,
+                                           body, slots[w], qx, qy, nFar,
+                                           nPair, nLeafTerm);
     }
     if (COUNT) {
       nVisit += 4;
@@ -410,10 +509,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
           sp = 0;
           break;
         }
-        if (lane == 0) {
-          stNode[w][sp] = bi[q].x;
-          stMask[w][sp] = open[q];
-        }
+        st.put(sp, bi[q].x, open[q]);
         ++sp;
       }
     }

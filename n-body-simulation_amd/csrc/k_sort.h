@@ -372,17 +372,27 @@ __device__ inline void gridBarrier(uint32_t *__restrict__ bar, uint32_t blocks,
 // ends in (keys0, vals0). table holds 256 * gridDim.x words, rowTotal 256;
 // chunk <= ITEMS * 1024. Global writes are bounds-checked, so even a stalled
 // barrier leaves every value in range.
-template <int ITEMS>
+//
+// BUILD: the tree build's sort, with the bounding box and keys folded in
+// (what k_bbox and k_keys do in two launches): each workgroup reduces its
+// chunk's bounding box and publishes it to parts[b]; after a grid barrier
+// every workgroup reduces all G partials to the root quad, computes its
+// bodies' float-descent keys (low words to keyLo, indices as payloads) and
+// ranks the high words of pass 0 straight from registers. Workgroup 0
+// publishes the root quad and clears the build's counters.
+template <int ITEMS, bool BUILD>
 __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
     uint32_t n, uint32_t chunk, uint32_t *keys0, uint32_t *vals0,
     uint32_t *keys1, uint32_t *vals1, uint32_t *table, uint32_t *rowTotal,
-    uint32_t *bar, Ctl *ctl) {
+    uint32_t *bar, Ctl *ctl, const float *px, const float *py,
+    uint32_t *parts, uint32_t *keyLo) {
   constexpr int kTile = ITEMS * kPsBlock;
   __shared__ uint32_t sk[kTile];
   __shared__ uint32_t sv[kTile];
   __shared__ uint32_t wh[kPsWaves][256]; // per wave: digit counts, then offsets
   __shared__ uint32_t dstart[256], goff[256];
   __shared__ uint32_t scratch[kPsWaves];
+  __shared__ uint4 red[kPsWaves];
 
   const uint32_t t = threadIdx.x;
   const uint32_t b = blockIdx.x, grid = gridDim.x;
@@ -401,6 +411,66 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
     seg <<= 1;
   }
 
+  uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+  if (BUILD) {
+    // Positions are read twice (bounding box, then keys after the barrier;
+    // the second read hits L2) rather than held across the barrier.
+    uint4 bb = bboxEmpty();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t p = first + j * kWave;
+      if (p < cnt) {
+        bb = bboxAdd(bb, px[lo + p], py[lo + p]);
+      }
+    }
+    bb = bboxWave(bb);
+    if (lane == 0) {
+      red[wave] = bb;
+    }
+    __syncthreads();
+    if (t == 0) {
+      for (int w = 1; w < kPsWaves; ++w) {
+        bb = bboxMerge(bb, red[w]);
+      }
+      storeAgent(parts + 4 * b, bb.x);
+      storeAgent(parts + 4 * b + 1, bb.y);
+      storeAgent(parts + 4 * b + 2, bb.z);
+      storeAgent(parts + 4 * b + 3, bb.w);
+    }
+    gridBarrier(bar, grid, status);
+    bb = bboxEmpty();
+    if (t < grid) {
+      bb = make_uint4(loadAgent(parts + 4 * t), loadAgent(parts + 4 * t + 1),
+                      loadAgent(parts + 4 * t + 2),
+                      loadAgent(parts + 4 * t + 3));
+    }
+    bb = bboxWave(bb);
+    if (lane == 0) {
+      red[wave] = bb;
+    }
+    __syncthreads();
+    bb = red[0];
+    for (int w = 1; w < kPsWaves; ++w) {
+      bb = bboxMerge(bb, red[w]);
+    }
+    float qcx, qcy, qs;
+    rootQuad(bb, qcx, qcy, qs);
+    if (b == 0 && t == 0) {
+      publishRoot(ctl, bb, qcx, qcy, qs);
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t p = first + j * kWave;
+      uint32_t h = 0, l = 0;
+      if (p < cnt) {
+        descentKeys(px[lo + p], py[lo + p], qcx, qcy, qs, h, l);
+        keyLo[lo + p] = l;
+      }
+      key[j] = h;
+      val[j] = lo + p;
+    }
+  }
+
   for (int pass = 0; pass < 4; ++pass) {
     const int shift = 8 * pass;
     const uint32_t *kin = (pass & 1) ? keys1 : keys0;
@@ -412,12 +482,13 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
     for (uint32_t i = t; i < kPsWaves * 256; i += kPsBlock) {
       (&wh[0][0])[i] = 0;
     }
-    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+    if (!BUILD || pass > 0) {
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      const uint32_t p = first + j * kWave;
-      key[j] = p < cnt ? loadAgent(kin + lo + p) : 0u;
-      val[j] = p < cnt ? loadAgent(vin + lo + p) : 0u;
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t p = first + j * kWave;
+        key[j] = p < cnt ? loadAgent(kin + lo + p) : 0u;
+        val[j] = p < cnt ? loadAgent(vin + lo + p) : 0u;
+      }
     }
     __syncthreads();
 #pragma unroll

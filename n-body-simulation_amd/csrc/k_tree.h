@@ -65,39 +65,19 @@ __global__ __launch_bounds__(kBlock) void k_bbox(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
     uint4 *__restrict__ parts) {
   __shared__ uint4 red[kBlock / kWave];
-  uint32_t mnx = 0xffffffffu, mxx = 0u, mny = 0xffffffffu, mxy = 0u;
+  uint4 b = bboxEmpty();
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n;
        i += gridDim.x * kBlock) {
-    const float x = px[i], y = py[i];
-    if (x == x) {
-      const uint32_t u = ordBits(x);
-      mnx = min(mnx, u);
-      mxx = max(mxx, u);
-    }
-    if (y == y) {
-      const uint32_t u = ordBits(y);
-      mny = min(mny, u);
-      mxy = max(mxy, u);
-    }
+    b = bboxAdd(b, px[i], py[i]);
   }
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    mnx = min(mnx, static_cast<uint32_t>(__shfl_xor(mnx, off, kWave)));
-    mxx = max(mxx, static_cast<uint32_t>(__shfl_xor(mxx, off, kWave)));
-    mny = min(mny, static_cast<uint32_t>(__shfl_xor(mny, off, kWave)));
-    mxy = max(mxy, static_cast<uint32_t>(__shfl_xor(mxy, off, kWave)));
-  }
+  b = bboxWave(b);
   if ((threadIdx.x & (kWave - 1)) == 0) {
-    red[threadIdx.x >> 6] = make_uint4(mnx, mxx, mny, mxy);
+    red[threadIdx.x >> 6] = b;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint4 b = red[0];
     for (int w = 1; w < kBlock / kWave; ++w) {
-      b.x = min(b.x, red[w].x);
-      b.y = max(b.y, red[w].y);
-      b.z = min(b.z, red[w].z);
-      b.w = max(b.w, red[w].w);
+      b = bboxMerge(b, red[w]);
     }
     parts[blockIdx.x] = b;
   }
@@ -114,69 +94,25 @@ __global__ __launch_bounds__(kBlock) void k_keys(
     uint32_t *__restrict__ vals) {
   __shared__ uint4 red[kBlock / kWave];
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint4 b = threadIdx.x < nparts ? parts[threadIdx.x]
-                                 : make_uint4(0xffffffffu, 0u, 0xffffffffu, 0u);
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    b.x = min(b.x, static_cast<uint32_t>(__shfl_xor(b.x, off, kWave)));
-    b.y = max(b.y, static_cast<uint32_t>(__shfl_xor(b.y, off, kWave)));
-    b.z = min(b.z, static_cast<uint32_t>(__shfl_xor(b.z, off, kWave)));
-    b.w = max(b.w, static_cast<uint32_t>(__shfl_xor(b.w, off, kWave)));
-  }
+  uint4 b = bboxWave(threadIdx.x < nparts ? parts[threadIdx.x] : bboxEmpty());
   if ((threadIdx.x & (kWave - 1)) == 0) {
     red[threadIdx.x >> 6] = b;
   }
   __syncthreads();
   b = red[0];
   for (int w = 1; w < kBlock / kWave; ++w) {
-    b.x = min(b.x, red[w].x);
-    b.y = max(b.y, red[w].y);
-    b.z = min(b.z, red[w].z);
-    b.w = max(b.w, red[w].w);
+    b = bboxMerge(b, red[w]);
   }
-  const float minX = ordFloat(b.x), maxX = ordFloat(b.y);
-  const float minY = ordFloat(b.z), maxY = ordFloat(b.w);
-  const float qcx = (minX + maxX) * 0.5f;
-  const float qcy = (minY + maxY) * 0.5f;
-  const float wx = maxX - minX, wy = maxY - minY;
-  const float qs = (wx < wy) ? wy : wx;
+  float qcx, qcy, qs;
+  rootQuad(b, qcx, qcy, qs);
   if (i == 0) {
-    ctl->bbox[0] = b.x;
-    ctl->bbox[1] = b.y;
-    ctl->bbox[2] = b.z;
-    ctl->bbox[3] = b.w;
-    ctl->quad[0] = qcx;
-    ctl->quad[1] = qcy;
-    ctl->quad[2] = qs;
-    ctl->deepCount = 0;
-    ctl->subtreeRoots = 0;
-    ctl->status &= ~(kStatusNodeOverflow | kStatusDeepTooLarge |
-                     kStatusTreeInconsistent);
+    publishRoot(ctl, b, qcx, qcy, qs);
   }
   if (i >= n) {
     return;
   }
-  const float x = px[i], y = py[i];
-  float cx = qcx, cy = qcy, s = qs;
-  uint32_t hi = 0, lo = 0;
-#pragma unroll 4
-  for (int d = 0; d < 16; ++d) {
-    const int bx = (x < cx) ? 0 : 1;
-    const int by = (y < cy) ? 0 : 1;
-    hi |= static_cast<uint32_t>(2 * by + bx) << (30 - 2 * d);
-    s *= 0.5f;
-    cx += (static_cast<float>(bx) - 0.5f) * s;
-    cy += (static_cast<float>(by) - 0.5f) * s;
-  }
-#pragma unroll 4
-  for (int d = 0; d < 16; ++d) {
-    const int bx = (x < cx) ? 0 : 1;
-    const int by = (y < cy) ? 0 : 1;
-    lo |= static_cast<uint32_t>(2 * by + bx) << (30 - 2 * d);
-    s *= 0.5f;
-    cx += (static_cast<float>(bx) - 0.5f) * s;
-    cy += (static_cast<float>(by) - 0.5f) * s;
-  }
+  uint32_t hi, lo;
+  descentKeys(px[i], py[i], qcx, qcy, qs, hi, lo);
   keyHi[i] = hi;
   keyLo[i] = lo;
   vals[i] = i;

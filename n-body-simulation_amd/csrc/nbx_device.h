@@ -100,6 +100,95 @@ __device__ inline uint64_t matchDigit8(uint32_t d, bool valid) {
   return m;
 }
 
+// Exact bounding box as order-preserving bits {min x, max x, min y, max y};
+// NaN coordinates are ignored, as std::min/max ignore them in
+// Quad::newContaining (QuadTree.cpp:30-51).
+__device__ inline uint4 bboxEmpty() {
+  return make_uint4(0xffffffffu, 0u, 0xffffffffu, 0u);
+}
+__device__ inline uint4 bboxAdd(uint4 b, float x, float y) {
+  if (x == x) {
+    const uint32_t u = ordBits(x);
+    b.x = min(b.x, u);
+    b.y = max(b.y, u);
+  }
+  if (y == y) {
+    const uint32_t u = ordBits(y);
+    b.z = min(b.z, u);
+    b.w = max(b.w, u);
+  }
+  return b;
+}
+__device__ inline uint4 bboxMerge(uint4 a, uint4 b) {
+  return make_uint4(min(a.x, b.x), max(a.y, b.y), min(a.z, b.z),
+                    max(a.w, b.w));
+}
+__device__ inline uint4 bboxWave(uint4 b) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    b.x = min(b.x, static_cast<uint32_t>(__shfl_xor(b.x, off, kWave)));
+    b.y = max(b.y, static_cast<uint32_t>(__shfl_xor(b.y, off, kWave)));
+    b.z = min(b.z, static_cast<uint32_t>(__shfl_xor(b.z, off, kWave)));
+    b.w = max(b.w, static_cast<uint32_t>(__shfl_xor(b.w, off, kWave)));
+  }
+  return b;
+}
+
+// Root quad from the bounding box (QuadTree.cpp:46-50): centre and size.
+__device__ inline void rootQuad(uint4 b, float &qcx, float &qcy, float &qs) {
+  const float minX = ordFloat(b.x), maxX = ordFloat(b.y);
+  const float minY = ordFloat(b.z), maxY = ordFloat(b.w);
+  qcx = (minX + maxX) * 0.5f;
+  qcy = (minY + maxY) * 0.5f;
+  const float wx = maxX - minX, wy = maxY - minY;
+  qs = (wx < wy) ? wy : wx;
+}
+
+// Publishes the build's bounding box and root quad and clears its counters
+// and status bits (one lane, before any build kernel after the keys).
+__device__ inline void publishRoot(Ctl *ctl, uint4 b, float qcx, float qcy,
+                                   float qs) {
+  ctl->bbox[0] = b.x;
+  ctl->bbox[1] = b.y;
+  ctl->bbox[2] = b.z;
+  ctl->bbox[3] = b.w;
+  ctl->quad[0] = qcx;
+  ctl->quad[1] = qcy;
+  ctl->quad[2] = qs;
+  ctl->deepCount = 0;
+  ctl->subtreeRoots = 0;
+  atomicAnd(&ctl->status, ~(kStatusNodeOverflow | kStatusDeepTooLarge |
+                            kStatusTreeInconsistent));
+}
+
+// Float-descent quadrant key of a body (QuadTree.cpp:57-90 partitions,
+// Quad::intoQuadrant QuadTree.h:14-25): digit 2*(y >= cy) + (x >= cx) at
+// each of 32 depths, the centre moved as the reference moves it; depths
+// 0-15 in `hi`, 16-31 in `lo`, two bits each from the top.
+__device__ inline void descentKeys(float x, float y, float cx, float cy,
+                                   float s, uint32_t &hi, uint32_t &lo) {
+  hi = 0;
+  lo = 0;
+#pragma unroll 4
+  for (int d = 0; d < 16; ++d) {
+    const int bx = (x < cx) ? 0 : 1;
+    const int by = (y < cy) ? 0 : 1;
+    hi |= static_cast<uint32_t>(2 * by + bx) << (30 - 2 * d);
+    s *= 0.5f;
+    cx += (static_cast<float>(bx) - 0.5f) * s;
+    cy += (static_cast<float>(by) - 0.5f) * s;
+  }
+#pragma unroll 4
+  for (int d = 0; d < 16; ++d) {
+    const int bx = (x < cx) ? 0 : 1;
+    const int by = (y < cy) ? 0 : 1;
+    lo |= static_cast<uint32_t>(2 * by + bx) << (30 - 2 * d);
+    s *= 0.5f;
+    cx += (static_cast<float>(bx) - 0.5f) * s;
+    cy += (static_cast<float>(by) - 0.5f) * s;
+  }
+}
+
 // QuadTree.cpp:11-26 fastInvSqrt: bit-level guess + one Newton step.
 __device__ inline float fisr(float x) {
   const float xhalf = 0.5f * x;

@@ -231,41 +231,62 @@ int launchScan(nbx_sim *s, uint32_t count, const uint32_t *in, uint32_t *out) {
   return NBX_OK;
 }
 
+// One launch of k_sort_onelaunch (one workgroup per 1024 x 1/2/4/8/16 keys,
+// the smallest tile that lets at most one workgroup per CU cover n).
+// Returns false when no tile fits (or the handle uses three launches a
+// pass); build = the tree build's sort with its keys computed in the launch.
+bool launchOneLaunchSort(nbx_sim *s, uint32_t n, uint32_t *const keys[2],
+                         uint32_t *const vals[2], bool build, int *rc) {
+  *rc = NBX_OK;
+  if (!s->sortGrid) {
+    return false;
+  }
+  const int c = s->cur;
+  const float *px = build ? s->px[c] : nullptr;
+  const float *py = build ? s->py[c] : nullptr;
+  uint32_t *parts = reinterpret_cast<uint32_t *>(s->bboxParts);
+#define SORT_LAUNCH(ITEMS)                                                    \
+  {                                                                           \
+    const uint32_t g = blocksFor(n, (ITEMS) * kPsBlock);                      \
+    if (g <= s->sortGrid) {                                                   \
+      const uint32_t chunk = (n + g - 1) / g;                                 \
+      if (build) {                                                            \
+        hipLaunchKernelGGL((k_sort_onelaunch<ITEMS, true>), dim3(g),          \
+                           dim3(kPsBlock), 0, s->stream, n, chunk, keys[0],   \
+                           vals[0], keys[1], vals[1], s->sortHist,            \
+                           s->sortRowTotal, s->sortBarrier, s->ctl, px, py,   \
+                           parts, s->keyLo);                                  \
+      } else {                                                                \
+        hipLaunchKernelGGL((k_sort_onelaunch<ITEMS, false>), dim3(g),         \
+                           dim3(kPsBlock), 0, s->stream, n, chunk, keys[0],   \
+                           vals[0], keys[1], vals[1], s->sortHist,            \
+                           s->sortRowTotal, s->sortBarrier, s->ctl, px, py,   \
+                           parts, s->keyLo);                                  \
+      }                                                                       \
+      *rc = checkLaunch();                                                    \
+      return true;                                                            \
+    }                                                                         \
+  }
+  SORT_LAUNCH(1)
+  SORT_LAUNCH(2)
+  SORT_LAUNCH(4)
+  SORT_LAUNCH(8)
+  SORT_LAUNCH(16)
+#undef SORT_LAUNCH
+  return false;
+}
+
 // Stable LSD radix sort of 32-bit keys with 32-bit payloads, 4 x 8 bits,
-// from buffers 0 to buffers 0 (through 1): one launch of k_sort_onelaunch
-// (one workgroup per 4096 keys, at most one per CU), or, where that kernel
-// cannot be resident on every CU, three launches per pass.
+// from buffers 0 to buffers 0 (through 1): one launch where it fits, else
+// three launches per pass.
 int launchRadixSort(nbx_sim *s, uint32_t n, uint32_t *const keys[2],
                     uint32_t *const vals[2]) {
   if (n == 0) {
     return NBX_OK;
   }
-  if (s->sortGrid) {
-    // Smallest tile that lets the grid cover n; above that, three launches.
-    const uint32_t g4 = blocksFor(n, 4 * kPsBlock);
-    const uint32_t g8 = blocksFor(n, 8 * kPsBlock);
-    const uint32_t g16 = blocksFor(n, 16 * kPsBlock);
-    if (g4 <= s->sortGrid) {
-      const uint32_t chunk = (n + g4 - 1) / g4;
-      LAUNCH(k_sort_onelaunch<4>, g4, kPsBlock, n, chunk, keys[0], vals[0],
-             keys[1], vals[1], s->sortHist, s->sortRowTotal, s->sortBarrier,
-             s->ctl);
-      return NBX_OK;
-    }
-    if (g8 <= s->sortGrid) {
-      const uint32_t chunk = (n + g8 - 1) / g8;
-      LAUNCH(k_sort_onelaunch<8>, g8, kPsBlock, n, chunk, keys[0], vals[0],
-             keys[1], vals[1], s->sortHist, s->sortRowTotal, s->sortBarrier,
-             s->ctl);
-      return NBX_OK;
-    }
-    if (g16 <= s->sortGrid) {
-      const uint32_t chunk = (n + g16 - 1) / g16;
-      LAUNCH(k_sort_onelaunch<16>, g16, kPsBlock, n, chunk, keys[0], vals[0],
-             keys[1], vals[1], s->sortHist, s->sortRowTotal, s->sortBarrier,
-             s->ctl);
-      return NBX_OK;
-    }
+  int rc = NBX_OK;
+  if (launchOneLaunchSort(s, n, keys, vals, false, &rc)) {
+    return rc;
   }
   const uint32_t tiles = blocksFor(n, kSortTile);
   for (int pass = 0; pass < 4; ++pass) {
@@ -374,11 +395,18 @@ int launchBuild(nbx_sim *s) {
   const int c = s->cur, o = c ^ 1;
   const uint32_t nb = blocksFor(n);
   const uint32_t parts = std::min<uint32_t>(nb, kBboxParts);
-  LAUNCH(k_bbox, parts, kBlock, n, s->px[c], s->py[c], s->bboxParts);
-  LAUNCH(k_keys, nb, kBlock, n, s->px[c], s->py[c], s->bboxParts, parts, s->ctl,
-         s->keyHi[0], s->keyLo, s->vals[0]);
-  // Stable LSD radix sort of the high key word; ends in buffer 0.
-  NBX_TRY(launchRadixSort(s, n, s->keyHi, s->vals));
+  // Bounding box, keys and the stable LSD radix sort of the high key words
+  // (ending in buffer 0): one launch, or k_bbox, k_keys and the sort's
+  // passes.
+  int rc = NBX_OK;
+  if (launchOneLaunchSort(s, n, s->keyHi, s->vals, true, &rc)) {
+    NBX_TRY(rc);
+  } else {
+    LAUNCH(k_bbox, parts, kBlock, n, s->px[c], s->py[c], s->bboxParts);
+    LAUNCH(k_keys, nb, kBlock, n, s->px[c], s->py[c], s->bboxParts, parts,
+           s->ctl, s->keyHi[0], s->keyLo, s->vals[0]);
+    NBX_TRY(launchRadixSort(s, n, s->keyHi, s->vals));
+  }
   const uint32_t *hi = s->keyHi[0];
   uint32_t *perm = s->vals[0];
   LAUNCH(k_deep_detect, nb, kBlock, n, hi, s->keyLo, perm, s->ctl, s->deepList);
@@ -759,7 +787,7 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                               dev) == hipSuccess &&
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &perCu, reinterpret_cast<const void *>(k_sort_onelaunch<16>),
+            &perCu, reinterpret_cast<const void *>(k_sort_onelaunch<16, true>),
             kPsBlock, 0) == hipSuccess &&
         perCu >= 1 && cus >= 1) {
       s->sortGrid = std::min<uint32_t>(static_cast<uint32_t>(cus), kPsMaxGrid);
@@ -782,7 +810,7 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->subtreeRoots, s->nodeCap, false)
   A(s->nodeDone, s->nodeCap, false)
   A(s->bodyPack, n + kBodyPad, true)
-  A(s->bboxParts, kBboxParts, false)
+  A(s->bboxParts, std::max<uint32_t>(kBboxParts, s->sortGrid), false)
   A(s->ctl, 1, true)
   {
     const uint64_t chunks = blocksFor(n, kJacChunk);

@@ -112,24 +112,26 @@ def test_build_bit_exact(nbx, oracle, states, name):
 
 
 @pytest.mark.parametrize("mode", ["one-launch", "multi"])
-@pytest.mark.parametrize("n", [4097, 300_000, 1_100_000, 2_500_000])
+@pytest.mark.parametrize("n", [4097, 300_000, 700_000, 1_100_000, 2_500_000])
 def test_sort_paths_bit_exact(nbx, oracle, monkeypatch, mode, n):
-    """Both radix sorts (one launch with grid barriers; three launches a
-    pass, selected by NBX_SORT=multi when the handle is created) against the
-    oracle's stable sort, with ragged chunks and each of the one-launch
-    kernel's tile sizes (4, 8 and 16 keys per lane: 2, 74, 135 and 153
-    workgroups)."""
+    """Both build sorts (one launch with the bounding box, keys and grid
+    barriers folded in; k_bbox, k_keys and three launches a pass, selected by
+    NBX_SORT=multi when the handle is created) against the oracle's keys and
+    stable sort, with ragged chunks and each of the one-launch kernel's tile
+    sizes: 1, 2, 4, 8 and 16 keys per lane (5, 147, 171, 135 and 153
+    workgroups on 256 CUs)."""
     if mode == "multi":
         monkeypatch.setenv("NBX_SORT", "multi")
     else:
         monkeypatch.delenv("NBX_SORT", raising=False)
     s = oracle.uniform_disc(n)
-    _, hi, lo, perm, _ = oracle.sorted_state(s)
+    _, hi, lo, perm, quad = oracle.sorted_state(s)
     with new_sim(nbx, s) as sim:
         # The build leaves the bodies in key order, so a second build sorts
         # sorted keys: same keys, identity permutation.
         for want_perm in (perm, np.arange(n, dtype=np.uint32)):
             sim.build()
+            assert_bits_equal(sim.root_quad(), quad, "root quad")
             ghi, glo, gperm = sim.keys()
             assert np.array_equal(ghi, hi), "sorted high key words"
             assert np.array_equal(glo, lo), "low key words in sorted order"

@@ -217,7 +217,8 @@ int nbx_download_contacts(nbx_sim *sim, int32_t *contacts,
  * walk's wavefront work: node visits summed over waves and leaf-body terms
  * evaluated per wave (each costs all 64 lanes), so lane utilisation is
  * (far + pairs) / (64 * (visits + leaf terms)). Any pointer may be null.
- * Does not change the state. (No reference counterpart: instrumentation.) */
+ * Does not change the state; NBX_ERR_STATE before any tree build (or after
+ * an upload). (No reference counterpart: instrumentation.) */
 int nbx_count_interactions(nbx_sim *sim, uint64_t *far_nodes,
                            uint64_t *body_pairs, uint64_t *wave_visits,
                            uint64_t *wave_leaf_terms);

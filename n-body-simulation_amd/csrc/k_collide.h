@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_cells(
     uint32_t mask, uint32_t *__restrict__ bucketOf,
     uint32_t *__restrict__ bucketRank,
     uint32_t *__restrict__ bucketCount, int32_t *__restrict__ largeList,
-    int32_t *__restrict__ ufParent) {
+    float4 *__restrict__ largeBox, int32_t *__restrict__ ufParent) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) {
     return;
@@ -296,6 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_cells(
   if (b.maxX - b.minX > cell || b.maxY - b.minY > cell) {
     const uint32_t slot = atomicAdd(&ctl->largeCount, 1u);
     largeList[slot] = static_cast<int32_t>(i);
+    largeBox[slot] = make_float4(b.minX, b.maxX, b.minY, b.maxY);
     bucketOf[i] = 0xffffffffu;
     return;
   }
@@ -407,7 +408,8 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
     uint32_t mask, const uint32_t *__restrict__ bucketOf,
     const uint32_t *__restrict__ bucketStart,
     const float4 *__restrict__ grid,
-    const int32_t *__restrict__ largeList, int4 *__restrict__ pairs,
+    const int32_t *__restrict__ largeList,
+    const float4 *__restrict__ largeBox, int4 *__restrict__ pairs,
     uint32_t cap) {
   __shared__ PairStageT<kFindStage> st;
   stageInit(st);
@@ -427,20 +429,54 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
       const uint32_t colMask = (1u << colBits) - 1u;
       const uint32_t col = static_cast<uint32_t>(mx) & colMask;
       const bool wraps = col == 0 || col == colMask;
-      uint32_t lo[9], hi[9];
-      int ranges = 0;
+      // Box and cell of an entry share one 32-byte record: both are
+      // requested before either is tested.
+      auto test = [&](const float4 &ob, const float4 &ocf, int32_t cy) {
+        const int32_t ox = __float_as_int(ocf.x), oy = __float_as_int(ocf.y);
+        const int other = __float_as_int(ocf.z);
+        if (oy != cy || ox < mx - 1 || ox > mx + 1 || other == self) {
+          return;
+        }
+        if (beforeBox(ob.x, other, me.minX, self) &&
+            overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
+          stagePair(st, ctl, pairs, cap, self, other, me.minX, ob.x);
+        }
+      };
+      if (!wraps) {
+        uint32_t lo[3], hi[3];
 #pragma unroll
-      for (int row = 0; row < 3; ++row) {
-        const int32_t cy = my + row - 1;
-        if (!wraps) {
-          const uint32_t h = rowBucket(mx, cy, colBits, mask);
+        for (int row = 0; row < 3; ++row) {
+          const uint32_t h = rowBucket(mx, my + row - 1, colBits, mask);
           lo[row] = bucketStart[h - 1];
           hi[row] = bucketStart[h + 2];
         }
-      }
-      if (!wraps) {
-        ranges = 3;
+        // The three rows' entries as one list, two entries per iteration
+        // (so a wave runs max-over-lanes of the total, not of each row, and
+        // two records are in flight per lane).
+        const uint32_t n0 = hi[0] - lo[0], n01 = n0 + (hi[1] - lo[1]);
+        const uint32_t total = n01 + (hi[2] - lo[2]);
+        auto at = [&](uint32_t q, int32_t &cy) {
+          cy = my + (q < n0 ? -1 : (q < n01 ? 0 : 1));
+          return q < n0 ? lo[0] + q : (q < n01 ? lo[1] + q - n0 : lo[2] + q - n01);
+        };
+        for (uint32_t q = 0; q < total; q += 2) {
+          int32_t cy0, cy1;
+          const uint32_t o0 = at(q, cy0);
+          const bool two = q + 1 < total;
+          const uint32_t o1 = two ? at(q + 1, cy1) : o0;
+          const float4 ob0 = grid[2 * o0], oc0 = grid[2 * o0 + 1];
+          const float4 ob1 = grid[2 * o1], oc1 = grid[2 * o1 + 1];
+          asm volatile("" ::"v"(ob0.x), "v"(ob0.y), "v"(ob0.z), "v"(ob0.w),
+                       "v"(oc0.x), "v"(oc0.y), "v"(oc0.z), "v"(ob1.x),
+                       "v"(ob1.y), "v"(ob1.z), "v"(ob1.w), "v"(oc1.x),
+                       "v"(oc1.y), "v"(oc1.z));
+          test(ob0, oc0, cy0);
+          if (two) {
+            test(ob1, oc1, cy1);
+          }
+        }
       } else {
+        uint32_t lo[9], hi[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
           const uint32_t h =
@@ -448,25 +484,15 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
           lo[k] = bucketStart[h];
           hi[k] = bucketStart[h + 1];
         }
-        ranges = 9;
-      }
-      for (int k = 0; k < ranges; ++k) {
-        const int32_t cy = my + (ranges == 3 ? k : k / 3) - 1;
-        for (uint32_t o = lo[k]; o < hi[k]; ++o) {
-          // Box and cell of an entry share one 32-byte record: both are
-          // requested before either is tested.
-          const float4 ob = grid[2 * o];
-          const float4 ocf = grid[2 * o + 1];
-          asm volatile("" ::"v"(ob.x), "v"(ob.y), "v"(ob.z), "v"(ob.w),
-                       "v"(ocf.x), "v"(ocf.y), "v"(ocf.z));
-          const int32_t ox = __float_as_int(ocf.x), oy = __float_as_int(ocf.y);
-          const int other = __float_as_int(ocf.z);
-          if (oy != cy || ox < mx - 1 || ox > mx + 1 || other == self) {
-            continue;
-          }
-          if (beforeBox(ob.x, other, me.minX, self) &&
-              overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
-            stagePair(st, ctl, pairs, cap, self, other, me.minX, ob.x);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          const int32_t cy = my + k / 3 - 1;
+          for (uint32_t o = lo[k]; o < hi[k]; ++o) {
+            const float4 ob = grid[2 * o];
+            const float4 ocf = grid[2 * o + 1];
+            asm volatile("" ::"v"(ob.x), "v"(ob.y), "v"(ob.z), "v"(ob.w),
+                         "v"(ocf.x), "v"(ocf.y), "v"(ocf.z));
+            test(ob, ocf, cy);
           }
         }
       }
@@ -474,12 +500,15 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
     // Large boxes (Collision.cpp:319-330): against every body, each pair
     // once -- a small body emits both orientations, a large one only the
     // pairs with later large boxes (the earlier one emits large-large pairs).
+    // Each large box's index and box were recorded together by k_cells: one
+    // round trip per large box.
     for (uint32_t k = 0; k < large; ++k) {
       const int l = largeList[k];
+      const float4 lbv = largeBox[k];
       if (l == self) {
         continue;
       }
-      const Box lb = bodyBox(px[l], py[l], r[l]);
+      const Box lb{lbv.x, lbv.y, lbv.z, lbv.w};
       if (beforeBox(lb.minX, l, me.minX, self)) {
         if (small && overlapsBox(lb, me)) {
           stagePair(st, ctl, pairs, cap, self, l, me.minX, lb.minX);

@@ -94,6 +94,7 @@ struct nbx_sim {
   uint32_t *scanTiles = nullptr;
   float4 *grid = nullptr; // per entry: box, then {cx, cy, index} (32 B)
   int32_t *largeList = nullptr, *ufParent = nullptr;
+  float4 *largeBox = nullptr; // {minX, maxX, minY, maxY} per large body
   uint32_t pairCap = 0;
   int4 *pairs = nullptr, *segPairs = nullptr;
   int32_t *pairComp = nullptr;
@@ -363,13 +364,14 @@ int launchCollide(nbx_sim *s) {
 
   LAUNCH(k_cell_size, 1, kBlock, n, px, py, r, s->ctl);
   LAUNCH(k_cells, nb, kBlock, n, px, py, r, s->ctl, colBits, mask, s->bucketOf,
-         s->bucketRank, s->bucketCount, s->largeList, s->ufParent);
+         s->bucketRank, s->bucketCount, s->largeList, s->largeBox,
+         s->ufParent);
   NBX_TRY(launchScan(s, buckets + 1, s->bucketCount, s->bucketStart));
   LAUNCH(k_grid_scatter, nb, kBlock, n, px, py, r, s->ctl, s->bucketOf,
          s->bucketRank, s->bucketCount, s->bucketStart, s->grid);
   LAUNCH(k_find_small, blocksFor(n, kFindBlock), kFindBlock, n, px, py, r,
          s->ctl, colBits, mask, s->bucketOf, s->bucketStart, s->grid,
-         s->largeList, s->pairs,
+         s->largeList, s->largeBox, s->pairs,
          s->pairCap);
   LAUNCH(k_union, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs, s->ufParent);
   LAUNCH(k_comp_count, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
@@ -770,7 +772,7 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->bucketStart, s->buckets + 1, false)
   A(s->scanTiles, blocksFor(s->buckets + 1, kScanTile) + 1, false)
   A(s->grid, 2 * n, false)
-  A(s->largeList, n, false) A(s->ufParent, n, false)
+  A(s->largeList, n, false) A(s->ufParent, n, false) A(s->largeBox, n, false)
   s->pairCap = static_cast<uint32_t>(std::min<uint64_t>(8 * n + 1024, 0xffffff00ull));
   A(s->pairs, s->pairCap, false) A(s->segPairs, s->pairCap, false)
   A(s->pairComp, s->pairCap, false)

@@ -600,4 +600,82 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
   }
 }
 
+// ---- exclusive scan in one launch ----
+//
+// G resident workgroups (as for k_sort_onelaunch); workgroup b owns the
+// chunk [b*C, b*C + C), C <= ITEMS * 1024 and a multiple of 4, each lane
+// ITEMS consecutive values (16-byte loads). The chunk is reduced and its
+// total published; after one grid barrier each workgroup adds the totals of
+// the workgroups before it and writes its chunk's prefixes from registers.
+// Replaces the reduce / tile scan / down-sweep launches.
+template <int ITEMS>
+__global__ __launch_bounds__(kPsBlock) void k_scan_onelaunch(
+    uint32_t count, uint32_t chunk, const uint32_t *__restrict__ in,
+    uint32_t *__restrict__ out, uint32_t *tileSums, uint32_t *bar,
+    Ctl *ctl) {
+  static_assert(ITEMS % 4 == 0, "16-byte loads");
+  __shared__ uint32_t scratch[kPsWaves];
+  const uint32_t t = threadIdx.x;
+  const uint32_t b = blockIdx.x, grid = gridDim.x;
+  const uint32_t lo = min(count, b * chunk);
+  const uint32_t hi = min(count, lo + chunk);
+  const uint32_t base = lo + t * ITEMS;
+  const bool full = base + ITEMS <= hi;
+  uint32_t v[ITEMS];
+  if (full) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+    for (int k = 0; k < ITEMS / 4; ++k) {
+      const uint4 q = p[k];
+      v[4 * k] = q.x;
+      v[4 * k + 1] = q.y;
+      v[4 * k + 2] = q.z;
+      v[4 * k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      v[k] = base + k < hi ? in[base + k] : 0u;
+    }
+  }
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t run = blockExclusiveScan<kPsBlock>(sum, scratch, &total);
+  if (t == 0) {
+    storeAgent(tileSums + b, total);
+  }
+  gridBarrier(bar, grid, &ctl->status);
+  // Totals of the workgroups before this one (b < G <= 1024 lanes).
+  {
+    uint32_t before;
+    blockExclusiveScan<kPsBlock>(t < b ? loadAgent(tileSums + t) : 0u,
+                                 scratch, &before);
+    run += before;
+  }
+  uint32_t o[ITEMS];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    o[k] = run;
+    run += v[k];
+  }
+  if (full) {
+    uint4 *p = reinterpret_cast<uint4 *>(out + base);
+#pragma unroll
+    for (int k = 0; k < ITEMS / 4; ++k) {
+      p[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      if (base + k < hi) {
+        out[base + k] = o[k];
+      }
+    }
+  }
+}
+
 } // namespace nbx

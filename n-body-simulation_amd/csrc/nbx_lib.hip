@@ -224,7 +224,28 @@ int launchIterate(nbx_sim *s) {
   return NBX_OK;
 }
 
+// Exclusive scan of `count` words: one launch of k_scan_onelaunch where its
+// grid fits the CUs, else reduce / tile scan / down-sweep launches.
 int launchScan(nbx_sim *s, uint32_t count, const uint32_t *in, uint32_t *out) {
+  if (count == 0) {
+    return NBX_OK;
+  }
+  if (s->sortGrid) {
+#define SCAN_LAUNCH(ITEMS)                                                    \
+  {                                                                           \
+    const uint32_t g = blocksFor(count, (ITEMS) * kPsBlock);                  \
+    if (g <= s->sortGrid) {                                                   \
+      const uint32_t chunk = ((count + g - 1) / g + 3) & ~3u;                 \
+      const uint32_t grid = (count + chunk - 1) / chunk;                      \
+      LAUNCH((k_scan_onelaunch<ITEMS>), grid, kPsBlock, count, chunk, in, out, \
+             s->scanTiles, s->sortBarrier, s->ctl);                           \
+      return NBX_OK;                                                          \
+    }                                                                         \
+  }
+    SCAN_LAUNCH(8)
+    SCAN_LAUNCH(16)
+#undef SCAN_LAUNCH
+  }
   const uint32_t tiles = blocksFor(count, kScanTile);
   LAUNCH(k_scan_reduce, tiles, kBlock, count, in, s->scanTiles);
   LAUNCH(k_scan_tiles, 1, kBlock, tiles, s->scanTiles);

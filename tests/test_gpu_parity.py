@@ -167,7 +167,23 @@ def test_forces_compensated_within_tolerance(nbx, oracle, states, theta):
 
 @pytest.mark.parametrize("name", ["collide10k_0", "collide20k_60"])
 def test_collide_bit_exact(nbx, oracle, states, name):
-    s = states[name]
+    check_collide(nbx, oracle, states[name])
+
+
+@pytest.mark.parametrize("mode", ["one-launch", "multi"])
+def test_collide_600k_bit_exact(nbx, oracle, monkeypatch, mode):
+    """A grid of 2^21 + 1 bucket counts: the bucket-start scan in one launch
+    (16 counts per lane, 129 workgroups) and, with NBX_SORT=multi, in three."""
+    if mode == "multi":
+        monkeypatch.setenv("NBX_SORT", "multi")
+    else:
+        monkeypatch.delenv("NBX_SORT", raising=False)
+    s = oracle.uniform_disc(600_000)
+    oracle.iterate(s)
+    check_collide(nbx, oracle, s)
+
+
+def check_collide(nbx, oracle, s):
     want = s.copy()
     later, earlier, cell = oracle.collide(want)
     with new_sim(nbx, s) as sim:

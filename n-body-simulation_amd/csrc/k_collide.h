@@ -639,6 +639,109 @@ __global__ __launch_bounds__(kBlock) void k_comp_alloc(
   }
 }
 
+// The same allocation in one launch (G resident 1024-lane workgroups, each
+// lane ITEMS consecutive bodies, 16-byte loads): every workgroup publishes
+// its pair, component and largest-component totals to tiles[3b..3b+2];
+// after one grid barrier it adds the totals of the workgroups before it
+// and writes its segments. Segments follow body order, with no atomics;
+// the last workgroup publishes the totals to ctl.
+template <int ITEMS>
+__global__ __launch_bounds__(kPsBlock) void k_comp_alloc_onelaunch(
+    uint32_t n, uint32_t chunk, Ctl *ctl, const uint32_t *__restrict__ compCount,
+    uint32_t *__restrict__ compStart, int4 *__restrict__ compInfo,
+    uint32_t *tiles, uint32_t *bar) {
+  static_assert(ITEMS % 4 == 0, "16-byte loads");
+  __shared__ uint32_t scratch[kPsWaves];
+  __shared__ uint32_t wmax[kPsWaves];
+  const uint32_t t = threadIdx.x;
+  const uint32_t b = blockIdx.x, grid = gridDim.x;
+  const uint32_t lo = min(n, b * chunk);
+  const uint32_t hi = min(n, lo + chunk);
+  const uint32_t first = lo + t * ITEMS;
+  uint32_t c[ITEMS];
+  if (first + ITEMS <= hi) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(compCount + first);
+#pragma unroll
+    for (int k = 0; k < ITEMS / 4; ++k) {
+      const uint4 q = p[k];
+      c[4 * k] = q.x;
+      c[4 * k + 1] = q.y;
+      c[4 * k + 2] = q.z;
+      c[4 * k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      c[k] = first + k < hi ? compCount[first + k] : 0u;
+    }
+  }
+  uint32_t pairs = 0, roots = 0, mx = 0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    pairs += c[k];
+    roots += c[k] > 0 ? 1u : 0u;
+    mx = max(mx, c[k]);
+  }
+  uint32_t totalPairs, totalRoots;
+  uint32_t p = blockExclusiveScan<kPsBlock>(pairs, scratch, &totalPairs);
+  uint32_t q = blockExclusiveScan<kPsBlock>(roots, scratch, &totalRoots);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    mx = max(mx, static_cast<uint32_t>(__shfl_xor(mx, off, kWave)));
+  }
+  if ((t & (kWave - 1)) == 0) {
+    wmax[t >> 6] = mx;
+  }
+  __syncthreads();
+  if (t == 0) {
+    uint32_t bm = 0;
+    for (int w = 0; w < kPsWaves; ++w) {
+      bm = max(bm, wmax[w]);
+    }
+    storeAgent(tiles + 3 * b, totalPairs);
+    storeAgent(tiles + 3 * b + 1, totalRoots);
+    storeAgent(tiles + 3 * b + 2, bm);
+  }
+  gridBarrier(bar, grid, &ctl->status);
+  uint32_t beforePairs, beforeRoots;
+  blockExclusiveScan<kPsBlock>(t < b ? loadAgent(tiles + 3 * t) : 0u, scratch,
+                               &beforePairs);
+  blockExclusiveScan<kPsBlock>(t < b ? loadAgent(tiles + 3 * t + 1) : 0u,
+                               scratch, &beforeRoots);
+  p += beforePairs;
+  q += beforeRoots;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    if (c[k] > 0) {
+      compStart[first + k] = p;
+      compInfo[q] = make_int4(static_cast<int>(first + k), static_cast<int>(p),
+                              static_cast<int>(c[k]), 0);
+      p += c[k];
+      q += 1;
+    }
+  }
+  if (b == grid - 1) {
+    uint32_t m = t < grid ? loadAgent(tiles + 3 * t + 2) : 0u;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      m = max(m, static_cast<uint32_t>(__shfl_xor(m, off, kWave)));
+    }
+    __syncthreads();
+    if ((t & (kWave - 1)) == 0) {
+      wmax[t >> 6] = m;
+    }
+    __syncthreads();
+    if (t == 0) {
+      for (int w = 0; w < kPsWaves; ++w) {
+        m = max(m, wmax[w]);
+      }
+      ctl->compPairs = beforePairs + totalPairs;
+      ctl->compCount = beforeRoots + totalRoots;
+      ctl->largest = m;
+    }
+  }
+}
+
 // Moves every pair into its component's segment; consumes the counts back
 // to zero.
 __global__ __launch_bounds__(kBlock) void k_comp_scatter(

@@ -397,8 +397,28 @@ int launchCollide(nbx_sim *s) {
   LAUNCH(k_union, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs, s->ufParent);
   LAUNCH(k_comp_count, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
          s->ufParent, s->pairComp, s->compCount);
-  LAUNCH(k_comp_alloc, blocksFor(n, kBlock * kAllocItems), kBlock, n, s->ctl,
-         s->compCount, s->compStart, s->compInfo);
+  // Component segments: one launch where 4/8/16 bodies per lane cover n
+  // with at most one workgroup per CU (block totals in the sort's table,
+  // unused during the collide), else one launch with per-workgroup atomics.
+  bool allocated = false;
+#define ALLOC_LAUNCH(ITEMS)                                                   \
+  if (!allocated && s->sortGrid &&                                            \
+      blocksFor(n, (ITEMS) * kPsBlock) <= s->sortGrid) {                      \
+    const uint32_t g = blocksFor(n, (ITEMS) * kPsBlock);                      \
+    const uint32_t chunk = ((n + g - 1) / g + 3) & ~3u;                       \
+    LAUNCH((k_comp_alloc_onelaunch<ITEMS>), (n + chunk - 1) / chunk,          \
+           kPsBlock, n, chunk, s->ctl, s->compCount, s->compStart,            \
+           s->compInfo, s->sortHist, s->sortBarrier);                         \
+    allocated = true;                                                         \
+  }
+  ALLOC_LAUNCH(4)
+  ALLOC_LAUNCH(8)
+  ALLOC_LAUNCH(16)
+#undef ALLOC_LAUNCH
+  if (!allocated) {
+    LAUNCH(k_comp_alloc, blocksFor(n, kBlock * kAllocItems), kBlock, n,
+           s->ctl, s->compCount, s->compStart, s->compInfo);
+  }
   LAUNCH(k_comp_scatter, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
          s->pairComp, s->compCount, s->compStart, s->segPairs);
   LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->segPairs,

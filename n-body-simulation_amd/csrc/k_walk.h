@@ -442,6 +442,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
   float qx = 0.0f, qy = 0.0f;                       // partial since flush
   uint32_t nFar = 0, nPair = 0, nLeafTerm = 0, nVisit = 0;
   const int nodeCount = ctl->nodeCount;
+  const uint32_t cbLimit =
+      nodeCount >= 5 ? static_cast<uint32_t>(nodeCount - 4) : 0u;
 
   // The root on its own, then blocks of four siblings from the stack.
   LaneStack st;
@@ -465,7 +467,11 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
     int cb;
     uint64_t mask;
     st.get(sp, cb, mask);
-    if (cb <= 0 || cb + 3 >= nodeCount || ++blocks > nodeCount) {
+    // A valid block lies in [1, nodeCount - 4] and each is popped at most
+    // once: one unsigned compare each (nodeCount < 5 leaves no valid block).
+    ++blocks;
+    if ((static_cast<uint32_t>(cb - 1) >= cbLimit) |
+        (static_cast<uint32_t>(blocks) > static_cast<uint32_t>(nodeCount))) {
       if (lane == 0) {
         atomicOr(&ctl->status, kStatusTreeInconsistent);
       }

@@ -272,6 +272,32 @@ struct LaneStack {
   }
 };
 
+// The stack of a tree of at most kRegStackLevels levels: all 64 entries in
+// VGPR lanes, no spill branch and no overflow check. Processing a block at
+// depth d leaves at most 3 pending blocks for each depth 2..d and pushes at
+// most 4, so the stack never holds more than 3 * levels - 5 entries (61 at
+// 22 levels). A corrupted tree can only make an entry miss the register (a
+// push at e >= 64 selects no lane) -- never a memory access -- and every
+// popped block index is validated before use.
+constexpr int kRegStackLevels = 21;
+struct RegStack {
+  int node = 0, lo = 0, hi = 0;
+
+  __device__ inline void put(int e, int nd, uint64_t m) {
+    const bool mine = static_cast<int>(threadIdx.x & (kWave - 1)) == e;
+    node = mine ? nd : node;
+    lo = mine ? static_cast<int>(static_cast<uint32_t>(m)) : lo;
+    hi = mine ? static_cast<int>(static_cast<uint32_t>(m >> 32)) : hi;
+  }
+  __device__ inline void get(int e, int &nd, uint64_t &m) const {
+    const int l = e & (kWave - 1);
+    nd = __builtin_amdgcn_readlane(node, l);
+    const uint32_t mlo = static_cast<uint32_t>(__builtin_amdgcn_readlane(lo, l));
+    const uint32_t mhi = static_cast<uint32_t>(__builtin_amdgcn_readlane(hi, l));
+    m = static_cast<uint64_t>(mlo) | (static_cast<uint64_t>(mhi) << 32);
+  }
+};
+
 // CAP = false drops the min(., FLT_MAX) guard; the host selects it only when
 // no factor can overflow (see capFree in the launch code).
 template <int LAW, bool CAP>
@@ -417,23 +443,18 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
   return 0;
 }
 
-template <int LAW, bool COUNT, bool CAP>
-__global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
-    uint32_t n, const float4 *__restrict__ nodeF,
-    const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
-    float tSq, float eSq, float *__restrict__ ax, float *__restrict__ ay,
-    Ctl *__restrict__ ctl) {
-  __shared__ float2 slots[kWalkBlock / kWave][kWave];
-  __shared__ int spillNode[kWalkBlock / kWave][kStackDepth - kWave];
-  __shared__ uint64_t spillMask[kWalkBlock / kWave][kStackDepth - kWave];
+// One wave's walk: 64 consecutive bodies from `base`, with stack type STACK
+// (RegStack for trees of at most kRegStackLevels levels, else LaneStack).
+template <int LAW, bool COUNT, bool CAP, class STACK>
+__device__ inline void walkWave(uint32_t n, uint32_t base,
+                                const float4 *__restrict__ nodeF,
+                                const int4 *__restrict__ nodeI,
+                                const float4 *__restrict__ body, float tSq,
+                                float eSq, float *__restrict__ ax,
+                                float *__restrict__ ay, Ctl *__restrict__ ctl,
+                                STACK &st, float2 *__restrict__ slot) {
+  constexpr bool kSpill = sizeof(STACK) > sizeof(RegStack);
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t w = threadIdx.x >> 6;
-  const uint32_t base =
-      xcdBlock() * kWalkBlock + (threadIdx.x & ~(kWave - 1));
-  if (base >= n ||
-      (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent))) {
-    return;
-  }
   const uint32_t i = base + lane;
   const bool valid = i < n;
   const float4 me = body[valid ? i : base];
@@ -446,15 +467,12 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
       nodeCount >= 5 ? static_cast<uint32_t>(nodeCount - 4) : 0u;
 
   // The root on its own, then blocks of four siblings from the stack.
-  LaneStack st;
-  st.spillNode = spillNode[w];
-  st.spillMask = spillMask[w];
   int sp = 0;
   {
     const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
     const uint64_t open =
         visitNode<LAW, COUNT, CAP>(nodeF[0], nodeI[0], all, x, y, tSq, eSq, body,
-                                   slots[w], qx, qy, nFar, nPair, nLeafTerm);
+                                   slot, qx, qy, nFar, nPair, nLeafTerm);
     nVisit += 1;
     if (open) {
       st.put(0, nodeI[0].x, open);
@@ -498,8 +516,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       open[q] = visitNode<LAW, COUNT, CAP>(bf[q], bi[q], mask, x, y, tSq, eSq,
-                                           body, slots[w], qx, qy, nFar,
-                                           nPair, nLeafTerm);
+                                           body, slot, qx, qy, nFar, nPair,
+                                           nLeafTerm);
     }
     if (COUNT) {
       nVisit += 4;
@@ -508,7 +526,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
 #pragma unroll
     for (int q = 3; q >= 0; --q) {
       if (open[q]) {
-        if (sp >= kStackDepth) {
+        if (kSpill && sp >= kStackDepth) {
           if (lane == 0) {
             atomicOr(&ctl->status, kStatusTreeInconsistent);
           }
@@ -545,6 +563,35 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
       atomicAdd(&ctl->waveVisits, static_cast<unsigned long long>(nVisit));
       atomicAdd(&ctl->waveLeafTerms, static_cast<unsigned long long>(nLeafTerm));
     }
+  }
+}
+
+template <int LAW, bool COUNT, bool CAP>
+__global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
+    uint32_t n, const float4 *__restrict__ nodeF,
+    const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
+    float tSq, float eSq, float *__restrict__ ax, float *__restrict__ ay,
+    Ctl *__restrict__ ctl) {
+  __shared__ float2 slots[kWalkBlock / kWave][kWave];
+  __shared__ int spillNode[kWalkBlock / kWave][kStackDepth - kWave];
+  __shared__ uint64_t spillMask[kWalkBlock / kWave][kStackDepth - kWave];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t base =
+      xcdBlock() * kWalkBlock + (threadIdx.x & ~(kWave - 1));
+  if (base >= n ||
+      (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent))) {
+    return;
+  }
+  if (ctl->levels <= kRegStackLevels) {
+    RegStack st;
+    walkWave<LAW, COUNT, CAP>(n, base, nodeF, nodeI, body, tSq, eSq, ax, ay,
+                              ctl, st, slots[w]);
+  } else {
+    LaneStack st;
+    st.spillNode = spillNode[w];
+    st.spillMask = spillMask[w];
+    walkWave<LAW, COUNT, CAP>(n, base, nodeF, nodeI, body, tSq, eSq, ax, ay,
+                              ctl, st, slots[w]);
   }
 }
 

@@ -293,6 +293,33 @@ def test_deep_cluster_below_depth_16(nbx, oracle):
     assert_bits_equal(got.ay, wy, "ay")
 
 
+def test_deep_tree_compensated_walk(nbx, oracle):
+    """A tree deeper than 21 levels: the compensated walk's traversal stack
+    outgrows its 64 register lanes and spills to LDS. Forces within the
+    compensated-mode tolerance of the oracle, interaction counts exact."""
+    from oracle_lib import Bodies
+    rng = np.random.default_rng(11)
+    n = 4000
+    b = Bodies.empty(n)
+    b.px[:] = rng.uniform(-1e4, 1e4, n).astype(np.float32)
+    b.py[:] = rng.uniform(-1e4, 1e4, n).astype(np.float32)
+    b.px[:400] = (1234.5 + rng.uniform(0, 2e-4, 400)).astype(np.float32)
+    b.py[:400] = (-321.25 + rng.uniform(0, 2e-4, 400)).astype(np.float32)
+    b.m[:] = 1.0
+    b.r[:] = 1e-5
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    srt, hi, lo, perm, quad = oracle.sorted_state(b)
+    tree = oracle.build_tree(srt, hi, lo, quad)
+    assert tree.levels > 21, f"state must need the spill stack ({tree.levels} levels)"
+    wx, wy, wfar, wpair = oracle.accelerations(srt, tree, 0.5, counts=True)
+    with new_sim(nbx, b, summation=nbx.SUM_COMPENSATED, theta=0.5) as sim:
+        sim.attract()
+        got = sim.download()
+        gf, gp = sim.count_interactions()
+    assert rms_rel(got.ax, got.ay, wx, wy) <= RMS_TOL
+    assert gf == int(wfar.sum()) and gp == int(wpair.sum())
+
+
 def test_large_contact_component(nbx, oracle):
     """A packed clump: one contact component far above 16 pairs (workgroup path)."""
     from oracle_lib import Bodies

@@ -366,7 +366,7 @@ __device__ inline void leafSpread(uint64_t near, uint32_t k, uint32_t logK,
 // only. The opening test is rounded exactly as the reference rounds it
 // (no contraction): the decisions, and so the interaction set, are the
 // reference's.
-template <int LAW, bool COUNT, bool CAP>
+template <int LAW, bool COUNT, bool CAP, bool MASS>
 __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
                                      uint64_t mask, float x, float y,
                                      float tSq, float eSq,
@@ -379,9 +379,14 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
   const float dSq = dx * dx + dy * dy;
   const uint64_t near = mask & ~__builtin_amdgcn_ballot_w64(f.w < dSq * tSq);
   // f.z > 0 on the scalar unit: positive floats, +inf included, are exactly
-  // the bit patterns in (0, 0x7f800000].
+  // the bit patterns in (0, 0x7f800000]. MASS = false (every body mass
+  // positive and finite) skips the test: a non-empty node's mass is then in
+  // (0, inf], and an empty node's far term is exactly 0 (its mass is 0, and
+  // a far lane has dSq > 0, so the inverse root and reciprocal are finite).
   const int32_t mb = __float_as_int(f.z);
-  const uint64_t farMask = (mb > 0 && mb <= 0x7f800000) ? (mask & ~near) : 0ull;
+  const uint64_t farMask = (!MASS || (mb > 0 && mb <= 0x7f800000))
+                               ? (mask & ~near)
+                               : 0ull;
   if (__builtin_amdgcn_inverse_ballot_w64(farMask)) {
     const float fac = fastFactor<LAW, CAP>(f.z, dSq, eSq);
     qx = fmaf(dx, fac, qx);
@@ -445,7 +450,7 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
 
 // One wave's walk: 64 consecutive bodies from `base`, with stack type STACK
 // (RegStack for trees of at most kRegStackLevels levels, else LaneStack).
-template <int LAW, bool COUNT, bool CAP, class STACK>
+template <int LAW, bool COUNT, bool CAP, bool MASS, class STACK>
 __device__ inline void walkWave(uint32_t n, uint32_t base,
                                 const float4 *__restrict__ nodeF,
                                 const int4 *__restrict__ nodeI,
@@ -471,7 +476,7 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
   {
     const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
     const uint64_t open =
-        visitNode<LAW, COUNT, CAP>(nodeF[0], nodeI[0], all, x, y, tSq, eSq, body,
+        visitNode<LAW, COUNT, CAP, MASS>(nodeF[0], nodeI[0], all, x, y, tSq, eSq, body,
                                    slot, qx, qy, nFar, nPair, nLeafTerm);
     nVisit += 1;
     if (open) {
@@ -515,7 +520,7 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
     uint64_t open[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      open[q] = visitNode<LAW, COUNT, CAP>(bf[q], bi[q], mask, x, y, tSq, eSq,
+      open[q] = visitNode<LAW, COUNT, CAP, MASS>(bf[q], bi[q], mask, x, y, tSq, eSq,
                                            body, slot, qx, qy, nFar, nPair,
                                            nLeafTerm);
     }
@@ -566,7 +571,7 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
   }
 }
 
-template <int LAW, bool COUNT, bool CAP>
+template <int LAW, bool COUNT, bool CAP, bool MASS>
 __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
     uint32_t n, const float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
@@ -584,13 +589,13 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
   }
   if (ctl->levels <= kRegStackLevels) {
     RegStack st;
-    walkWave<LAW, COUNT, CAP>(n, base, nodeF, nodeI, body, tSq, eSq, ax, ay,
+    walkWave<LAW, COUNT, CAP, MASS>(n, base, nodeF, nodeI, body, tSq, eSq, ax, ay,
                               ctl, st, slots[w]);
   } else {
     LaneStack st;
     st.spillNode = spillNode[w];
     st.spillMask = spillMask[w];
-    walkWave<LAW, COUNT, CAP>(n, base, nodeF, nodeI, body, tSq, eSq, ax, ay,
+    walkWave<LAW, COUNT, CAP, MASS>(n, base, nodeF, nodeI, body, tSq, eSq, ax, ay,
                               ctl, st, slots[w]);
   }
 }

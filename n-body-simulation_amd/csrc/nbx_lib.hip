@@ -75,6 +75,7 @@ struct nbx_sim {
   uint64_t frame = 0;
   nbx_params params{};
   double massAbsSum = 0; // sum of |m| of the bodies held (masses never change)
+  bool massPositive = false; // every body mass in (0, FLT_MAX]
   float jacobiCell = 1.0f; // reference GPU collide cell (radii never change)
   JacAcc *jacPartials = nullptr;
   uint32_t jacMaxLarge = 0;
@@ -507,14 +508,25 @@ int launchForces(nbx_sim *s, bool count) {
   const double eps2 = static_cast<double>(eSq);
   const bool capFree = !acc && eps2 > 0 &&
                        s->massAbsSum * 2e19 * std::max(1.0, 1.0 / eps2) < 1e37;
+  // The node-mass test is needed only when some body mass is not positive
+  // and finite (and always by the counting walk, which counts far nodes of
+  // positive mass).
+#define FAST_M(LAWV, CNT, CAPV)                                               \
+  do {                                                                        \
+    if (!(CNT) && s->massPositive) {                                          \
+      LAUNCH((k_walk_fast<LAWV, CNT, CAPV, false>), walkGrid, kWalkBlock, n,  \
+             s->nodeF, s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);\
+    } else {                                                                  \
+      LAUNCH((k_walk_fast<LAWV, CNT, CAPV, true>), walkGrid, kWalkBlock, n,   \
+             s->nodeF, s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);\
+    }                                                                         \
+  } while (0)
 #define FAST(LAWV, CNT)                                                       \
   do {                                                                        \
     if (capFree) {                                                            \
-      LAUNCH((k_walk_fast<LAWV, CNT, false>), walkGrid, kWalkBlock, n,        \
-             s->nodeF, s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);\
+      FAST_M(LAWV, CNT, false);                                               \
     } else {                                                                  \
-      LAUNCH((k_walk_fast<LAWV, CNT, true>), walkGrid, kWalkBlock, n,         \
-             s->nodeF, s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);\
+      FAST_M(LAWV, CNT, true);                                                \
     }                                                                         \
   } while (0)
   if (ref) {
@@ -535,6 +547,7 @@ int launchForces(nbx_sim *s, bool count) {
     }
   }
 #undef FAST
+#undef FAST_M
 #undef WALK
   return NBX_OK;
 }
@@ -908,8 +921,10 @@ int nbx_upload(nbx_sim *s, uint64_t n, const float *px, const float *py,
   s->cur = 0;
   s->n = n;
   s->massAbsSum = 0;
+  s->massPositive = true;
   for (uint64_t k = 0; k < n; ++k) {
     s->massAbsSum += std::fabs(static_cast<double>(m[k]));
+    s->massPositive = s->massPositive && m[k] > 0.0f && m[k] <= FLT_MAX;
   }
   // CudaCollider's cell (CudaCollision.cu:197-209): 2 x median radius x 1.01.
   s->jacobiCell = 1.0f;

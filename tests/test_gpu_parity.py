@@ -320,6 +320,31 @@ def test_deep_tree_compensated_walk(nbx, oracle):
     assert gf == int(wfar.sum()) and gp == int(wpair.sum())
 
 
+@pytest.mark.parametrize("summation", ["reference", "compensated"])
+def test_nonpositive_masses(nbx, oracle, states, summation):
+    """Massless and negative-mass bodies: far nodes whose mass is not
+    positive contribute nothing (the fast walk keeps its node-mass test only
+    for such systems). Reference summation bit-exact, compensated within
+    tolerance, interaction counts exact."""
+    s = states["disc10k"].copy()
+    s.m[::7] = 0.0
+    s.m[3::11] = -s.m[3::11]
+    srt, hi, lo, perm, quad = oracle.sorted_state(s)
+    tree = oracle.build_tree(srt, hi, lo, quad)
+    wx, wy, wfar, wpair = oracle.accelerations(srt, tree, 0.5, counts=True)
+    mode = nbx.SUM_REFERENCE if summation == "reference" else nbx.SUM_COMPENSATED
+    with new_sim(nbx, s, summation=mode, theta=0.5) as sim:
+        sim.attract()
+        got = sim.download()
+        gf, gp = sim.count_interactions()
+    if summation == "reference":
+        assert_bits_equal(got.ax, wx, "ax")
+        assert_bits_equal(got.ay, wy, "ay")
+    else:
+        assert rms_rel(got.ax, got.ay, wx, wy) <= RMS_TOL
+    assert gf == int(wfar.sum()) and gp == int(wpair.sum())
+
+
 def test_large_contact_component(nbx, oracle):
     """A packed clump: one contact component far above 16 pairs (workgroup path)."""
     from oracle_lib import Bodies

@@ -317,6 +317,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_scatter(
     const uint32_t *__restrict__ bucketRank,
     uint32_t *__restrict__ bucketCount,
     const uint32_t *__restrict__ bucketStart, float4 *__restrict__ grid) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) {
     return;
@@ -411,6 +414,9 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
     const int32_t *__restrict__ largeList,
     const float4 *__restrict__ largeBox, int4 *__restrict__ pairs,
     uint32_t cap) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   __shared__ PairStageT<kFindStage> st;
   stageInit(st);
   const uint32_t i = xcdBlock() * kFindBlock + threadIdx.x;
@@ -549,6 +555,9 @@ __device__ inline int ufFind(int32_t *parent, int x) {
 __global__ __launch_bounds__(kBlock) void k_union(
     const Ctl *__restrict__ ctl, uint32_t cap, const int4 *__restrict__ pairs,
     int32_t *__restrict__ parent) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t count = min(ctl->pairCount, cap);
   for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;
        k += gridDim.x * kBlock) {
@@ -572,6 +581,9 @@ __global__ __launch_bounds__(kBlock) void k_comp_count(
     const Ctl *__restrict__ ctl, uint32_t cap, const int4 *__restrict__ pairs,
     int32_t *__restrict__ parent, int32_t *__restrict__ pairComp,
     uint32_t *__restrict__ compCount) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t count = min(ctl->pairCount, cap);
   for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;
        k += gridDim.x * kBlock) {
@@ -649,7 +661,7 @@ template <int ITEMS>
 __global__ __launch_bounds__(kPsBlock) void k_comp_alloc_onelaunch(
     uint32_t n, uint32_t chunk, Ctl *ctl, const uint32_t *__restrict__ compCount,
     uint32_t *__restrict__ compStart, int4 *__restrict__ compInfo,
-    uint32_t *tiles, uint32_t *bar) {
+    uint32_t *tiles, uint32_t *bar, uint32_t spinLimit) {
   static_assert(ITEMS % 4 == 0, "16-byte loads");
   __shared__ uint32_t scratch[kPsWaves];
   __shared__ uint32_t wmax[kPsWaves];
@@ -702,7 +714,7 @@ __global__ __launch_bounds__(kPsBlock) void k_comp_alloc_onelaunch(
     storeAgent(tiles + 3 * b + 1, totalRoots);
     storeAgent(tiles + 3 * b + 2, bm);
   }
-  gridBarrier(bar, grid, &ctl->status);
+  gridBarrier(bar, grid, &ctl->status, spinLimit);
   uint32_t beforePairs, beforeRoots;
   blockExclusiveScan<kPsBlock>(t < b ? loadAgent(tiles + 3 * t) : 0u, scratch,
                                &beforePairs);
@@ -748,6 +760,9 @@ __global__ __launch_bounds__(kBlock) void k_comp_scatter(
     const Ctl *__restrict__ ctl, uint32_t cap, const int4 *__restrict__ pairs,
     const int32_t *__restrict__ pairComp, uint32_t *__restrict__ compCount,
     const uint32_t *__restrict__ compStart, int4 *__restrict__ segPairs) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t count = min(ctl->pairCount, cap);
   for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;
        k += gridDim.x * kBlock) {
@@ -814,6 +829,9 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
     float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
     float *__restrict__ vy, const float *__restrict__ m,
     const float *__restrict__ r) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= ctl->compCount || (ctl->status & kStatusPairOverflow)) {
     return;
@@ -845,6 +863,9 @@ __global__ __launch_bounds__(kBlock) void k_resolve_big(
     float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
     float *__restrict__ vy, const float *__restrict__ m,
     const float *__restrict__ r) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   __shared__ int4 sp[kBigCompMax];
   __shared__ int16_t order[kBigCompMax];
   const uint32_t big = ctl->bigCount;

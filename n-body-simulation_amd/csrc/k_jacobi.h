@@ -234,6 +234,9 @@ __global__ __launch_bounds__(kBlock) void k_jac_sorted(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
     const float *__restrict__ r, const uint32_t *__restrict__ perm,
     Ctl *__restrict__ ctl, uint64_t *__restrict__ keysSorted) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= n) {
     return;
@@ -278,6 +281,9 @@ __global__ __launch_bounds__(kBlock) void k_jac_small(
     const float *__restrict__ vy, const float *__restrict__ m,
     const float *__restrict__ r, float4 *__restrict__ out,
     int32_t *__restrict__ contacts, int32_t *__restrict__ asLater) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const int s = static_cast<int>(xcdBlock() * kBlock + threadIdx.x);
   const int nn = static_cast<int>(n);
   const int nSmall = nn - static_cast<int>(ctl->largeCount);
@@ -333,6 +339,9 @@ __global__ __launch_bounds__(kJacLanes) void k_jac_large_partial(
     const float *__restrict__ py, const float *__restrict__ vx,
     const float *__restrict__ vy, const float *__restrict__ m,
     const float *__restrict__ r, JacAcc *__restrict__ partials) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   __shared__ float shF[kJacLanes];
   __shared__ int shI[kJacLanes];
   const uint32_t large = ctl->largeCount;
@@ -382,6 +391,9 @@ __global__ __launch_bounds__(kBlock) void k_jac_large_combine(
     const float *__restrict__ r, const JacAcc *__restrict__ partials,
     float4 *__restrict__ out, int32_t *__restrict__ contacts,
     int32_t *__restrict__ asLater) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t large = ctl->largeCount;
   if (large > maxLarge) {
     return;
@@ -413,6 +425,9 @@ __global__ __launch_bounds__(kBlock) void k_jac_commit(
     uint32_t n, const Ctl *__restrict__ ctl, const float4 *__restrict__ out,
     float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
     float *__restrict__ vy) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n || (ctl->status & kStatusJacobiTooManyLarge)) {
     return;

@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(
 constexpr int kPsBlock = 1024;
 constexpr int kPsWaves = kPsBlock / kWave;
 constexpr uint32_t kPsMaxGrid = 1024; // phase B scans one row entry per lane
-constexpr uint32_t kBarrierSpins = 1u << 20; // ~1 s of polling
+constexpr uint32_t kBarrierSpins = 1u << 20; // default poll bound, ~1 s
 // Barrier words: arrival counter [0], generation [32], one arrival counter
 // per XCD at [64 + 32 x] (128 bytes apart).
 constexpr int kBarrierWords = 64 + 32 * kXcds;
@@ -328,10 +328,11 @@ __device__ inline void storeAgent(uint32_t *p, uint32_t v) {
 // 256 on one address. The last top arrival resets the counters and bumps
 // the generation; the others poll it. Counters are zero at rest. The poll is
 // bounded: a barrier that does not complete (workgroups not co-resident)
-// sets kStatusSortStalled and returns, so the kernel always ends; the host
+// sets kStatusBarrierStalled and returns, so the kernel always ends; the host
 // reports the error and clears the barrier words.
 __device__ inline void gridBarrier(uint32_t *__restrict__ bar, uint32_t blocks,
-                                   uint32_t *__restrict__ status) {
+                                   uint32_t *__restrict__ status,
+                                   uint32_t spinLimit) {
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -356,8 +357,8 @@ __device__ inline void gridBarrier(uint32_t *__restrict__ bar, uint32_t blocks,
     } else {
       uint32_t spins = 0;
       while (loadAgent(gen) == g) {
-        if (++spins > kBarrierSpins || (loadAgent(status) & kStatusSortStalled)) {
-          __hip_atomic_fetch_or(status, kStatusSortStalled, __ATOMIC_RELAXED,
+        if (++spins > spinLimit || (loadAgent(status) & kStatusBarrierStalled)) {
+          __hip_atomic_fetch_or(status, kStatusBarrierStalled, __ATOMIC_RELAXED,
                                 __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
@@ -384,8 +385,8 @@ template <int ITEMS, bool BUILD>
 __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
     uint32_t n, uint32_t chunk, uint32_t *keys0, uint32_t *vals0,
     uint32_t *keys1, uint32_t *vals1, uint32_t *table, uint32_t *rowTotal,
-    uint32_t *bar, Ctl *ctl, const float *px, const float *py,
-    uint32_t *parts, uint32_t *keyLo) {
+    uint32_t *bar, uint32_t spinLimit, Ctl *ctl, const float *px,
+    const float *py, uint32_t *parts, uint32_t *keyLo) {
   constexpr int kTile = ITEMS * kPsBlock;
   __shared__ uint32_t sk[kTile];
   __shared__ uint32_t sv[kTile];
@@ -437,7 +438,7 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
       storeAgent(parts + 4 * b + 2, bb.z);
       storeAgent(parts + 4 * b + 3, bb.w);
     }
-    gridBarrier(bar, grid, status);
+    gridBarrier(bar, grid, status, spinLimit);
     bb = bboxEmpty();
     if (t < grid) {
       bb = make_uint4(loadAgent(parts + 4 * t), loadAgent(parts + 4 * t + 1),
@@ -532,7 +533,7 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
         sv[pos] = val[j];
       }
     }
-    gridBarrier(bar, grid, status);
+    gridBarrier(bar, grid, status, spinLimit);
 
     // B. row scans, all of this workgroup's rows (b, b + G, ...) at once:
     // lane t takes entry t % S of row b + G * (t / S), S = segment, and a
@@ -572,7 +573,7 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
         storeAgent(rowTotal + row, x);
       }
     }
-    gridBarrier(bar, grid, status);
+    gridBarrier(bar, grid, status, spinLimit);
 
     // C. write the staged runs.
     {
@@ -595,7 +596,7 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
       }
     }
     if (pass < 3) {
-      gridBarrier(bar, grid, status);
+      gridBarrier(bar, grid, status, spinLimit);
     }
   }
 }
@@ -612,7 +613,7 @@ template <int ITEMS>
 __global__ __launch_bounds__(kPsBlock) void k_scan_onelaunch(
     uint32_t count, uint32_t chunk, const uint32_t *__restrict__ in,
     uint32_t *__restrict__ out, uint32_t *tileSums, uint32_t *bar,
-    Ctl *ctl) {
+    uint32_t spinLimit, Ctl *ctl) {
   static_assert(ITEMS % 4 == 0, "16-byte loads");
   __shared__ uint32_t scratch[kPsWaves];
   const uint32_t t = threadIdx.x;
@@ -648,7 +649,7 @@ __global__ __launch_bounds__(kPsBlock) void k_scan_onelaunch(
   if (t == 0) {
     storeAgent(tileSums + b, total);
   }
-  gridBarrier(bar, grid, &ctl->status);
+  gridBarrier(bar, grid, &ctl->status, spinLimit);
   // Totals of the workgroups before this one (b < G <= 1024 lanes).
   {
     uint32_t before;

@@ -127,6 +127,9 @@ __global__ __launch_bounds__(kBlock) void k_deep_detect(
     uint32_t n, const uint32_t *__restrict__ hi,
     const uint32_t *__restrict__ keyLo, uint32_t *__restrict__ perm,
     Ctl *__restrict__ ctl, int2 *__restrict__ deepList) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j + 1 >= n) {
     return;
@@ -174,6 +177,9 @@ __global__ __launch_bounds__(kBlock) void k_deep_detect(
 __global__ __launch_bounds__(kBlock) void k_deep_sort(
     Ctl *__restrict__ ctl, const int2 *__restrict__ deepList,
     const uint32_t *__restrict__ keyLo, uint32_t *__restrict__ perm) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   __shared__ uint32_t sk[kDeepMax];
   __shared__ uint32_t sv[kDeepMax];
   __shared__ int16_t order[kDeepMax];
@@ -333,6 +339,9 @@ __global__ __launch_bounds__(kBlock) void k_lcp_count(
 // depth total.
 __global__ __launch_bounds__(1024) void k_level_scan(
     uint32_t tiles, uint32_t *__restrict__ tileCount, Ctl *__restrict__ ctl) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   __shared__ uint32_t scratch[1024 / kWave];
   const int d = blockIdx.x;
   uint32_t *row = tileCount + static_cast<size_t>(d) * tiles;
@@ -362,6 +371,9 @@ __global__ void k_level_finalize(uint32_t n, int64_t capacity,
                                  float *__restrict__ centerY,
                                  float *__restrict__ size,
                                  int32_t *__restrict__ nodeOfSplit) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   int32_t base = 0;
   for (int d = 0; d < kMaxDepth; ++d) {
     ctl->splitBase[d] = base;
@@ -404,6 +416,9 @@ __global__ __launch_bounds__(kBlock) void k_emit(
     const uint32_t *__restrict__ lo, const uint32_t *__restrict__ perm,
     uint32_t tiles, const uint32_t *__restrict__ tileOffset,
     const Ctl *__restrict__ ctl, int32_t *__restrict__ splitList) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   __shared__ uint32_t waveCnt[kBlock / kWave][kMaxDepth];
   __shared__ uint32_t tileBase[kMaxDepth];
   if (ctl->status & kStatusNodeOverflow) {
@@ -447,6 +462,9 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
     float4 *__restrict__ nodeF, int4 *__restrict__ nodeI,
     float *__restrict__ centerX, float *__restrict__ centerY,
     float *__restrict__ size, int32_t *__restrict__ nodeOfSplit) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   if (ctl->status & kStatusNodeOverflow) {
     return;
   }
@@ -697,6 +715,9 @@ __global__ __launch_bounds__(kBlock) void k_node_finish(
     const int32_t *__restrict__ nodeOfSplit, float4 *__restrict__ nodeF,
     int4 *__restrict__ nodeI, float2 *__restrict__ wsum,
     uint8_t *__restrict__ done, int32_t *__restrict__ roots) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   if (ctl->status & kStatusNodeOverflow) {
     return;
   }
@@ -742,6 +763,9 @@ __global__ __launch_bounds__(kBlock) void k_subtree(
     const Ctl *__restrict__ ctl, const int32_t *__restrict__ roots,
     float4 *__restrict__ nodeF, const int4 *__restrict__ nodeI,
     float2 *__restrict__ wsum, uint8_t *__restrict__ done) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   __shared__ int list[kSubtreeList];
   __shared__ int levelOff[kMaxDepth + 3];
   __shared__ int appended;
@@ -811,6 +835,9 @@ __global__ __launch_bounds__(1024) void k_upward_top(
     const Ctl *__restrict__ ctl, float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, float2 *__restrict__ wsum,
     const uint8_t *__restrict__ done, int32_t *__restrict__ big) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   __shared__ int levelOff[kMaxDepth + 3];
   __shared__ int appended;
   if (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent)) {

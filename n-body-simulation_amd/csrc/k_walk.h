@@ -52,6 +52,9 @@ __global__ __launch_bounds__(kBlock) void k_walk(
     const float *__restrict__ py, const float *__restrict__ pm, float tSq,
     float eSq, float *__restrict__ ax, float *__restrict__ ay,
     Ctl *__restrict__ ctl) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1));
   if (base >= n || (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent))) {
@@ -577,6 +580,9 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
     const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
     float tSq, float eSq, float *__restrict__ ax, float *__restrict__ ay,
     Ctl *__restrict__ ctl) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
   __shared__ float2 slots[kWalkBlock / kWave][kWave];
   __shared__ int spillNode[kWalkBlock / kWave][kStackDepth - kWave];
   __shared__ uint64_t spillMask[kWalkBlock / kWave][kStackDepth - kWave];

@@ -31,7 +31,7 @@ enum : uint32_t {
   kStatusComponentTooLarge = 8u,
   kStatusTreeInconsistent = 16u, // internal check failed (never expected)
   kStatusJacobiTooManyLarge = 32u,
-  kStatusSortStalled = 64u, // a grid barrier of the one-launch sort timed out
+  kStatusBarrierStalled = 64u, // a grid barrier of a one-launch kernel timed out
 };
 
 // Per-handle control block in device memory: everything the host would
@@ -55,6 +55,13 @@ struct Ctl {
   unsigned long long farCount, bodyPairCount;
   unsigned long long waveVisits, waveLeafTerms; // counting walk only
 };
+
+// A one-launch kernel's grid barrier timed out earlier in this step: the
+// kernels after it skip their work (their inputs may be partial), so a
+// failed step never indexes outside its buffers.
+__device__ inline bool barrierStalled(const Ctl *ctl) {
+  return (ctl->status & kStatusBarrierStalled) != 0;
+}
 
 // Order-preserving float <-> uint32 map (so exact float min/max can use
 // integer atomics).

@@ -108,6 +108,7 @@ struct nbx_sim {
   uint32_t *keyLo = nullptr, *sortHist = nullptr, *sortRowTotal = nullptr;
   uint32_t *sortBarrier = nullptr; // grid barrier words of k_sort_onelaunch
   uint32_t sortGrid = 0; // its maximum workgroups; 0: three launches a pass
+  uint32_t barrierSpins = kBarrierSpins; // poll bound of the grid barriers
   int2 *deepList = nullptr;
   uint32_t *tileCount = nullptr;
   int64_t nodeCap = 0;
@@ -239,7 +240,7 @@ int launchScan(nbx_sim *s, uint32_t count, const uint32_t *in, uint32_t *out) {
       const uint32_t chunk = ((count + g - 1) / g + 3) & ~3u;                 \
       const uint32_t grid = (count + chunk - 1) / chunk;                      \
       LAUNCH((k_scan_onelaunch<ITEMS>), grid, kPsBlock, count, chunk, in, out, \
-             s->scanTiles, s->sortBarrier, s->ctl);                           \
+             s->scanTiles, s->sortBarrier, s->barrierSpins, s->ctl);          \
       return NBX_OK;                                                          \
     }                                                                         \
   }
@@ -277,13 +278,15 @@ bool launchOneLaunchSort(nbx_sim *s, uint32_t n, uint32_t *const keys[2],
         hipLaunchKernelGGL((k_sort_onelaunch<ITEMS, true>), dim3(g),          \
                            dim3(kPsBlock), 0, s->stream, n, chunk, keys[0],   \
                            vals[0], keys[1], vals[1], s->sortHist,            \
-                           s->sortRowTotal, s->sortBarrier, s->ctl, px, py,   \
+                           s->sortRowTotal, s->sortBarrier, s->barrierSpins,  \
+                           s->ctl, px, py,                                    \
                            parts, s->keyLo);                                  \
       } else {                                                                \
         hipLaunchKernelGGL((k_sort_onelaunch<ITEMS, false>), dim3(g),         \
                            dim3(kPsBlock), 0, s->stream, n, chunk, keys[0],   \
                            vals[0], keys[1], vals[1], s->sortHist,            \
-                           s->sortRowTotal, s->sortBarrier, s->ctl, px, py,   \
+                           s->sortRowTotal, s->sortBarrier, s->barrierSpins,  \
+                           s->ctl, px, py,                                    \
                            parts, s->keyLo);                                  \
       }                                                                       \
       *rc = checkLaunch();                                                    \
@@ -409,7 +412,7 @@ int launchCollide(nbx_sim *s) {
     const uint32_t chunk = ((n + g - 1) / g + 3) & ~3u;                       \
     LAUNCH((k_comp_alloc_onelaunch<ITEMS>), (n + chunk - 1) / chunk,          \
            kPsBlock, n, chunk, s->ctl, s->compCount, s->compStart,            \
-           s->compInfo, s->sortHist, s->sortBarrier);                         \
+           s->compInfo, s->sortHist, s->sortBarrier, s->barrierSpins);        \
     allocated = true;                                                         \
   }
   ALLOC_LAUNCH(4)
@@ -698,17 +701,17 @@ int syncAndCheck(nbx_sim *s) {
     return setError(NBX_ERR_CAPACITY,
                     "too many large boxes for the Jacobi collide's partial sums");
   }
-  if (status & kStatusSortStalled) {
+  if (status & kStatusBarrierStalled) {
     // Leave the barrier words at rest for the next sort.
-    const uint32_t cleared = status & ~kStatusSortStalled;
+    const uint32_t cleared = status & ~kStatusBarrierStalled;
     NBX_HIP(hipMemsetAsync(s->sortBarrier, 0, kBarrierWords * sizeof(uint32_t),
                            s->stream));
     NBX_HIP(hipMemcpyAsync(&s->ctl->status, &cleared, sizeof(uint32_t),
                            hipMemcpyHostToDevice, s->stream));
     NBX_HIP(hipStreamSynchronize(s->stream));
     return setError(NBX_ERR_STATE,
-                    "radix sort grid barrier timed out (workgroups were not "
-                    "all resident)");
+                    "grid barrier timed out (workgroups were not all "
+                    "resident)");
   }
   if (status & kStatusComponentTooLarge) {
     return setError(NBX_ERR_CAPACITY,
@@ -837,6 +840,10 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
     // The one-launch sort needs every one of its workgroups resident at
     // once: at most one per CU, and only where one fits a CU.
     int dev = 0, cus = 0, perCu = 0;
+    // Test hook: a small poll bound forces the barrier-timeout path.
+    if (const char *sp = std::getenv("NBX_BARRIER_SPINS")) {
+      s->barrierSpins = static_cast<uint32_t>(std::strtoul(sp, nullptr, 10));
+    }
     const char *mode = std::getenv("NBX_SORT");
     const bool multi = mode && std::strcmp(mode, "multi") == 0;
     if (!multi && hipGetDevice(&dev) == hipSuccess &&

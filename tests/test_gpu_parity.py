@@ -345,6 +345,28 @@ def test_nonpositive_masses(nbx, oracle, states, summation):
     assert gf == int(wfar.sum()) and gp == int(wpair.sum())
 
 
+def test_barrier_timeout_fails_the_step(nbx, oracle, monkeypatch):
+    """A grid barrier of a one-launch kernel that never completes -- forced
+    from the host with a zero poll bound, so every kernel still ends -- fails
+    the step with NBX_ERR_STATE (4); the kernels after it skip their work, and
+    a handle with the normal bound steps on afterwards."""
+    s = oracle.uniform_disc(20000)
+    monkeypatch.setenv("NBX_BARRIER_SPINS", "0")
+    with new_sim(nbx, s) as sim:
+        with pytest.raises(nbx.NbxError) as err:
+            sim.step(3)
+        assert err.value.code == 4 and "grid barrier timed out" in str(err.value)
+    monkeypatch.delenv("NBX_BARRIER_SPINS")
+    want = s.copy()
+    for _ in range(2):
+        oracle.step(want)
+    with new_sim(nbx, s, summation=nbx.SUM_REFERENCE) as sim:
+        sim.step(2)
+        got = sim.download()
+    for f in ("px", "py", "vx", "vy"):
+        assert_bits_equal(getattr(got, f), getattr(want, f), f)
+
+
 def test_large_contact_component(nbx, oracle):
     """A packed clump: one contact component far above 16 pairs (workgroup path)."""
     from oracle_lib import Bodies

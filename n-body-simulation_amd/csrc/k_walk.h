@@ -286,11 +286,17 @@ constexpr int kRegStackLevels = 21;
 struct RegStack {
   int node = 0, lo = 0, hi = 0;
 
+  // v_writelane, one per field, with the lane select in M0 (gfx9 reads one
+  // SGPR per VALU instruction besides M0); e is computed on the scalar unit.
   __device__ inline void put(int e, int nd, uint64_t m) {
-    const bool mine = static_cast<int>(threadIdx.x & (kWave - 1)) == e;
-    node = mine ? nd : node;
-    lo = mine ? static_cast<int>(static_cast<uint32_t>(m)) : lo;
-    hi = mine ? static_cast<int>(static_cast<uint32_t>(m >> 32)) : hi;
+    const int mlo = static_cast<int>(static_cast<uint32_t>(m));
+    const int mhi = static_cast<int>(static_cast<uint32_t>(m >> 32));
+    asm volatile(
+        "v_writelane_b32 %0, %4, %3\n\t"
+        "v_writelane_b32 %1, %5, %3\n\t"
+        "v_writelane_b32 %2, %6, %3"
+        : "+v"(node), "+v"(lo), "+v"(hi)
+        : "{m0}"(e), "s"(nd), "s"(mlo), "s"(mhi));
   }
   __device__ inline void get(int e, int &nd, uint64_t &m) const {
     const int l = e & (kWave - 1);

@@ -240,6 +240,16 @@ int nbx_phase_times(nbx_sim *sim, double ms[NBX_PHASES], uint64_t *steps);
  * work after it. */
 int nbx_stream(nbx_sim *sim, void **stream);
 
+/* ---- FP64 energy and momentum of the current state (no reference
+ *      counterpart: the diagnostics of the precision study). Kinetic
+ *      sum m|v|^2/2 and momentum sum m v are summed in FP64 on the host; the
+ *      potential sum over pairs of m_i m_j phi(r_ij) is evaluated in FP64 on
+ *      the GPU (O(N^2)) with phi(r) = -(pi/2 - atan(r/eps))/eps, the
+ *      potential of the walk's law m*d/(|d|(|d|^2+eps^2)) (-1/r when
+ *      eps2 = 0, coincident bodies then skipped). eps2 = epsilon^2. ---- */
+int nbx_energy(nbx_sim *sim, double eps2, double *kinetic, double *potential,
+               double momentum[2]);
+
 /* ---- FP64 direct sum (CheckCuda.cu:456-475 exactTreeLawForces) on the
  *      GPU, the accuracy judge: accelerations of the listed target bodies
  *      (device indices) from all bodies, law m*d/(|d|(|d|^2+eps^2)). ---- */

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_size(
     ctl->compPairs = 0;
     ctl->bigCount = 0;
     ctl->largest = 0;
-    ctl->status &= ~(kStatusPairOverflow | kStatusComponentTooLarge);
+    ctl->status &= ~kStatusPairOverflow;
   }
 }
 
@@ -855,69 +855,226 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
   }
 }
 
-// One workgroup per large component: bitonic sort of its pairs in LDS, then
-// lane 0 replays them in order.
+// ---- contact components of more than kSmallComp pairs ----
+//
+// One workgroup per component. Sort into pairLess order: up to kBigCompMax
+// pairs by a bitonic sort in LDS; longer components as runs of kBigCompMax
+// sorted in LDS, then merge-path rounds in global memory, ping-ponging with
+// `tmp` (the same offsets of the detection's pair array, no longer needed
+// once the segments are filled). Replay: every body of the component claims
+// a local slot in `bodySlot` (all -1 at rest; the claims are cleared again),
+// and when they fit, the bodies are staged in LDS and one lane replays the
+// pairs on the staged copies -- the same resolvePair arithmetic in the same
+// order -- a few hundred cycles per pair instead of a memory round trip.
+// Components with more than kReplayBodies bodies are replayed in global
+// memory.
+constexpr int kReplayBodies = 5120;
+constexpr int kReplayChunk = 1024; // pairs whose slots are staged at once
+constexpr int kMergeItems = 8;     // merge outputs per lane per round step
+constexpr int kResolveLds =        // words: sort area or replay area
+    (kReplayBodies * 7 + kReplayChunk * 2 > kBigCompMax * 4 + kBigCompMax / 2)
+        ? kReplayBodies * 7 + kReplayChunk * 2
+        : kBigCompMax * 4 + kBigCompMax / 2;
+
+// Sorts run[0, len) (len <= kBigCompMax) into pairLess order in LDS.
+__device__ inline void sortRunInLds(int4 *__restrict__ run, int len,
+                                    int4 *__restrict__ sp,
+                                    int16_t *__restrict__ order) {
+  int p2 = 1;
+  while (p2 < len) {
+    p2 <<= 1;
+  }
+  for (int k = threadIdx.x; k < p2; k += kBlock) {
+    if (k < len) {
+      sp[k] = run[k];
+    }
+    order[k] = static_cast<int16_t>(k);
+  }
+  __syncthreads();
+  for (int width = 2; width <= p2; width <<= 1) {
+    for (int stride = width >> 1; stride > 0; stride >>= 1) {
+      for (int k = threadIdx.x; k < p2; k += kBlock) {
+        const int partner = k ^ stride;
+        if (partner > k) {
+          const bool up = (k & width) == 0;
+          const int a = order[k], c = order[partner];
+          // Padding (index >= len) sorts after every real pair.
+          const bool aGreater = a >= len ? (c < len || a > c)
+                                         : (c < len && pairLess(sp[c], sp[a]));
+          if (aGreater == up) {
+            order[k] = static_cast<int16_t>(c);
+            order[partner] = static_cast<int16_t>(a);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = threadIdx.x; k < len; k += kBlock) {
+    run[k] = sp[order[k]];
+  }
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  __syncthreads();
+}
+
+// Merge path: how many of the first `diag` merged outputs come from A.
+__device__ inline int mergeSplit(const int4 *__restrict__ a, int na,
+                                 const int4 *__restrict__ b, int nb, int diag) {
+  int lo = max(0, diag - nb), hi = min(diag, na);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (pairLess(b[diag - 1 - mid], a[mid])) {
+      hi = mid;
+    } else {
+      lo = mid + 1;
+    }
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(kBlock) void k_resolve_big(
     Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
     const int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
+    int4 *__restrict__ tmpPairs, int32_t *__restrict__ bodySlot,
     float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
     float *__restrict__ vy, const float *__restrict__ m,
     const float *__restrict__ r) {
   if (barrierStalled(ctl)) {
     return;
   }
-  __shared__ int4 sp[kBigCompMax];
-  __shared__ int16_t order[kBigCompMax];
+  __shared__ __align__(16) uint32_t lds[kResolveLds];
+  __shared__ int nBodies;
+  // Sort area.
+  int4 *sp = reinterpret_cast<int4 *>(lds);
+  int16_t *order = reinterpret_cast<int16_t *>(lds + 4 * kBigCompMax);
+  // Replay area.
+  float *lx = reinterpret_cast<float *>(lds);
+  float *ly = lx + kReplayBodies, *lvx = ly + kReplayBodies;
+  float *lvy = lvx + kReplayBodies, *lm = lvy + kReplayBodies;
+  float *lr = lm + kReplayBodies;
+  int32_t *bodyIdx = reinterpret_cast<int32_t *>(lr + kReplayBodies);
+  int2 *chunk = reinterpret_cast<int2 *>(bodyIdx + kReplayBodies);
+
   const uint32_t big = ctl->bigCount;
-  for (uint32_t b = blockIdx.x; b < big; b += gridDim.x) {
-    const int4 info = compInfo[bigList[b]];
+  for (uint32_t g = blockIdx.x; g < big; g += gridDim.x) {
+    const int4 info = compInfo[bigList[g]];
     const int start = info.y, size = info.z;
-    if (size > kBigCompMax) {
-      if (threadIdx.x == 0) {
-        atomicOr(&ctl->status, kStatusComponentTooLarge);
+    int4 *seg = segPairs + start;
+
+    // 1. pairLess order.
+    if (size <= kBigCompMax) {
+      sortRunInLds(seg, size, sp, order);
+    } else {
+      for (int c0 = 0; c0 < size; c0 += kBigCompMax) {
+        sortRunInLds(seg + c0, min(kBigCompMax, size - c0), sp, order);
       }
-      continue;
-    }
-    int p2 = 1;
-    while (p2 < size) {
-      p2 <<= 1;
-    }
-    for (int k = threadIdx.x; k < p2; k += kBlock) {
-      if (k < size) {
-        sp[k] = segPairs[start + k];
-      }
-      order[k] = static_cast<int16_t>(k);
-    }
-    __syncthreads();
-    for (int len = 2; len <= p2; len <<= 1) {
-      for (int stride = len >> 1; stride > 0; stride >>= 1) {
-        for (int k = threadIdx.x; k < p2; k += kBlock) {
-          const int partner = k ^ stride;
-          if (partner > k) {
-            const bool up = (k & len) == 0;
-            const int a = order[k], c = order[partner];
-            // Padding (index >= size) sorts after every real pair.
-            const bool aGreater =
-                a >= size ? (c < size || a > c)
-                          : (c < size && pairLess(sp[c], sp[a]));
-            if (aGreater == up) {
-              order[k] = static_cast<int16_t>(c);
-              order[partner] = static_cast<int16_t>(a);
+      int4 *src = seg, *dst = tmpPairs + start;
+      for (int w = kBigCompMax; w < size; w <<= 1) {
+        for (int base = 0; base < size; base += 2 * w) {
+          const int na = min(w, size - base);
+          const int nb = max(0, min(w, size - base - na));
+          const int4 *a = src + base, *b = src + base + na;
+          const int total = na + nb;
+          for (int o = threadIdx.x * kMergeItems; o < total;
+               o += kBlock * kMergeItems) {
+            int ia = mergeSplit(a, na, b, nb, o);
+            int ib = o - ia;
+            const int cnt = min(kMergeItems, total - o);
+            for (int q = 0; q < cnt; ++q) {
+              const bool takeA =
+                  ib >= nb || (ia < na && !pairLess(b[ib], a[ia]));
+              dst[base + o + q] = takeA ? a[ia++] : b[ib++];
             }
           }
         }
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        __syncthreads();
+        int4 *t = src;
+        src = dst;
+        dst = t;
+      }
+      if (src != seg) {
+        for (int k = threadIdx.x; k < size; k += kBlock) {
+          seg[k] = src[k];
+        }
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         __syncthreads();
       }
     }
-    for (int k = threadIdx.x; k < size; k += kBlock) {
-      segPairs[start + k] = sp[order[k]];
-    }
+
+    // 2. Local slots for the component's bodies.
     if (threadIdx.x == 0) {
-      for (int k = 0; k < size; ++k) {
-        const int4 q = sp[order[k]];
-        resolvePair(px, py, vx, vy, m, r, q.x, q.y);
+      nBodies = 0;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < size; k += kBlock) {
+      const int4 q = seg[k];
+      const int ends[2] = {q.x, q.y};
+      for (int e = 0; e < 2; ++e) {
+        if (atomicCAS(&bodySlot[ends[e]], -1, -2) == -1) {
+          const int slot = atomicAdd(&nBodies, 1);
+          storeAgent(reinterpret_cast<uint32_t *>(&bodySlot[ends[e]]),
+                     static_cast<uint32_t>(slot));
+          if (slot < kReplayBodies) {
+            bodyIdx[slot] = ends[e];
+          }
+        }
       }
     }
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    __syncthreads();
+    const int nb = nBodies;
+
+    // 3. Replay in pairLess order.
+    if (nb <= kReplayBodies) {
+      for (int s = threadIdx.x; s < nb; s += kBlock) {
+        const int i = bodyIdx[s];
+        lx[s] = px[i];
+        ly[s] = py[i];
+        lvx[s] = vx[i];
+        lvy[s] = vy[i];
+        lm[s] = m[i];
+        lr[s] = r[i];
+      }
+      for (int c0 = 0; c0 < size; c0 += kReplayChunk) {
+        const int cnt = min(kReplayChunk, size - c0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < cnt; k += kBlock) {
+          const int4 q = seg[c0 + k];
+          chunk[k] = make_int2(
+              static_cast<int>(loadAgent(reinterpret_cast<const uint32_t *>(&bodySlot[q.x]))),
+              static_cast<int>(loadAgent(reinterpret_cast<const uint32_t *>(&bodySlot[q.y]))));
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          for (int k = 0; k < cnt; ++k) {
+            resolvePair(lx, ly, lvx, lvy, lm, lr, chunk[k].x, chunk[k].y);
+          }
+        }
+      }
+      __syncthreads();
+      for (int s = threadIdx.x; s < nb; s += kBlock) {
+        const int i = bodyIdx[s];
+        px[i] = lx[s];
+        py[i] = ly[s];
+        vx[i] = lvx[s];
+        vy[i] = lvy[s];
+        bodySlot[i] = -1;
+      }
+    } else {
+      if (threadIdx.x == 0) {
+        for (int k = 0; k < size; ++k) {
+          const int4 q = seg[k];
+          resolvePair(px, py, vx, vy, m, r, q.x, q.y);
+        }
+      }
+      for (int k = threadIdx.x; k < size; k += kBlock) {
+        const int4 q = seg[k];
+        bodySlot[q.x] = -1;
+        bodySlot[q.y] = -1;
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     __syncthreads();
   }
 }

@@ -658,6 +658,65 @@ __global__ __launch_bounds__(kBlock) void k_exact_partial(
   }
 }
 
+// Per-body FP64 potential of the tree law, the partial sum over one chunk
+// of sources: sum over j != i of m_j phi(|x_j - x_i|), with
+// phi(r) = -(pi/2 - atan(r / eps)) / eps, whose gradient is the walk's
+// force m d / (|d| (|d|^2 + eps^2)); phi(r) = -1/r when eps = 0 (then
+// coincident bodies are skipped, as in the direct-sum forces).
+__global__ __launch_bounds__(kBlock) void k_energy_partial(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ pm, double eps, double *__restrict__ part) {
+  __shared__ float sx[kBlock], sy[kBlock], sm[kBlock];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = i < n;
+  const double x = valid ? px[i] : 0.0, y = valid ? py[i] : 0.0;
+  const uint32_t chunk = (n + kExactChunks - 1) / kExactChunks;
+  const uint32_t j0 = blockIdx.y * chunk;
+  const uint32_t j1 = min(n, j0 + chunk);
+  const double halfPi = 1.5707963267948966;
+  double phi = 0.0;
+  for (uint32_t b = j0; b < j1; b += kBlock) {
+    const uint32_t j = b + threadIdx.x;
+    __syncthreads();
+    if (j < j1) {
+      sx[threadIdx.x] = px[j];
+      sy[threadIdx.x] = py[j];
+      sm[threadIdx.x] = pm[j];
+    }
+    __syncthreads();
+    const uint32_t cnt = min<uint32_t>(kBlock, j1 - b);
+    for (uint32_t k = 0; k < cnt; ++k) {
+      if (b + k == i) {
+        continue;
+      }
+      const double dx = static_cast<double>(sx[k]) - x;
+      const double dy = static_cast<double>(sy[k]) - y;
+      const double r = sqrt(dx * dx + dy * dy);
+      if (eps > 0.0) {
+        phi -= static_cast<double>(sm[k]) * (halfPi - atan(r / eps)) / eps;
+      } else if (r > 0.0) {
+        phi -= static_cast<double>(sm[k]) / r;
+      }
+    }
+  }
+  if (valid) {
+    part[static_cast<size_t>(blockIdx.y) * n + i] = phi;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy_reduce(
+    uint32_t n, const double *__restrict__ part, double *__restrict__ phi) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) {
+    return;
+  }
+  double s = 0.0;
+  for (int c = 0; c < kExactChunks; ++c) {
+    s += part[static_cast<size_t>(c) * n + i];
+  }
+  phi[i] = s;
+}
+
 __global__ __launch_bounds__(kBlock) void k_exact_reduce(
     uint32_t nt, const double2 *__restrict__ part, double *__restrict__ ax,
     double *__restrict__ ay) {

@@ -19,7 +19,7 @@ constexpr int kLeafCapacity = 16;  // Simulation.cpp:9 leafCapacity
 constexpr int kMaxDepth = 32;      // key depth: 2 words x 16 levels
 constexpr int kDeepMax = 4096;     // bodies per equal-hi-key run sortable in LDS
 constexpr int kSmallComp = 16;     // contact components replayed by one lane
-constexpr int kBigCompMax = 4096;  // pairs per component sortable in LDS
+constexpr int kBigCompMax = 4096;  // pairs per component sorted in LDS at once
 constexpr uint32_t kDone = 0x7fffffffu;
 
 // Device status bits (Ctl::status). Any set bit makes the step fail with
@@ -28,7 +28,6 @@ enum : uint32_t {
   kStatusNodeOverflow = 1u,
   kStatusPairOverflow = 2u,
   kStatusDeepTooLarge = 4u,
-  kStatusComponentTooLarge = 8u,
   kStatusTreeInconsistent = 16u, // internal check failed (never expected)
   kStatusJacobiTooManyLarge = 32u,
   kStatusBarrierStalled = 64u, // a grid barrier of a one-launch kernel timed out

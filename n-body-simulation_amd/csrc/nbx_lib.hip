@@ -102,6 +102,7 @@ struct nbx_sim {
   uint32_t *compCount = nullptr, *compStart = nullptr;
   int4 *compInfo = nullptr;
   int32_t *bigList = nullptr;
+  int32_t *bodySlot = nullptr; // per body: slot in a big component's replay, -1 at rest
 
   // tree
   uint32_t *keyHi[2]{}, *vals[2]{};
@@ -428,7 +429,7 @@ int launchCollide(nbx_sim *s) {
   LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->segPairs,
          s->bigList, px, py, vx, vy, m, r);
   LAUNCH(k_resolve_big, 256, kBlock, s->ctl, s->compInfo, s->bigList,
-         s->segPairs, px, py, vx, vy, m, r);
+         s->segPairs, s->pairs, s->bodySlot, px, py, vx, vy, m, r);
   s->collided = true;
   return NBX_OK;
 }
@@ -713,10 +714,6 @@ int syncAndCheck(nbx_sim *s) {
                     "grid barrier timed out (workgroups were not all "
                     "resident)");
   }
-  if (status & kStatusComponentTooLarge) {
-    return setError(NBX_ERR_CAPACITY,
-                    "a collision contact component exceeds 4096 pairs");
-  }
   return NBX_OK;
 }
 
@@ -835,6 +832,10 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->pairComp, s->pairCap, false)
   A(s->compCount, n, true) A(s->compStart, n, false) A(s->compInfo, n, false)
   A(s->bigList, n, false)
+  if ((rc = devAlloc(s, &s->bodySlot, n, false)) != NBX_OK ||
+      hipMemset(s->bodySlot, 0xff, n * sizeof(int32_t)) != hipSuccess) {
+    return fail(rc != NBX_OK ? rc : setError(NBX_ERR_HIP, "hipMemset failed"));
+  }
   A(s->keyLo, n, false)
   {
     // The one-launch sort needs every one of its workgroups resident at
@@ -1375,6 +1376,65 @@ int nbx_stream(nbx_sim *s, void **stream) {
     return setError(NBX_ERR_ARG, "null argument");
   }
   *stream = s->stream;
+  return NBX_OK;
+}
+
+int nbx_energy(nbx_sim *s, double eps2, double *kinetic, double *potential,
+               double momentum[2]) {
+  if (!s || !kinetic || !potential || !momentum) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (!(eps2 >= 0.0)) {
+    return setError(NBX_ERR_ARG, "eps2 must be >= 0");
+  }
+  *kinetic = *potential = momentum[0] = momentum[1] = 0.0;
+  const uint64_t n = s->n;
+  if (n == 0) {
+    return NBX_OK;
+  }
+  NBX_TRY(syncAndCheck(s));
+  const int c = s->cur;
+  std::vector<float> m(n), vx(n), vy(n);
+  NBX_TRY(copyOut(s, m.data(), s->m[c], n));
+  NBX_TRY(copyOut(s, vx.data(), s->vx[c], n));
+  NBX_TRY(copyOut(s, vy.data(), s->vy[c], n));
+  double *part = nullptr, *phi = nullptr;
+  if (hipMalloc(&part, static_cast<size_t>(kExactChunks) * n * sizeof(double)) !=
+          hipSuccess ||
+      hipMalloc(&phi, n * sizeof(double)) != hipSuccess) {
+    (void)hipFree(part);
+    (void)hipFree(phi);
+    return setError(NBX_ERR_HIP, "hipMalloc failed in nbx_energy");
+  }
+  const uint32_t nn = static_cast<uint32_t>(n);
+  hipLaunchKernelGGL(k_energy_partial, dim3(blocksFor(nn), kExactChunks),
+                     dim3(kBlock), 0, s->stream, nn, s->px[c], s->py[c],
+                     s->m[c], std::sqrt(eps2), part);
+  hipLaunchKernelGGL(k_energy_reduce, dim3(blocksFor(nn)), dim3(kBlock), 0,
+                     s->stream, nn, part, phi);
+  std::vector<double> hphi(n);
+  int rc = NBX_OK;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(hphi.data(), phi, n * sizeof(double), hipMemcpyDeviceToHost,
+                     s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess) {
+    rc = setError(NBX_ERR_HIP, "energy kernels failed");
+  }
+  (void)hipFree(part);
+  (void)hipFree(phi);
+  NBX_TRY(rc);
+  double k = 0.0, u = 0.0, p0 = 0.0, p1 = 0.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const double mi = m[i], ux = vx[i], uy = vy[i];
+    k += 0.5 * mi * (ux * ux + uy * uy);
+    u += 0.5 * mi * hphi[i];
+    p0 += mi * ux;
+    p1 += mi * uy;
+  }
+  *kinetic = k;
+  *potential = u;
+  momentum[0] = p0;
+  momentum[1] = p1;
   return NBX_OK;
 }
 

@@ -37,6 +37,7 @@ EXPORTS = (
     "nbx_root_quad", "nbx_download_keys", "nbx_collide_info", "nbx_download_pairs",
     "nbx_count_interactions", "nbx_set_profiling", "nbx_phase_times", "nbx_stream",
     "nbx_exact_forces", "nbx_init_scenario", "nbx_download_contacts",
+    "nbx_energy",
 )
 
 COLLIDE_SEQUENTIAL, COLLIDE_JACOBI = 0, 1
@@ -107,6 +108,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_phase_times.argtypes = [_vp, _f64p, C.POINTER(C.c_uint64)]
     L.nbx_stream.argtypes = [_vp, C.POINTER(_vp)]
     L.nbx_exact_forces.argtypes = [_vp, C.c_uint64, _u32p, C.c_double, _f64p, _f64p]
+    L.nbx_energy.argtypes = [_vp, C.c_double, C.POINTER(C.c_double),
+                             C.POINTER(C.c_double), _f64p]
     _lib = L
     return L
 
@@ -335,6 +338,14 @@ class Simulation:
         steps = C.c_uint64()
         self._check(self.L.nbx_phase_times(self.h, ms, C.byref(steps)))
         return dict(zip(PHASES, ms.tolist())), int(steps.value)
+
+    def energy(self, eps2: float = 1.0) -> dict:
+        """FP64 kinetic and potential energy (the walk's law) and momentum."""
+        k, u = C.c_double(), C.c_double()
+        p = np.zeros(2, np.float64)
+        self._check(self.L.nbx_energy(self.h, eps2, C.byref(k), C.byref(u), p))
+        return dict(kinetic=k.value, potential=u.value, total=k.value + u.value,
+                    momentum=(float(p[0]), float(p[1])))
 
     def exact_forces(self, targets, eps2: float = 1.0):
         t = np.ascontiguousarray(targets, np.uint32)

@@ -367,11 +367,43 @@ def test_barrier_timeout_fails_the_step(nbx, oracle, monkeypatch):
         assert_bits_equal(getattr(got, f), getattr(want, f), f)
 
 
-def test_large_contact_component(nbx, oracle):
-    """A packed clump: one contact component far above 16 pairs (workgroup path)."""
+@pytest.mark.parametrize("eps", [1.0, 0.0])
+def test_energy_fp64(nbx, oracle, eps):
+    """FP64 energy and momentum diagnostics against a numpy FP64 O(N^2) sum:
+    the potential of the walk's law, phi(r) = -(pi/2 - atan(r/eps))/eps
+    (-1/r at eps = 0)."""
+    s = oracle.uniform_disc(2000)
+    for _ in range(3):
+        oracle.step(s)
+    with new_sim(nbx, s) as sim:
+        e = sim.energy(eps * eps)
+    x, y = s.px.astype(np.float64), s.py.astype(np.float64)
+    m = s.m.astype(np.float64)
+    vx, vy = s.vx.astype(np.float64), s.vy.astype(np.float64)
+    iu = np.triu_indices(s.n, 1)
+    r = np.hypot(x[iu[1]] - x[iu[0]], y[iu[1]] - y[iu[0]])
+    if eps > 0:
+        phi = -(np.pi / 2 - np.arctan(r / eps)) / eps
+    else:
+        phi = -1.0 / np.where(r > 0, r, np.inf)
+    pot = np.sum(m[iu[0]] * m[iu[1]] * phi)
+    kin = 0.5 * np.sum(m * (vx * vx + vy * vy))
+    scale = np.sum(m * np.hypot(vx, vy))
+    assert abs(e["potential"] - pot) <= 1e-10 * abs(pot)
+    assert abs(e["kinetic"] - kin) <= 1e-12 * kin
+    assert abs(e["momentum"][0] - np.sum(m * vx)) <= 1e-12 * scale
+    assert abs(e["momentum"][1] - np.sum(m * vy)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("side", [30, 45, 80])
+def test_large_contact_component(nbx, oracle, side):
+    """A packed clump: one contact component far above 16 pairs (workgroup
+    path). side 30: ~3.5k pairs, sorted in LDS, replayed on LDS copies;
+    side 45: ~8k pairs, LDS-sorted runs merged in global memory; side 80:
+    6400 bodies, more than the replay stages in LDS, replayed in global
+    memory."""
     from oracle_lib import Bodies
     rng = np.random.default_rng(3)
-    side = 30
     g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)
     n = g.shape[0] + 500
     b = Bodies.empty(n)
@@ -390,7 +422,7 @@ def test_large_contact_component(nbx, oracle):
         sim.collide()
         info = sim.collide_info()
         got = sim.download()
-    assert info["largest"] > 16
+    assert info["largest"] > (4096 if side > 30 else 16)
     for f in ("px", "py", "vx", "vy"):
         assert_bits_equal(getattr(got, f), getattr(want, f), f)
 

@@ -200,8 +200,9 @@ int nbx_collide_info(nbx_sim *sim, float *cell, int64_t *pairs,
 
 /* The last collide's pairs (device indices, later/earlier as in
  * Collision.cpp:120-127) in the order they were resolved, grouped by
- * component: comp[k] is the component's smallest body index. Pairs of one
- * component appear in the reference's pairLess order. */
+ * component: comp[k] is the component's smallest body index. Every body's
+ * pairs appear in the reference's pairLess order (resolve() touches only its
+ * two bodies, so any such order gives the sequential replay's result). */
 int nbx_download_pairs(nbx_sim *sim, int64_t cap, int32_t *later,
                        int32_t *earlier, int32_t *comp, int64_t *count);
 

@@ -273,6 +273,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_size(
     ctl->compCount = 0;
     ctl->compPairs = 0;
     ctl->bigCount = 0;
+    for (int t = 0; t < kWaveTiers; ++t) {
+      ctl->waveCount[t] = 0;
+      ctl->waveNext[t] = 0;
+    }
     ctl->largest = 0;
     ctl->status &= ~kStatusPairOverflow;
   }
@@ -822,24 +826,55 @@ __device__ inline void resolveComponent(int start, int size,
 }
 
 // One lane per component of up to kSmallComp pairs: pairLess order, then the
-// sequential replay. Larger components are queued for k_resolve_big.
+// sequential replay. Larger components are queued by size for
+// k_resolve_wave's tiers, beyond those for k_resolve_big.
 __global__ __launch_bounds__(kBlock) void k_resolve_small(
     Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
-    int4 *__restrict__ segPairs, int32_t *__restrict__ bigList,
-    float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
-    float *__restrict__ vy, const float *__restrict__ m,
-    const float *__restrict__ r) {
-  if (barrierStalled(ctl)) {
+    int4 *__restrict__ segPairs, int32_t *__restrict__ tierList,
+    uint32_t listCap, int32_t *__restrict__ bigList, float *__restrict__ px,
+    float *__restrict__ py, float *__restrict__ vx, float *__restrict__ vy,
+    const float *__restrict__ m, const float *__restrict__ r) {
+  if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
     return;
   }
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= ctl->compCount || (ctl->status & kStatusPairOverflow)) {
-    return;
+  const bool live = j < ctl->compCount;
+  int4 info = make_int4(0, 0, 0, 0);
+  if (live) {
+    info = compInfo[j];
   }
-  const int4 info = compInfo[j];
   const int start = info.y, size = info.z;
+  // Queue class: 0..kWaveTiers-1 a wave tier, kWaveTiers the workgroup
+  // path, -1 none. One atomic per class and wave.
+  int cls = -1;
   if (size > kSmallComp) {
-    bigList[atomicAdd(&ctl->bigCount, 1u)] = static_cast<int32_t>(j);
+    cls = 0;
+    while (cls < kWaveTiers && size > waveTierPairs(cls)) {
+      ++cls;
+    }
+  }
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (int c = 0; c <= kWaveTiers; ++c) {
+    const bool mine = cls == c;
+    const uint64_t mask = __ballot(mine);
+    if (mask == 0) {
+      continue;
+    }
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (static_cast<int>(lane) == leader) {
+      base = atomicAdd(c < kWaveTiers ? &ctl->waveCount[c] : &ctl->bigCount,
+                       static_cast<uint32_t>(__popcll(mask)));
+    }
+    base = __shfl(base, leader, kWave);
+    if (mine) {
+      int32_t *dst = c < kWaveTiers ? tierList + static_cast<size_t>(c) * listCap
+                                    : bigList;
+      dst[base + static_cast<uint32_t>(__popcll(mask & laneMaskLt()))] =
+          static_cast<int32_t>(j);
+    }
+  }
+  if (!live || size > kSmallComp) {
     return;
   }
   static_assert(kSmallComp == 16, "size classes below cover 16 pairs");
@@ -852,6 +887,379 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
     resolveComponent<8>(start, size, segPairs, px, py, vx, vy, m, r);
   } else {
     resolveComponent<16>(start, size, segPairs, px, py, vx, vy, m, r);
+  }
+}
+
+// ---- contact components of kSmallComp+1 .. 4096 pairs: one wave each ----
+//
+// resolve() touches only its two bodies, so the sequential replay of a
+// component is fixed by the order of the pairs on each body: any order that
+// keeps, for every body, its pairs in pairLess order gives every pair
+// exactly the inputs the sequential replay gives it -- a bit-identical
+// result. Pairs are oriented by before() (Collision.cpp:88-90): the later
+// body has the larger (minX, index), so a body's pairs in pairLess order are
+// its pairs ordered by the other body's (minX, index). The replay is then a
+// topological sort of the per-body chains: rounds of pairs whose earlier
+// pairs on both bodies are done (Kahn's algorithm), the pairs of a round in
+// parallel -- as many rounds as the longest chain (~0.3 x the pairs in the
+// packed clumps a long collisional run builds) instead of one step a pair.
+//
+// One 64-lane workgroup per component, all in its own LDS region so that
+// many components are in flight on each CU (the rounds are latency-bound):
+// an open-addressing table gives the bodies dense slots, counting sort
+// builds each body's list of pairs, one lane a body sorts its list (at most
+// kMaxDegree pairs, in registers) and links each pair to the next pair on
+// that body, the bodies are staged in LDS and the rounds replay the pairs.
+// The pairs are written back in the order they were replayed. A component
+// with more than BMAX bodies goes on to the next tier's queue, one with a
+// body of more than kMaxDegree pairs to k_resolve_big.
+constexpr int kMaxDegree = 16;
+constexpr uint32_t kEmptyKey = 0xffffffffu;
+constexpr uint16_t kNoPair = 0xffffu;
+
+template <int PMAX, int BMAX> struct WaveLds {
+  static_assert(PMAX < 65535 && BMAX <= 32768 && BMAX % 2 == 0, "16-bit fields");
+  static_assert(2 * BMAX >= PMAX + 2, "the body table always has a free entry");
+  static constexpr int kHash = 2 * BMAX;            // table entries
+  static constexpr int kPairSlot = 0;               // u32[P]: slot A | slot B << 16
+  static constexpr int kSucc = PMAX;                // u16[2P]: next pair on A, on B
+  static constexpr int kIndeg = 2 * PMAX;           // u32[P]: earlier pairs not done
+  static constexpr int kInc = 3 * PMAX;             // u16[2P]: lists, then frontiers
+  static constexpr int kBodyIdx = 4 * PMAX;         // i32[B]
+  static constexpr int kMinX = 4 * PMAX + BMAX;     // u32[B]: bits of the box minX
+  static constexpr int kU = 4 * PMAX + 2 * BMAX;    // 6B words, in turn: table
+  static constexpr int kWords = kU + 6 * BMAX;      // (3B), lists (2B+1), state (6B)
+};
+
+// before() order (Collision.cpp:88-90) as one integer: minX in
+// order-preserving bits (-0 made +0, as float < treats them alike), then the
+// index.
+__device__ inline uint64_t beforeKey(uint32_t minXBits, int32_t index) {
+  const float f = __uint_as_float(minXBits) + 0.0f;
+  return (static_cast<uint64_t>(ordBits(f)) << 32) | static_cast<uint32_t>(index);
+}
+
+// Inserts key e into the table; h is where it lives. True when this call
+// added it. Terminates: the table never fills (more entries than bodies).
+__device__ inline bool hashInsert(uint32_t *__restrict__ key, uint32_t size,
+                                  uint32_t e, uint32_t &h) {
+  h = static_cast<uint32_t>((static_cast<uint64_t>(e * 0x9E3779B1u) * size) >> 32);
+  while (true) {
+    const uint32_t old = atomicCAS(&key[h], kEmptyKey, e);
+    if (old == kEmptyKey) {
+      return true;
+    }
+    if (old == e) {
+      return false;
+    }
+    h = (h + 1 == size) ? 0u : h + 1;
+  }
+}
+
+// Sorts (key, pair) by key with Batcher's odd-even merge network (static
+// indices only, so everything stays in registers).
+template <int N>
+__device__ inline void sortIncident(uint64_t (&key)[N], uint32_t (&pr)[N]) {
+#pragma unroll
+  for (int w = 1; w < N; w <<= 1) {
+#pragma unroll
+    for (int k = w; k >= 1; k >>= 1) {
+#pragma unroll
+      for (int j = k % w; j + k < N; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+          if (i + j + k < N && (i + j) / (2 * w) == (i + j + k) / (2 * w)) {
+            const int a = i + j, b = i + j + k;
+            if (key[b] < key[a]) {
+              const uint64_t tk = key[a];
+              key[a] = key[b];
+              key[b] = tk;
+              const uint32_t tp = pr[a];
+              pr[a] = pr[b];
+              pr[b] = tp;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Sorts body s's list (d <= N entries at inc[b0..]) into pairLess order and
+// links each pair to the next one on s; every pair after the first gets one
+// more earlier pair to wait for.
+template <int N>
+__device__ inline void linkBody(int s, uint32_t b0, uint32_t d,
+                                const uint16_t *__restrict__ inc,
+                                const uint32_t *__restrict__ pairSlot,
+                                const uint32_t *__restrict__ minX,
+                                const int32_t *__restrict__ bodyIdx,
+                                uint16_t *__restrict__ succ,
+                                uint32_t *__restrict__ indeg) {
+  uint64_t key[N];
+  uint32_t pr[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    key[i] = ~0ull;
+    pr[i] = 0;
+    if (static_cast<uint32_t>(i) < d) {
+      const uint32_t k = inc[b0 + i];
+      const uint32_t sl = pairSlot[k];
+      const uint32_t a = sl & 0xffffu, b = sl >> 16;
+      const uint32_t o = a == static_cast<uint32_t>(s) ? b : a;
+      key[i] = beforeKey(minX[o], bodyIdx[o]);
+      pr[i] = k | (a == static_cast<uint32_t>(s) ? 0u : 0x10000u); // side of s
+    }
+  }
+  sortIncident<N>(key, pr);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (static_cast<uint32_t>(i) < d) {
+      const uint32_t k = pr[i] & 0xffffu, side = pr[i] >> 16;
+      const uint16_t next =
+          static_cast<uint32_t>(i + 1) < d ? static_cast<uint16_t>(pr[i + 1 < N ? i + 1 : i] & 0xffffu)
+                                           : kNoPair;
+      succ[2 * k + side] = next;
+      if (i > 0) {
+        atomicAdd(&indeg[k], 1u);
+      }
+    }
+  }
+}
+
+template <int PMAX, int BMAX>
+__global__ __launch_bounds__(kWave) void k_resolve_wave(
+    Ctl *__restrict__ ctl, int tier, const int4 *__restrict__ compInfo,
+    int32_t *__restrict__ tierList, uint32_t listCap,
+    int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
+    float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
+    float *__restrict__ vy, const float *__restrict__ m,
+    const float *__restrict__ r) {
+  using L = WaveLds<PMAX, BMAX>;
+  if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
+    return;
+  }
+  const uint32_t count = ctl->waveCount[tier];
+  if (blockIdx.x >= count) {
+    return;
+  }
+  __shared__ __align__(16) uint32_t lds[L::kWords];
+  __shared__ uint32_t frontierCount[2];
+  uint32_t *pairSlot = lds + L::kPairSlot;
+  uint16_t *succ = reinterpret_cast<uint16_t *>(lds + L::kSucc);
+  uint32_t *indeg = lds + L::kIndeg;
+  uint16_t *inc = reinterpret_cast<uint16_t *>(lds + L::kInc);
+  int32_t *bodyIdx = reinterpret_cast<int32_t *>(lds + L::kBodyIdx);
+  uint32_t *minX = lds + L::kMinX;
+  uint32_t *hkey = lds + L::kU;
+  uint16_t *hslot = reinterpret_cast<uint16_t *>(lds + L::kU + 2 * BMAX);
+  uint32_t *start = lds + L::kU;
+  uint32_t *cursor = lds + L::kU + BMAX + 1;
+  float *lx = reinterpret_cast<float *>(lds + L::kU);
+  float *ly = lx + BMAX, *lvx = ly + BMAX, *lvy = lvx + BMAX;
+  float *lm = lvy + BMAX, *lr = lm + BMAX;
+  const int lane = static_cast<int>(threadIdx.x);
+  const uint64_t lt = laneMaskLt();
+  const int32_t *list = tierList + static_cast<size_t>(tier) * listCap;
+
+  for (uint32_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const int comp = list[g];
+    const int4 info = compInfo[comp];
+    const int size = info.z; // kSmallComp < size <= PMAX
+    int4 *seg = segPairs + info.y;
+
+    // 1. A dense slot per body; pairSlot holds table positions meanwhile.
+    for (int t = lane; t < L::kHash; t += kWave) {
+      hkey[t] = kEmptyKey;
+    }
+    __syncthreads();
+    uint32_t nb = 0;
+    for (int k0 = 0; k0 < size; k0 += kWave) {
+      const int k = k0 + lane;
+      int4 q = make_int4(0, 0, 0, 0);
+      uint32_t h0 = 0, h1 = 0;
+      bool w0 = false, w1 = false;
+      if (k < size) {
+        q = seg[k];
+        w0 = hashInsert(hkey, L::kHash, static_cast<uint32_t>(q.x), h0);
+        w1 = hashInsert(hkey, L::kHash, static_cast<uint32_t>(q.y), h1);
+        pairSlot[k] = h0 | (h1 << 16);
+      }
+      const uint64_t b0 = __ballot(w0), b1 = __ballot(w1);
+      const uint32_t n0 = static_cast<uint32_t>(__popcll(b0));
+      if (w0) {
+        const uint32_t slot = nb + static_cast<uint32_t>(__popcll(b0 & lt));
+        hslot[h0] = static_cast<uint16_t>(slot);
+        if (slot < BMAX) {
+          bodyIdx[slot] = q.x;
+          minX[slot] = static_cast<uint32_t>(q.z);
+        }
+      }
+      if (w1) {
+        const uint32_t slot = nb + n0 + static_cast<uint32_t>(__popcll(b1 & lt));
+        hslot[h1] = static_cast<uint16_t>(slot);
+        if (slot < BMAX) {
+          bodyIdx[slot] = q.y;
+          minX[slot] = static_cast<uint32_t>(q.w);
+        }
+      }
+      nb += n0 + static_cast<uint32_t>(__popcll(b1));
+    }
+    __syncthreads();
+    if (nb > BMAX) {
+      if (lane == 0) {
+        if (tier + 1 < kWaveTiers) {
+          tierList[static_cast<size_t>(tier + 1) * listCap +
+                   atomicAdd(&ctl->waveCount[tier + 1], 1u)] = comp;
+        } else {
+          bigList[atomicAdd(&ctl->bigCount, 1u)] = comp;
+        }
+      }
+      continue;
+    }
+    for (int k = lane; k < size; k += kWave) {
+      const uint32_t h = pairSlot[k];
+      pairSlot[k] = static_cast<uint32_t>(hslot[h & 0xffffu]) |
+                    (static_cast<uint32_t>(hslot[h >> 16]) << 16);
+    }
+    __syncthreads(); // the table is dead from here on
+
+    // 2. Each body's list of pairs (counting sort by slot).
+    for (int t = lane; t <= static_cast<int>(nb); t += kWave) {
+      start[t] = 0;
+    }
+    for (int k = lane; k < size; k += kWave) {
+      indeg[k] = 0;
+    }
+    __syncthreads();
+    for (int k = lane; k < size; k += kWave) {
+      const uint32_t sl = pairSlot[k];
+      atomicAdd(&start[sl & 0xffffu], 1u);
+      atomicAdd(&start[sl >> 16], 1u);
+    }
+    __syncthreads();
+    uint32_t carry = 0, maxDeg = 0;
+    for (uint32_t s0 = 0; s0 < nb; s0 += kWave) {
+      const uint32_t t = s0 + static_cast<uint32_t>(lane);
+      const uint32_t own = t < nb ? start[t] : 0u;
+      maxDeg = max(maxDeg, own);
+      uint32_t v = own;
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off, kWave);
+        if (lane >= off) {
+          v += u;
+        }
+      }
+      if (t < nb) {
+        start[t] = carry + v - own;
+        cursor[t] = carry + v - own;
+      }
+      carry += __shfl(v, kWave - 1, kWave);
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      maxDeg = max(maxDeg, static_cast<uint32_t>(__shfl_xor(maxDeg, off, kWave)));
+    }
+    if (lane == 0) {
+      start[nb] = carry;
+    }
+    if (maxDeg > static_cast<uint32_t>(kMaxDegree)) {
+      if (lane == 0) {
+        bigList[atomicAdd(&ctl->bigCount, 1u)] = comp;
+      }
+      __syncthreads();
+      continue;
+    }
+    __syncthreads();
+    for (int k = lane; k < size; k += kWave) {
+      const uint32_t sl = pairSlot[k];
+      inc[atomicAdd(&cursor[sl & 0xffffu], 1u)] = static_cast<uint16_t>(k);
+      inc[atomicAdd(&cursor[sl >> 16], 1u)] = static_cast<uint16_t>(k);
+    }
+    __syncthreads();
+
+    // 3. Lists into pairLess order; links to the next pair on each body.
+    for (int t = lane; t < static_cast<int>(nb); t += kWave) {
+      const uint32_t b0 = start[t], d = start[t + 1] - b0;
+      if (d <= 4) {
+        linkBody<4>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+      } else if (d <= 8) {
+        linkBody<8>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+      } else {
+        linkBody<kMaxDegree>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+      }
+    }
+    __syncthreads(); // the lists are dead from here on
+
+    // 4. Bodies into LDS; the first round: pairs with no earlier pair.
+    for (int t = lane; t < static_cast<int>(nb); t += kWave) {
+      const int i = bodyIdx[t];
+      lx[t] = px[i];
+      ly[t] = py[i];
+      lvx[t] = vx[i];
+      lvy[t] = vy[i];
+      lm[t] = m[i];
+      lr[t] = r[i];
+    }
+    uint16_t *front = inc;
+    uint32_t cnt = 0;
+    for (int k0 = 0; k0 < size; k0 += kWave) {
+      const int k = k0 + lane;
+      const bool ready = k < size && indeg[k] == 0u;
+      const uint64_t bm = __ballot(ready);
+      if (ready) {
+        front[cnt + static_cast<uint32_t>(__popcll(bm & lt))] = static_cast<uint16_t>(k);
+      }
+      cnt += static_cast<uint32_t>(__popcll(bm));
+    }
+    __syncthreads();
+
+    // 5. Rounds. Pairs are written back in the order they are replayed.
+    uint32_t done = 0;
+    int cur = 0;
+    while (cnt > 0) {
+      const uint16_t *f = front + cur * PMAX;
+      uint16_t *nf = front + (cur ^ 1) * PMAX;
+      if (lane == 0) {
+        frontierCount[cur ^ 1] = 0;
+      }
+      __syncthreads();
+      for (uint32_t idx = static_cast<uint32_t>(lane); idx < cnt; idx += kWave) {
+        const uint32_t k = f[idx];
+        const uint32_t sl = pairSlot[k];
+        const int a = static_cast<int>(sl & 0xffffu), b = static_cast<int>(sl >> 16);
+        resolvePair(lx, ly, lvx, lvy, lm, lr, a, b);
+        if (done + idx < static_cast<uint32_t>(size)) {
+          seg[done + idx] = make_int4(bodyIdx[a], bodyIdx[b], static_cast<int>(minX[a]),
+                                      static_cast<int>(minX[b]));
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const uint32_t nx = succ[2 * k + side];
+          if (nx != kNoPair && atomicSub(&indeg[nx], 1u) == 1u) {
+            const uint32_t pos = atomicAdd(&frontierCount[cur ^ 1], 1u);
+            if (pos < PMAX) {
+              nf[pos] = static_cast<uint16_t>(nx);
+            }
+          }
+        }
+      }
+      done += cnt;
+      __syncthreads();
+      cnt = min(frontierCount[cur ^ 1], static_cast<uint32_t>(PMAX));
+      cur ^= 1;
+    }
+
+    // 6. Bodies back.
+    for (int t = lane; t < static_cast<int>(nb); t += kWave) {
+      const int i = bodyIdx[t];
+      px[i] = lx[t];
+      py[i] = ly[t];
+      vx[i] = lvx[t];
+      vy[i] = lvy[t];
+    }
+    __syncthreads();
   }
 }
 

@@ -20,6 +20,12 @@ constexpr int kMaxDepth = 32;      // key depth: 2 words x 16 levels
 constexpr int kDeepMax = 4096;     // bodies per equal-hi-key run sortable in LDS
 constexpr int kSmallComp = 16;     // contact components replayed by one lane
 constexpr int kBigCompMax = 4096;  // pairs per component sorted in LDS at once
+// Contact components of kSmallComp+1 .. 4096 pairs are replayed by one wave
+// each, in tiers by pair count (k_resolve_wave).
+constexpr int kWaveTiers = 3;
+__host__ __device__ constexpr int waveTierPairs(int tier) {
+  return tier == 0 ? 256 : tier == 1 ? 1024 : 4096;
+}
 constexpr uint32_t kDone = 0x7fffffffu;
 
 // Device status bits (Ctl::status). Any set bit makes the step fail with
@@ -50,6 +56,8 @@ struct Ctl {
   int32_t totalSplits, nodeCount, levels, deepCount;
   int32_t subtreeRoots;
   uint32_t status;
+  uint32_t waveCount[kWaveTiers]; // components queued per k_resolve_wave tier
+  uint32_t waveNext[kWaveTiers];  // next queue entry per tier
   uint32_t pad[2];
   unsigned long long farCount, bodyPairCount;
   unsigned long long waveVisits, waveLeafTerms; // counting walk only

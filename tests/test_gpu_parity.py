@@ -199,14 +199,25 @@ def check_collide(nbx, oracle, s):
     # Same unordered pair set, each pair oriented (later, earlier) the same way.
     key = lambda a, b: (a.astype(np.int64) << 32) | b.astype(np.int64)
     assert np.array_equal(np.sort(key(glater, gearlier)), np.sort(key(later, earlier)))
-    # Within each contact component, the reference's resolution order.
-    seq = {int(k): i for i, k in enumerate(key(later, earlier))}
-    gseq = np.array([seq[int(k)] for k in key(glater, gearlier)])
-    for c in np.unique(gcomp):
-        order = gseq[gcomp == c]
-        assert np.all(np.diff(order) > 0), f"component {c} replayed out of order"
+    # The resolution order: every body's pairs in the reference's order
+    # (resolve() touches only its two bodies, so this fixes the result).
+    ref = key(later, earlier)
+    rank = np.argsort(ref)
+    pos = np.searchsorted(ref[rank], key(glater, gearlier))
+    gseq = rank[pos]  # reference position of each downloaded pair
+    assert np.array_equal(ref[gseq], key(glater, gearlier))
+    body = np.concatenate([glater, gearlier])
+    at = np.concatenate([np.arange(gseq.size)] * 2)
+    o = np.lexsort((at, body))
+    same = body[o][1:] == body[o][:-1]
+    seqs = np.concatenate([gseq, gseq])[o]
+    assert np.all(np.diff(seqs)[same] > 0), "a body's pairs replayed out of order"
+    # Pairs of one component are contiguous.
+    starts = np.flatnonzero(np.diff(gcomp)) + 1
+    assert np.unique(gcomp).size == starts.size + (gcomp.size > 0)
     for f in ("px", "py", "vx", "vy"):
         assert_bits_equal(getattr(got, f), getattr(want, f), f)
+    return info, gcomp
 
 
 def test_step_trajectory_bit_exact(nbx, oracle, states):
@@ -397,11 +408,11 @@ def test_energy_fp64(nbx, oracle, eps):
 
 @pytest.mark.parametrize("side", [30, 45, 80])
 def test_large_contact_component(nbx, oracle, side):
-    """A packed clump: one contact component far above 16 pairs (workgroup
-    path). side 30: ~3.5k pairs, sorted in LDS, replayed on LDS copies;
-    side 45: ~8k pairs, LDS-sorted runs merged in global memory; side 80:
-    6400 bodies, more than the replay stages in LDS, replayed in global
-    memory."""
+    """A packed clump (boxes overlap with 8 neighbours): one contact
+    component far above 16 pairs. side 30: ~3.4k pairs, one wave (largest
+    tier, level-parallel replay); side 45: ~7.8k pairs, workgroup path,
+    LDS-sorted runs merged in global memory; side 80: 6400 bodies, more than
+    the workgroup replay stages in LDS, replayed in global memory."""
     from oracle_lib import Bodies
     rng = np.random.default_rng(3)
     g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)
@@ -425,6 +436,44 @@ def test_large_contact_component(nbx, oracle, side):
     assert info["largest"] > (4096 if side > 30 else 16)
     for f in ("px", "py", "vx", "vy"):
         assert_bits_equal(getattr(got, f), getattr(want, f), f)
+
+
+def test_contact_component_tiers(nbx, oracle):
+    """Packed square clumps of side 3..25 (12 to ~1200 pairs: the lane path
+    and each one-wave tier by pair count) and straight chains of 257, 700 and
+    3000 bodies (256, 699, 2999 pairs, one long dependency chain each, with
+    more bodies than the tier their pair count selects holds, so they are
+    passed on to the next tier or the workgroup path), all in one collide,
+    bit-exact against the sequential replay."""
+    from oracle_lib import Bodies
+    rng = np.random.default_rng(11)
+    xs, ys = [], []
+    for k, side in enumerate(range(3, 26)):
+        g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)
+        xs.append(g[:, 0] * 1.9 + rng.uniform(-0.08, 0.08, g.shape[0]) + 200.0 * k)
+        ys.append(g[:, 1] * 1.9 + rng.uniform(-0.08, 0.08, g.shape[0]))
+    for k, length in enumerate((257, 700, 3000)):
+        t = np.arange(length)
+        xs.append(t * 1.9 + rng.uniform(-0.03, 0.03, length))
+        ys.append(rng.uniform(-0.3, 0.3, length) + 300.0 + 100.0 * k)
+    xs.append(rng.uniform(-5000, 5000, 500))
+    ys.append(rng.uniform(-5000, -2000, 500))
+    px, py = np.concatenate(xs), np.concatenate(ys)
+    n = px.shape[0]
+    order = rng.permutation(n)  # bodies of one clump not adjacent in index
+    b = Bodies.empty(n)
+    b.px[:] = px[order].astype(np.float32)
+    b.py[:] = py[order].astype(np.float32)
+    b.vx[:] = rng.standard_normal(n).astype(np.float32)
+    b.vy[:] = rng.standard_normal(n).astype(np.float32)
+    b.m[:] = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    b.r[:] = 1.0
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    info, comp = check_collide(nbx, oracle, b)
+    sizes = np.unique(comp, return_counts=True)[1]
+    assert sizes.max() > 1024 and np.any((sizes > 16) & (sizes <= 256))
+    assert np.any((sizes > 256) & (sizes <= 1024))
+    assert info["largest"] == sizes.max()
 
 
 def test_exact_forces_fp64(nbx, oracle, states):

@@ -581,6 +581,30 @@ __global__ __launch_bounds__(kBlock) void k_union(
   }
 }
 
+// Runs of equal roots among a wave's consecutive lanes (valid lanes are a
+// prefix of the wave): the run's first lane, its length and, for every lane,
+// its run's first lane -- one atomic a run instead of one a pair (the pairs
+// of a clump come out of the search next to each other).
+struct RootRun {
+  bool head;
+  uint32_t length;
+  int headLane;
+};
+__device__ inline RootRun rootRun(int root, bool valid) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const int prev = __shfl_up(root, 1, kWave);
+  const bool head = valid && (lane == 0 || prev != root);
+  const uint64_t heads = __ballot(head);
+  const uint32_t nvalid = static_cast<uint32_t>(__popcll(__ballot(valid)));
+  RootRun run;
+  run.head = head;
+  const uint64_t above = lane == kWave - 1 ? 0ull : heads >> (lane + 1);
+  run.length = (above ? lane + 1 + static_cast<uint32_t>(__builtin_ctzll(above)) : nvalid) - lane;
+  const uint64_t upto = heads & (lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull));
+  run.headLane = upto ? 63 - __builtin_clzll(upto) : 0;
+  return run;
+}
+
 __global__ __launch_bounds__(kBlock) void k_comp_count(
     const Ctl *__restrict__ ctl, uint32_t cap, const int4 *__restrict__ pairs,
     int32_t *__restrict__ parent, int32_t *__restrict__ pairComp,
@@ -589,11 +613,19 @@ __global__ __launch_bounds__(kBlock) void k_comp_count(
     return;
   }
   const uint32_t count = min(ctl->pairCount, cap);
-  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;
-       k += gridDim.x * kBlock) {
-    const int root = ufFind(parent, pairs[k].x);
-    pairComp[k] = root;
-    atomicAdd(&compCount[root], 1u);
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (uint32_t k0 = blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1u));
+       k0 < count; k0 += gridDim.x * kBlock) {
+    const uint32_t k = k0 + lane;
+    int root = -1;
+    if (k < count) {
+      root = ufFind(parent, pairs[k].x);
+      pairComp[k] = root;
+    }
+    const RootRun run = rootRun(root, k < count);
+    if (run.head) {
+      atomicAdd(&compCount[root], run.length);
+    }
   }
 }
 
@@ -768,12 +800,23 @@ __global__ __launch_bounds__(kBlock) void k_comp_scatter(
     return;
   }
   const uint32_t count = min(ctl->pairCount, cap);
-  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;
-       k += gridDim.x * kBlock) {
-    const int root = pairComp[k];
-    const uint32_t slot =
-        compStart[root] + atomicSub(&compCount[root], 1u) - 1u;
-    segPairs[slot] = pairs[k];
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  for (uint32_t k0 = blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1u));
+       k0 < count; k0 += gridDim.x * kBlock) {
+    const uint32_t k = k0 + lane;
+    const bool valid = k < count;
+    const int root = valid ? pairComp[k] : -1;
+    const RootRun run = rootRun(root, valid);
+    uint32_t old = 0;
+    if (run.head) {
+      old = atomicSub(&compCount[root], run.length);
+    }
+    // The run takes slots [old - length, old) of its component's segment.
+    const uint32_t top = __shfl(old, run.headLane, kWave);
+    const uint32_t len = __shfl(run.length, run.headLane, kWave);
+    if (valid) {
+      segPairs[compStart[root] + top - len + (lane - run.headLane)] = pairs[k];
+    }
   }
 }
 
@@ -830,7 +873,7 @@ __device__ inline void resolveComponent(int start, int size,
 // k_resolve_wave's tiers, beyond those for k_resolve_big.
 __global__ __launch_bounds__(kBlock) void k_resolve_small(
     Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
-    int4 *__restrict__ segPairs, int32_t *__restrict__ tierList,
+    int4 *__restrict__ segPairs, int4 *__restrict__ tierList,
     uint32_t listCap, int32_t *__restrict__ bigList, float *__restrict__ px,
     float *__restrict__ py, float *__restrict__ vx, float *__restrict__ vy,
     const float *__restrict__ m, const float *__restrict__ r) {
@@ -868,10 +911,13 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
     }
     base = __shfl(base, leader, kWave);
     if (mine) {
-      int32_t *dst = c < kWaveTiers ? tierList + static_cast<size_t>(c) * listCap
-                                    : bigList;
-      dst[base + static_cast<uint32_t>(__popcll(mask & laneMaskLt()))] =
-          static_cast<int32_t>(j);
+      const uint32_t at = base + static_cast<uint32_t>(__popcll(mask & laneMaskLt()));
+      if (c < kWaveTiers) {
+        tierList[static_cast<size_t>(c) * listCap + at] =
+            make_int4(start, size, static_cast<int>(j), 0);
+      } else {
+        bigList[at] = static_cast<int32_t>(j);
+      }
     }
   }
   if (!live || size > kSmallComp) {
@@ -1027,24 +1073,36 @@ __device__ inline void linkBody(int s, uint32_t b0, uint32_t d,
   }
 }
 
+// Orders the wave's LDS accesses around this point: a workgroup of one wave
+// needs no barrier (a wave's LDS operations complete in order), and no wait
+// for its global stores, which nothing in the kernel reads back.
+__device__ inline void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Queue entry of a component: {first pair, pair count, component index, 0}.
+template <int PMAX, int BMAX> struct WaveCtx {
+  static constexpr int kPf = PMAX <= 4 * kWave ? PMAX / kWave : 0; // pairs/lane in registers
+  static constexpr int kSf = BMAX <= 4 * kWave ? BMAX / kWave : 0; // bodies/lane in registers
+  struct Pairs {
+    int4 v[kPf > 0 ? kPf : 1];
+  };
+};
+
 template <int PMAX, int BMAX>
-__global__ __launch_bounds__(kWave) void k_resolve_wave(
-    Ctl *__restrict__ ctl, int tier, const int4 *__restrict__ compInfo,
-    int32_t *__restrict__ tierList, uint32_t listCap,
+__device__ __forceinline__ void resolveWaveComponent(
+    uint32_t *__restrict__ lds, uint32_t *__restrict__ frontierCount,
+    Ctl *__restrict__ ctl, int tier, int4 info,
+    typename WaveCtx<PMAX, BMAX>::Pairs pre,
+    int4 *__restrict__ tierList, uint32_t listCap,
     int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
     float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
     float *__restrict__ vy, const float *__restrict__ m,
     const float *__restrict__ r) {
   using L = WaveLds<PMAX, BMAX>;
-  if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
-    return;
-  }
-  const uint32_t count = ctl->waveCount[tier];
-  if (blockIdx.x >= count) {
-    return;
-  }
-  __shared__ __align__(16) uint32_t lds[L::kWords];
-  __shared__ uint32_t frontierCount[2];
+  using W = WaveCtx<PMAX, BMAX>;
   uint32_t *pairSlot = lds + L::kPairSlot;
   uint16_t *succ = reinterpret_cast<uint16_t *>(lds + L::kSucc);
   uint32_t *indeg = lds + L::kIndeg;
@@ -1060,139 +1118,173 @@ __global__ __launch_bounds__(kWave) void k_resolve_wave(
   float *lm = lvy + BMAX, *lr = lm + BMAX;
   const int lane = static_cast<int>(threadIdx.x);
   const uint64_t lt = laneMaskLt();
-  const int32_t *list = tierList + static_cast<size_t>(tier) * listCap;
+  const int size = info.y; // kSmallComp < size <= PMAX
+  int4 *seg = segPairs + info.x;
 
-  for (uint32_t g = blockIdx.x; g < count; g += gridDim.x) {
-    const int comp = list[g];
-    const int4 info = compInfo[comp];
-    const int size = info.z; // kSmallComp < size <= PMAX
-    int4 *seg = segPairs + info.y;
-
-    // 1. A dense slot per body; pairSlot holds table positions meanwhile.
-    for (int t = lane; t < L::kHash; t += kWave) {
-      hkey[t] = kEmptyKey;
+  // 1. A dense slot per body; pairSlot holds table positions meanwhile.
+  for (int t = lane; t < L::kHash; t += kWave) {
+    hkey[t] = kEmptyKey;
+  }
+  waveSync();
+  uint32_t nb = 0;
+  auto insertPair = [&](int k, int4 q) __attribute__((always_inline)) {
+    uint32_t h0 = 0, h1 = 0;
+    bool w0 = false, w1 = false;
+    if (k < size) {
+      w0 = hashInsert(hkey, L::kHash, static_cast<uint32_t>(q.x), h0);
+      w1 = hashInsert(hkey, L::kHash, static_cast<uint32_t>(q.y), h1);
+      pairSlot[k] = h0 | (h1 << 16);
     }
-    __syncthreads();
-    uint32_t nb = 0;
+    const uint64_t b0 = __ballot(w0), b1 = __ballot(w1);
+    const uint32_t n0 = static_cast<uint32_t>(__popcll(b0));
+    if (w0) {
+      const uint32_t slot = nb + static_cast<uint32_t>(__popcll(b0 & lt));
+      hslot[h0] = static_cast<uint16_t>(slot);
+      if (slot < BMAX) {
+        bodyIdx[slot] = q.x;
+        minX[slot] = static_cast<uint32_t>(q.z);
+      }
+    }
+    if (w1) {
+      const uint32_t slot = nb + n0 + static_cast<uint32_t>(__popcll(b1 & lt));
+      hslot[h1] = static_cast<uint16_t>(slot);
+      if (slot < BMAX) {
+        bodyIdx[slot] = q.y;
+        minX[slot] = static_cast<uint32_t>(q.w);
+      }
+    }
+    nb += n0 + static_cast<uint32_t>(__popcll(b1));
+  };
+  if constexpr (W::kPf > 0) {
+#pragma unroll
+    for (int j = 0; j < W::kPf; ++j) {
+      insertPair(lane + j * kWave, pre.v[j]);
+    }
+  } else {
     for (int k0 = 0; k0 < size; k0 += kWave) {
       const int k = k0 + lane;
-      int4 q = make_int4(0, 0, 0, 0);
-      uint32_t h0 = 0, h1 = 0;
-      bool w0 = false, w1 = false;
-      if (k < size) {
-        q = seg[k];
-        w0 = hashInsert(hkey, L::kHash, static_cast<uint32_t>(q.x), h0);
-        w1 = hashInsert(hkey, L::kHash, static_cast<uint32_t>(q.y), h1);
-        pairSlot[k] = h0 | (h1 << 16);
-      }
-      const uint64_t b0 = __ballot(w0), b1 = __ballot(w1);
-      const uint32_t n0 = static_cast<uint32_t>(__popcll(b0));
-      if (w0) {
-        const uint32_t slot = nb + static_cast<uint32_t>(__popcll(b0 & lt));
-        hslot[h0] = static_cast<uint16_t>(slot);
-        if (slot < BMAX) {
-          bodyIdx[slot] = q.x;
-          minX[slot] = static_cast<uint32_t>(q.z);
-        }
-      }
-      if (w1) {
-        const uint32_t slot = nb + n0 + static_cast<uint32_t>(__popcll(b1 & lt));
-        hslot[h1] = static_cast<uint16_t>(slot);
-        if (slot < BMAX) {
-          bodyIdx[slot] = q.y;
-          minX[slot] = static_cast<uint32_t>(q.w);
-        }
-      }
-      nb += n0 + static_cast<uint32_t>(__popcll(b1));
+      insertPair(k, k < size ? seg[k] : make_int4(0, 0, 0, 0));
     }
-    __syncthreads();
-    if (nb > BMAX) {
-      if (lane == 0) {
-        if (tier + 1 < kWaveTiers) {
-          tierList[static_cast<size_t>(tier + 1) * listCap +
-                   atomicAdd(&ctl->waveCount[tier + 1], 1u)] = comp;
-        } else {
-          bigList[atomicAdd(&ctl->bigCount, 1u)] = comp;
-        }
-      }
-      continue;
-    }
-    for (int k = lane; k < size; k += kWave) {
-      const uint32_t h = pairSlot[k];
-      pairSlot[k] = static_cast<uint32_t>(hslot[h & 0xffffu]) |
-                    (static_cast<uint32_t>(hslot[h >> 16]) << 16);
-    }
-    __syncthreads(); // the table is dead from here on
-
-    // 2. Each body's list of pairs (counting sort by slot).
-    for (int t = lane; t <= static_cast<int>(nb); t += kWave) {
-      start[t] = 0;
-    }
-    for (int k = lane; k < size; k += kWave) {
-      indeg[k] = 0;
-    }
-    __syncthreads();
-    for (int k = lane; k < size; k += kWave) {
-      const uint32_t sl = pairSlot[k];
-      atomicAdd(&start[sl & 0xffffu], 1u);
-      atomicAdd(&start[sl >> 16], 1u);
-    }
-    __syncthreads();
-    uint32_t carry = 0, maxDeg = 0;
-    for (uint32_t s0 = 0; s0 < nb; s0 += kWave) {
-      const uint32_t t = s0 + static_cast<uint32_t>(lane);
-      const uint32_t own = t < nb ? start[t] : 0u;
-      maxDeg = max(maxDeg, own);
-      uint32_t v = own;
-#pragma unroll
-      for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t u = __shfl_up(v, off, kWave);
-        if (lane >= off) {
-          v += u;
-        }
-      }
-      if (t < nb) {
-        start[t] = carry + v - own;
-        cursor[t] = carry + v - own;
-      }
-      carry += __shfl(v, kWave - 1, kWave);
-    }
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-      maxDeg = max(maxDeg, static_cast<uint32_t>(__shfl_xor(maxDeg, off, kWave)));
-    }
+  }
+  waveSync();
+  if (nb > BMAX) {
     if (lane == 0) {
-      start[nb] = carry;
-    }
-    if (maxDeg > static_cast<uint32_t>(kMaxDegree)) {
-      if (lane == 0) {
-        bigList[atomicAdd(&ctl->bigCount, 1u)] = comp;
-      }
-      __syncthreads();
-      continue;
-    }
-    __syncthreads();
-    for (int k = lane; k < size; k += kWave) {
-      const uint32_t sl = pairSlot[k];
-      inc[atomicAdd(&cursor[sl & 0xffffu], 1u)] = static_cast<uint16_t>(k);
-      inc[atomicAdd(&cursor[sl >> 16], 1u)] = static_cast<uint16_t>(k);
-    }
-    __syncthreads();
-
-    // 3. Lists into pairLess order; links to the next pair on each body.
-    for (int t = lane; t < static_cast<int>(nb); t += kWave) {
-      const uint32_t b0 = start[t], d = start[t + 1] - b0;
-      if (d <= 4) {
-        linkBody<4>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
-      } else if (d <= 8) {
-        linkBody<8>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+      if (tier + 1 < kWaveTiers) {
+        tierList[static_cast<size_t>(tier + 1) * listCap +
+                 atomicAdd(&ctl->waveCount[tier + 1], 1u)] = info;
       } else {
-        linkBody<kMaxDegree>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+        bigList[atomicAdd(&ctl->bigCount, 1u)] = info.z;
       }
     }
-    __syncthreads(); // the lists are dead from here on
+    return;
+  }
+  // The bodies' state is fetched now and lands in LDS after the links.
+  float sx[W::kSf > 0 ? W::kSf : 1], sy[W::kSf > 0 ? W::kSf : 1];
+  float svx[W::kSf > 0 ? W::kSf : 1], svy[W::kSf > 0 ? W::kSf : 1];
+  float sm[W::kSf > 0 ? W::kSf : 1], sr[W::kSf > 0 ? W::kSf : 1];
+  if constexpr (W::kSf > 0) {
+#pragma unroll
+    for (int j = 0; j < W::kSf; ++j) {
+      const int t = lane + j * kWave;
+      const int i = t < static_cast<int>(nb) ? bodyIdx[t] : bodyIdx[0];
+      sx[j] = px[i];
+      sy[j] = py[i];
+      svx[j] = vx[i];
+      svy[j] = vy[i];
+      sm[j] = m[i];
+      sr[j] = r[i];
+    }
+  }
+  for (int k = lane; k < size; k += kWave) {
+    const uint32_t h = pairSlot[k];
+    pairSlot[k] = static_cast<uint32_t>(hslot[h & 0xffffu]) |
+                  (static_cast<uint32_t>(hslot[h >> 16]) << 16);
+  }
+  waveSync(); // the table is dead from here on
 
-    // 4. Bodies into LDS; the first round: pairs with no earlier pair.
+  // 2. Each body's list of pairs (counting sort by slot).
+  for (int t = lane; t <= static_cast<int>(nb); t += kWave) {
+    start[t] = 0;
+  }
+  for (int k = lane; k < size; k += kWave) {
+    indeg[k] = 0;
+  }
+  waveSync();
+  for (int k = lane; k < size; k += kWave) {
+    const uint32_t sl = pairSlot[k];
+    atomicAdd(&start[sl & 0xffffu], 1u);
+    atomicAdd(&start[sl >> 16], 1u);
+  }
+  waveSync();
+  uint32_t carry = 0, maxDeg = 0;
+  for (uint32_t s0 = 0; s0 < nb; s0 += kWave) {
+    const uint32_t t = s0 + static_cast<uint32_t>(lane);
+    const uint32_t own = t < nb ? start[t] : 0u;
+    maxDeg = max(maxDeg, own);
+    uint32_t v = own;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t u = __shfl_up(v, off, kWave);
+      if (lane >= off) {
+        v += u;
+      }
+    }
+    if (t < nb) {
+      start[t] = carry + v - own;
+      cursor[t] = carry + v - own;
+    }
+    carry += __shfl(v, kWave - 1, kWave);
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    maxDeg = max(maxDeg, static_cast<uint32_t>(__shfl_xor(maxDeg, off, kWave)));
+  }
+  if (lane == 0) {
+    start[nb] = carry;
+  }
+  if (maxDeg > static_cast<uint32_t>(kMaxDegree)) {
+    if (lane == 0) {
+      bigList[atomicAdd(&ctl->bigCount, 1u)] = info.z;
+    }
+    waveSync();
+    return;
+  }
+  waveSync();
+  for (int k = lane; k < size; k += kWave) {
+    const uint32_t sl = pairSlot[k];
+    inc[atomicAdd(&cursor[sl & 0xffffu], 1u)] = static_cast<uint16_t>(k);
+    inc[atomicAdd(&cursor[sl >> 16], 1u)] = static_cast<uint16_t>(k);
+  }
+  waveSync();
+
+  // 3. Lists into pairLess order; links to the next pair on each body.
+  for (int t = lane; t < static_cast<int>(nb); t += kWave) {
+    const uint32_t b0 = start[t], d = start[t + 1] - b0;
+    if (d <= 4) {
+      linkBody<4>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+    } else if (d <= 8) {
+      linkBody<8>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+    } else {
+      linkBody<kMaxDegree>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+    }
+  }
+  waveSync(); // the lists are dead from here on
+
+  // 4. Bodies into LDS; the first round: pairs with no earlier pair.
+  if constexpr (W::kSf > 0) {
+#pragma unroll
+    for (int j = 0; j < W::kSf; ++j) {
+      const int t = lane + j * kWave;
+      if (t < static_cast<int>(nb)) {
+        lx[t] = sx[j];
+        ly[t] = sy[j];
+        lvx[t] = svx[j];
+        lvy[t] = svy[j];
+        lm[t] = sm[j];
+        lr[t] = sr[j];
+      }
+    }
+  } else {
     for (int t = lane; t < static_cast<int>(nb); t += kWave) {
       const int i = bodyIdx[t];
       lx[t] = px[i];
@@ -1202,64 +1294,120 @@ __global__ __launch_bounds__(kWave) void k_resolve_wave(
       lm[t] = m[i];
       lr[t] = r[i];
     }
-    uint16_t *front = inc;
-    uint32_t cnt = 0;
-    for (int k0 = 0; k0 < size; k0 += kWave) {
-      const int k = k0 + lane;
-      const bool ready = k < size && indeg[k] == 0u;
-      const uint64_t bm = __ballot(ready);
-      if (ready) {
-        front[cnt + static_cast<uint32_t>(__popcll(bm & lt))] = static_cast<uint16_t>(k);
-      }
-      cnt += static_cast<uint32_t>(__popcll(bm));
+  }
+  uint16_t *front = inc;
+  uint32_t cnt = 0;
+  for (int k0 = 0; k0 < size; k0 += kWave) {
+    const int k = k0 + lane;
+    const bool ready = k < size && indeg[k] == 0u;
+    const uint64_t bm = __ballot(ready);
+    if (ready) {
+      front[cnt + static_cast<uint32_t>(__popcll(bm & lt))] = static_cast<uint16_t>(k);
     }
-    __syncthreads();
+    cnt += static_cast<uint32_t>(__popcll(bm));
+  }
+  waveSync();
 
-    // 5. Rounds. Pairs are written back in the order they are replayed.
-    uint32_t done = 0;
-    int cur = 0;
-    while (cnt > 0) {
-      const uint16_t *f = front + cur * PMAX;
-      uint16_t *nf = front + (cur ^ 1) * PMAX;
-      if (lane == 0) {
-        frontierCount[cur ^ 1] = 0;
+  // 5. Rounds. Pairs are written back in the order they are replayed.
+  uint32_t done = 0;
+  int cur = 0;
+  while (cnt > 0) {
+    const uint16_t *f = front + cur * PMAX;
+    uint16_t *nf = front + (cur ^ 1) * PMAX;
+    if (lane == 0) {
+      frontierCount[cur ^ 1] = 0;
+    }
+    waveSync();
+    for (uint32_t idx = static_cast<uint32_t>(lane); idx < cnt; idx += kWave) {
+      const uint32_t k = f[idx];
+      const uint32_t sl = pairSlot[k];
+      const int a = static_cast<int>(sl & 0xffffu), b = static_cast<int>(sl >> 16);
+      resolvePair(lx, ly, lvx, lvy, lm, lr, a, b);
+      if (done + idx < static_cast<uint32_t>(size)) {
+        seg[done + idx] = make_int4(bodyIdx[a], bodyIdx[b], static_cast<int>(minX[a]),
+                                    static_cast<int>(minX[b]));
       }
-      __syncthreads();
-      for (uint32_t idx = static_cast<uint32_t>(lane); idx < cnt; idx += kWave) {
-        const uint32_t k = f[idx];
-        const uint32_t sl = pairSlot[k];
-        const int a = static_cast<int>(sl & 0xffffu), b = static_cast<int>(sl >> 16);
-        resolvePair(lx, ly, lvx, lvy, lm, lr, a, b);
-        if (done + idx < static_cast<uint32_t>(size)) {
-          seg[done + idx] = make_int4(bodyIdx[a], bodyIdx[b], static_cast<int>(minX[a]),
-                                      static_cast<int>(minX[b]));
-        }
 #pragma unroll
-        for (int side = 0; side < 2; ++side) {
-          const uint32_t nx = succ[2 * k + side];
-          if (nx != kNoPair && atomicSub(&indeg[nx], 1u) == 1u) {
-            const uint32_t pos = atomicAdd(&frontierCount[cur ^ 1], 1u);
-            if (pos < PMAX) {
-              nf[pos] = static_cast<uint16_t>(nx);
-            }
+      for (int side = 0; side < 2; ++side) {
+        const uint32_t nx = succ[2 * k + side];
+        if (nx != kNoPair && atomicSub(&indeg[nx], 1u) == 1u) {
+          const uint32_t pos = atomicAdd(&frontierCount[cur ^ 1], 1u);
+          if (pos < PMAX) {
+            nf[pos] = static_cast<uint16_t>(nx);
           }
         }
       }
-      done += cnt;
-      __syncthreads();
-      cnt = min(frontierCount[cur ^ 1], static_cast<uint32_t>(PMAX));
-      cur ^= 1;
     }
+    done += cnt;
+    waveSync();
+    cnt = min(frontierCount[cur ^ 1], static_cast<uint32_t>(PMAX));
+    cur ^= 1;
+  }
 
-    // 6. Bodies back.
-    for (int t = lane; t < static_cast<int>(nb); t += kWave) {
-      const int i = bodyIdx[t];
-      px[i] = lx[t];
-      py[i] = ly[t];
-      vx[i] = lvx[t];
-      vy[i] = lvy[t];
+  // 6. Bodies back.
+  for (int t = lane; t < static_cast<int>(nb); t += kWave) {
+    const int i = bodyIdx[t];
+    px[i] = lx[t];
+    py[i] = ly[t];
+    vx[i] = lvx[t];
+    vy[i] = lvy[t];
+  }
+  waveSync();
+}
+
+// Components are taken in a fixed stride (no shared counter); the next
+// component's queue entry, and for the smallest tier its pairs, are loaded
+// while the current one is resolved.
+template <int PMAX, int BMAX>
+__global__ __launch_bounds__(kWave) void k_resolve_wave(
+    Ctl *__restrict__ ctl, int tier, int4 *__restrict__ tierList,
+    uint32_t listCap, int32_t *__restrict__ bigList,
+    int4 *__restrict__ segPairs, float *__restrict__ px,
+    float *__restrict__ py, float *__restrict__ vx, float *__restrict__ vy,
+    const float *__restrict__ m, const float *__restrict__ r) {
+  using L = WaveLds<PMAX, BMAX>;
+  using W = WaveCtx<PMAX, BMAX>;
+  if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
+    return;
+  }
+  const uint32_t count = ctl->waveCount[tier];
+  const uint32_t stride = gridDim.x;
+  if (blockIdx.x >= count) {
+    return;
+  }
+  __shared__ __align__(16) uint32_t lds[L::kWords];
+  __shared__ uint32_t frontierCount[2];
+  const int lane = static_cast<int>(threadIdx.x);
+  const int4 *list = tierList + static_cast<size_t>(tier) * listCap;
+  constexpr int kP = W::kPf > 0 ? W::kPf : 1;
+  const int4 none = make_int4(0, 0, -1, 0);
+  // Loads from a clamped index and selects values (a select of pointers
+  // would turn the load into a flat one through a private copy of `none`).
+  auto entry = [&](uint32_t g) {
+    const int4 e = list[min(g, count - 1)];
+    return g < count ? e : none;
+  };
+  int4 info0 = list[blockIdx.x];
+  int4 info1 = entry(blockIdx.x + stride);
+  typename W::Pairs q0, q1;
+#pragma unroll
+  for (int j = 0; j < kP; ++j) {
+    const int k = lane + j * kWave;
+    q0.v[j] = (W::kPf > 0 && k < info0.y) ? segPairs[info0.x + k] : make_int4(0, 0, 0, 0);
+  }
+  for (uint32_t g = blockIdx.x; g < count; g += stride) {
+    const int4 info2 = entry(g + 2 * stride);
+#pragma unroll
+    for (int j = 0; j < kP; ++j) {
+      const int k = lane + j * kWave;
+      q1.v[j] = (W::kPf > 0 && k < info1.y) ? segPairs[info1.x + k] : make_int4(0, 0, 0, 0);
     }
-    __syncthreads();
+    resolveWaveComponent<PMAX, BMAX>(lds, frontierCount, ctl, tier, info0, q0,
+                                     tierList, listCap, bigList, segPairs, px,
+                                     py, vx, vy, m, r);
+    info0 = info1;
+    info1 = info2;
+    q0 = q1;
   }
 }
 

@@ -102,7 +102,8 @@ struct nbx_sim {
   uint32_t *compCount = nullptr, *compStart = nullptr;
   int4 *compInfo = nullptr;
   int32_t *bigList = nullptr;
-  int32_t *tierList = nullptr; // kWaveTiers queues of n components each
+  int4 *tierList = nullptr; // kWaveTiers queues of tierCap {start, size, component}
+  uint32_t tierCap = 0;
   int32_t *bodySlot = nullptr; // per body: slot in a big component's replay, -1 at rest
 
   // tree
@@ -428,20 +429,19 @@ int launchCollide(nbx_sim *s) {
   LAUNCH(k_comp_scatter, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
          s->pairComp, s->compCount, s->compStart, s->segPairs);
   LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->segPairs,
-         s->tierList, n, s->bigList, px, py, vx, vy, m, r);
+         s->tierList, s->tierCap, s->bigList, px, py, vx, vy, m, r);
   // Components of 17..4096 pairs, one wave each, in tiers by size; the
   // grids are the LDS residency of each tier on 256 CUs (a workgroup
   // beyond the queued count returns at once).
   static_assert(waveTierPairs(0) == 256 && waveTierPairs(1) == 1024 &&
                     waveTierPairs(2) == 4096 && kWaveTiers == 3,
                 "tier launches below");
-  LAUNCH((k_resolve_wave<256, 256>), 256 * 12, kWave, s->ctl, 0, s->compInfo,
-         s->tierList, n, s->bigList, s->segPairs, px, py, vx, vy, m, r);
-  LAUNCH((k_resolve_wave<1024, 640>), 256 * 4, kWave, s->ctl, 1,
-         s->compInfo, s->tierList, n, s->bigList, s->segPairs, px, py, vx, vy,
-         m, r);
-  LAUNCH((k_resolve_wave<4096, 2560>), 256, kWave, s->ctl, 2, s->compInfo,
-         s->tierList, n, s->bigList, s->segPairs, px, py, vx, vy, m, r);
+  LAUNCH((k_resolve_wave<256, 256>), 256 * 12, kWave, s->ctl, 0, s->tierList,
+         s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
+  LAUNCH((k_resolve_wave<1024, 640>), 256 * 4, kWave, s->ctl, 1, s->tierList,
+         s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
+  LAUNCH((k_resolve_wave<4096, 2560>), 256, kWave, s->ctl, 2, s->tierList,
+         s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
   LAUNCH(k_resolve_big, 256, kBlock, s->ctl, s->compInfo, s->bigList,
          s->segPairs, s->pairs, s->bodySlot, px, py, vx, vy, m, r);
   s->collided = true;
@@ -845,7 +845,10 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->pairs, s->pairCap, false) A(s->segPairs, s->pairCap, false)
   A(s->pairComp, s->pairCap, false)
   A(s->compCount, n, true) A(s->compStart, n, false) A(s->compInfo, n, false)
-  A(s->bigList, n, false) A(s->tierList, kWaveTiers * n, false)
+  A(s->bigList, n, false)
+  // A queued component has more than kSmallComp pairs.
+  s->tierCap = s->pairCap / (kSmallComp + 1) + 1;
+  A(s->tierList, static_cast<uint64_t>(kWaveTiers) * s->tierCap, false)
   if ((rc = devAlloc(s, &s->bodySlot, n, false)) != NBX_OK ||
       hipMemset(s->bodySlot, 0xff, n * sizeof(int32_t)) != hipSuccess) {
     return fail(rc != NBX_OK ? rc : setError(NBX_ERR_HIP, "hipMemset failed"));

@@ -1083,26 +1083,26 @@ __device__ inline void waveSync() {
 }
 
 // Queue entry of a component: {first pair, pair count, component index, 0}.
-template <int PMAX, int BMAX> struct WaveCtx {
-  static constexpr int kPf = PMAX <= 4 * kWave ? PMAX / kWave : 0; // pairs/lane in registers
-  static constexpr int kSf = BMAX <= 4 * kWave ? BMAX / kWave : 0; // bodies/lane in registers
+template <int PMAX, int BMAX, int G> struct WaveCtx {
+  static_assert(G == 16 || G == 32 || G == 64, "lanes per component");
+  static constexpr int kPf = PMAX <= 4 * G && PMAX % G == 0 ? PMAX / G : 0; // pairs/lane in registers
+  static constexpr int kSf = BMAX <= 4 * G && BMAX % G == 0 ? BMAX / G : 0; // bodies/lane in registers
   struct Pairs {
     int4 v[kPf > 0 ? kPf : 1];
   };
 };
 
-template <int PMAX, int BMAX>
+template <int PMAX, int BMAX, int G>
 __device__ __forceinline__ void resolveWaveComponent(
-    uint32_t *__restrict__ lds, uint32_t *__restrict__ frontierCount,
-    Ctl *__restrict__ ctl, int tier, int4 info,
-    typename WaveCtx<PMAX, BMAX>::Pairs pre,
+    uint32_t *__restrict__ lds, Ctl *__restrict__ ctl, int tier, int4 info,
+    typename WaveCtx<PMAX, BMAX, G>::Pairs pre,
     int4 *__restrict__ tierList, uint32_t listCap,
     int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
     float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
     float *__restrict__ vy, const float *__restrict__ m,
     const float *__restrict__ r) {
   using L = WaveLds<PMAX, BMAX>;
-  using W = WaveCtx<PMAX, BMAX>;
+  using W = WaveCtx<PMAX, BMAX, G>;
   uint32_t *pairSlot = lds + L::kPairSlot;
   uint16_t *succ = reinterpret_cast<uint16_t *>(lds + L::kSucc);
   uint32_t *indeg = lds + L::kIndeg;
@@ -1116,13 +1116,17 @@ __device__ __forceinline__ void resolveWaveComponent(
   float *lx = reinterpret_cast<float *>(lds + L::kU);
   float *ly = lx + BMAX, *lvx = ly + BMAX, *lvy = lvx + BMAX;
   float *lm = lvy + BMAX, *lr = lm + BMAX;
-  const int lane = static_cast<int>(threadIdx.x);
-  const uint64_t lt = laneMaskLt();
+  // This component's G lanes of the wave.
+  const int lane = static_cast<int>(threadIdx.x) % G;
+  const uint64_t gmask =
+      G == kWave ? ~0ull : (((1ull << (G % kWave)) - 1ull) << (threadIdx.x - lane));
+  const uint64_t lt = laneMaskLt() & gmask;
+  auto gballot = [&](bool p) { return __ballot(p) & gmask; };
   const int size = info.y; // kSmallComp < size <= PMAX
   int4 *seg = segPairs + info.x;
 
   // 1. A dense slot per body; pairSlot holds table positions meanwhile.
-  for (int t = lane; t < L::kHash; t += kWave) {
+  for (int t = lane; t < L::kHash; t += G) {
     hkey[t] = kEmptyKey;
   }
   waveSync();
@@ -1135,7 +1139,7 @@ __device__ __forceinline__ void resolveWaveComponent(
       w1 = hashInsert(hkey, L::kHash, static_cast<uint32_t>(q.y), h1);
       pairSlot[k] = h0 | (h1 << 16);
     }
-    const uint64_t b0 = __ballot(w0), b1 = __ballot(w1);
+    const uint64_t b0 = gballot(w0), b1 = gballot(w1);
     const uint32_t n0 = static_cast<uint32_t>(__popcll(b0));
     if (w0) {
       const uint32_t slot = nb + static_cast<uint32_t>(__popcll(b0 & lt));
@@ -1158,10 +1162,10 @@ __device__ __forceinline__ void resolveWaveComponent(
   if constexpr (W::kPf > 0) {
 #pragma unroll
     for (int j = 0; j < W::kPf; ++j) {
-      insertPair(lane + j * kWave, pre.v[j]);
+      insertPair(lane + j * G, pre.v[j]);
     }
   } else {
-    for (int k0 = 0; k0 < size; k0 += kWave) {
+    for (int k0 = 0; k0 < size; k0 += G) {
       const int k = k0 + lane;
       insertPair(k, k < size ? seg[k] : make_int4(0, 0, 0, 0));
     }
@@ -1185,7 +1189,7 @@ __device__ __forceinline__ void resolveWaveComponent(
   if constexpr (W::kSf > 0) {
 #pragma unroll
     for (int j = 0; j < W::kSf; ++j) {
-      const int t = lane + j * kWave;
+      const int t = lane + j * G;
       const int i = t < static_cast<int>(nb) ? bodyIdx[t] : bodyIdx[0];
       sx[j] = px[i];
       sy[j] = py[i];
@@ -1195,7 +1199,7 @@ __device__ __forceinline__ void resolveWaveComponent(
       sr[j] = r[i];
     }
   }
-  for (int k = lane; k < size; k += kWave) {
+  for (int k = lane; k < size; k += G) {
     const uint32_t h = pairSlot[k];
     pairSlot[k] = static_cast<uint32_t>(hslot[h & 0xffffu]) |
                   (static_cast<uint32_t>(hslot[h >> 16]) << 16);
@@ -1203,28 +1207,28 @@ __device__ __forceinline__ void resolveWaveComponent(
   waveSync(); // the table is dead from here on
 
   // 2. Each body's list of pairs (counting sort by slot).
-  for (int t = lane; t <= static_cast<int>(nb); t += kWave) {
+  for (int t = lane; t <= static_cast<int>(nb); t += G) {
     start[t] = 0;
   }
-  for (int k = lane; k < size; k += kWave) {
+  for (int k = lane; k < size; k += G) {
     indeg[k] = 0;
   }
   waveSync();
-  for (int k = lane; k < size; k += kWave) {
+  for (int k = lane; k < size; k += G) {
     const uint32_t sl = pairSlot[k];
     atomicAdd(&start[sl & 0xffffu], 1u);
     atomicAdd(&start[sl >> 16], 1u);
   }
   waveSync();
   uint32_t carry = 0, maxDeg = 0;
-  for (uint32_t s0 = 0; s0 < nb; s0 += kWave) {
+  for (uint32_t s0 = 0; s0 < nb; s0 += G) {
     const uint32_t t = s0 + static_cast<uint32_t>(lane);
     const uint32_t own = t < nb ? start[t] : 0u;
     maxDeg = max(maxDeg, own);
     uint32_t v = own;
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const uint32_t u = __shfl_up(v, off, kWave);
+    for (int off = 1; off < G; off <<= 1) {
+      const uint32_t u = __shfl_up(v, off, G);
       if (lane >= off) {
         v += u;
       }
@@ -1233,11 +1237,11 @@ __device__ __forceinline__ void resolveWaveComponent(
       start[t] = carry + v - own;
       cursor[t] = carry + v - own;
     }
-    carry += __shfl(v, kWave - 1, kWave);
+    carry += __shfl(v, G - 1, G);
   }
 #pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    maxDeg = max(maxDeg, static_cast<uint32_t>(__shfl_xor(maxDeg, off, kWave)));
+  for (int off = G / 2; off > 0; off >>= 1) {
+    maxDeg = max(maxDeg, static_cast<uint32_t>(__shfl_xor(maxDeg, off, G)));
   }
   if (lane == 0) {
     start[nb] = carry;
@@ -1250,7 +1254,7 @@ __device__ __forceinline__ void resolveWaveComponent(
     return;
   }
   waveSync();
-  for (int k = lane; k < size; k += kWave) {
+  for (int k = lane; k < size; k += G) {
     const uint32_t sl = pairSlot[k];
     inc[atomicAdd(&cursor[sl & 0xffffu], 1u)] = static_cast<uint16_t>(k);
     inc[atomicAdd(&cursor[sl >> 16], 1u)] = static_cast<uint16_t>(k);
@@ -1258,7 +1262,7 @@ __device__ __forceinline__ void resolveWaveComponent(
   waveSync();
 
   // 3. Lists into pairLess order; links to the next pair on each body.
-  for (int t = lane; t < static_cast<int>(nb); t += kWave) {
+  for (int t = lane; t < static_cast<int>(nb); t += G) {
     const uint32_t b0 = start[t], d = start[t + 1] - b0;
     if (d <= 4) {
       linkBody<4>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
@@ -1274,7 +1278,7 @@ __device__ __forceinline__ void resolveWaveComponent(
   if constexpr (W::kSf > 0) {
 #pragma unroll
     for (int j = 0; j < W::kSf; ++j) {
-      const int t = lane + j * kWave;
+      const int t = lane + j * G;
       if (t < static_cast<int>(nb)) {
         lx[t] = sx[j];
         ly[t] = sy[j];
@@ -1285,7 +1289,7 @@ __device__ __forceinline__ void resolveWaveComponent(
       }
     }
   } else {
-    for (int t = lane; t < static_cast<int>(nb); t += kWave) {
+    for (int t = lane; t < static_cast<int>(nb); t += G) {
       const int i = bodyIdx[t];
       lx[t] = px[i];
       ly[t] = py[i];
@@ -1297,10 +1301,10 @@ __device__ __forceinline__ void resolveWaveComponent(
   }
   uint16_t *front = inc;
   uint32_t cnt = 0;
-  for (int k0 = 0; k0 < size; k0 += kWave) {
+  for (int k0 = 0; k0 < size; k0 += G) {
     const int k = k0 + lane;
     const bool ready = k < size && indeg[k] == 0u;
-    const uint64_t bm = __ballot(ready);
+    const uint64_t bm = gballot(ready);
     if (ready) {
       front[cnt + static_cast<uint32_t>(__popcll(bm & lt))] = static_cast<uint16_t>(k);
     }
@@ -1308,44 +1312,62 @@ __device__ __forceinline__ void resolveWaveComponent(
   }
   waveSync();
 
-  // 5. Rounds. Pairs are written back in the order they are replayed.
+  // 5. Rounds. Pairs are written back in the order they are replayed. A
+  //    pair's successors are released before its resolve (they run in the
+  //    next round, after it); the next frontier is placed by ballots.
   uint32_t done = 0;
   int cur = 0;
   while (cnt > 0) {
     const uint16_t *f = front + cur * PMAX;
     uint16_t *nf = front + (cur ^ 1) * PMAX;
-    if (lane == 0) {
-      frontierCount[cur ^ 1] = 0;
-    }
-    waveSync();
-    for (uint32_t idx = static_cast<uint32_t>(lane); idx < cnt; idx += kWave) {
-      const uint32_t k = f[idx];
-      const uint32_t sl = pairSlot[k];
-      const int a = static_cast<int>(sl & 0xffffu), b = static_cast<int>(sl >> 16);
-      resolvePair(lx, ly, lvx, lvy, lm, lr, a, b);
-      if (done + idx < static_cast<uint32_t>(size)) {
-        seg[done + idx] = make_int4(bodyIdx[a], bodyIdx[b], static_cast<int>(minX[a]),
-                                    static_cast<int>(minX[b]));
-      }
-#pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        const uint32_t nx = succ[2 * k + side];
-        if (nx != kNoPair && atomicSub(&indeg[nx], 1u) == 1u) {
-          const uint32_t pos = atomicAdd(&frontierCount[cur ^ 1], 1u);
-          if (pos < PMAX) {
-            nf[pos] = static_cast<uint16_t>(nx);
-          }
+    uint32_t next = 0;
+    for (uint32_t base = 0; base < cnt; base += G) {
+      const uint32_t idx = base + static_cast<uint32_t>(lane);
+      const bool act = idx < cnt;
+      uint32_t nx0 = kNoPair, nx1 = kNoPair, r0 = 0, r1 = 0;
+      if (act) {
+        const uint32_t k = f[idx];
+        const uint32_t sl = pairSlot[k];
+        nx0 = succ[2 * k];
+        nx1 = succ[2 * k + 1];
+        if (nx0 != kNoPair) {
+          r0 = atomicSub(&indeg[nx0], 1u);
+        }
+        if (nx1 != kNoPair) {
+          r1 = atomicSub(&indeg[nx1], 1u);
+        }
+        const int a = static_cast<int>(sl & 0xffffu), b = static_cast<int>(sl >> 16);
+        resolvePair(lx, ly, lvx, lvy, lm, lr, a, b);
+        if (done + idx < static_cast<uint32_t>(size)) {
+          seg[done + idx] = make_int4(bodyIdx[a], bodyIdx[b], static_cast<int>(minX[a]),
+                                      static_cast<int>(minX[b]));
         }
       }
+      const bool go0 = r0 == 1u, go1 = r1 == 1u;
+      const uint64_t m0 = gballot(go0), m1 = gballot(go1);
+      const uint32_t c0 = static_cast<uint32_t>(__popcll(m0));
+      if (go0) {
+        const uint32_t pos = next + static_cast<uint32_t>(__popcll(m0 & lt));
+        if (pos < PMAX) {
+          nf[pos] = static_cast<uint16_t>(nx0);
+        }
+      }
+      if (go1) {
+        const uint32_t pos = next + c0 + static_cast<uint32_t>(__popcll(m1 & lt));
+        if (pos < PMAX) {
+          nf[pos] = static_cast<uint16_t>(nx1);
+        }
+      }
+      next += c0 + static_cast<uint32_t>(__popcll(m1));
     }
     done += cnt;
-    waveSync();
-    cnt = min(frontierCount[cur ^ 1], static_cast<uint32_t>(PMAX));
+    cnt = min(next, static_cast<uint32_t>(PMAX));
     cur ^= 1;
+    waveSync();
   }
 
   // 6. Bodies back.
-  for (int t = lane; t < static_cast<int>(nb); t += kWave) {
+  for (int t = lane; t < static_cast<int>(nb); t += G) {
     const int i = bodyIdx[t];
     px[i] = lx[t];
     py[i] = ly[t];
@@ -1355,10 +1377,11 @@ __device__ __forceinline__ void resolveWaveComponent(
   waveSync();
 }
 
-// Components are taken in a fixed stride (no shared counter); the next
-// component's queue entry, and for the smallest tier its pairs, are loaded
-// while the current one is resolved.
-template <int PMAX, int BMAX>
+// Components are taken in a fixed stride (no shared counter), G lanes each
+// (a wave resolves 64 / G components side by side); the next component's
+// queue entry, and for the small tiers its pairs, are loaded while the
+// current one is resolved.
+template <int PMAX, int BMAX, int G>
 __global__ __launch_bounds__(kWave) void k_resolve_wave(
     Ctl *__restrict__ ctl, int tier, int4 *__restrict__ tierList,
     uint32_t listCap, int32_t *__restrict__ bigList,
@@ -1366,18 +1389,20 @@ __global__ __launch_bounds__(kWave) void k_resolve_wave(
     float *__restrict__ py, float *__restrict__ vx, float *__restrict__ vy,
     const float *__restrict__ m, const float *__restrict__ r) {
   using L = WaveLds<PMAX, BMAX>;
-  using W = WaveCtx<PMAX, BMAX>;
+  using W = WaveCtx<PMAX, BMAX, G>;
+  constexpr int kGroups = kWave / G;
   if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
     return;
   }
   const uint32_t count = ctl->waveCount[tier];
-  const uint32_t stride = gridDim.x;
-  if (blockIdx.x >= count) {
+  if (blockIdx.x * kGroups >= count) {
     return;
   }
-  __shared__ __align__(16) uint32_t lds[L::kWords];
-  __shared__ uint32_t frontierCount[2];
-  const int lane = static_cast<int>(threadIdx.x);
+  __shared__ __align__(16) uint32_t lds[L::kWords * kGroups];
+  const int group = static_cast<int>(threadIdx.x) / G;
+  const int lane = static_cast<int>(threadIdx.x) % G;
+  const uint32_t stride = gridDim.x * kGroups;
+  const uint32_t first = blockIdx.x * kGroups + static_cast<uint32_t>(group);
   const int4 *list = tierList + static_cast<size_t>(tier) * listCap;
   constexpr int kP = W::kPf > 0 ? W::kPf : 1;
   const int4 none = make_int4(0, 0, -1, 0);
@@ -1387,24 +1412,24 @@ __global__ __launch_bounds__(kWave) void k_resolve_wave(
     const int4 e = list[min(g, count - 1)];
     return g < count ? e : none;
   };
-  int4 info0 = list[blockIdx.x];
-  int4 info1 = entry(blockIdx.x + stride);
+  int4 info0 = entry(first);
+  int4 info1 = entry(first + stride);
   typename W::Pairs q0, q1;
 #pragma unroll
   for (int j = 0; j < kP; ++j) {
-    const int k = lane + j * kWave;
+    const int k = lane + j * G;
     q0.v[j] = (W::kPf > 0 && k < info0.y) ? segPairs[info0.x + k] : make_int4(0, 0, 0, 0);
   }
-  for (uint32_t g = blockIdx.x; g < count; g += stride) {
+  for (uint32_t g = first; g < count; g += stride) {
     const int4 info2 = entry(g + 2 * stride);
 #pragma unroll
     for (int j = 0; j < kP; ++j) {
-      const int k = lane + j * kWave;
+      const int k = lane + j * G;
       q1.v[j] = (W::kPf > 0 && k < info1.y) ? segPairs[info1.x + k] : make_int4(0, 0, 0, 0);
     }
-    resolveWaveComponent<PMAX, BMAX>(lds, frontierCount, ctl, tier, info0, q0,
-                                     tierList, listCap, bigList, segPairs, px,
-                                     py, vx, vy, m, r);
+    resolveWaveComponent<PMAX, BMAX, G>(lds + group * L::kWords, ctl, tier,
+                                        info0, q0, tierList, listCap, bigList,
+                                        segPairs, px, py, vx, vy, m, r);
     info0 = info1;
     info1 = info2;
     q0 = q1;

@@ -22,9 +22,9 @@ constexpr int kSmallComp = 16;     // contact components replayed by one lane
 constexpr int kBigCompMax = 4096;  // pairs per component sorted in LDS at once
 // Contact components of kSmallComp+1 .. 4096 pairs are replayed by one wave
 // each, in tiers by pair count (k_resolve_wave).
-constexpr int kWaveTiers = 3;
+constexpr int kWaveTiers = 4;
 __host__ __device__ constexpr int waveTierPairs(int tier) {
-  return tier == 0 ? 256 : tier == 1 ? 1024 : 4096;
+  return tier == 0 ? 64 : tier == 1 ? 256 : tier == 2 ? 1024 : 4096;
 }
 constexpr uint32_t kDone = 0x7fffffffu;
 

@@ -430,18 +430,21 @@ int launchCollide(nbx_sim *s) {
          s->pairComp, s->compCount, s->compStart, s->segPairs);
   LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->segPairs,
          s->tierList, s->tierCap, s->bigList, px, py, vx, vy, m, r);
-  // Components of 17..4096 pairs, one wave each, in tiers by size; the
+  // Components of 17..4096 pairs, in tiers by size (4, 2, 1, 1 a wave); the
   // grids are the LDS residency of each tier on 256 CUs (a workgroup
   // beyond the queued count returns at once).
-  static_assert(waveTierPairs(0) == 256 && waveTierPairs(1) == 1024 &&
-                    waveTierPairs(2) == 4096 && kWaveTiers == 3,
+  static_assert(waveTierPairs(0) == 64 && waveTierPairs(1) == 256 &&
+                    waveTierPairs(2) == 1024 && waveTierPairs(3) == 4096 &&
+                    kWaveTiers == 4,
                 "tier launches below");
-  LAUNCH((k_resolve_wave<256, 256>), 256 * 12, kWave, s->ctl, 0, s->tierList,
-         s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
-  LAUNCH((k_resolve_wave<1024, 640>), 256 * 4, kWave, s->ctl, 1, s->tierList,
-         s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
-  LAUNCH((k_resolve_wave<4096, 2560>), 256, kWave, s->ctl, 2, s->tierList,
-         s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
+  LAUNCH((k_resolve_wave<64, 64, 16>), 256 * 12, kWave, s->ctl, 0,
+         s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
+  LAUNCH((k_resolve_wave<256, 192, 32>), 256 * 8, kWave, s->ctl, 1,
+         s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
+  LAUNCH((k_resolve_wave<1024, 640, 64>), 256 * 4, kWave, s->ctl, 2,
+         s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
+  LAUNCH((k_resolve_wave<4096, 2560, 64>), 256, kWave, s->ctl, 3,
+         s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
   LAUNCH(k_resolve_big, 256, kBlock, s->ctl, s->compInfo, s->bigList,
          s->segPairs, s->pairs, s->bodySlot, px, py, vx, vy, m, r);
   s->collided = true;

@@ -439,12 +439,12 @@ def test_large_contact_component(nbx, oracle, side):
 
 
 def test_contact_component_tiers(nbx, oracle):
-    """Packed square clumps of side 3..25 (12 to ~1200 pairs: the lane path
-    and each one-wave tier by pair count) and straight chains of 257, 700 and
-    3000 bodies (256, 699, 2999 pairs, one long dependency chain each, with
-    more bodies than the tier their pair count selects holds, so they are
-    passed on to the next tier or the workgroup path), all in one collide,
-    bit-exact against the sequential replay."""
+    """Packed square clumps of side 3..25 (20 to ~2300 pairs: every tier of
+    the wave path by pair count) and straight chains of 65, 257, 700 and
+    3000 bodies (64, 256, 699, 2999 pairs, one long dependency chain each,
+    with more bodies than the tier their pair count selects holds, so they
+    are passed on to the next tier or the workgroup path), all in one
+    collide, bit-exact against the sequential replay."""
     from oracle_lib import Bodies
     rng = np.random.default_rng(11)
     xs, ys = [], []
@@ -452,7 +452,7 @@ def test_contact_component_tiers(nbx, oracle):
         g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)
         xs.append(g[:, 0] * 1.9 + rng.uniform(-0.08, 0.08, g.shape[0]) + 200.0 * k)
         ys.append(g[:, 1] * 1.9 + rng.uniform(-0.08, 0.08, g.shape[0]))
-    for k, length in enumerate((257, 700, 3000)):
+    for k, length in enumerate((65, 257, 700, 3000)):
         t = np.arange(length)
         xs.append(t * 1.9 + rng.uniform(-0.03, 0.03, length))
         ys.append(rng.uniform(-0.3, 0.3, length) + 300.0 + 100.0 * k)
@@ -471,8 +471,8 @@ def test_contact_component_tiers(nbx, oracle):
     b.id[:] = np.arange(n, dtype=np.uint32)
     info, comp = check_collide(nbx, oracle, b)
     sizes = np.unique(comp, return_counts=True)[1]
-    assert sizes.max() > 1024 and np.any((sizes > 16) & (sizes <= 256))
-    assert np.any((sizes > 256) & (sizes <= 1024))
+    assert sizes.max() > 1024 and np.any((sizes > 16) & (sizes <= 64))
+    assert np.any((sizes > 64) & (sizes <= 256)) and np.any((sizes > 256) & (sizes <= 1024))
     assert info["largest"] == sizes.max()
 
 

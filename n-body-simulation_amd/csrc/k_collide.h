@@ -365,11 +365,19 @@ __device__ inline void stageInit(PairStageT<CAP> &st) {
   __syncthreads();
 }
 
+// Also hooks the larger-index body of the pair under the smaller one
+// (fire-and-forget atomicMin on the union-find parents, all roots at this
+// point): every link joins two bodies in contact and points to a smaller
+// index, so it is a valid union-find link, and k_union, which processes
+// every pair again, restores any link a later atomicMin replaced. The union
+// pass then starts from a mostly linked forest instead of singletons.
 template <int CAP>
 __device__ inline void stagePair(PairStageT<CAP> &st, Ctl *__restrict__ ctl,
                                  int4 *__restrict__ pairs, uint32_t cap,
-                                 int later, int earlier, float laterMinX,
+                                 int32_t *__restrict__ parent, int later,
+                                 int earlier, float laterMinX,
                                  float earlierMinX) {
+  atomicMin(&parent[max(later, earlier)], min(later, earlier));
   const int4 p = make_int4(later, earlier, __float_as_int(laterMinX),
                            __float_as_int(earlierMinX));
   const uint32_t slot = atomicAdd(&st.count, 1u);
@@ -417,7 +425,7 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
     const float4 *__restrict__ grid,
     const int32_t *__restrict__ largeList,
     const float4 *__restrict__ largeBox, int4 *__restrict__ pairs,
-    uint32_t cap) {
+    uint32_t cap, int32_t *__restrict__ parent) {
   if (barrierStalled(ctl)) {
     return;
   }
@@ -449,7 +457,7 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
         }
         if (beforeBox(ob.x, other, me.minX, self) &&
             overlapsBox(Box{ob.x, ob.y, ob.z, ob.w}, me)) {
-          stagePair(st, ctl, pairs, cap, self, other, me.minX, ob.x);
+          stagePair(st, ctl, pairs, cap, parent, self, other, me.minX, ob.x);
         }
       };
       if (!wraps) {
@@ -521,10 +529,10 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
       const Box lb{lbv.x, lbv.y, lbv.z, lbv.w};
       if (beforeBox(lb.minX, l, me.minX, self)) {
         if (small && overlapsBox(lb, me)) {
-          stagePair(st, ctl, pairs, cap, self, l, me.minX, lb.minX);
+          stagePair(st, ctl, pairs, cap, parent, self, l, me.minX, lb.minX);
         }
       } else if (overlapsBox(me, lb)) {
-        stagePair(st, ctl, pairs, cap, l, self, lb.minX, me.minX);
+        stagePair(st, ctl, pairs, cap, parent, l, self, lb.minX, me.minX);
       }
     }
   }

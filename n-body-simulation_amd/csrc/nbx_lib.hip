@@ -400,7 +400,7 @@ int launchCollide(nbx_sim *s) {
   LAUNCH(k_find_small, blocksFor(n, kFindBlock), kFindBlock, n, px, py, r,
          s->ctl, colBits, mask, s->bucketOf, s->bucketStart, s->grid,
          s->largeList, s->largeBox, s->pairs,
-         s->pairCap);
+         s->pairCap, s->ufParent);
   LAUNCH(k_union, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs, s->ufParent);
   LAUNCH(k_comp_count, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
          s->ufParent, s->pairComp, s->compCount);

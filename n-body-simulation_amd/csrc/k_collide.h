@@ -1093,8 +1093,11 @@ __device__ inline void waveSync() {
 // Queue entry of a component: {first pair, pair count, component index, 0}.
 template <int PMAX, int BMAX, int G> struct WaveCtx {
   static_assert(G == 16 || G == 32 || G == 64, "lanes per component");
-  static constexpr int kPf = PMAX <= 4 * G && PMAX % G == 0 ? PMAX / G : 0; // pairs/lane in registers
-  static constexpr int kSf = BMAX <= 4 * G && BMAX % G == 0 ? BMAX / G : 0; // bodies/lane in registers
+  // Pairs and bodies per lane held in registers (prefetched pairs of the
+  // next component, staged bodies) where the tier's LDS already limits the
+  // occupancy more than these registers do.
+  static constexpr int kPf = PMAX <= 8 * G && PMAX % G == 0 ? PMAX / G : 0;
+  static constexpr int kSf = BMAX <= 6 * G && BMAX % G == 0 ? BMAX / G : 0;
   struct Pairs {
     int4 v[kPf > 0 ? kPf : 1];
   };

@@ -164,6 +164,15 @@ int nbx_frame(nbx_sim *sim, uint64_t *frame);
 /* Waits for queued work and reports any device-side error. */
 int nbx_sync(nbx_sim *sim);
 
+/* nbx_step without the final host sync: queues `steps` steps on the
+ * handle's stream and returns; device-side errors of these steps surface
+ * at the next nbx_sync / nbx_step. No reference counterpart (CudaSimulation
+ * ::step always waits): for callers that keep several handles busy at once,
+ * each on its own stream, e.g. an ensemble of independent simulations.
+ * Handles stepped concurrently must be created with NBX_SORT=multi in the
+ * environment (no kernel that waits on a grid barrier). */
+int nbx_step_async(nbx_sim *sim, uint32_t steps);
+
 /* ---- Inspection, for checks (CudaQuadTree::download / nodeCount /
  *      levelCount, src/cuda/CudaQuadTree.cuh:42-63). ---- */
 

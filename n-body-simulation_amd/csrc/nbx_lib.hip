@@ -1137,6 +1137,22 @@ int nbx_step(nbx_sim *s, uint32_t steps) {
   return syncAndCheck(s);
 }
 
+int nbx_step_async(nbx_sim *s, uint32_t steps) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  const bool graphs = s->params.use_graph && s->n > 0;
+  for (uint32_t k = 0; k < steps; ++k) {
+    if (graphs) {
+      NBX_TRY(graphStep(s));
+    } else {
+      NBX_TRY(launchStep(s));
+    }
+    s->frame += 1;
+  }
+  return NBX_OK;
+}
+
 int nbx_iterate(nbx_sim *s) {
   if (!s) {
     return setError(NBX_ERR_ARG, "null handle");

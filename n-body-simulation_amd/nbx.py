@@ -37,7 +37,7 @@ EXPORTS = (
     "nbx_root_quad", "nbx_download_keys", "nbx_collide_info", "nbx_download_pairs",
     "nbx_count_interactions", "nbx_set_profiling", "nbx_phase_times", "nbx_stream",
     "nbx_exact_forces", "nbx_init_scenario", "nbx_download_contacts",
-    "nbx_energy",
+    "nbx_energy", "nbx_step_async",
 )
 
 COLLIDE_SEQUENTIAL, COLLIDE_JACOBI = 0, 1
@@ -92,6 +92,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_set_params.argtypes = [_vp, C.POINTER(Params)]
     L.nbx_get_params.argtypes = [_vp, C.POINTER(Params)]
     L.nbx_step.argtypes = [_vp, C.c_uint32]
+    L.nbx_step_async.argtypes = [_vp, C.c_uint32]
     for f in ("nbx_iterate", "nbx_collide", "nbx_attract", "nbx_build", "nbx_forces",
               "nbx_sync"):
         getattr(L, f).argtypes = [_vp]
@@ -242,6 +243,11 @@ class Simulation:
     def step(self, steps: int = 1) -> None:
         self._push_params()
         self._check(self.L.nbx_step(self.h, steps))
+
+    def step_async(self, steps: int = 1) -> None:
+        """Queues steps without waiting; errors surface at sync()."""
+        self._push_params()
+        self._check(self.L.nbx_step_async(self.h, steps))
 
     def iterate(self) -> None:
         self._push_params()

@@ -476,6 +476,36 @@ def test_contact_component_tiers(nbx, oracle):
     assert info["largest"] == sizes.max()
 
 
+def test_step_async_handles_side_by_side(nbx, monkeypatch):
+    """Several handles (created with NBX_SORT=multi, as concurrent handles
+    must be) queued together with nbx_step_async end in exactly the states
+    the same handles reach stepped one at a time with nbx_step."""
+    monkeypatch.setenv("NBX_SORT", "multi")
+    specs = [(3000, nbx.SCENARIO_DISC), (20000, nbx.SCENARIO_COLD_SQUARE),
+             (9000, nbx.SCENARIO_DISC), (40000, nbx.SCENARIO_COLD_SQUARE)]
+    sims = [nbx.Simulation(n, seed=i, scenario=sc) for i, (n, sc) in enumerate(specs)]
+    try:
+        starts = [sim.download() for sim in sims]
+        for sim in sims:
+            sim.theta = 0.5
+            sim.step(30)
+        want = [sim.download_by_id() for sim in sims]
+        for sim, st in zip(sims, starts):
+            sim.upload(st)
+        for _ in range(3):
+            for sim in sims:
+                sim.step_async(10)
+        for sim in sims:
+            sim.sync()
+        for sim, w in zip(sims, want):
+            got = sim.download_by_id()
+            for f in ("px", "py", "vx", "vy"):
+                assert_bits_equal(got[f], w[f], f)
+    finally:
+        for sim in sims:
+            sim.close()
+
+
 def test_exact_forces_fp64(nbx, oracle, states):
     s = states["disc10k"]
     targets = np.arange(0, s.n, 37, dtype=np.uint32)

@@ -38,9 +38,15 @@ extern "C" {
 #define NBX_ERR_STATE 4    /* call not valid in the handle's current state */
 
 /* Force law: the reference's (fastInvSqrt, src/QuadTree.cpp:11-26) or an
- * accurate inverse square root. */
+ * accurate inverse square root. NBX_LAW_ACCURATE_QUAD (no reference
+ * counterpart; compensated summation only) adds, for every far node, the
+ * quadrupole term of the law m*d/(|d|(|d|^2+eps^2)) expanded about the
+ * node's centre of mass, from the node's second mass moments -- a
+ * closed-form far-field correction, so a coarser theta reaches a given
+ * force error. */
 #define NBX_LAW_REFERENCE 0
 #define NBX_LAW_ACCURATE 1
+#define NBX_LAW_ACCURATE_QUAD 2
 
 /* Per-body force summation: bit-exact replay of the reference's FP32 order
  * (src/QuadTree.cpp:264-389, four interleaved leaf sums), or FP32 with

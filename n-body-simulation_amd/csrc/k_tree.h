@@ -690,14 +690,49 @@ __device__ inline void writeSum(int x, const Sum3 &v, float4 *__restrict__ nodeF
   nodeF[x].z = v.m;
 }
 
-// Branch x from its four children (QuadTree.cpp:243-253).
+// Second mass moments of a leaf about its centre of mass (the quadrupole
+// law's node data; no reference counterpart): sum m (x - c)(x - c)^T over
+// the leaf's bodies, as {xx, xy, yy}.
+__device__ inline void leafMoments(int x, const float *__restrict__ px,
+                                   const float *__restrict__ py,
+                                   const float *__restrict__ m, int i, int e,
+                                   const float4 *__restrict__ nodeF,
+                                   float4 *__restrict__ nodeQ) {
+  const float cx = nodeF[x].x, cy = nodeF[x].y;
+  float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+  for (; i < e; ++i) {
+    const float dx = px[i] - cx, dy = py[i] - cy, w = m[i];
+    sxx += w * dx * dx;
+    sxy += w * dx * dy;
+    syy += w * dy * dy;
+  }
+  nodeQ[x] = make_float4(sxx, sxy, syy, 0.0f);
+}
+
+// Branch x from its four children (QuadTree.cpp:243-253); with nodeQ, also
+// its second moments from the children's, moved to x's centre of mass
+// (parallel axes: S_x = sum over children of S_c + m_c (c_c - c_x)(c_c - c_x)^T).
 __device__ inline void sumBranch(int x, int c, float4 *__restrict__ nodeF,
-                                 float2 *__restrict__ wsum) {
+                                 float2 *__restrict__ wsum,
+                                 float4 *__restrict__ nodeQ) {
   const float2 w0 = wsum[c], w1 = wsum[c + 1], w2 = wsum[c + 2],
                w3 = wsum[c + 3];
   const Sum3 v{w0.x + w1.x + w2.x + w3.x, w0.y + w1.y + w2.y + w3.y,
                nodeF[c].z + nodeF[c + 1].z + nodeF[c + 2].z + nodeF[c + 3].z};
   writeSum(x, v, nodeF, wsum);
+  if (nodeQ) {
+    const float cx = nodeF[x].x, cy = nodeF[x].y;
+    float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+    for (int q = 0; q < 4; ++q) {
+      const float4 f = nodeF[c + q];
+      const float4 k = nodeQ[c + q];
+      const float dx = f.x - cx, dy = f.y - cy;
+      sxx += k.x + f.z * dx * dx;
+      sxy += k.y + f.z * dx * dy;
+      syy += k.z + f.z * dy * dy;
+    }
+    nodeQ[x] = make_float4(sxx, sxy, syy, 0.0f);
+  }
 }
 
 // Parent of node x >= 1: the splitting node of depth D-1 whose children
@@ -714,7 +749,8 @@ __global__ __launch_bounds__(kBlock) void k_node_finish(
     const float *__restrict__ py, const float *__restrict__ m,
     const int32_t *__restrict__ nodeOfSplit, float4 *__restrict__ nodeF,
     int4 *__restrict__ nodeI, float2 *__restrict__ wsum,
-    uint8_t *__restrict__ done, int32_t *__restrict__ roots) {
+    uint8_t *__restrict__ done, int32_t *__restrict__ roots,
+    float4 *__restrict__ nodeQ) {
   if (barrierStalled(ctl)) {
     return;
   }
@@ -739,6 +775,9 @@ __global__ __launch_bounds__(kBlock) void k_node_finish(
     nodeI[x].y = y == 0 ? 0 : y + 1;
     if (ni.x == 0) {
       writeSum(x, leafSum(px, py, m, ni.z, ni.w), nodeF, wsum);
+      if (nodeQ) {
+        leafMoments(x, px, py, m, ni.z, ni.w, nodeF, nodeQ);
+      }
       done[x] = 1;
       continue;
     }
@@ -762,7 +801,8 @@ __global__ __launch_bounds__(kBlock) void k_node_finish(
 __global__ __launch_bounds__(kBlock) void k_subtree(
     const Ctl *__restrict__ ctl, const int32_t *__restrict__ roots,
     float4 *__restrict__ nodeF, const int4 *__restrict__ nodeI,
-    float2 *__restrict__ wsum, uint8_t *__restrict__ done) {
+    float2 *__restrict__ wsum, uint8_t *__restrict__ done,
+    float4 *__restrict__ nodeQ) {
   if (barrierStalled(ctl)) {
     return;
   }
@@ -817,7 +857,7 @@ __global__ __launch_bounds__(kBlock) void k_subtree(
         for (int k = levelOff[d] + threadIdx.x; k < levelOff[d + 1];
              k += kBlock) {
           const int x = list[k];
-          sumBranch(x, nodeI[x].x, nodeF, wsum);
+          sumBranch(x, nodeI[x].x, nodeF, wsum, nodeQ);
           done[x] = 1;
         }
         __syncthreads();
@@ -834,7 +874,8 @@ __global__ __launch_bounds__(kBlock) void k_subtree(
 __global__ __launch_bounds__(1024) void k_upward_top(
     const Ctl *__restrict__ ctl, float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, float2 *__restrict__ wsum,
-    const uint8_t *__restrict__ done, int32_t *__restrict__ big) {
+    const uint8_t *__restrict__ done, int32_t *__restrict__ big,
+    float4 *__restrict__ nodeQ) {
   if (barrierStalled(ctl)) {
     return;
   }
@@ -881,7 +922,7 @@ __global__ __launch_bounds__(1024) void k_upward_top(
   for (int d = levels - 1; d >= 0; --d) {
     for (int k = levelOff[d] + tid; k < levelOff[d + 1]; k += blockDim.x) {
       const int x = big[k];
-      sumBranch(x, nodeI[x].x, nodeF, wsum);
+      sumBranch(x, nodeI[x].x, nodeF, wsum, nodeQ);
     }
     __syncthreads();
   }

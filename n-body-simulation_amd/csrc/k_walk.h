@@ -24,6 +24,7 @@ constexpr int kSumReference = 0;
 constexpr int kSumCompensated = 1;
 constexpr int kLawReference = 0;
 constexpr int kLawAccurate = 1;
+constexpr int kLawQuadrupole = 2; // accurate law + far-node quadrupole term
 
 template <int SUM, int LAW>
 __device__ inline float lawFactor(float mass, float dSq, float eSq) {
@@ -327,6 +328,31 @@ __device__ inline void leafTerm(const float4 &b, float x, float y, float eSq,
   qy = fmaf(by, fac, qy);
 }
 
+// Quadrupole term of a far node (no reference counterpart): the law is
+// f(d) = d g(r), g(r) = 1 / (r (r^2 + eps^2)), d = COM - x. Expanding the
+// node's bodies about its COM, the first-order term vanishes and the second
+// is 1/2 sum_ab S_ab d_a d_b f(d) with S the node's second mass moments:
+//   h1 (S d) + 1/2 (h1 tr S + h2 d^T S d) d,
+// h1 = g'/r, h2 = (g'' - g'/r) / r^2. With u = 1/(r (r^2 + eps^2)) and
+// a = 3 r^2 + eps^2: g' = -a u^2, g'' = u^2 (2 a^2 u - 6 r). (eps = 0:
+// h1 = -3/r^5, h2 = 15/r^7, the Newtonian quadrupole.)
+__device__ inline void quadTerm(const float4 &sm, float dx, float dy,
+                                float dSq, float eSq, float &qx, float &qy) {
+  const float invR = __builtin_amdgcn_rsqf(dSq);
+  const float r = dSq * invR;
+  const float u = __builtin_amdgcn_rcpf(r * (dSq + eSq));
+  const float a = 3.0f * dSq + eSq;
+  const float u2 = u * u;
+  const float h1 = -a * u2 * invR;
+  const float g2 = u2 * (2.0f * a * a * u - 6.0f * r);
+  const float h2 = (g2 - h1) * (invR * invR);
+  const float sdx = sm.x * dx + sm.y * dy;
+  const float sdy = sm.y * dx + sm.z * dy;
+  const float c = 0.5f * (h1 * (sm.x + sm.z) + h2 * (dx * sdx + dy * sdy));
+  qx += h1 * sdx + c * dx;
+  qy += h1 * sdy + c * dy;
+}
+
 // A leaf's bodies for the lanes of `near` when few lanes need them. The
 // (lane, record) pairs are spread over the wave: with k near lanes and K the
 // smallest power of two >= k, lane slot r = lane % K takes the r-th near
@@ -377,7 +403,8 @@ __device__ inline void leafSpread(uint64_t near, uint32_t k, uint32_t logK,
 // reference's.
 template <int LAW, bool COUNT, bool CAP, bool MASS>
 __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
-                                     uint64_t mask, float x, float y,
+                                     const float4 &sm, uint64_t mask, float x,
+                                     float y,
                                      float tSq, float eSq,
                                      const float4 *__restrict__ body,
                                      float2 *__restrict__ slot, float &qx,
@@ -400,6 +427,9 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
     const float fac = fastFactor<LAW, CAP>(f.z, dSq, eSq);
     qx = fmaf(dx, fac, qx);
     qy = fmaf(dy, fac, qy);
+    if constexpr (LAW == kLawQuadrupole) {
+      quadTerm(sm, dx, dy, dSq, eSq, qx, qy);
+    }
     if (COUNT) {
       ++nFar;
     }
@@ -463,6 +493,7 @@ template <int LAW, bool COUNT, bool CAP, bool MASS, class STACK>
 __device__ inline void walkWave(uint32_t n, uint32_t base,
                                 const float4 *__restrict__ nodeF,
                                 const int4 *__restrict__ nodeI,
+                                const float4 *__restrict__ nodeQ,
                                 const float4 *__restrict__ body, float tSq,
                                 float eSq, float *__restrict__ ax,
                                 float *__restrict__ ay, Ctl *__restrict__ ctl,
@@ -484,9 +515,10 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
   int sp = 0;
   {
     const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
+    const float4 q0 = LAW == kLawQuadrupole ? nodeQ[0] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint64_t open =
-        visitNode<LAW, COUNT, CAP, MASS>(nodeF[0], nodeI[0], all, x, y, tSq, eSq, body,
-                                   slot, qx, qy, nFar, nPair, nLeafTerm);
+        visitNode<LAW, COUNT, CAP, MASS>(nodeF[0], nodeI[0], q0, all, x, y, tSq, eSq,
+                                   body, slot, qx, qy, nFar, nPair, nLeafTerm);
     nVisit += 1;
     if (open) {
       st.put(0, nodeI[0].x, open);
@@ -526,11 +558,16 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
                  "s"(bi[1].x), "s"(bi[1].y), "s"(bi[1].z), "s"(bi[1].w),
                  "s"(bi[2].x), "s"(bi[2].y), "s"(bi[2].z), "s"(bi[2].w),
                  "s"(bi[3].x), "s"(bi[3].y), "s"(bi[3].z), "s"(bi[3].w));
+    float4 bq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      bq[q] = LAW == kLawQuadrupole ? nodeQ[cb + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     uint64_t open[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      open[q] = visitNode<LAW, COUNT, CAP, MASS>(bf[q], bi[q], mask, x, y, tSq, eSq,
-                                           body, slot, qx, qy, nFar, nPair,
+      open[q] = visitNode<LAW, COUNT, CAP, MASS>(bf[q], bi[q], bq[q], mask, x, y, tSq,
+                                           eSq, body, slot, qx, qy, nFar, nPair,
                                            nLeafTerm);
     }
     if (COUNT) {
@@ -583,7 +620,8 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
 template <int LAW, bool COUNT, bool CAP, bool MASS>
 __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
     uint32_t n, const float4 *__restrict__ nodeF,
-    const int4 *__restrict__ nodeI, const float4 *__restrict__ body,
+    const int4 *__restrict__ nodeI, const float4 *__restrict__ nodeQ,
+    const float4 *__restrict__ body,
     float tSq, float eSq, float *__restrict__ ax, float *__restrict__ ay,
     Ctl *__restrict__ ctl) {
   if (barrierStalled(ctl)) {
@@ -601,13 +639,13 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
   }
   if (ctl->levels <= kRegStackLevels) {
     RegStack st;
-    walkWave<LAW, COUNT, CAP, MASS>(n, base, nodeF, nodeI, body, tSq, eSq, ax, ay,
+    walkWave<LAW, COUNT, CAP, MASS>(n, base, nodeF, nodeI, nodeQ, body, tSq, eSq, ax, ay,
                               ctl, st, slots[w]);
   } else {
     LaneStack st;
     st.spillNode = spillNode[w];
     st.spillMask = spillMask[w];
-    walkWave<LAW, COUNT, CAP, MASS>(n, base, nodeF, nodeI, body, tSq, eSq, ax, ay,
+    walkWave<LAW, COUNT, CAP, MASS>(n, base, nodeF, nodeI, nodeQ, body, tSq, eSq, ax, ay,
                               ctl, st, slots[w]);
   }
 }

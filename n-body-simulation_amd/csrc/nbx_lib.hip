@@ -117,6 +117,7 @@ struct nbx_sim {
   int64_t nodeCap = 0;
   int32_t *splitList = nullptr, *nodeOfSplit = nullptr;
   float4 *nodeF = nullptr;
+  float4 *nodeQ = nullptr; // second mass moments about the COM (quadrupole law)
   int4 *nodeI = nullptr;
   float *centerX = nullptr, *centerY = nullptr, *size = nullptr;
   float2 *wsum = nullptr;
@@ -492,14 +493,16 @@ int launchBuild(nbx_sim *s) {
   LAUNCH(k_nodes, nodeGrid, kBlock, n, hi, s->keyLo, perm, s->ctl,
          s->splitList, s->nodeF, s->nodeI, s->centerX, s->centerY, s->size,
          s->nodeOfSplit);
+  // Second mass moments only for the quadrupole law (null: not computed).
+  float4 *nodeQ = s->params.law == NBX_LAW_ACCURATE_QUAD ? s->nodeQ : nullptr;
   LAUNCH(k_node_finish, nodeGrid, kBlock, s->ctl, s->px[o], s->py[o], s->m[o],
          s->nodeOfSplit, s->nodeF, s->nodeI, s->wsum, s->nodeDone,
-         s->subtreeRoots);
+         s->subtreeRoots, nodeQ);
   LAUNCH(k_subtree, std::min<uint32_t>(1024, std::max<uint32_t>(1, n / 1024)),
          kBlock, s->ctl, s->subtreeRoots, s->nodeF, s->nodeI, s->wsum,
-         s->nodeDone);
+         s->nodeDone, nodeQ);
   LAUNCH(k_upward_top, 1, 1024, s->ctl, s->nodeF, s->nodeI, s->wsum,
-         s->nodeDone, s->bigNodes);
+         s->nodeDone, s->bigNodes, nodeQ);
   s->treeBuilt = true;
   return NBX_OK;
 }
@@ -519,6 +522,7 @@ int launchForces(nbx_sim *s, bool count) {
   const uint32_t walkGrid = blocksFor(n, kWalkBlock);
   const bool ref = s->params.summation == NBX_SUM_REFERENCE;
   const bool acc = s->params.law == NBX_LAW_ACCURATE;
+  const bool quad = s->params.law == NBX_LAW_ACCURATE_QUAD;
 #define WALK(SUMV, LAWV, CNT)                                                 \
   LAUNCH((k_walk<SUMV, LAWV, CNT>), grid, kBlock, n, s->nodeF, s->nodeI,      \
          s->px[c], s->py[c], s->m[c], tSq, eSq, s->ax, s->ay, s->ctl)
@@ -527,7 +531,7 @@ int launchForces(nbx_sim *s, bool count) {
   // the min(., FLT_MAX) guard cannot change a value. (The accurate law keeps
   // the guard: its rsqrt(0) is infinite for a body's own term.)
   const double eps2 = static_cast<double>(eSq);
-  const bool capFree = !acc && eps2 > 0 &&
+  const bool capFree = !acc && !quad && eps2 > 0 &&
                        s->massAbsSum * 2e19 * std::max(1.0, 1.0 / eps2) < 1e37;
   // The node-mass test is needed only when some body mass is not positive
   // and finite (and always by the counting walk, which counts far nodes of
@@ -536,10 +540,12 @@ int launchForces(nbx_sim *s, bool count) {
   do {                                                                        \
     if (!(CNT) && s->massPositive) {                                          \
       LAUNCH((k_walk_fast<LAWV, CNT, CAPV, false>), walkGrid, kWalkBlock, n,  \
-             s->nodeF, s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);\
+             s->nodeF, s->nodeI, s->nodeQ, s->bodyPack, tSq, eSq, s->ax,     \
+             s->ay, s->ctl);                                                  \
     } else {                                                                  \
       LAUNCH((k_walk_fast<LAWV, CNT, CAPV, true>), walkGrid, kWalkBlock, n,   \
-             s->nodeF, s->nodeI, s->bodyPack, tSq, eSq, s->ax, s->ay, s->ctl);\
+             s->nodeF, s->nodeI, s->nodeQ, s->bodyPack, tSq, eSq, s->ax,     \
+             s->ay, s->ctl);                                                  \
     }                                                                         \
   } while (0)
 #define FAST(LAWV, CNT)                                                       \
@@ -560,10 +566,12 @@ int launchForces(nbx_sim *s, bool count) {
     }
   } else {
     if (count) {
-      if (acc) FAST(kLawAccurate, true);
+      if (quad) FAST_M(kLawQuadrupole, true, true);
+      else if (acc) FAST(kLawAccurate, true);
       else FAST(kLawReference, true);
     } else {
-      if (acc) FAST(kLawAccurate, false);
+      if (quad) FAST_M(kLawQuadrupole, false, true);
+      else if (acc) FAST(kLawAccurate, false);
       else FAST(kLawReference, false);
     }
   }
@@ -770,8 +778,13 @@ int validParams(const nbx_params *p) {
   if (!(p->dt == p->dt) || !(p->theta >= 0.0f) || !(p->epsilon == p->epsilon)) {
     return setError(NBX_ERR_ARG, "dt/theta/epsilon must be numbers, theta >= 0");
   }
-  if (p->law != NBX_LAW_REFERENCE && p->law != NBX_LAW_ACCURATE) {
+  if (p->law != NBX_LAW_REFERENCE && p->law != NBX_LAW_ACCURATE &&
+      p->law != NBX_LAW_ACCURATE_QUAD) {
     return setError(NBX_ERR_ARG, "unknown force law");
+  }
+  if (p->law == NBX_LAW_ACCURATE_QUAD && p->summation != NBX_SUM_COMPENSATED) {
+    return setError(NBX_ERR_ARG,
+                    "the quadrupole law needs compensated summation");
   }
   if (p->summation != NBX_SUM_REFERENCE &&
       p->summation != NBX_SUM_COMPENSATED) {
@@ -888,6 +901,7 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->splitList, s->nodeCap / 4 + 1, false)
   A(s->nodeOfSplit, s->nodeCap / 4 + 1, false)
   A(s->nodeF, s->nodeCap, false) A(s->nodeI, s->nodeCap, false)
+  A(s->nodeQ, s->nodeCap, false)
   A(s->centerX, s->nodeCap, false) A(s->centerY, s->nodeCap, false)
   A(s->size, s->nodeCap, false) A(s->wsum, s->nodeCap, false)
   A(s->bigNodes, s->nodeCap / 4 + 1, false)

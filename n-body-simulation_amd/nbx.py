@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("NBX_LIBRARY") or os.path.join(HERE, "build", "libnbx.so")
 
 NBX_OK = 0
-LAW_REFERENCE, LAW_ACCURATE = 0, 1
+LAW_REFERENCE, LAW_ACCURATE, LAW_ACCURATE_QUAD = 0, 1, 2
 SUM_REFERENCE, SUM_COMPENSATED = 0, 1
 PHASES = ("iterate", "collide", "build", "walk")
 

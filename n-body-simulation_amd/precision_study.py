@@ -7,6 +7,11 @@ degrees in-plane; nbx_init_scenario's merger).
   (nbx_exact_forces, the law m d / (|d| (|d|^2 + eps^2))) on seeded targets
   at several theta, for both force laws: the norm-weighted RMS
   sqrt(sum |da|^2 / sum |a|^2), the median and the maximum of |da| / |a|.
+* Equal-error comparison: the quadrupole law (NBX_LAW_ACCURATE_QUAD) at
+  --quad-theta against the accurate law at the largest theta of a grid whose
+  RMS error is no larger; both timed over --time-steps steps of the full
+  step (collisions on) from the merger's initial state, with the walk's time
+  from per-phase events.
 * Energy and momentum in FP64 (nbx_energy) before and after a run of the
   full step from the merger's initial state, once without collisions (the
   integrator's and the walk's drift alone) and once with them (the energy
@@ -63,6 +68,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--drift-theta", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--quad-theta", type=float, default=0.5)
+    ap.add_argument("--plain-thetas", type=float, nargs="+",
+                    default=[0.5, 0.45, 0.4, 0.35, 0.3, 0.275, 0.25, 0.225, 0.2, 0.175, 0.15])
+    ap.add_argument("--time-steps", type=int, default=50)
     args = ap.parse_args()
 
     nbx = load_nbx()
@@ -74,7 +83,8 @@ def main() -> int:
                                  replace=False)).astype(np.uint32)
 
     errors = {}
-    for law_name, law in (("reference", nbx.LAW_REFERENCE), ("accurate", nbx.LAW_ACCURATE)):
+    for law_name, law in (("reference", nbx.LAW_REFERENCE), ("accurate", nbx.LAW_ACCURATE),
+                          ("accurate+quadrupole", nbx.LAW_ACCURATE_QUAD)):
         sim.law = law
         for theta in args.thetas:
             sim.theta = theta
@@ -82,7 +92,50 @@ def main() -> int:
             print(f"[precision] {law_name} theta={theta}: {errors[f'{law_name}@{theta}']}",
                   file=sys.stderr, flush=True)
 
+    # Equal error: the quadrupole law at quad_theta against the coarsest
+    # plain theta that is at least as accurate.
+    sim.law = nbx.LAW_ACCURATE_QUAD
+    sim.theta = args.quad_theta
+    quad_err = force_errors(sim, targets, eps2)
+    plain = None
+    for theta in sorted(args.plain_thetas, reverse=True):
+        sim.law = nbx.LAW_ACCURATE
+        sim.theta = theta
+        e = force_errors(sim, targets, eps2)
+        print(f"[precision] plain theta={theta}: rms {e['rms']:.3e} (quadrupole at "
+              f"{args.quad_theta}: {quad_err['rms']:.3e})", file=sys.stderr, flush=True)
+        if e["rms"] <= quad_err["rms"]:
+            plain = (theta, e)
+            break
     sim.close()
+    equal = {"quad_theta": args.quad_theta, "quad_error": quad_err}
+    configs = [("quadrupole", nbx.LAW_ACCURATE_QUAD, args.quad_theta)]
+    if plain is not None:
+        equal["plain_theta"], equal["plain_error"] = plain
+        configs.append(("plain", nbx.LAW_ACCURATE, plain[0]))
+    for name, law, theta in configs:
+        t = nbx.Simulation(args.n, seed=args.seed, scenario=nbx.SCENARIO_MERGER)
+        t.summation = nbx.SUM_COMPENSATED
+        t.law = law
+        t.theta = theta
+        t.step(3)
+        t.sync()
+        t0 = time.perf_counter()
+        t.step(args.time_steps)
+        t.sync()
+        sps = args.time_steps / (time.perf_counter() - t0)
+        t.set_profiling(True)
+        t.step(args.time_steps)
+        t.sync()
+        ph, ns = t.phase_times()
+        t.set_profiling(False)
+        far, pairs = t.count_interactions()
+        t.close()
+        equal[name] = {"theta": theta, "steps_per_s": sps,
+                       "walk_ms": ph["walk"] / max(ns, 1),
+                       "interactions_per_body": (far + pairs) / args.n}
+        print(f"[precision] {name} theta={theta}: {sps:.1f} steps/s, walk "
+              f"{equal[name]['walk_ms']:.3f} ms", file=sys.stderr, flush=True)
 
     drift = {}
     for collisions in (False, True):
@@ -117,6 +170,7 @@ def main() -> int:
         "workload": (f"two-galaxy merger, N={args.n} (2 x {args.n // 2}), seed {args.seed}, "
                      f"compensated walk, {len(targets)} seeded targets"),
         "force_error_vs_fp64": errors,
+        "equal_error": equal,
         "drift": drift,
     }
     print(json.dumps(out), flush=True)

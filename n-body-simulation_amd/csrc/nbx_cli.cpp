@@ -8,7 +8,7 @@
 //
 // The physics parameters the reference fixes at compile time can be set
 // from the environment: NBX_THETA, NBX_EPSILON, NBX_DT, NBX_COLLISIONS (0/1),
-// NBX_LAW (reference|accurate), NBX_SUM (reference|compensated), NBX_COLLIDE
+// NBX_LAW (reference|accurate|quadrupole), NBX_SUM (reference|compensated), NBX_COLLIDE
 // (sequential|jacobi: the reference CPU's or GPU's resolution). NBX_SCENARIO
 // (disc|two-galaxies|cold-square|merger) and NBX_SEED pick the initial
 // condition (default: the reference's uniformDisc, seed 0).
@@ -77,7 +77,9 @@ int main(int argc, char **argv) {
     p.collisions = std::atoi(v) != 0;
   }
   if (const char *v = std::getenv("NBX_LAW")) {
-    p.law = std::strcmp(v, "accurate") == 0 ? NBX_LAW_ACCURATE : NBX_LAW_REFERENCE;
+    p.law = std::strcmp(v, "accurate") == 0     ? NBX_LAW_ACCURATE
+            : std::strcmp(v, "quadrupole") == 0 ? NBX_LAW_ACCURATE_QUAD
+                                                : NBX_LAW_REFERENCE;
   }
   if (const char *v = std::getenv("NBX_COLLIDE")) {
     p.collide_mode =

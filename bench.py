@@ -124,7 +124,7 @@ def main() -> int:
                     help="first run one device copy of MIB MiB (PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; a multi-GPU run is not part of it "
+        print("bench.py measures one GPU; a multi-GPU run is not supported "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
 

@@ -267,7 +267,7 @@ int nbx_energy(nbx_sim *sim, double eps2, double *kinetic, double *potential,
                double momentum[2]);
 
 /* ---- FP64 direct sum (CheckCuda.cu:456-475 exactTreeLawForces) on the
- *      GPU, the accuracy judge: accelerations of the listed target bodies
+ *      GPU, the accuracy reference: accelerations of the listed target bodies
  *      (device indices) from all bodies, law m*d/(|d|(|d|^2+eps^2)). ---- */
 int nbx_exact_forces(nbx_sim *sim, uint64_t n_targets,
                      const uint32_t *targets, double eps2, double *ax,

@@ -1,5 +1,5 @@
 // k_walk.h -- Barnes-Hut force walk (QuadTree::acceleration,
-// QuadTree.cpp:264-389) and the FP64 direct sum used to judge accuracy.
+// QuadTree.cpp:264-389) and the FP64 direct sum that measures its accuracy.
 //
 // Wavefront-coherent traversal. A wave takes 64 consecutive bodies (the
 // build leaves them in key order, so they are spatial neighbours) and walks

@@ -79,19 +79,21 @@ __device__ inline void resolvePair(float *__restrict__ px,
                                    float *__restrict__ vy,
                                    const float *__restrict__ m,
                                    const float *__restrict__ r, int i, int j) {
+  // Every input is loaded up front (one memory round trip, not two); the
+  // arithmetic is the reference's, in its order.
   float p1x = px[i], p1y = py[i];
   float p2x = px[j], p2y = py[j];
   const float r1 = r[i], r2 = r[j];
+  const float v1x = vx[i], v1y = vy[i];
+  const float v2x = vx[j], v2y = vy[j];
+  const float m1 = m[i], m2 = m[j];
   float dx = p2x - p1x, dy = p2y - p1y;
   const float rr = r1 + r2;
   if (dx * dx + dy * dy > rr * rr) {
     return;
   }
-  const float v1x = vx[i], v1y = vy[i];
-  const float v2x = vx[j], v2y = vy[j];
   const float wx = v2x - v1x, wy = v2y - v1y;
   float dDotV = dx * wx + dy * wy;
-  const float m1 = m[i], m2 = m[j];
   const float weight1 = m2 / (m1 + m2);
   const float weight2 = m1 / (m1 + m2);
   const bool dNonZero = !(dx == 0.0f && dy == 0.0f);
@@ -1341,12 +1343,11 @@ __device__ __forceinline__ void resolveWaveComponent(
         const uint32_t sl = pairSlot[k];
         nx0 = succ[2 * k];
         nx1 = succ[2 * k + 1];
-        if (nx0 != kNoPair) {
-          r0 = atomicSub(&indeg[nx0], 1u);
-        }
-        if (nx1 != kNoPair) {
-          r1 = atomicSub(&indeg[nx1], 1u);
-        }
+        // Both decrements issued back to back, results used after the
+        // resolve (their latency overlaps it); a missing successor counts
+        // down this pair's own entry instead, which is never read again.
+        r0 = atomicSub(&indeg[nx0 != kNoPair ? nx0 : k], 1u);
+        r1 = atomicSub(&indeg[nx1 != kNoPair ? nx1 : k], 1u);
         const int a = static_cast<int>(sl & 0xffffu), b = static_cast<int>(sl >> 16);
         resolvePair(lx, ly, lvx, lvy, lm, lr, a, b);
         if (done + idx < static_cast<uint32_t>(size)) {
@@ -1354,7 +1355,8 @@ __device__ __forceinline__ void resolveWaveComponent(
                                       static_cast<int>(minX[b]));
         }
       }
-      const bool go0 = r0 == 1u, go1 = r1 == 1u;
+      const bool go0 = nx0 != kNoPair && r0 == 1u;
+      const bool go1 = nx1 != kNoPair && r1 == 1u;
       const uint64_t m0 = gballot(go0), m1 = gballot(go1);
       const uint32_t c0 = static_cast<uint32_t>(__popcll(m0));
       if (go0) {

@@ -692,7 +692,7 @@ __device__ inline void writeSum(int x, const Sum3 &v, float4 *__restrict__ nodeF
 
 // Second mass moments of a leaf about its centre of mass (the quadrupole
 // law's node data; no reference counterpart): sum m (x - c)(x - c)^T over
-// the leaf's bodies, as {xx, xy, yy}.
+// the leaf's bodies, as {xx, xy, yy, xx + yy}.
 __device__ inline void leafMoments(int x, const float *__restrict__ px,
                                    const float *__restrict__ py,
                                    const float *__restrict__ m, int i, int e,
@@ -706,7 +706,7 @@ __device__ inline void leafMoments(int x, const float *__restrict__ px,
     sxy += w * dx * dy;
     syy += w * dy * dy;
   }
-  nodeQ[x] = make_float4(sxx, sxy, syy, 0.0f);
+  nodeQ[x] = make_float4(sxx, sxy, syy, sxx + syy);
 }
 
 // Branch x from its four children (QuadTree.cpp:243-253); with nodeQ, also
@@ -731,7 +731,7 @@ __device__ inline void sumBranch(int x, int c, float4 *__restrict__ nodeF,
       sxy += k.y + f.z * dx * dy;
       syy += k.z + f.z * dy * dy;
     }
-    nodeQ[x] = make_float4(sxx, sxy, syy, 0.0f);
+    nodeQ[x] = make_float4(sxx, sxy, syy, sxx + syy);
   }
 }
 

@@ -331,26 +331,28 @@ __device__ inline void leafTerm(const float4 &b, float x, float y, float eSq,
 // Quadrupole term of a far node (no reference counterpart): the law is
 // f(d) = d g(r), g(r) = 1 / (r (r^2 + eps^2)), d = COM - x. Expanding the
 // node's bodies about its COM, the first-order term vanishes and the second
-// is 1/2 sum_ab S_ab d_a d_b f(d) with S the node's second mass moments:
+// is 1/2 sum_ab S_ab d_a d_b f(d) with S the node's second mass moments
+// ({xx, xy, yy, trace}):
 //   h1 (S d) + 1/2 (h1 tr S + h2 d^T S d) d,
 // h1 = g'/r, h2 = (g'' - g'/r) / r^2. With u = 1/(r (r^2 + eps^2)) and
 // a = 3 r^2 + eps^2: g' = -a u^2, g'' = u^2 (2 a^2 u - 6 r). (eps = 0:
-// h1 = -3/r^5, h2 = 15/r^7, the Newtonian quadrupole.)
+// h1 = -3/r^5, h2 = 15/r^7, the Newtonian quadrupole.) invR = 1/r and
+// w = 1/(r^2 + eps^2) are the monopole's; this term is free to contract.
 __device__ inline void quadTerm(const float4 &sm, float dx, float dy,
-                                float dSq, float eSq, float &qx, float &qy) {
-  const float invR = __builtin_amdgcn_rsqf(dSq);
-  const float r = dSq * invR;
-  const float u = __builtin_amdgcn_rcpf(r * (dSq + eSq));
-  const float a = 3.0f * dSq + eSq;
+                                float dSq, float eSq, float invR, float w,
+                                float &qx, float &qy) {
+  const float u = invR * w;
+  const float a = fmaf(3.0f, dSq, eSq);
   const float u2 = u * u;
   const float h1 = -a * u2 * invR;
-  const float g2 = u2 * (2.0f * a * a * u - 6.0f * r);
+  const float r = dSq * invR;
+  const float g2 = u2 * fmaf(2.0f * a * a, u, -6.0f * r);
   const float h2 = (g2 - h1) * (invR * invR);
-  const float sdx = sm.x * dx + sm.y * dy;
-  const float sdy = sm.y * dx + sm.z * dy;
-  const float c = 0.5f * (h1 * (sm.x + sm.z) + h2 * (dx * sdx + dy * sdy));
-  qx += h1 * sdx + c * dx;
-  qy += h1 * sdy + c * dy;
+  const float sdx = fmaf(sm.x, dx, sm.y * dy);
+  const float sdy = fmaf(sm.y, dx, sm.z * dy);
+  const float c = 0.5f * fmaf(h2, fmaf(dx, sdx, dy * sdy), h1 * sm.w);
+  qx = fmaf(h1, sdx, fmaf(c, dx, qx));
+  qy = fmaf(h1, sdy, fmaf(c, dy, qy));
 }
 
 // A leaf's bodies for the lanes of `near` when few lanes need them. The
@@ -424,11 +426,18 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
                                ? (mask & ~near)
                                : 0ull;
   if (__builtin_amdgcn_inverse_ballot_w64(farMask)) {
-    const float fac = fastFactor<LAW, CAP>(f.z, dSq, eSq);
-    qx = fmaf(dx, fac, qx);
-    qy = fmaf(dy, fac, qy);
     if constexpr (LAW == kLawQuadrupole) {
-      quadTerm(sm, dx, dy, dSq, eSq, qx, qy);
+      const float invR = __builtin_amdgcn_rsqf(dSq);
+      const float w = __builtin_amdgcn_rcpf(dSq + eSq);
+      const float fac0 = f.z * invR * w;
+      const float fac = CAP ? capMax(fac0) : fac0;
+      qx = fmaf(dx, fac, qx);
+      qy = fmaf(dy, fac, qy);
+      quadTerm(sm, dx, dy, dSq, eSq, invR, w, qx, qy);
+    } else {
+      const float fac = fastFactor<LAW, CAP>(f.z, dSq, eSq);
+      qx = fmaf(dx, fac, qx);
+      qy = fmaf(dy, fac, qy);
     }
     if (COUNT) {
       ++nFar;

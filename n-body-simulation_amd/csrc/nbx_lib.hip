@@ -127,6 +127,7 @@ struct nbx_sim {
   float4 *bodyPack = nullptr; // {x, y, m, 0} in tree order, for the walk
   uint4 *bboxParts = nullptr;
   bool treeBuilt = false;
+  bool momentsBuilt = false; // the last build computed nodeQ
   bool collided = false;
 
   Ctl *ctl = nullptr;
@@ -504,6 +505,7 @@ int launchBuild(nbx_sim *s) {
   LAUNCH(k_upward_top, 1, 1024, s->ctl, s->nodeF, s->nodeI, s->wsum,
          s->nodeDone, s->bigNodes, nodeQ);
   s->treeBuilt = true;
+  s->momentsBuilt = nodeQ != nullptr;
   return NBX_OK;
 }
 
@@ -514,6 +516,11 @@ int launchForces(nbx_sim *s, bool count) {
   }
   if (!s->treeBuilt) {
     return setError(NBX_ERR_STATE, "forces requested before any tree build");
+  }
+  if (s->params.law == NBX_LAW_ACCURATE_QUAD && !s->momentsBuilt) {
+    return setError(NBX_ERR_STATE,
+                    "the quadrupole law needs a tree built under it "
+                    "(nbx_build, nbx_attract or nbx_step)");
   }
   const int c = s->cur;
   const float tSq = s->params.theta * s->params.theta;
@@ -695,6 +702,7 @@ int graphStep(nbx_sim *s) {
   }
   s->cur = c ^ 1;
   s->treeBuilt = true;
+  s->momentsBuilt = s->params.law == NBX_LAW_ACCURATE_QUAD;
   s->collided = s->params.collisions != 0 && s->n >= 2;
   return NBX_OK;
 }
@@ -901,7 +909,7 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
   A(s->splitList, s->nodeCap / 4 + 1, false)
   A(s->nodeOfSplit, s->nodeCap / 4 + 1, false)
   A(s->nodeF, s->nodeCap, false) A(s->nodeI, s->nodeCap, false)
-  A(s->nodeQ, s->nodeCap, false)
+  A(s->nodeQ, s->nodeCap, true)
   A(s->centerX, s->nodeCap, false) A(s->centerY, s->nodeCap, false)
   A(s->size, s->nodeCap, false) A(s->wsum, s->nodeCap, false)
   A(s->bigNodes, s->nodeCap / 4 + 1, false)

@@ -573,6 +573,21 @@ def test_quadrupole_disc_error(nbx, states, theta):
     assert_bits_equal(a.ax, b2.ax, "ax")
 
 
+def test_quadrupole_needs_moments(nbx, states):
+    """Forces under the quadrupole law on a tree built under another law
+    (no second moments) are refused; a build under it serves them."""
+    with new_sim(nbx, states["disc10k"], summation=nbx.SUM_COMPENSATED, theta=0.5,
+                 law=nbx.LAW_ACCURATE) as sim:
+        sim.attract()
+        sim.law = nbx.LAW_ACCURATE_QUAD
+        with pytest.raises(nbx.NbxError, match="quadrupole law needs a tree"):
+            sim.forces()
+        sim.build()
+        sim.forces()
+        got = sim.download()
+    assert np.all(np.isfinite(got.ax)) and np.all(np.isfinite(got.ay))
+
+
 def test_exact_forces_fp64(nbx, oracle, states):
     s = states["disc10k"]
     targets = np.arange(0, s.n, 37, dtype=np.uint32)

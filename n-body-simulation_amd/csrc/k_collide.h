@@ -9,9 +9,11 @@
 // resolves all pairs sequentially in pairLess order. resolve() touches only
 // its two bodies, so pairs in different connected components of the contact
 // graph commute: components are found with a lock-free union-find, and each
-// component replays its own pairs, in pairLess order, on one lane (or one
-// workgroup for large components). The result is bit-identical to the
-// sequential replay.
+// component replays its own pairs: in pairLess order on one lane (up to
+// kSmallComp pairs), as a topological sort of its per-body pair chains on a
+// group of 16-64 lanes (up to 4096 pairs), or in pairLess order on one
+// workgroup (beyond, and components a lane group cannot hold). The result
+// is bit-identical to the sequential replay.
 #pragma once
 
 #include "k_sort.h"
@@ -1449,7 +1451,8 @@ __global__ __launch_bounds__(kWave) void k_resolve_wave(
   }
 }
 
-// ---- contact components of more than kSmallComp pairs ----
+// ---- contact components of more than 4096 pairs, or that a wave tier
+//      cannot hold ----
 //
 // One workgroup per component. Sort into pairLess order: up to kBigCompMax
 // pairs by a bitonic sort in LDS; longer components as runs of kBigCompMax

@@ -549,11 +549,18 @@ __device__ inline int ufLoad(const int32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Root of x with path halving. Parent links only ever move to a smaller
-// index, so the walk terminates and the final root is the component's
-// smallest body index.
+// Root of x. Parent links only ever move to a smaller index, so the walk
+// terminates and the final root is the component's smallest body index.
+// The first kUfReadSteps steps only read; a walk that goes on halves its
+// path (each link it passes is pointed at its grandparent). Most finds are
+// a few links long, and halving those too makes many lanes store to the
+// few cache lines a clump's parents occupy, which cost k_union more than
+// the reads (145 vs 84 us at 1.9M pairs). A long chain -- bodies in a line,
+// numbered along it by the tree order -- is still cut down by the finds
+// that cross it.
+constexpr int kUfReadSteps = 2;
 __device__ inline int ufFind(int32_t *parent, int x) {
-  while (true) {
+  for (int step = 0;; ++step) {
     const int p = ufLoad(&parent[x]);
     if (p == x) {
       return x;
@@ -562,9 +569,37 @@ __device__ inline int ufFind(int32_t *parent, int x) {
     if (gp == p) {
       return p;
     }
-    __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    if (step >= kUfReadSteps) {
+      __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
     x = gp;
+  }
+}
+
+// One thread per body, between the search's hooks and k_union: points
+// every hooked body at the root of its hook tree. The hooks alone leave
+// chains (each body under its smallest partner, that one under its own...);
+// flattened, most pairs then see one common parent at their first two
+// loads and leave k_union at once. Links only move to ancestors, so the
+// forest stays valid whatever the order of the stores.
+__global__ __launch_bounds__(kBlock) void k_uf_flatten(
+    const Ctl *__restrict__ ctl, uint32_t n, int32_t *__restrict__ parent) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= n) {
+    return;
+  }
+  const int p = ufLoad(&parent[v]);
+  if (p == static_cast<int>(v)) {
+    return;
+  }
+  const int root = ufFind(parent, p);
+  if (root != p) {
+    __hip_atomic_store(&parent[v], root, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -578,8 +613,14 @@ __global__ __launch_bounds__(kBlock) void k_union(
   for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;
        k += gridDim.x * kBlock) {
     const int4 p = pairs[k];
-    int a = ufFind(parent, p.x);
-    int b = ufFind(parent, p.y);
+    // Already in one tree: a common parent, or one body the other's parent.
+    const int pa = ufLoad(&parent[p.x]);
+    const int pb = ufLoad(&parent[p.y]);
+    if (pa == pb || pa == p.y || pb == p.x) {
+      continue;
+    }
+    int a = ufFind(parent, pa);
+    int b = ufFind(parent, pb);
     while (a != b) {
       const int hi = a > b ? a : b;
       const int lo = a > b ? b : a;

@@ -476,6 +476,54 @@ def test_contact_component_tiers(nbx, oracle):
     assert info["largest"] == sizes.max()
 
 
+def test_contact_chains_numbered_along(nbx, oracle):
+    """Straight chains whose bodies are numbered along their length, the
+    order a tree build gives a line of bodies, so the search's hooks (each
+    body under its smallest partner) form chains thousands of links deep for
+    the union-find: 3000 bodies numbered forwards, 1000 backwards and 257
+    forwards, beside a 40x40 clump numbered row by row. Bit-exact against
+    the sequential replay, and the pair components are exactly the contact
+    graph's connected components."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    from oracle_lib import Bodies
+    rng = np.random.default_rng(5)
+    xs, ys = [], []
+    for k, (length, step) in enumerate(((3000, 1.9), (1000, -1.9), (257, 1.9))):
+        t = np.arange(length)
+        xs.append(t * step + rng.uniform(-0.03, 0.03, length))
+        ys.append(rng.uniform(-0.3, 0.3, length) + 50.0 * k)
+    g = np.stack(np.meshgrid(np.arange(40), np.arange(40)), -1).reshape(-1, 2)
+    xs.append(g[:, 0] * 1.9 + rng.uniform(-0.05, 0.05, g.shape[0]))
+    ys.append(g[:, 1] * 1.9 + rng.uniform(-0.05, 0.05, g.shape[0]) + 200.0)
+    px, py = np.concatenate(xs), np.concatenate(ys)
+    n = px.shape[0]
+    b = Bodies.empty(n)
+    b.px[:] = px.astype(np.float32)
+    b.py[:] = py.astype(np.float32)
+    b.vx[:] = rng.standard_normal(n).astype(np.float32)
+    b.vy[:] = rng.standard_normal(n).astype(np.float32)
+    b.m[:] = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    b.r[:] = 1.0
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    info, _ = check_collide(nbx, oracle, b)
+    with new_sim(nbx, b) as sim:
+        sim.collide()
+        later, earlier, comp = sim.pairs()
+    ones = np.ones(later.size)
+    ncc, label = connected_components(
+        coo_matrix((ones, (later, earlier)), shape=(n, n)), directed=False)
+    pair_cc = label[later]
+    # comp and the connected-component label determine each other.
+    assert np.unique(np.stack([comp, pair_cc]), axis=1).shape[1] == np.unique(comp).size
+    assert np.unique(comp).size == np.unique(pair_cc).size
+    sizes = np.bincount(pair_cc)
+    assert info["largest"] == sizes.max() and info["components"] == np.unique(pair_cc).size
+    # Each chain is one component (its bodies' hooks: one chain as deep as
+    # the line is long), the clump another.
+    assert {2999, 999, 256} <= set(sizes[sizes > 0].tolist())
+
+
 def test_step_async_handles_side_by_side(nbx, monkeypatch):
     """Several handles (created with NBX_SORT=multi, as concurrent handles
     must be) queued together with nbx_step_async end in exactly the states

@@ -472,29 +472,42 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
           lo[row] = bucketStart[h - 1];
           hi[row] = bucketStart[h + 2];
         }
-        // The three rows' entries as one list, two entries per iteration
+        // The three rows' entries as one list, four entries per iteration
         // (so a wave runs max-over-lanes of the total, not of each row, and
-        // two records are in flight per lane).
+        // four records are in flight per lane; a lane past its end reloads
+        // its first entry of the iteration and skips the test).
         const uint32_t n0 = hi[0] - lo[0], n01 = n0 + (hi[1] - lo[1]);
         const uint32_t total = n01 + (hi[2] - lo[2]);
         auto at = [&](uint32_t q, int32_t &cy) {
           cy = my + (q < n0 ? -1 : (q < n01 ? 0 : 1));
           return q < n0 ? lo[0] + q : (q < n01 ? lo[1] + q - n0 : lo[2] + q - n01);
         };
-        for (uint32_t q = 0; q < total; q += 2) {
-          int32_t cy0, cy1;
+        for (uint32_t q = 0; q < total; q += 4) {
+          int32_t cy0, cy1, cy2, cy3;
           const uint32_t o0 = at(q, cy0);
-          const bool two = q + 1 < total;
-          const uint32_t o1 = two ? at(q + 1, cy1) : o0;
+          const uint32_t o1 = q + 1 < total ? at(q + 1, cy1) : o0;
+          const uint32_t o2 = q + 2 < total ? at(q + 2, cy2) : o0;
+          const uint32_t o3 = q + 3 < total ? at(q + 3, cy3) : o0;
           const float4 ob0 = grid[2 * o0], oc0 = grid[2 * o0 + 1];
           const float4 ob1 = grid[2 * o1], oc1 = grid[2 * o1 + 1];
+          const float4 ob2 = grid[2 * o2], oc2 = grid[2 * o2 + 1];
+          const float4 ob3 = grid[2 * o3], oc3 = grid[2 * o3 + 1];
           asm volatile("" ::"v"(ob0.x), "v"(ob0.y), "v"(ob0.z), "v"(ob0.w),
                        "v"(oc0.x), "v"(oc0.y), "v"(oc0.z), "v"(ob1.x),
                        "v"(ob1.y), "v"(ob1.z), "v"(ob1.w), "v"(oc1.x),
-                       "v"(oc1.y), "v"(oc1.z));
+                       "v"(oc1.y), "v"(oc1.z), "v"(ob2.x), "v"(ob2.y),
+                       "v"(ob2.z), "v"(ob2.w), "v"(oc2.x), "v"(oc2.y),
+                       "v"(oc2.z), "v"(ob3.x), "v"(ob3.y), "v"(ob3.z),
+                       "v"(ob3.w), "v"(oc3.x), "v"(oc3.y), "v"(oc3.z));
           test(ob0, oc0, cy0);
-          if (two) {
+          if (q + 1 < total) {
             test(ob1, oc1, cy1);
+          }
+          if (q + 2 < total) {
+            test(ob2, oc2, cy2);
+          }
+          if (q + 3 < total) {
+            test(ob3, oc3, cy3);
           }
         }
       } else {

@@ -962,28 +962,54 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
       ++cls;
     }
   }
+  // Queue slots: the workgroup's count per class in LDS, then one atomic
+  // per class and workgroup, the classes' atomics issued together (a wave
+  // waiting on each class's contended counter in turn cost more than the
+  // replays).
+  constexpr int kClasses = kWaveTiers + 1;
+  constexpr int kWaves = kBlock / kWave;
+  __shared__ uint32_t waveCnt[kWaves][kClasses];
+  __shared__ uint32_t waveBase[kWaves][kClasses];
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  for (int c = 0; c <= kWaveTiers; ++c) {
-    const bool mine = cls == c;
-    const uint64_t mask = __ballot(mine);
-    if (mask == 0) {
-      continue;
+  const int wave = static_cast<int>(threadIdx.x / kWave);
+  uint64_t mine = 0;
+#pragma unroll
+  for (int c = 0; c < kClasses; ++c) {
+    const uint64_t mask = __ballot(cls == c);
+    if (cls == c) {
+      mine = mask;
     }
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    if (lane == static_cast<uint32_t>(c)) {
+      waveCnt[wave][c] = static_cast<uint32_t>(__popcll(mask));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < static_cast<uint32_t>(kClasses)) {
+    const int c = static_cast<int>(threadIdx.x);
+    uint32_t total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      total += waveCnt[w][c];
+    }
     uint32_t base = 0;
-    if (static_cast<int>(lane) == leader) {
-      base = atomicAdd(c < kWaveTiers ? &ctl->waveCount[c] : &ctl->bigCount,
-                       static_cast<uint32_t>(__popcll(mask)));
+    if (total > 0) {
+      base = atomicAdd(c < kWaveTiers ? &ctl->waveCount[c] : &ctl->bigCount, total);
     }
-    base = __shfl(base, leader, kWave);
-    if (mine) {
-      const uint32_t at = base + static_cast<uint32_t>(__popcll(mask & laneMaskLt()));
-      if (c < kWaveTiers) {
-        tierList[static_cast<size_t>(c) * listCap + at] =
-            make_int4(start, size, static_cast<int>(j), 0);
-      } else {
-        bigList[at] = static_cast<int32_t>(j);
-      }
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      waveBase[w][c] = base;
+      base += waveCnt[w][c];
+    }
+  }
+  __syncthreads();
+  if (cls >= 0) {
+    const uint32_t at = waveBase[wave][cls] +
+                        static_cast<uint32_t>(__popcll(mine & laneMaskLt()));
+    if (cls < kWaveTiers) {
+      tierList[static_cast<size_t>(cls) * listCap + at] =
+          make_int4(start, size, static_cast<int>(j), 0);
+    } else {
+      bigList[at] = static_cast<int32_t>(j);
     }
   }
   if (!live || size > kSmallComp) {

@@ -590,21 +590,13 @@ __device__ inline int ufFind(int32_t *parent, int x) {
   }
 }
 
-// One thread per body, between the search's hooks and k_union: points
-// every hooked body at the root of its hook tree. The hooks alone leave
-// chains (each body under its smallest partner, that one under its own...);
-// flattened, most pairs then see one common parent at their first two
-// loads and leave k_union at once. Links only move to ancestors, so the
-// forest stays valid whatever the order of the stores.
-__global__ __launch_bounds__(kBlock) void k_uf_flatten(
-    const Ctl *__restrict__ ctl, uint32_t n, int32_t *__restrict__ parent) {
-  if (barrierStalled(ctl)) {
-    return;
-  }
-  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-  if (v >= n) {
-    return;
-  }
+// Points hooked body v at the root of its hook tree. The search's hooks
+// alone leave chains (each body under its smallest partner, that one under
+// its own...); flattened, most pairs then see one common parent at their
+// first two loads and leave k_union at once. Links only move to ancestors,
+// so the forest stays valid whatever the order of the stores, also while
+// other lanes join trees.
+__device__ inline void flattenBody(int32_t *parent, uint32_t v) {
   const int p = ufLoad(&parent[v]);
   if (p == static_cast<int>(v)) {
     return;
@@ -616,11 +608,18 @@ __global__ __launch_bounds__(kBlock) void k_uf_flatten(
   }
 }
 
+// Each thread flattens its share of the bodies, then joins its share of
+// the pairs (one launch: the flattening of one part overlaps the joins of
+// another, and a step has one kernel less).
 __global__ __launch_bounds__(kBlock) void k_union(
-    const Ctl *__restrict__ ctl, uint32_t cap, const int4 *__restrict__ pairs,
-    int32_t *__restrict__ parent) {
+    const Ctl *__restrict__ ctl, uint32_t n, uint32_t cap,
+    const int4 *__restrict__ pairs, int32_t *__restrict__ parent) {
   if (barrierStalled(ctl)) {
     return;
+  }
+  for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n;
+       v += gridDim.x * kBlock) {
+    flattenBody(parent, v);
   }
   const uint32_t count = min(ctl->pairCount, cap);
   for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < count;

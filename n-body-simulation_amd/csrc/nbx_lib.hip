@@ -403,9 +403,8 @@ int launchCollide(nbx_sim *s) {
          s->ctl, colBits, mask, s->bucketOf, s->bucketStart, s->grid,
          s->largeList, s->largeBox, s->pairs,
          s->pairCap, s->ufParent);
-  LAUNCH(k_uf_flatten, nb, kBlock, s->ctl, static_cast<uint32_t>(n),
-         s->ufParent);
-  LAUNCH(k_union, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs, s->ufParent);
+  LAUNCH(k_union, pairGrid, kBlock, s->ctl, static_cast<uint32_t>(n),
+         s->pairCap, s->pairs, s->ufParent);
   LAUNCH(k_comp_count, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
          s->ufParent, s->pairComp, s->compCount);
   // Component segments: one launch where 4/8/16 bodies per lane cover n

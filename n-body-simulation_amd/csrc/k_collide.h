@@ -890,7 +890,7 @@ __global__ __launch_bounds__(kBlock) void k_comp_scatter(
 // (Batcher) -- static indices only, so nothing spills to scratch -- then
 // written back in order and replayed.
 template <int N>
-__device__ inline void resolveComponent(int start, int size,
+__device__ __forceinline__ void resolveComponent(int start, int size,
                                         int4 *__restrict__ segPairs,
                                         float *__restrict__ px,
                                         float *__restrict__ py,
@@ -898,11 +898,15 @@ __device__ inline void resolveComponent(int start, int size,
                                         float *__restrict__ vy,
                                         const float *__restrict__ m,
                                         const float *__restrict__ r) {
+  // Loads from a clamped index and selects values (a select of pointers
+  // would load through a private copy of `pad`); inlined, the body
+  // accesses below are global ones, not flat.
   const int4 pad = make_int4(INT_MAX, INT_MAX, 0x7f800000, 0x7f800000);
   int4 p[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    p[k] = k < size ? segPairs[start + k] : pad;
+    const int4 q = segPairs[start + min(k, size - 1)];
+    p[k] = k < size ? q : pad;
   }
 #pragma unroll
   for (int w = 1; w < N; w <<= 1) {

@@ -86,15 +86,27 @@ def walk_traffic():
     return float(m.group(1)) if m else None
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n: int, theta: float, steps: int):
-    """Reference OpenMP build on this host's cores, bounded sample."""
+    """Reference OpenMP build on every core this process may run on, over
+    steps 1..`steps` of the same workload (a bounded sample)."""
     if not os.path.exists(REF_BENCH):
         return None
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    threads = max(1, min(cores, 16))
+    threads = max(1, cores)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores",
                OMP_PROC_BIND="close")
     t0 = time.time()
@@ -105,10 +117,30 @@ def cpu_baseline(n: int, theta: float, steps: int):
     print(f"[bench] cpu baseline: {out.stdout.strip()} (wall {time.time() - t0:.1f} s)",
           file=sys.stderr, flush=True)
     return {"value": round(1000.0 / ms, 4), "unit": "steps/s", "cores": threads,
-            "kind": "reference",
-            "sample": (f"reference Simulation (QuadTree/Collision/Utils.cpp, OpenMP, "
-                       f"-Ofast -march=x86-64-v3), same {n}-body disc at theta={theta}, "
-                       f"steps 1..{steps} from the initial state; {ms:.1f} ms/step")}
+            "kind": "reference", "cpu_model": cpu_model(), "steps": steps,
+            "sample": (f"reference Simulation (QuadTree/Collision/Utils.cpp, OpenMP on "
+                       f"{threads} threads, -Ofast -march=x86-64-v3), same {n}-body disc "
+                       f"at theta={theta}, steps 1..{steps} from the initial state; "
+                       f"{ms:.1f} ms/step")}
+
+
+def force_accuracy(nbx, sim, targets: int, seed: int = 12345) -> dict:
+    """Walk accelerations of the current state against the FP64 direct sum
+    on `targets` seeded bodies: norm-weighted RMS sqrt(sum|da|^2/sum|a|^2),
+    median and max of |da|/|a|."""
+    import numpy as np
+    n = sim.n
+    rng = np.random.default_rng(seed)
+    t = np.sort(rng.choice(n, size=min(targets, n), replace=False)).astype(np.uint32)
+    got = sim.download()
+    ex, ey = sim.exact_forces(t, float(sim.epsilon) ** 2)
+    dx = got.ax[t].astype(np.float64) - ex
+    dy = got.ay[t].astype(np.float64) - ey
+    d2, a2 = dx * dx + dy * dy, ex * ex + ey * ey
+    rel = np.sqrt(d2 / np.where(a2 > 0, a2, np.inf))
+    return {"rms": float(np.sqrt(d2.sum() / a2.sum())), "median_rel": float(np.median(rel)),
+            "max_rel": float(rel.max()), "targets": int(t.size),
+            "reference": "FP64 direct sum m*d/(|d|(|d|^2+eps^2)) over all bodies"}
 
 
 def main() -> int:
@@ -118,7 +150,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=100,
+                    help="CPU baseline window: steps 1..min(K, this)")
+    ap.add_argument("--targets", type=int, default=4096,
+                    help="seeded targets of the accuracy check")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (PMC scaling)")
@@ -178,6 +213,21 @@ def main() -> int:
 
     info = sim.collide_info()
     nodes, levels = sim.tree_info()
+    # Accuracy of the final state's accelerations (outside every timed region).
+    accuracy = force_accuracy(nbx, sim, args.targets)
+    print(f"[bench] accuracy vs FP64 direct sum: {accuracy}", file=sys.stderr, flush=True)
+
+    # The CPU baseline's window is steps 1..cpu_steps; when that is shorter
+    # than the timed run, the GPU is timed over the same window too.
+    cpu_steps = min(args.steps, args.cpu_steps)
+    same_window = value_same = None
+    if cpu_steps < args.steps and not args.no_cpu:
+        sim.upload(start)
+        device_sync()
+        t4 = time.perf_counter()
+        sim.step(cpu_steps)
+        device_sync()
+        value_same = cpu_steps / (time.perf_counter() - t4)
     sim.close()
 
     phase_ms = {k: v / max(psteps, 1) for k, v in phases.items()}
@@ -186,12 +236,11 @@ def main() -> int:
     interactions = 0.25 * ((far0 + pair0) + 2 * (farM + pairM) + (far1 + pair1))
     achieved = interactions * FLOP_PER_INTERACTION / walk_s / 1e12
     value = args.steps / elapsed
-    # Compute roof: 157.3 TFLOP/s is the MI355X FP32 peak of its matrix and
-    # vector paths alike; the walk runs on the vector ALUs (no MFMA), and it
-    # is not HBM-bound (see "traffic").
+    # Compute roof: the walk runs on the FP32 vector ALUs (no MFMA), priced
+    # at the 157.3 TFLOP/s FP32 peak; it is not HBM-bound (see "traffic").
     roofline = {
-        "bound": "mfma",
-        "compute": "fp32 VALU (FP32 peak 157.3 TFLOP/s, matrix and vector alike)",
+        "bound": "valu",
+        "compute": "fp32 VALU, no MFMA (MI355X FP32 peak 157.3 TFLOP/s)",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
@@ -205,9 +254,12 @@ def main() -> int:
         "walk_leaf_util": round(pair1 / (64.0 * max(terms1, 1)), 4),
         "walk_wave_slots": {"visits": visits1, "leaf_terms": terms1},
     }
-    cpu = None if args.no_cpu else cpu_baseline(args.n, args.theta, args.cpu_steps)
+    cpu = None if args.no_cpu else cpu_baseline(args.n, args.theta, cpu_steps)
+    if cpu is not None:
+        cpu["gpu_value_same_window"] = round(value if value_same is None else value_same, 3)
     line = {
-        "metric": "steps/s at 1M bodies (plain Barnes-Hut, theta=0.5, collisions on, 1000 steps)",
+        "metric": (f"steps/s at {args.n} bodies (plain Barnes-Hut, theta={args.theta}, "
+                   f"collisions on, steps 1..{args.steps})"),
         "value": round(value, 3),
         "unit": "steps/s",
         "n_gpus": 1,
@@ -233,6 +285,7 @@ def main() -> int:
         "instrumented_steps_per_s": round(profiled_sps, 3),
         "tree": {"nodes": nodes, "levels": levels},
         "collisions": info,
+        "accuracy": accuracy,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -96,6 +96,18 @@ void nbx_default_params(nbx_params *p);
  * allocate. */
 int nbx_create(uint64_t capacity, nbx_sim **out);
 
+/* nbx_create with creation flags (no reference counterpart: the reference
+ * has one way to build its CudaSimulation). NBX_CREATE_NO_GRID_BARRIERS
+ * replaces the kernels that wait on a grid barrier (the one-launch radix
+ * sort with the bounding box and keys folded in, the collide's bucket scan
+ * and component allocation) by their multi-launch forms, for handles that
+ * share the GPU with concurrent work and cannot promise that all of a
+ * launch's workgroups are resident at once. nbx_create(c, out) is
+ * nbx_create_ex(c, flags, out) with this flag set when the environment
+ * variable NBX_SORT is "multi". */
+#define NBX_CREATE_NO_GRID_BARRIERS 1u
+int nbx_create_ex(uint64_t capacity, uint32_t flags, nbx_sim **out);
+
 /* Releases all device memory and the stream. Null is accepted. */
 int nbx_destroy(nbx_sim *sim);
 
@@ -175,8 +187,9 @@ int nbx_sync(nbx_sim *sim);
  * at the next nbx_sync / nbx_step. No reference counterpart (CudaSimulation
  * ::step always waits): for callers that keep several handles busy at once,
  * each on its own stream, e.g. an ensemble of independent simulations.
- * Handles stepped concurrently must be created with NBX_SORT=multi in the
- * environment (no kernel that waits on a grid barrier). */
+ * Handles stepped this way must be created with NBX_CREATE_NO_GRID_BARRIERS
+ * (no kernel that waits on a grid barrier); any other handle is refused
+ * with NBX_ERR_STATE. */
 int nbx_step_async(nbx_sim *sim, uint32_t steps);
 
 /* ---- Inspection, for checks (CudaQuadTree::download / nodeCount /

@@ -28,6 +28,7 @@
 
 using namespace nbx;
 
+
 namespace {
 
 thread_local std::string g_error;
@@ -65,6 +66,7 @@ uint64_t pow2AtLeast(uint64_t v) {
   }
   return p;
 }
+
 
 } // namespace
 
@@ -829,12 +831,21 @@ void nbx_default_params(nbx_params *p) {
 }
 
 int nbx_create(uint64_t capacity, nbx_sim **out) {
+  const char *mode = std::getenv("NBX_SORT");
+  const bool multi = mode && std::strcmp(mode, "multi") == 0;
+  return nbx_create_ex(capacity, multi ? NBX_CREATE_NO_GRID_BARRIERS : 0u, out);
+}
+
+int nbx_create_ex(uint64_t capacity, uint32_t flags, nbx_sim **out) {
   if (!out) {
     return setError(NBX_ERR_ARG, "out is null");
   }
   *out = nullptr;
   if (capacity == 0 || capacity > (1ull << 27)) {
     return setError(NBX_ERR_ARG, "capacity must be in [1, 2^27]");
+  }
+  if (flags & ~static_cast<uint32_t>(NBX_CREATE_NO_GRID_BARRIERS)) {
+    return setError(NBX_ERR_ARG, "unknown creation flag");
   }
   nbx_sim *s = new nbx_sim();
   s->cap = capacity;
@@ -887,8 +898,7 @@ int nbx_create(uint64_t capacity, nbx_sim **out) {
     if (const char *sp = std::getenv("NBX_BARRIER_SPINS")) {
       s->barrierSpins = static_cast<uint32_t>(std::strtoul(sp, nullptr, 10));
     }
-    const char *mode = std::getenv("NBX_SORT");
-    const bool multi = mode && std::strcmp(mode, "multi") == 0;
+    const bool multi = (flags & NBX_CREATE_NO_GRID_BARRIERS) != 0;
     if (!multi && hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                               dev) == hipSuccess &&
@@ -1163,6 +1173,13 @@ int nbx_step(nbx_sim *s, uint32_t steps) {
 int nbx_step_async(nbx_sim *s, uint32_t steps) {
   if (!s) {
     return setError(NBX_ERR_ARG, "null handle");
+  }
+  if (s->sortGrid) {
+    // Its one-launch kernels wait on grid barriers, which need every
+    // workgroup resident: not promised while other streams hold CUs.
+    return setError(NBX_ERR_STATE,
+                    "nbx_step_async needs a handle created with "
+                    "NBX_CREATE_NO_GRID_BARRIERS");
   }
   const bool graphs = s->params.use_graph && s->n > 0;
   for (uint32_t k = 0; k < steps; ++k) {

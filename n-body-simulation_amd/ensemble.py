@@ -14,8 +14,9 @@ steps with nbx_step_async and syncs all handles at the end, so the GPU runs
 the instances' step graphs side by side (as many at once as it has hardware
 queues for the process). Both runs start from the same initial states and
 must end in bit-identical states, which the script checks. The handles are
-created with NBX_SORT=multi: concurrent handles must not use the kernels
-that wait on a grid barrier (those need all of their workgroups resident).
+created with NBX_CREATE_NO_GRID_BARRIERS: concurrent handles must not use
+the kernels that wait on a grid barrier (those need all of their workgroups
+resident), and nbx_step_async refuses handles that would.
 
     python n-body-simulation_amd/ensemble.py [--instances 256] [--steps 500]
         [--nmin 2000] [--nmax 64000] [--theta 0.5] [--chunk 10] [--seed 7]
@@ -62,13 +63,12 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=7)
     args = ap.parse_args()
 
-    os.environ["NBX_SORT"] = "multi"  # read when each handle is created
     nbx = load_nbx()
     ns = sizes(args.instances, args.nmin, args.nmax, args.seed)
     sims, starts = [], []
     for i, n in enumerate(ns):
         scenario = nbx.SCENARIO_DISC if i % 2 == 0 else nbx.SCENARIO_COLD_SQUARE
-        sim = nbx.Simulation(n, seed=i, scenario=scenario)
+        sim = nbx.Simulation(n, seed=i, scenario=scenario, no_grid_barriers=True)
         sim.theta = args.theta
         sim.collisions = True
         sims.append(sim)

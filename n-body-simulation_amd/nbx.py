@@ -37,8 +37,10 @@ EXPORTS = (
     "nbx_root_quad", "nbx_download_keys", "nbx_collide_info", "nbx_download_pairs",
     "nbx_count_interactions", "nbx_set_profiling", "nbx_phase_times", "nbx_stream",
     "nbx_exact_forces", "nbx_init_scenario", "nbx_download_contacts",
-    "nbx_energy", "nbx_step_async",
+    "nbx_energy", "nbx_step_async", "nbx_create_ex",
 )
+
+CREATE_NO_GRID_BARRIERS = 1
 
 COLLIDE_SEQUENTIAL, COLLIDE_JACOBI = 0, 1
 
@@ -81,6 +83,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_default_params.argtypes = [C.POINTER(Params)]
     L.nbx_default_params.restype = None
     L.nbx_create.argtypes = [C.c_uint64, C.POINTER(_vp)]
+    L.nbx_create_ex.argtypes = [C.c_uint64, C.c_uint32, C.POINTER(_vp)]
     L.nbx_destroy.argtypes = [_vp]
     L.nbx_init_disc.argtypes = [_vp, C.c_uint64, C.c_uint32]
     L.nbx_init_scenario.argtypes = [_vp, C.c_int32, C.c_uint64, C.c_uint32]
@@ -141,11 +144,13 @@ class Simulation:
     of makeScenario's initial conditions (SCENARIO_*); Simulation(state=...)
     loads given bodies. Attributes dt, theta, epsilon, collisions, law,
     summation and use_graph are read on every call, like CudaSimulation's
-    public members.
+    public members. no_grid_barriers=True creates the handle with
+    NBX_CREATE_NO_GRID_BARRIERS (required by step_async).
     """
 
     def __init__(self, n: int | None = None, seed: int = 0, state=None,
-                 capacity: int | None = None, scenario: int = SCENARIO_DISC):
+                 capacity: int | None = None, scenario: int = SCENARIO_DISC,
+                 no_grid_barriers: bool = False):
         self.L = load_library()
         if state is not None:
             n_bodies = int(np.asarray(state.px).shape[0])
@@ -155,7 +160,10 @@ class Simulation:
             raise ValueError("give n or state")
         cap = max(1, capacity if capacity is not None else n_bodies)
         h = _vp()
-        self._check(self.L.nbx_create(cap, C.byref(h)))
+        if no_grid_barriers:
+            self._check(self.L.nbx_create_ex(cap, CREATE_NO_GRID_BARRIERS, C.byref(h)))
+        else:
+            self._check(self.L.nbx_create(cap, C.byref(h)))
         self.h = h
         p = Params()
         self.L.nbx_default_params(C.byref(p))

@@ -44,6 +44,9 @@ def test_argument_errors_without_gpu(nbx):
     assert L.nbx_create(1, None) == 1
     assert L.nbx_destroy(None) == 0
     assert L.nbx_step(None, 1) == 1
+    assert L.nbx_step_async(None, 1) == 1
+    assert L.nbx_create_ex(16, 0x2, C.byref(C.c_void_p())) == 1  # unknown flag
+    assert b"flag" in L.nbx_last_error()
 
 
 @pytest.mark.parametrize("args", [[], ["10"], ["10", "5"], ["1", "2", "3", "4", "5"]])

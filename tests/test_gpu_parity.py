@@ -524,14 +524,29 @@ def test_contact_chains_numbered_along(nbx, oracle):
     assert {2999, 999, 256} <= set(sizes[sizes > 0].tolist())
 
 
+def test_step_async_refuses_grid_barrier_handles(nbx, monkeypatch):
+    """A handle that uses the one-launch (grid-barrier) kernels is refused by
+    nbx_step_async up front, with NBX_ERR_STATE, and stays usable."""
+    monkeypatch.delenv("NBX_SORT", raising=False)
+    with nbx.Simulation(3000) as sim:
+        with pytest.raises(nbx.NbxError) as e:
+            sim.step_async(1)
+        assert e.value.code == 4 and "NBX_CREATE_NO_GRID_BARRIERS" in str(e.value)
+        assert sim.frame == 0
+        sim.step(1)
+        assert sim.frame == 1
+
+
 def test_step_async_handles_side_by_side(nbx, monkeypatch):
-    """Several handles (created with NBX_SORT=multi, as concurrent handles
-    must be) queued together with nbx_step_async end in exactly the states
-    the same handles reach stepped one at a time with nbx_step."""
-    monkeypatch.setenv("NBX_SORT", "multi")
+    """Several handles (created with NBX_CREATE_NO_GRID_BARRIERS, as
+    concurrent handles must be) queued together with nbx_step_async end in
+    exactly the states the same handles reach stepped one at a time with
+    nbx_step."""
+    monkeypatch.delenv("NBX_SORT", raising=False)
     specs = [(3000, nbx.SCENARIO_DISC), (20000, nbx.SCENARIO_COLD_SQUARE),
              (9000, nbx.SCENARIO_DISC), (40000, nbx.SCENARIO_COLD_SQUARE)]
-    sims = [nbx.Simulation(n, seed=i, scenario=sc) for i, (n, sc) in enumerate(specs)]
+    sims = [nbx.Simulation(n, seed=i, scenario=sc, no_grid_barriers=True)
+            for i, (n, sc) in enumerate(specs)]
     try:
         starts = [sim.download() for sim in sims]
         for sim in sims:

@@ -48,6 +48,11 @@ def test_bench_json_line():
     assert d["value"] > 0 and d["higher_is_better"] is True
     assert abs(d["value"] * d["ms_per_step"] / 1e3 - 1.0) < 1e-2
     assert "workload" in d["config"] and d["config"]["n_bodies"] == 20000
+    # The metric names what was timed: the body count and the step window.
+    assert "20000 bodies" in d["metric"] and "steps 1..5" in d["metric"]
+    acc = d["accuracy"]
+    # Reference law (fastInvSqrt): its ~1.1e-3 RMS floor against FP64.
+    assert acc["targets"] == 4096 and 0 < acc["rms"] < 5e-3 and acc["max_rel"] > 0
     rf = d["roofline"]
     for k in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
         assert k in rf, k
@@ -56,3 +61,4 @@ def test_bench_json_line():
     for k in ("value", "unit", "cores", "kind", "sample"):
         assert k in cb, k
     assert cb["value"] > 0 and cb["cores"] >= 1 and cb["kind"] in ("reference", "port")
+    assert cb["steps"] == 2 and cb["cpu_model"] and cb["gpu_value_same_window"] > 0

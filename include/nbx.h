@@ -286,6 +286,20 @@ int nbx_exact_forces(nbx_sim *sim, uint64_t n_targets,
                      const uint32_t *targets, double eps2, double *ax,
                      double *ay);
 
+/* nbx_exact_forces without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (harnesses that
+ * forbid hipMalloc, tight serving or checking loops). Its partial sums,
+ * results and target list live in one statically sized device array held by
+ * each GPU until the process ends (about 8.5 MB), so at most
+ * NBX_NOALLOC_MAX_TARGETS targets per call (more: NBX_ERR_ARG). Same kernels,
+ * same results as nbx_exact_forces. Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other threads, other handles'
+ * streams) race and must be serialised by the caller. */
+#define NBX_NOALLOC_MAX_TARGETS 8192
+int nbx_exact_forces_noalloc(nbx_sim *sim, uint64_t n_targets,
+                             const uint32_t *targets, double eps2, double *ax,
+                             double *ay);
+
 #ifdef __cplusplus
 }
 #endif

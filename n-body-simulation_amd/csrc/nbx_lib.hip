@@ -23,11 +23,22 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace nbx;
 
+// Device memory of nbx_exact_forces_noalloc, whose callers cannot allocate:
+// per-(body chunk, target) FP64 partial sums, the per-target results and the
+// target list, sized for kNoAllocTargets targets. Each GPU holds its own
+// copy until the process ends; in host code the name is only a stand-in, so
+// its per-device address (resolved once per device with hipGetSymbolAddress)
+// is what the host passes to kernels and copies.
+constexpr uint32_t kNoAllocTargets = 8192;
+constexpr size_t kScratchPartials = static_cast<size_t>(kExactChunks) * kNoAllocTargets;
+constexpr size_t kScratchEntries = kScratchPartials + kNoAllocTargets + kNoAllocTargets / 4;
+static __device__ __align__(256) double2 scratch[kScratchEntries];
 
 namespace {
 
@@ -67,10 +78,31 @@ uint64_t pow2AtLeast(uint64_t v) {
   return p;
 }
 
+// Per-device address of `scratch`, resolved with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+double2 *g_scratchPtr[kMaxDevices] = {};
+
+int resolveScratch(int dev, double2 **out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    return setError(NBX_ERR_ARG, "device index out of range");
+  }
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchPtr[dev]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+      return setError(NBX_ERR_HIP, "hipGetSymbolAddress(scratch) failed");
+    }
+    g_scratchPtr[dev] = static_cast<double2 *>(ptr);
+  }
+  *out = g_scratchPtr[dev];
+  return NBX_OK;
+}
 
 } // namespace
 
 struct nbx_sim {
+  int device = 0; // the HIP device current at nbx_create
   uint64_t cap = 0;
   uint64_t n = 0;
   int cur = 0; // body buffer holding the state
@@ -860,6 +892,15 @@ int nbx_create_ex(uint64_t capacity, uint32_t flags, nbx_sim **out) {
   }
   const uint64_t n = capacity;
   int rc = NBX_OK;
+  {
+    double2 *unused = nullptr;
+    if (hipGetDevice(&s->device) != hipSuccess) {
+      return fail(setError(NBX_ERR_HIP, "hipGetDevice failed"));
+    }
+    if ((rc = resolveScratch(s->device, &unused)) != NBX_OK) {
+      return fail(rc);
+    }
+  }
 #define A(ptr, count, zero)                                                   \
   if ((rc = devAlloc(s, &(ptr), (count), (zero))) != NBX_OK)                  \
     return fail(rc);
@@ -1570,6 +1611,56 @@ int nbx_exact_forces(nbx_sim *s, uint64_t n_targets, const uint32_t *targets,
   }
   cleanup();
   return rc;
+}
+
+int nbx_exact_forces_noalloc(nbx_sim *s, uint64_t n_targets,
+                             const uint32_t *targets, double eps2, double *ax,
+                             double *ay) {
+  if (!s || (n_targets && (!targets || !ax || !ay))) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (n_targets > kNoAllocTargets) {
+    return setError(NBX_ERR_ARG, "nbx_exact_forces_noalloc takes at most " +
+                                     std::to_string(kNoAllocTargets) +
+                                     " targets");
+  }
+  if (n_targets == 0 || s->n == 0) {
+    return NBX_OK;
+  }
+  for (uint64_t t = 0; t < n_targets; ++t) {
+    if (targets[t] >= s->n) {
+      return setError(NBX_ERR_ARG, "target index out of range");
+    }
+  }
+  double2 *base = nullptr;
+  NBX_TRY(resolveScratch(s->device, &base));
+  NBX_TRY(syncAndCheck(s));
+  // Every (chunk, target) partial the reduction reads is written by the
+  // partial-sum kernel of this call, so nothing needs clearing first.
+  double2 *part = base;
+  double *dax = reinterpret_cast<double *>(base + kScratchPartials);
+  double *day = dax + kNoAllocTargets;
+  uint32_t *dT = reinterpret_cast<uint32_t *>(base + kScratchPartials + kNoAllocTargets);
+  const uint32_t nt = static_cast<uint32_t>(n_targets);
+  const int c = s->cur;
+  if (hipMemcpyAsync(dT, targets, nt * sizeof(uint32_t), hipMemcpyHostToDevice,
+                     s->stream) != hipSuccess) {
+    return setError(NBX_ERR_HIP, "target upload failed");
+  }
+  hipLaunchKernelGGL(k_exact_partial, dim3(blocksFor(nt), kExactChunks),
+                     dim3(kBlock), 0, s->stream, static_cast<uint32_t>(s->n), nt,
+                     dT, s->px[c], s->py[c], s->m[c], eps2, part);
+  hipLaunchKernelGGL(k_exact_reduce, dim3(blocksFor(nt)), dim3(kBlock), 0,
+                     s->stream, nt, part, dax, day);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(ax, dax, nt * sizeof(double), hipMemcpyDeviceToHost,
+                     s->stream) != hipSuccess ||
+      hipMemcpyAsync(ay, day, nt * sizeof(double), hipMemcpyDeviceToHost,
+                     s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess) {
+    return setError(NBX_ERR_HIP, "exact-force kernels failed");
+  }
+  return NBX_OK;
 }
 
 } // extern "C"

@@ -37,7 +37,7 @@ EXPORTS = (
     "nbx_root_quad", "nbx_download_keys", "nbx_collide_info", "nbx_download_pairs",
     "nbx_count_interactions", "nbx_set_profiling", "nbx_phase_times", "nbx_stream",
     "nbx_exact_forces", "nbx_init_scenario", "nbx_download_contacts",
-    "nbx_energy", "nbx_step_async", "nbx_create_ex",
+    "nbx_energy", "nbx_step_async", "nbx_create_ex", "nbx_exact_forces_noalloc",
 )
 
 CREATE_NO_GRID_BARRIERS = 1
@@ -112,6 +112,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_phase_times.argtypes = [_vp, _f64p, C.POINTER(C.c_uint64)]
     L.nbx_stream.argtypes = [_vp, C.POINTER(_vp)]
     L.nbx_exact_forces.argtypes = [_vp, C.c_uint64, _u32p, C.c_double, _f64p, _f64p]
+    L.nbx_exact_forces_noalloc.argtypes = [_vp, C.c_uint64, _u32p, C.c_double, _f64p,
+                                           _f64p]
     L.nbx_energy.argtypes = [_vp, C.c_double, C.POINTER(C.c_double),
                              C.POINTER(C.c_double), _f64p]
     _lib = L
@@ -361,9 +363,12 @@ class Simulation:
         return dict(kinetic=k.value, potential=u.value, total=k.value + u.value,
                     momentum=(float(p[0]), float(p[1])))
 
-    def exact_forces(self, targets, eps2: float = 1.0):
+    def exact_forces(self, targets, eps2: float = 1.0, noalloc: bool = False):
+        """FP64 direct-sum accelerations of the targets (device indices);
+        noalloc=True goes through nbx_exact_forces_noalloc."""
         t = np.ascontiguousarray(targets, np.uint32)
         ax = np.zeros(t.shape[0], np.float64)
         ay = np.zeros(t.shape[0], np.float64)
-        self._check(self.L.nbx_exact_forces(self.h, t.shape[0], t, eps2, ax, ay))
+        f = self.L.nbx_exact_forces_noalloc if noalloc else self.L.nbx_exact_forces
+        self._check(f(self.h, t.shape[0], t, eps2, ax, ay))
         return ax, ay

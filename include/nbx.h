@@ -47,6 +47,12 @@ extern "C" {
 #define NBX_LAW_REFERENCE 0
 #define NBX_LAW_ACCURATE 1
 #define NBX_LAW_ACCURATE_QUAD 2
+/* NBX_LAW_LEARNED (no reference counterpart; compensated summation only):
+ * the accurate law, where every far node of positive mass inside the
+ * network's trained range adds its monopole plus the residual predicted by
+ * the learned far-field network (nbx_set_mlp), and every far node outside
+ * that range is opened instead, as a finer theta would open it. */
+#define NBX_LAW_LEARNED 3
 
 /* Per-body force summation: bit-exact replay of the reference's FP32 order
  * (src/QuadTree.cpp:264-389, four interleaved leaf sums), or FP32 with
@@ -299,6 +305,85 @@ int nbx_exact_forces(nbx_sim *sim, uint64_t n_targets,
 int nbx_exact_forces_noalloc(nbx_sim *sim, uint64_t n_targets,
                              const uint32_t *targets, double eps2, double *ax,
                              double *ay);
+
+/* ---- Learned far-field correction (no reference counterpart). ----
+ *
+ * A network 10 -> 64 -> 64 -> 64 -> 2 with GELU (tanh form) after each
+ * hidden layer, BF16 weights and activations with FP32 accumulation on the
+ * matrix cores, FP32 master weights trained with Adam on the GPU. For a far
+ * node (mass M > 0, centre of mass c, size s, second mass moments S about c)
+ * and a target body at x it predicts the residual between the node's
+ * monopole (the far term of QuadTree.cpp:276-296) and the exact force of
+ * the node's bodies (the law of CheckCuda.cu:456-475). Inputs, with
+ * d = c - x, r = |d|, w = 1/(r^2 + eps^2), q = S/M, k = 8 w:
+ *   log2(M)/16, d.x sqrt(w), d.y sqrt(w), s/r, k q_xx, k q_xy, k q_yy,
+ *   k d.q.d/r^2 (radial), k (d x q d)/r^2 (mixed), k tr(q) - radial;
+ * output: the residual times (r^2 + eps^2) / M * 16. Trained range: s <= 1.2
+ * r and the target at least s/4 outside the node's square.
+ *
+ * Parameters (NBX_MLP_PARAMS floats, row-major [out][in]): W1 (64x10), b1,
+ * W2 (64x64), b2, W3 (64x64), b3, W4 (2x64), b4. */
+#define NBX_MLP_PARAMS 9154
+typedef struct nbx_mlp nbx_mlp;
+
+/* A network handle with room for `pair_capacity` training pairs (48 bytes
+ * each) in device memory. */
+int nbx_mlp_create(uint64_t pair_capacity, nbx_mlp **out);
+int nbx_mlp_destroy(nbx_mlp *mlp);
+
+/* Master parameters in and out; setting them resets the Adam state. */
+int nbx_mlp_set_params(nbx_mlp *mlp, const float *params);
+int nbx_mlp_get_params(nbx_mlp *mlp, float *params);
+
+/* Training set: number of pairs held, emptied by nbx_mlp_clear. */
+int nbx_mlp_count(nbx_mlp *mlp, uint64_t *count);
+int nbx_mlp_clear(nbx_mlp *mlp);
+
+/* Appends the training pairs of the simulation's last tree (built with
+ * node moments: law NBX_LAW_ACCURATE_QUAD or NBX_LAW_LEARNED): for each
+ * listed target body (device index), every node of positive mass that a
+ * walk at `theta` accepts inside the trained range (nodes outside it are
+ * opened); inputs as the walk computes them, the target residual in FP64
+ * over the node's bodies. *added (may be null) gets the number appended;
+ * NBX_ERR_CAPACITY if the set overflowed (the excess is dropped). At most
+ * 2^22 targets per call. */
+int nbx_mlp_generate(nbx_mlp *mlp, nbx_sim *sim, float theta,
+                     uint64_t n_targets, const uint32_t *targets,
+                     uint64_t *added);
+
+/* The (target body, node) pairs the last nbx_mlp_generate appended, in the
+ * set's order (device indices and node indices of that tree): up to cap
+ * written, *count = how many it appended. For checks. */
+int nbx_mlp_last_pairs(nbx_mlp *mlp, uint64_t cap, uint32_t *target,
+                       int32_t *node, uint64_t *count);
+
+/* Replaces the training set with host pairs (inputs count x 10, targets
+ * count x 2), or reads pairs [first, first + count) back. */
+int nbx_mlp_upload(nbx_mlp *mlp, uint64_t count, const float *features,
+                   const float *labels);
+int nbx_mlp_download(nbx_mlp *mlp, uint64_t first, uint64_t count,
+                     float *features, float *labels);
+
+/* Loss (mean over pairs of the summed squared output error) and its
+ * gradient (NBX_MLP_PARAMS floats) over pairs [first, first + count) of the
+ * set, without changing the parameters: the training kernels' backward
+ * pass, for checks. */
+int nbx_mlp_gradient(nbx_mlp *mlp, uint64_t first, uint64_t count,
+                     float *grad, float *loss);
+
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8, learning rate lr) on
+ * batches of `batch` pairs taken in the order of a seeded permutation of
+ * the set; losses (may be null) gets each step's batch loss. */
+int nbx_mlp_train(nbx_mlp *mlp, uint32_t steps, uint32_t batch, float lr,
+                  uint64_t seed, float *losses);
+
+/* Network outputs (count x 2) for host inputs (count x 10). */
+int nbx_mlp_predict(nbx_mlp *mlp, uint64_t count, const float *features,
+                    float *out);
+
+/* Copies the network's current weights into the simulation, for
+ * NBX_LAW_LEARNED (later changes to the network need another call). */
+int nbx_set_mlp(nbx_sim *sim, nbx_mlp *mlp);
 
 #ifdef __cplusplus
 }

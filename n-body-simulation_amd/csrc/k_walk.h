@@ -16,6 +16,7 @@
 // hardware reciprocal and Kahan accumulation.
 #pragma once
 
+#include "k_mlp.h"
 #include "nbx_device.h"
 
 namespace nbx {
@@ -25,6 +26,7 @@ constexpr int kSumCompensated = 1;
 constexpr int kLawReference = 0;
 constexpr int kLawAccurate = 1;
 constexpr int kLawQuadrupole = 2; // accurate law + far-node quadrupole term
+constexpr int kLawLearned = 3;    // accurate law + learned far-node residual
 
 template <int SUM, int LAW>
 __device__ inline float lawFactor(float mass, float dSq, float eSq) {
@@ -398,6 +400,92 @@ __device__ inline void leafSpread(uint64_t near, uint32_t k, uint32_t logK,
   }
 }
 
+// ---- Learned far-field correction inside the walk (kLawLearned). ----
+// A far node inside the trained range adds its monopole at once and queues
+// (features, lane) in this wave's LDS queue; every 32 queued pairs the wave
+// runs the network on the matrix cores (mlpForward, weights staged in LDS)
+// and adds each output, scaled by the node's M / (r^2 + eps^2), to its
+// lane's correction sum. Far nodes outside the range are opened instead.
+constexpr int kMlpQueueCap = 96; // < 32 pending + up to 64 new
+struct MlpQueue {
+  bf16x8 feat[kMlpQueueCap][2];
+  float scale[kMlpQueueCap];
+  int owner[kMlpQueueCap];
+  float2 corr[kWave];
+};
+
+struct LearnedArgs {
+  const float4 *__restrict__ nodeL = nullptr;
+  const float *__restrict__ centerX = nullptr;
+  const float *__restrict__ centerY = nullptr;
+  const float *__restrict__ size = nullptr;
+  MlpQueue *q = nullptr;
+  const bf16x8 *pk = nullptr; // forward operands (LDS)
+  const float *bc = nullptr;  // biases (LDS)
+};
+
+// Runs the network on queue entries 0 .. n-1 (n <= 32) and adds the scaled
+// outputs to their lanes' correction sums. Called by the whole wave.
+__device__ __forceinline__ void mlpFlush(const LearnedArgs &la, int n) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t j = lane & 31, h = lane >> 5;
+  bf16x8 x = {};
+  if (static_cast<int>(j) < n) {
+    x = la.q->feat[j][h];
+  }
+  const f32x16 y = mlpForward<false>(la.pk, la.bc, x, lane);
+  if (h == 0 && static_cast<int>(j) < n) {
+    const int o = la.q->owner[j];
+    const float s = la.q->scale[j];
+    atomicAdd(&la.q->corr[o].x, s * y[0]);
+    atomicAdd(&la.q->corr[o].y, s * y[1]);
+  }
+}
+
+// Queues the pairs of `farMask` (each lane its own features) and runs full
+// tiles of 32. qn: the wave-uniform queue length, < 32 on entry and exit.
+__device__ __forceinline__ void mlpEnqueue(const LearnedArgs &la, int &qn,
+                                           uint64_t farMask, bf16x8 x0,
+                                           bf16x8 x1, float scale) {
+  const uint32_t cnt = static_cast<uint32_t>(__popcll(farMask));
+  if (cnt == 0) {
+    return;
+  }
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  if (__builtin_amdgcn_inverse_ballot_w64(farMask)) {
+    const int slot = qn + __popcll(farMask & laneMaskLt());
+    la.q->feat[slot][0] = x0;
+    la.q->feat[slot][1] = x1;
+    la.q->scale[slot] = scale;
+    la.q->owner[slot] = static_cast<int>(lane);
+  }
+  qn += static_cast<int>(cnt);
+  while (qn >= kMlpTile) {
+    ldsFence();
+    mlpFlush(la, kMlpTile);
+    const int rem = qn - kMlpTile;
+    bf16x8 a0 = {}, a1 = {};
+    float sc = 0.0f;
+    int ow = 0;
+    ldsFence();
+    if (static_cast<int>(lane) < rem) {
+      a0 = la.q->feat[kMlpTile + lane][0];
+      a1 = la.q->feat[kMlpTile + lane][1];
+      sc = la.q->scale[kMlpTile + lane];
+      ow = la.q->owner[kMlpTile + lane];
+    }
+    ldsFence();
+    if (static_cast<int>(lane) < rem) {
+      la.q->feat[lane][0] = a0;
+      la.q->feat[lane][1] = a1;
+      la.q->scale[lane] = sc;
+      la.q->owner[lane] = ow;
+    }
+    ldsFence();
+    qn = rem;
+  }
+}
+
 // One node for the lanes of `mask`. Lane membership is carried as wave
 // masks (inverse ballots), so the branches below cost scalar instructions
 // only. The opening test is rounded exactly as the reference rounds it
@@ -411,21 +499,56 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
                                      const float4 *__restrict__ body,
                                      float2 *__restrict__ slot, float &qx,
                                      float &qy, uint32_t &nFar,
-                                     uint32_t &nPair, uint32_t &nLeafTerm) {
+                                     uint32_t &nPair, uint32_t &nLeafTerm,
+                                     const float4 &geo = float4{},
+                                     const LearnedArgs &la = LearnedArgs{},
+                                     int *qn = nullptr) {
   const float dx = f.x - x;
   const float dy = f.y - y;
   const float dSq = dx * dx + dy * dy;
-  const uint64_t near = mask & ~__builtin_amdgcn_ballot_w64(f.w < dSq * tSq);
+  uint64_t near = mask & ~__builtin_amdgcn_ballot_w64(f.w < dSq * tSq);
   // f.z > 0 on the scalar unit: positive floats, +inf included, are exactly
   // the bit patterns in (0, 0x7f800000]. MASS = false (every body mass
   // positive and finite) skips the test: a non-empty node's mass is then in
   // (0, inf], and an empty node's far term is exactly 0 (its mass is 0, and
   // a far lane has dSq > 0, so the inverse root and reciprocal are finite).
   const int32_t mb = __float_as_int(f.z);
-  const uint64_t farMask = (!MASS || (mb > 0 && mb <= 0x7f800000))
-                               ? (mask & ~near)
-                               : 0ull;
-  if (__builtin_amdgcn_inverse_ballot_w64(farMask)) {
+  uint64_t farMask = (!MASS || (mb > 0 && mb <= 0x7f800000))
+                         ? (mask & ~near)
+                         : 0ull;
+  if constexpr (LAW == kLawLearned) {
+    // Only nodes of positive mass take the network; far lanes outside the
+    // trained range open the node instead (geo = {square centre, size}).
+    if (!(mb > 0 && mb <= 0x7f800000)) {
+      farMask = 0ull;
+    } else if (farMask) {
+      const uint64_t inR =
+          farMask & __builtin_amdgcn_ballot_w64(
+                        mlpInRange(x, y, geo.x, geo.y, geo.z, dSq));
+      near |= farMask & ~inR;
+      farMask = inR;
+    }
+  }
+  if constexpr (LAW == kLawLearned) {
+    bf16x8 x0 = {}, x1 = {};
+    float scale = 0.0f;
+    if (__builtin_amdgcn_inverse_ballot_w64(farMask)) {
+      const float invR = __builtin_amdgcn_rsqf(dSq);
+      const float w = __builtin_amdgcn_rcpf(dSq + eSq);
+      const float fac = capMax(f.z * invR * w);
+      qx = fmaf(dx, fac, qx);
+      qy = fmaf(dy, fac, qy);
+      float ft[kMlpIn];
+      mlpFeatures(dx, dy, dSq, eSq, geo.z, sm, ft);
+      x0 = mlpInput(ft, 0);
+      x1 = mlpInput(ft, 1);
+      scale = f.z * w * (1.0f / kMlpOutScale);
+      if (COUNT) {
+        ++nFar;
+      }
+    }
+    mlpEnqueue(la, *qn, farMask, x0, x1, scale);
+  } else if (__builtin_amdgcn_inverse_ballot_w64(farMask)) {
     if constexpr (LAW == kLawQuadrupole) {
       const float invR = __builtin_amdgcn_rsqf(dSq);
       const float w = __builtin_amdgcn_rcpf(dSq + eSq);
@@ -506,8 +629,11 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
                                 const float4 *__restrict__ body, float tSq,
                                 float eSq, float *__restrict__ ax,
                                 float *__restrict__ ay, Ctl *__restrict__ ctl,
-                                STACK &st, float2 *__restrict__ slot) {
+                                STACK &st, float2 *__restrict__ slot,
+                                const LearnedArgs &la = LearnedArgs{}) {
   constexpr bool kSpill = sizeof(STACK) > sizeof(RegStack);
+  constexpr bool kMoments = LAW == kLawQuadrupole || LAW == kLawLearned;
+  int qn = 0; // learned law: queued pairs
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t i = base + lane;
   const bool valid = i < n;
@@ -519,15 +645,23 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
   const int nodeCount = ctl->nodeCount;
   const uint32_t cbLimit =
       nodeCount >= 5 ? static_cast<uint32_t>(nodeCount - 4) : 0u;
+  if constexpr (LAW == kLawLearned) {
+    la.q->corr[lane] = make_float2(0.0f, 0.0f);
+  }
 
   // The root on its own, then blocks of four siblings from the stack.
   int sp = 0;
   {
     const uint64_t all = __builtin_amdgcn_ballot_w64(valid);
-    const float4 q0 = LAW == kLawQuadrupole ? nodeQ[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 q0 = kMoments ? nodeQ[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (LAW == kLawLearned) {
+      g0 = make_float4(la.centerX[0], la.centerY[0], la.size[0], 0.0f);
+    }
     const uint64_t open =
         visitNode<LAW, COUNT, CAP, MASS>(nodeF[0], nodeI[0], q0, all, x, y, tSq, eSq,
-                                   body, slot, qx, qy, nFar, nPair, nLeafTerm);
+                                   body, slot, qx, qy, nFar, nPair, nLeafTerm,
+                                   g0, la, &qn);
     nVisit += 1;
     if (open) {
       st.put(0, nodeI[0].x, open);
@@ -567,17 +701,21 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
                  "s"(bi[1].x), "s"(bi[1].y), "s"(bi[1].z), "s"(bi[1].w),
                  "s"(bi[2].x), "s"(bi[2].y), "s"(bi[2].z), "s"(bi[2].w),
                  "s"(bi[3].x), "s"(bi[3].y), "s"(bi[3].z), "s"(bi[3].w));
-    float4 bq[4];
+    float4 bq[4], bg[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      bq[q] = LAW == kLawQuadrupole ? nodeQ[cb + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+      bq[q] = kMoments ? nodeQ[cb + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+      bg[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if constexpr (LAW == kLawLearned) {
+        bg[q] = make_float4(la.centerX[cb + q], la.centerY[cb + q], la.size[cb + q], 0.0f);
+      }
     }
     uint64_t open[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       open[q] = visitNode<LAW, COUNT, CAP, MASS>(bf[q], bi[q], bq[q], mask, x, y, tSq,
                                            eSq, body, slot, qx, qy, nFar, nPair,
-                                           nLeafTerm);
+                                           nLeafTerm, bg[q], la, &qn);
     }
     if (COUNT) {
       nVisit += 4;
@@ -606,7 +744,18 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
   }
   kahanAdd(sx, cx, qx);
   kahanAdd(sy, cy, qy);
-  if (valid) {
+  if constexpr (LAW == kLawLearned) {
+    if (qn > 0) {
+      ldsFence();
+      mlpFlush(la, qn);
+    }
+    ldsFence();
+    const float2 corr = la.q->corr[lane];
+    if (valid) {
+      ax[i] = (sx - cx) + corr.x;
+      ay[i] = (sy - cy) + corr.y;
+    }
+  } else if (valid) {
     ax[i] = sx - cx;
     ay[i] = sy - cy;
   }
@@ -656,6 +805,64 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
     st.spillMask = spillMask[w];
     walkWave<LAW, COUNT, CAP, MASS>(n, base, nodeF, nodeI, nodeQ, body, tSq, eSq, ax, ay,
                               ctl, st, slots[w]);
+  }
+}
+
+// The learned-law walk: k_walk_fast's traversal with the network evaluated
+// for in-range far nodes (nodeL = {Sxx/M, Sxy/M, Syy/M, log2(M)/16} per
+// node; pkG / bcG the packed forward weights and biases, staged in LDS).
+constexpr int kWalkLearnedBlock = 256;
+template <bool COUNT, bool MASS>
+__global__ __launch_bounds__(kWalkLearnedBlock) void k_walk_learned(
+    uint32_t n, const float4 *__restrict__ nodeF,
+    const int4 *__restrict__ nodeI, const float4 *__restrict__ nodeL,
+    const float *__restrict__ centerX, const float *__restrict__ centerY,
+    const float *__restrict__ size, const float4 *__restrict__ body,
+    const bf16x8 *__restrict__ pkG, const float *__restrict__ bcG, float tSq,
+    float eSq, float *__restrict__ ax, float *__restrict__ ay,
+    Ctl *__restrict__ ctl) {
+  constexpr int W = kWalkLearnedBlock / kWave;
+  __shared__ bf16x8 pk[kPkFwd];
+  __shared__ float bc[kBcAll];
+  __shared__ MlpQueue queues[W];
+  __shared__ float2 slots[W][kWave];
+  __shared__ int spillNode[W][kStackDepth - kWave];
+  __shared__ uint64_t spillMask[W][kStackDepth - kWave];
+  for (int k = threadIdx.x; k < kPkFwd; k += kWalkLearnedBlock) {
+    pk[k] = pkG[k];
+  }
+  for (int k = threadIdx.x; k < kBcAll; k += kWalkLearnedBlock) {
+    bc[k] = bcG[k];
+  }
+  __syncthreads();
+  if (barrierStalled(ctl)) {
+    return;
+  }
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t base =
+      xcdBlock() * kWalkLearnedBlock + (threadIdx.x & ~(kWave - 1));
+  if (base >= n ||
+      (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent))) {
+    return;
+  }
+  LearnedArgs la;
+  la.nodeL = nodeL;
+  la.centerX = centerX;
+  la.centerY = centerY;
+  la.size = size;
+  la.q = &queues[w];
+  la.pk = pk;
+  la.bc = bc;
+  if (ctl->levels <= kRegStackLevels) {
+    RegStack st;
+    walkWave<kLawLearned, COUNT, true, MASS>(n, base, nodeF, nodeI, nodeL, body, tSq,
+                                             eSq, ax, ay, ctl, st, slots[w], la);
+  } else {
+    LaneStack st;
+    st.spillNode = spillNode[w];
+    st.spillMask = spillMask[w];
+    walkWave<kLawLearned, COUNT, true, MASS>(n, base, nodeF, nodeI, nodeL, body, tSq,
+                                             eSq, ax, ay, ctl, st, slots[w], la);
   }
 }
 

@@ -152,6 +152,10 @@ struct nbx_sim {
   int32_t *splitList = nullptr, *nodeOfSplit = nullptr;
   float4 *nodeF = nullptr;
   float4 *nodeQ = nullptr; // second mass moments about the COM (quadrupole law)
+  float4 *nodeL = nullptr; // {Sxx/M, Sxy/M, Syy/M, log2(M)/16} (learned law)
+  bf16x8 *mlpPk = nullptr; // packed forward weights of the learned law
+  float *mlpBc = nullptr;  // its biases, accumulator order
+  bool mlpLoaded = false;  // nbx_set_mlp has been called
   int4 *nodeI = nullptr;
   float *centerX = nullptr, *centerY = nullptr, *size = nullptr;
   float2 *wsum = nullptr;
@@ -162,6 +166,7 @@ struct nbx_sim {
   uint4 *bboxParts = nullptr;
   bool treeBuilt = false;
   bool momentsBuilt = false; // the last build computed nodeQ
+  bool learnBuilt = false;   // ... and nodeL
   bool collided = false;
 
   Ctl *ctl = nullptr;
@@ -529,8 +534,11 @@ int launchBuild(nbx_sim *s) {
   LAUNCH(k_nodes, nodeGrid, kBlock, n, hi, s->keyLo, perm, s->ctl,
          s->splitList, s->nodeF, s->nodeI, s->centerX, s->centerY, s->size,
          s->nodeOfSplit);
-  // Second mass moments only for the quadrupole law (null: not computed).
-  float4 *nodeQ = s->params.law == NBX_LAW_ACCURATE_QUAD ? s->nodeQ : nullptr;
+  // Second mass moments only for the quadrupole and learned laws (null: not
+  // computed).
+  const bool learned = s->params.law == NBX_LAW_LEARNED;
+  float4 *nodeQ =
+      (s->params.law == NBX_LAW_ACCURATE_QUAD || learned) ? s->nodeQ : nullptr;
   LAUNCH(k_node_finish, nodeGrid, kBlock, s->ctl, s->px[o], s->py[o], s->m[o],
          s->nodeOfSplit, s->nodeF, s->nodeI, s->wsum, s->nodeDone,
          s->subtreeRoots, nodeQ);
@@ -539,8 +547,13 @@ int launchBuild(nbx_sim *s) {
          s->nodeDone, nodeQ);
   LAUNCH(k_upward_top, 1, 1024, s->ctl, s->nodeF, s->nodeI, s->wsum,
          s->nodeDone, s->bigNodes, nodeQ);
+  if (learned) {
+    LAUNCH(k_node_learn, blocksFor(s->nodeCap), kBlock, s->nodeF, s->nodeQ,
+           s->ctl, s->nodeL);
+  }
   s->treeBuilt = true;
   s->momentsBuilt = nodeQ != nullptr;
+  s->learnBuilt = learned;
   return NBX_OK;
 }
 
@@ -556,6 +569,12 @@ int launchForces(nbx_sim *s, bool count) {
     return setError(NBX_ERR_STATE,
                     "the quadrupole law needs a tree built under it "
                     "(nbx_build, nbx_attract or nbx_step)");
+  }
+  if (s->params.law == NBX_LAW_LEARNED && (!s->learnBuilt || !s->mlpLoaded)) {
+    return setError(NBX_ERR_STATE,
+                    s->mlpLoaded ? "the learned law needs a tree built under it "
+                                   "(nbx_build, nbx_attract or nbx_step)"
+                                 : "the learned law needs weights (nbx_set_mlp)");
   }
   const int c = s->cur;
   const float tSq = s->params.theta * s->params.theta;
@@ -598,7 +617,16 @@ int launchForces(nbx_sim *s, bool count) {
       FAST_M(LAWV, CNT, true);                                                \
     }                                                                         \
   } while (0)
-  if (ref) {
+#define LEARNED(CNT, MASSV)                                                   \
+  LAUNCH((k_walk_learned<CNT, MASSV>), blocksFor(n, kWalkLearnedBlock),       \
+         kWalkLearnedBlock, n, s->nodeF, s->nodeI, s->nodeL, s->centerX,      \
+         s->centerY, s->size, s->bodyPack, s->mlpPk, s->mlpBc, tSq, eSq,      \
+         s->ax, s->ay, s->ctl)
+  if (s->params.law == NBX_LAW_LEARNED) {
+    if (count) LEARNED(true, true);
+    else if (s->massPositive) LEARNED(false, false);
+    else LEARNED(false, true);
+  } else if (ref) {
     if (count) {
       if (acc) WALK(kSumReference, kLawAccurate, true);
       else WALK(kSumReference, kLawReference, true);
@@ -617,6 +645,7 @@ int launchForces(nbx_sim *s, bool count) {
       else FAST(kLawReference, false);
     }
   }
+#undef LEARNED
 #undef FAST
 #undef FAST_M
 #undef WALK
@@ -822,12 +851,13 @@ int validParams(const nbx_params *p) {
     return setError(NBX_ERR_ARG, "dt/theta/epsilon must be numbers, theta >= 0");
   }
   if (p->law != NBX_LAW_REFERENCE && p->law != NBX_LAW_ACCURATE &&
-      p->law != NBX_LAW_ACCURATE_QUAD) {
+      p->law != NBX_LAW_ACCURATE_QUAD && p->law != NBX_LAW_LEARNED) {
     return setError(NBX_ERR_ARG, "unknown force law");
   }
-  if (p->law == NBX_LAW_ACCURATE_QUAD && p->summation != NBX_SUM_COMPENSATED) {
+  if ((p->law == NBX_LAW_ACCURATE_QUAD || p->law == NBX_LAW_LEARNED) &&
+      p->summation != NBX_SUM_COMPENSATED) {
     return setError(NBX_ERR_ARG,
-                    "the quadrupole law needs compensated summation");
+                    "the quadrupole and learned laws need compensated summation");
   }
   if (p->summation != NBX_SUM_REFERENCE &&
       p->summation != NBX_SUM_COMPENSATED) {
@@ -962,6 +992,9 @@ int nbx_create_ex(uint64_t capacity, uint32_t flags, nbx_sim **out) {
   A(s->nodeOfSplit, s->nodeCap / 4 + 1, false)
   A(s->nodeF, s->nodeCap, false) A(s->nodeI, s->nodeCap, false)
   A(s->nodeQ, s->nodeCap, true)
+  A(s->nodeL, s->nodeCap, true)
+  A(s->mlpPk, kPkFwd, true)
+  A(s->mlpBc, kBcAll, true)
   A(s->centerX, s->nodeCap, false) A(s->centerY, s->nodeCap, false)
   A(s->size, s->nodeCap, false) A(s->wsum, s->nodeCap, false)
   A(s->bigNodes, s->nodeCap / 4 + 1, false)
@@ -1660,6 +1693,468 @@ int nbx_exact_forces_noalloc(nbx_sim *s, uint64_t n_targets,
       hipStreamSynchronize(s->stream) != hipSuccess) {
     return setError(NBX_ERR_HIP, "exact-force kernels failed");
   }
+  return NBX_OK;
+}
+
+} // extern "C"
+
+// ---- The learned far-field correction: a network handle (weights, Adam
+//      state and a training set in HBM) and its training kernels. ----
+
+struct nbx_mlp {
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  uint64_t cap = 0;   // training pairs the set can hold
+  uint64_t count = 0; // pairs held
+  float4 *data = nullptr; // 3 float4 per pair: {f0..f3} {f4..f7} {f8 f9 t0 t1}
+  float *params = nullptr, *adamM = nullptr, *adamV = nullptr, *grad = nullptr;
+  bf16x8 *pk = nullptr; // packed operands (forward + transposed)
+  float *bc = nullptr;  // biases, accumulator order
+  float *partial = nullptr; // per-workgroup gradient sums
+  float *loss = nullptr;    // per-step losses of the last nbx_mlp_train
+  uint32_t lossCap = 0;
+  uint64_t adamStep = 0;
+  bool hasParams = false;
+  int2 *pairs = nullptr;    // generation scratch
+  uint32_t pairCap = 0;
+  uint32_t *pairCount = nullptr;
+  uint32_t *targets = nullptr;
+  uint32_t targetCap = 0;
+  uint32_t lastTake = 0;    // pairs of the last generate call (in `pairs`)
+  float *io = nullptr;      // staging of predict inputs/outputs
+  uint64_t ioCap = 0;
+};
+
+namespace {
+
+constexpr uint32_t kMlpGroups = 256;        // workgroups of a training step
+constexpr uint32_t kMlpLossCap = 1u << 20;  // steps per nbx_mlp_train call
+constexpr uint32_t kMlpPairScratch = 1u << 24;
+constexpr uint32_t kMlpTargetScratch = 1u << 22;
+
+template <class T>
+int mlpAlloc(nbx_mlp *m, T **p, uint64_t count) {
+  void *q = nullptr;
+  const size_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+  if (hipMalloc(&q, bytes) != hipSuccess) {
+    return setError(NBX_ERR_HIP, "hipMalloc of " + std::to_string(bytes) +
+                                     " bytes failed");
+  }
+  m->allocs.push_back(q);
+  if (hipMemset(q, 0, bytes) != hipSuccess) {
+    return setError(NBX_ERR_HIP, "hipMemset failed");
+  }
+  *p = static_cast<T *>(q);
+  return NBX_OK;
+}
+
+int mlpPack(nbx_mlp *m) {
+  hipLaunchKernelGGL(k_mlp_pack, dim3(blocksFor(kPkAll * 8)), dim3(kBlock), 0,
+                     m->stream, m->params,
+                     reinterpret_cast<__bf16 *>(m->pk), m->bc);
+  return checkLaunch();
+}
+
+uint64_t gcd64(uint64_t a, uint64_t b) {
+  while (b) {
+    const uint64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// One gradient pass over `batch` positions of the permutation (mulA, addC)
+// starting at k0, summed into m->partial.
+int mlpGradLaunch(nbx_mlp *m, uint64_t mulA, uint64_t addC, uint64_t k0,
+                  uint32_t batch) {
+  const uint32_t per =
+      ((batch + kMlpGroups - 1) / kMlpGroups + kMlpTile - 1) / kMlpTile * kMlpTile;
+  const uint32_t groups = (batch + per - 1) / per;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(groups), dim3(kMlpTrainBlock), 0,
+                     m->stream, m->pk, m->bc, m->data, m->count, mulA, addC,
+                     k0, batch, per, m->partial);
+  return checkLaunch();
+}
+
+uint32_t mlpGroups(uint32_t batch) {
+  const uint32_t per =
+      ((batch + kMlpGroups - 1) / kMlpGroups + kMlpTile - 1) / kMlpTile * kMlpTile;
+  return (batch + per - 1) / per;
+}
+
+} // namespace
+
+extern "C" {
+
+int nbx_mlp_create(uint64_t pair_capacity, nbx_mlp **out) {
+  if (!out) {
+    return setError(NBX_ERR_ARG, "out is null");
+  }
+  *out = nullptr;
+  if (pair_capacity == 0 || pair_capacity > (1ull << 31)) {
+    return setError(NBX_ERR_ARG, "pair capacity must be in [1, 2^31]");
+  }
+  nbx_mlp *m = new nbx_mlp();
+  auto fail = [&](int rc) {
+    nbx_mlp_destroy(m);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete m;
+    return setError(NBX_ERR_HIP, "hipStreamCreateWithFlags failed");
+  }
+  m->cap = pair_capacity;
+  m->pairCap = kMlpPairScratch;
+  m->targetCap = kMlpTargetScratch;
+  m->lossCap = kMlpLossCap;
+  int rc = NBX_OK;
+  if ((rc = mlpAlloc(m, &m->data, 3 * pair_capacity)) ||
+      (rc = mlpAlloc(m, &m->params, kMlpParams)) ||
+      (rc = mlpAlloc(m, &m->adamM, kMlpParams)) ||
+      (rc = mlpAlloc(m, &m->adamV, kMlpParams)) ||
+      (rc = mlpAlloc(m, &m->grad, kMlpParams)) ||
+      (rc = mlpAlloc(m, &m->pk, kPkAll)) || (rc = mlpAlloc(m, &m->bc, kBcAll)) ||
+      (rc = mlpAlloc(m, &m->partial,
+                     static_cast<uint64_t>(kMlpGroups) * (kMlpParams + 1))) ||
+      (rc = mlpAlloc(m, &m->loss, m->lossCap)) ||
+      (rc = mlpAlloc(m, &m->pairs, m->pairCap)) ||
+      (rc = mlpAlloc(m, &m->pairCount, 1)) ||
+      (rc = mlpAlloc(m, &m->targets, m->targetCap))) {
+    return fail(rc);
+  }
+  *out = m;
+  return NBX_OK;
+}
+
+int nbx_mlp_destroy(nbx_mlp *m) {
+  if (!m) {
+    return NBX_OK;
+  }
+  if (m->stream) {
+    (void)hipStreamSynchronize(m->stream);
+  }
+  for (void *p : m->allocs) {
+    (void)hipFree(p);
+  }
+  if (m->stream) {
+    (void)hipStreamDestroy(m->stream);
+  }
+  delete m;
+  return NBX_OK;
+}
+
+int nbx_mlp_set_params(nbx_mlp *m, const float *params) {
+  if (!m || !params) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  for (int i = 0; i < kMlpParams; ++i) {
+    if (!std::isfinite(params[i])) {
+      return setError(NBX_ERR_ARG, "parameters must be finite");
+    }
+  }
+  NBX_HIP(hipMemcpyAsync(m->params, params, kMlpParams * sizeof(float),
+                         hipMemcpyHostToDevice, m->stream));
+  NBX_HIP(hipMemsetAsync(m->adamM, 0, kMlpParams * sizeof(float), m->stream));
+  NBX_HIP(hipMemsetAsync(m->adamV, 0, kMlpParams * sizeof(float), m->stream));
+  m->adamStep = 0;
+  NBX_TRY(mlpPack(m));
+  NBX_HIP(hipStreamSynchronize(m->stream));
+  m->hasParams = true;
+  return NBX_OK;
+}
+
+int nbx_mlp_get_params(nbx_mlp *m, float *params) {
+  if (!m || !params) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  NBX_HIP(hipMemcpyAsync(params, m->params, kMlpParams * sizeof(float),
+                         hipMemcpyDeviceToHost, m->stream));
+  NBX_HIP(hipStreamSynchronize(m->stream));
+  return NBX_OK;
+}
+
+int nbx_mlp_clear(nbx_mlp *m) {
+  if (!m) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  m->count = 0;
+  return NBX_OK;
+}
+
+int nbx_mlp_count(nbx_mlp *m, uint64_t *count) {
+  if (!m || !count) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  *count = m->count;
+  return NBX_OK;
+}
+
+int nbx_mlp_generate(nbx_mlp *m, nbx_sim *s, float theta, uint64_t n_targets,
+                     const uint32_t *targets, uint64_t *added) {
+  if (!m || !s || (n_targets && !targets)) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (added) {
+    *added = 0;
+  }
+  if (!(theta > 0.0f)) {
+    return setError(NBX_ERR_ARG, "theta must be > 0");
+  }
+  if (n_targets > m->targetCap) {
+    return setError(NBX_ERR_ARG, "at most " + std::to_string(m->targetCap) +
+                                     " targets per call");
+  }
+  for (uint64_t t = 0; t < n_targets; ++t) {
+    if (targets[t] >= s->n) {
+      return setError(NBX_ERR_ARG, "target index out of range");
+    }
+  }
+  if (!s->treeBuilt || !s->momentsBuilt) {
+    return setError(NBX_ERR_STATE,
+                    "training pairs need a tree built with node moments (law "
+                    "NBX_LAW_ACCURATE_QUAD or NBX_LAW_LEARNED)");
+  }
+  if (n_targets == 0) {
+    return NBX_OK;
+  }
+  NBX_TRY(syncAndCheck(s));
+  const uint32_t nt = static_cast<uint32_t>(n_targets);
+  NBX_HIP(hipMemcpyAsync(m->targets, targets, nt * sizeof(uint32_t),
+                         hipMemcpyHostToDevice, s->stream));
+  NBX_HIP(hipMemsetAsync(m->pairCount, 0, sizeof(uint32_t), s->stream));
+  if (!s->learnBuilt) {
+    hipLaunchKernelGGL(k_node_learn, dim3(blocksFor(s->nodeCap)), dim3(kBlock), 0,
+                       s->stream, s->nodeF, s->nodeQ, s->ctl, s->nodeL);
+    NBX_TRY(checkLaunch());
+  }
+  hipLaunchKernelGGL(k_mlp_gen_pairs, dim3(blocksFor(nt)), dim3(kBlock), 0,
+                     s->stream, nt, m->targets, s->bodyPack, s->nodeF, s->nodeI,
+                     s->centerX, s->centerY, s->size, s->ctl, theta * theta,
+                     m->pairs, m->pairCap, m->pairCount);
+  NBX_TRY(checkLaunch());
+  uint32_t found = 0;
+  NBX_HIP(hipMemcpyAsync(&found, m->pairCount, sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, s->stream));
+  NBX_HIP(hipStreamSynchronize(s->stream));
+  const uint64_t room = m->cap - m->count;
+  const uint32_t take = static_cast<uint32_t>(
+      std::min<uint64_t>(std::min<uint64_t>(found, m->pairCap), room));
+  if (take > 0) {
+    const float eSq = s->params.epsilon * s->params.epsilon;
+    hipLaunchKernelGGL(k_mlp_label, dim3(blocksFor(static_cast<uint64_t>(take) * 64)),
+                       dim3(kBlock), 0, s->stream, take, m->pairs, s->bodyPack,
+                       s->nodeF, s->nodeI, s->nodeL, s->size, eSq,
+                       static_cast<double>(eSq), m->data + 3 * m->count);
+    NBX_TRY(checkLaunch());
+    NBX_HIP(hipStreamSynchronize(s->stream));
+  }
+  m->count += take;
+  m->lastTake = take;
+  if (added) {
+    *added = take;
+  }
+  if (found > take) {
+    return setError(NBX_ERR_CAPACITY,
+                    "training pairs beyond the set's capacity were dropped (" +
+                        std::to_string(found - take) + ")");
+  }
+  return NBX_OK;
+}
+
+int nbx_mlp_last_pairs(nbx_mlp *m, uint64_t cap, uint32_t *target,
+                       int32_t *node, uint64_t *count) {
+  if (!m || !count || (cap && (!target || !node))) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  const uint64_t k = std::min<uint64_t>(cap, m->lastTake);
+  std::vector<int2> h(k);
+  if (k) {
+    NBX_HIP(hipMemcpyAsync(h.data(), m->pairs, k * sizeof(int2),
+                           hipMemcpyDeviceToHost, m->stream));
+    NBX_HIP(hipStreamSynchronize(m->stream));
+  }
+  for (uint64_t i = 0; i < k; ++i) {
+    target[i] = static_cast<uint32_t>(h[i].x);
+    node[i] = h[i].y;
+  }
+  *count = m->lastTake;
+  return NBX_OK;
+}
+
+int nbx_mlp_upload(nbx_mlp *m, uint64_t count, const float *features,
+                   const float *labels) {
+  if (!m || (count && (!features || !labels))) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (count > m->cap) {
+    return setError(NBX_ERR_ARG, "count above the set's capacity");
+  }
+  std::vector<float4> h(3 * count);
+  for (uint64_t k = 0; k < count; ++k) {
+    const float *f = features + k * kMlpIn;
+    h[3 * k] = make_float4(f[0], f[1], f[2], f[3]);
+    h[3 * k + 1] = make_float4(f[4], f[5], f[6], f[7]);
+    h[3 * k + 2] = make_float4(f[8], f[9], labels[2 * k], labels[2 * k + 1]);
+  }
+  if (count) {
+    NBX_HIP(hipMemcpyAsync(m->data, h.data(), h.size() * sizeof(float4),
+                           hipMemcpyHostToDevice, m->stream));
+    NBX_HIP(hipStreamSynchronize(m->stream));
+  }
+  m->count = count;
+  return NBX_OK;
+}
+
+int nbx_mlp_download(nbx_mlp *m, uint64_t first, uint64_t count,
+                     float *features, float *labels) {
+  if (!m || (count && (!features || !labels))) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (first > m->count || count > m->count - first) {
+    return setError(NBX_ERR_ARG, "range outside the training set");
+  }
+  std::vector<float4> h(3 * count);
+  if (count) {
+    NBX_HIP(hipMemcpyAsync(h.data(), m->data + 3 * first, h.size() * sizeof(float4),
+                           hipMemcpyDeviceToHost, m->stream));
+    NBX_HIP(hipStreamSynchronize(m->stream));
+  }
+  for (uint64_t k = 0; k < count; ++k) {
+    float *f = features + k * kMlpIn;
+    const float4 a = h[3 * k], b = h[3 * k + 1], c = h[3 * k + 2];
+    f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
+    f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
+    f[8] = c.x; f[9] = c.y;
+    labels[2 * k] = c.z;
+    labels[2 * k + 1] = c.w;
+  }
+  return NBX_OK;
+}
+
+int nbx_mlp_gradient(nbx_mlp *m, uint64_t first, uint64_t count, float *grad,
+                     float *loss) {
+  if (!m || !grad || !loss) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (!m->hasParams) {
+    return setError(NBX_ERR_STATE, "no parameters set");
+  }
+  if (count == 0 || count > 0xffffffffull || first > m->count ||
+      count > m->count - first) {
+    return setError(NBX_ERR_ARG, "range outside the training set");
+  }
+  const uint32_t batch = static_cast<uint32_t>(count);
+  NBX_TRY(mlpGradLaunch(m, 1, first, 0, batch));
+  hipLaunchKernelGGL(k_mlp_adam, dim3(blocksFor(kMlpParams + 1)), dim3(kBlock), 0,
+                     m->stream, m->partial, mlpGroups(batch), batch, 0, 0.0f,
+                     0.0f, 0.0f, m->params, m->adamM, m->adamV, m->grad, m->loss);
+  NBX_TRY(checkLaunch());
+  NBX_HIP(hipMemcpyAsync(grad, m->grad, kMlpParams * sizeof(float),
+                         hipMemcpyDeviceToHost, m->stream));
+  NBX_HIP(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost,
+                         m->stream));
+  NBX_HIP(hipStreamSynchronize(m->stream));
+  return NBX_OK;
+}
+
+int nbx_mlp_train(nbx_mlp *m, uint32_t steps, uint32_t batch, float lr,
+                  uint64_t seed, float *losses) {
+  if (!m) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  if (!m->hasParams) {
+    return setError(NBX_ERR_STATE, "no parameters set");
+  }
+  if (m->count == 0) {
+    return setError(NBX_ERR_STATE, "empty training set");
+  }
+  if (batch == 0 || steps > m->lossCap || !(lr > 0.0f)) {
+    return setError(NBX_ERR_ARG, "batch must be > 0, steps <= 2^20, lr > 0");
+  }
+  // A permutation of the set, k -> (a k + c) mod count, a coprime to count;
+  // consecutive steps take consecutive batches of it.
+  const uint64_t count = m->count;
+  uint64_t a = (seed * 0x9e3779b97f4a7c15ull) % count;
+  a = a | 1u;
+  while (gcd64(a, count) != 1) {
+    a = (a + 2) % count;
+    if (a == 0) a = 1;
+  }
+  const uint64_t c = (seed * 0xbf58476d1ce4e5b9ull + 12345) % count;
+  const uint32_t groups = mlpGroups(batch);
+  for (uint32_t t = 0; t < steps; ++t) {
+    NBX_TRY(mlpGradLaunch(m, a, c, static_cast<uint64_t>(t) * batch % count, batch));
+    ++m->adamStep;
+    const double st = static_cast<double>(m->adamStep);
+    const float bc1 = static_cast<float>(1.0 / (1.0 - std::pow(0.9, st)));
+    const float bc2 = static_cast<float>(1.0 / (1.0 - std::pow(0.999, st)));
+    hipLaunchKernelGGL(k_mlp_adam, dim3(blocksFor(kMlpParams + 1)), dim3(kBlock), 0,
+                       m->stream, m->partial, groups, batch, 1, lr, bc1, bc2,
+                       m->params, m->adamM, m->adamV, m->grad, m->loss + t);
+    NBX_TRY(checkLaunch());
+    NBX_TRY(mlpPack(m));
+  }
+  if (losses && steps) {
+    NBX_HIP(hipMemcpyAsync(losses, m->loss, steps * sizeof(float),
+                           hipMemcpyDeviceToHost, m->stream));
+  }
+  NBX_HIP(hipStreamSynchronize(m->stream));
+  return NBX_OK;
+}
+
+int nbx_mlp_predict(nbx_mlp *m, uint64_t count, const float *features,
+                    float *out) {
+  if (!m || (count && (!features || !out))) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (!m->hasParams) {
+    return setError(NBX_ERR_STATE, "no parameters set");
+  }
+  if (count == 0) {
+    return NBX_OK;
+  }
+  if (count > 0xffffffffull / 16) {
+    return setError(NBX_ERR_ARG, "too many pairs");
+  }
+  const uint64_t need = count * (kMlpIn + kMlpOut);
+  if (need > m->ioCap) {
+    float *p = nullptr;
+    if (hipMalloc(&p, need * sizeof(float)) != hipSuccess) {
+      return setError(NBX_ERR_HIP, "hipMalloc failed in nbx_mlp_predict");
+    }
+    m->allocs.push_back(p);
+    m->io = p;
+    m->ioCap = need;
+  }
+  float *fin = m->io, *fout = m->io + count * kMlpIn;
+  NBX_HIP(hipMemcpyAsync(fin, features, count * kMlpIn * sizeof(float),
+                         hipMemcpyHostToDevice, m->stream));
+  const uint64_t waves = (count + kMlpTile - 1) / kMlpTile;
+  hipLaunchKernelGGL(k_mlp_predict, dim3(blocksFor(waves * 64)), dim3(kBlock), 0,
+                     m->stream, m->pk, m->bc, fin, static_cast<uint32_t>(count),
+                     fout);
+  NBX_TRY(checkLaunch());
+  NBX_HIP(hipMemcpyAsync(out, fout, count * kMlpOut * sizeof(float),
+                         hipMemcpyDeviceToHost, m->stream));
+  NBX_HIP(hipStreamSynchronize(m->stream));
+  return NBX_OK;
+}
+
+int nbx_set_mlp(nbx_sim *s, nbx_mlp *m) {
+  if (!s || !m) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (!m->hasParams) {
+    return setError(NBX_ERR_STATE, "no parameters set");
+  }
+  NBX_HIP(hipStreamSynchronize(m->stream));
+  NBX_HIP(hipMemcpyAsync(s->mlpPk, m->pk, kPkFwd * sizeof(bf16x8),
+                         hipMemcpyDeviceToDevice, s->stream));
+  NBX_HIP(hipMemcpyAsync(s->mlpBc, m->bc, kBcAll * sizeof(float),
+                         hipMemcpyDeviceToDevice, s->stream));
+  NBX_HIP(hipStreamSynchronize(s->stream));
+  s->mlpLoaded = true;
   return NBX_OK;
 }
 

@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("NBX_LIBRARY") or os.path.join(HERE, "build", "libnbx.so")
 
 NBX_OK = 0
-LAW_REFERENCE, LAW_ACCURATE, LAW_ACCURATE_QUAD = 0, 1, 2
+LAW_REFERENCE, LAW_ACCURATE, LAW_ACCURATE_QUAD, LAW_LEARNED = 0, 1, 2, 3
 SUM_REFERENCE, SUM_COMPENSATED = 0, 1
 PHASES = ("iterate", "collide", "build", "walk")
 
@@ -38,7 +38,13 @@ EXPORTS = (
     "nbx_count_interactions", "nbx_set_profiling", "nbx_phase_times", "nbx_stream",
     "nbx_exact_forces", "nbx_init_scenario", "nbx_download_contacts",
     "nbx_energy", "nbx_step_async", "nbx_create_ex", "nbx_exact_forces_noalloc",
+    "nbx_mlp_create", "nbx_mlp_destroy", "nbx_mlp_set_params", "nbx_mlp_get_params",
+    "nbx_mlp_count", "nbx_mlp_clear", "nbx_mlp_generate", "nbx_mlp_upload",
+    "nbx_mlp_download", "nbx_mlp_gradient", "nbx_mlp_train", "nbx_mlp_predict",
+    "nbx_set_mlp", "nbx_mlp_last_pairs",
 )
+
+MLP_PARAMS = 9154
 
 CREATE_NO_GRID_BARRIERS = 1
 
@@ -116,6 +122,23 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                            _f64p]
     L.nbx_energy.argtypes = [_vp, C.c_double, C.POINTER(C.c_double),
                              C.POINTER(C.c_double), _f64p]
+    L.nbx_mlp_create.argtypes = [C.c_uint64, C.POINTER(_vp)]
+    L.nbx_mlp_destroy.argtypes = [_vp]
+    L.nbx_mlp_set_params.argtypes = [_vp, _f32p]
+    L.nbx_mlp_get_params.argtypes = [_vp, _f32p]
+    L.nbx_mlp_count.argtypes = [_vp, C.POINTER(C.c_uint64)]
+    L.nbx_mlp_clear.argtypes = [_vp]
+    L.nbx_mlp_generate.argtypes = [_vp, _vp, C.c_float, C.c_uint64, _u32p,
+                                   C.POINTER(C.c_uint64)]
+    L.nbx_mlp_upload.argtypes = [_vp, C.c_uint64, _f32p, _f32p]
+    L.nbx_mlp_download.argtypes = [_vp, C.c_uint64, C.c_uint64, _f32p, _f32p]
+    L.nbx_mlp_gradient.argtypes = [_vp, C.c_uint64, C.c_uint64, _f32p,
+                                   C.POINTER(C.c_float)]
+    L.nbx_mlp_train.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
+                                _vp]
+    L.nbx_mlp_predict.argtypes = [_vp, C.c_uint64, _f32p, _f32p]
+    L.nbx_set_mlp.argtypes = [_vp, _vp]
+    L.nbx_mlp_last_pairs.argtypes = [_vp, C.c_uint64, _vp, _vp, C.POINTER(C.c_uint64)]
     _lib = L
     return L
 
@@ -372,3 +395,124 @@ class Simulation:
         f = self.L.nbx_exact_forces_noalloc if noalloc else self.L.nbx_exact_forces
         self._check(f(self.h, t.shape[0], t, eps2, ax, ay))
         return ax, ay
+
+
+class FarFieldNet:
+    """The learned far-field correction's network (nbx_mlp): parameters,
+    a training set in device memory generated from simulations' trees, and
+    GPU training (BF16 matrix-core forward/backward, Adam)."""
+
+    def __init__(self, pair_capacity: int = 1 << 20):
+        self.L = load_library()
+        h = _vp()
+        self._check(self.L.nbx_mlp_create(int(pair_capacity), C.byref(h)))
+        self.h = h
+
+    def _check(self, rc: int) -> None:
+        if rc != NBX_OK:
+            raise NbxError(rc, self.L.nbx_last_error().decode())
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.nbx_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.zeros(MLP_PARAMS, np.float32)
+        self._check(self.L.nbx_mlp_get_params(self.h, p))
+        return p
+
+    @params.setter
+    def params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        if p.shape != (MLP_PARAMS,):
+            raise ValueError(f"expected {MLP_PARAMS} parameters")
+        self._check(self.L.nbx_mlp_set_params(self.h, p))
+
+    @property
+    def count(self) -> int:
+        v = C.c_uint64()
+        self._check(self.L.nbx_mlp_count(self.h, C.byref(v)))
+        return int(v.value)
+
+    def clear(self) -> None:
+        self._check(self.L.nbx_mlp_clear(self.h))
+
+    def generate(self, sim: "Simulation", targets, theta: float = 1.2) -> int:
+        """Appends the training pairs of sim's last tree for the targets."""
+        t = np.ascontiguousarray(targets, np.uint32)
+        added = C.c_uint64()
+        self._check(self.L.nbx_mlp_generate(self.h, sim.h, float(theta), t.size, t,
+                                            C.byref(added)))
+        return int(added.value)
+
+    def last_pairs(self):
+        """(target, node) of the pairs the last generate() appended."""
+        cnt = C.c_uint64()
+        self._check(self.L.nbx_mlp_last_pairs(self.h, 0, None, None, C.byref(cnt)))
+        k = int(cnt.value)
+        t = np.zeros(max(k, 1), np.uint32)
+        nd = np.zeros(max(k, 1), np.int32)
+        self._check(self.L.nbx_mlp_last_pairs(self.h, k, t.ctypes.data, nd.ctypes.data,
+                                              C.byref(cnt)))
+        return t[:k], nd[:k]
+
+    def upload(self, features, labels) -> None:
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 10)
+        y = np.ascontiguousarray(labels, np.float32).reshape(-1, 2)
+        self._check(self.L.nbx_mlp_upload(self.h, f.shape[0], f.reshape(-1),
+                                          y.reshape(-1)))
+
+    def download(self, first: int = 0, count: int | None = None):
+        count = self.count - first if count is None else count
+        f = np.zeros(count * 10, np.float32)
+        y = np.zeros(count * 2, np.float32)
+        self._check(self.L.nbx_mlp_download(self.h, first, count, f, y))
+        return f.reshape(count, 10), y.reshape(count, 2)
+
+    def gradient(self, first: int = 0, count: int | None = None):
+        count = self.count - first if count is None else count
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        self._check(self.L.nbx_mlp_gradient(self.h, first, count, g, C.byref(loss)))
+        return float(loss.value), g
+
+    def train(self, steps: int, batch: int = 16384, lr: float = 1e-3,
+              seed: int = 0) -> np.ndarray:
+        losses = np.zeros(max(steps, 1), np.float32)
+        self._check(self.L.nbx_mlp_train(self.h, steps, batch, lr, seed,
+                                         losses.ctypes.data))
+        return losses[:steps]
+
+    def predict(self, features) -> np.ndarray:
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 10)
+        out = np.zeros(f.shape[0] * 2, np.float32)
+        self._check(self.L.nbx_mlp_predict(self.h, f.shape[0], f.reshape(-1), out))
+        return out.reshape(-1, 2)
+
+    def attach(self, sim: "Simulation") -> None:
+        """Loads the current weights into sim for LAW_LEARNED."""
+        self._check(self.L.nbx_set_mlp(sim.h, self.h))
+
+
+def init_mlp_params(seed: int = 0) -> np.ndarray:
+    """Uniform +-1/sqrt(fan_in) weights and zero biases, seeded."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MLP_PARAMS, np.float32)
+    for (a, b), fan in (((0, 640), 10), ((704, 4800), 64), ((4864, 8960), 64),
+                        ((9024, 9152), 64)):
+        p[a:b] = rng.uniform(-1.0, 1.0, b - a) / np.sqrt(fan)
+    return p

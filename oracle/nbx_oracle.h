@@ -34,6 +34,8 @@
  *                      (oracle/_ref) driven in CudaCollision.cu's order
  *   or_step            reference Simulation::step (per phase), invariants
  *   or_exact_forces    closed-form two/three-body cases
+ *   or_mlp_*           see nbx_oracle_mlp.cpp (finite differences, closed
+ *                      forms, reduction to the plain walk)
  */
 #ifndef NBX_ORACLE_H
 #define NBX_ORACLE_H
@@ -145,6 +147,89 @@ void or_exact_forces(uint64_t n, const float *px, const float *py,
                      const float *m, uint64_t n_targets,
                      const uint32_t *targets, double eps2, double *ax,
                      double *ay);
+
+/* ---- Learned far-field correction (nbx_oracle_mlp.cpp). No reference
+ * counterpart; the term it corrects is the far-node monopole of
+ * QuadTree.cpp:276-296 and its judge the exact sum of CheckCuda.cu:456-475.
+ *
+ * Network: 10 -> 64 -> 64 -> 64 -> 2, GELU (tanh form) after each hidden
+ * layer. Parameters, flat, row-major [out][in]: W1 (64x10), b1, W2 (64x64),
+ * b2, W3 (64x64), b3, W4 (2x64), b4 -- OR_MLP_PARAMS floats.
+ *
+ * A pair is (target body t, accepted node k with mass M > 0, centre of mass
+ * c, square centre C and size s, second mass moments S about c). With
+ * d = c - t, r^2 = |d|^2, w = 1/(r^2 + eps^2), q = S / M and
+ * k = OR_MLP_MOMENT_SCALE * w, the inputs are
+ *   f0 = log2(M) / 16, f1 = d.x sqrt(w), f2 = d.y sqrt(w), f3 = s / r,
+ *   f4..f6 = k (qxx, qxy, qyy),
+ *   f7 = k d^T q d / r^2, f8 = k (d x q d) / r^2, f9 = k tr q - f7
+ * (q in the frame of d: radial, mixed, tangential), and the target is the
+ * residual R = sum over the node's bodies of the law minus the node's
+ * monopole, scaled as R (r^2 + eps^2) / M * OR_MLP_OUT_SCALE. The trained
+ * range: s^2 <= OR_MLP_MAX_RATIO^2 r^2 and the target at least
+ * OR_MLP_BOX_MARGIN * s from the node's square (FP32, as the walk tests it);
+ * far nodes outside it are opened instead. */
+#define OR_MLP_IN 10
+#define OR_MLP_HIDDEN 64
+#define OR_MLP_OUT 2
+#define OR_MLP_PARAMS 9154
+#define OR_MLP_MAX_RATIO 1.2
+#define OR_MLP_BOX_MARGIN 0.25
+#define OR_MLP_MOMENT_SCALE 8.0
+#define OR_MLP_OUT_SCALE 16.0
+
+/* Stackless preorder walk (QuadTree.cpp:264-389's order) at `theta` for
+ * each target, recording every accepted node of positive mass; with
+ * use_range, far nodes outside the trained range are opened instead (and
+ * not recorded). Writes up to cap pairs; returns the pair count. */
+int64_t or_mlp_pairs(uint64_t n, const float *px, const float *py,
+                     int64_t nodes, const float *comX, const float *comY,
+                     const float *mass, const float *centerX,
+                     const float *centerY, const float *size,
+                     const int32_t *children, const int32_t *next,
+                     uint64_t n_targets, const uint32_t *targets, float theta,
+                     int use_range, int64_t cap, uint32_t *out_target,
+                     int32_t *out_node);
+
+/* Inputs (count x 10) and regression targets (count x 2) of the pairs,
+ * moments and residuals in FP64 from the node's bodies [start, end). */
+void or_mlp_features(const float *px, const float *py, const float *m,
+                     const float *comX, const float *comY, const float *mass,
+                     const float *size, const int32_t *start,
+                     const int32_t *end, uint64_t count,
+                     const uint32_t *targets, const int32_t *nodes,
+                     double eps2, float *feat, float *label);
+
+/* Network outputs (count x 2). bf16 != 0 rounds weights, inputs and hidden
+ * activations to BF16 (round to nearest even) and computes in FP32; 0
+ * computes exactly in double. */
+void or_mlp_forward(const float *params, uint64_t count, const float *feat,
+                    int bf16, float *out);
+
+/* Mean over pairs of the summed squared output error, and its gradient with
+ * respect to every parameter (OR_MLP_PARAMS doubles). bf16 as above, with
+ * the backward operands (output and hidden deltas) rounded to BF16 too. */
+void or_mlp_loss_grad(const float *params, uint64_t count, const float *feat,
+                      const float *label, int bf16, double *loss,
+                      double *grad);
+
+/* The learned-law walk for the listed targets, in FP64: accurate law; far
+ * nodes inside the trained range add their monopole plus the network's
+ * residual (params may be null: monopole only), far nodes outside it are
+ * opened; leaves add their bodies. Per-target interaction counts optional. */
+void or_mlp_accelerations(uint64_t n, const float *px, const float *py,
+                          const float *m, int64_t nodes, const float *comX,
+                          const float *comY, const float *mass,
+                          const float *centerX, const float *centerY,
+                          const float *size, const int32_t *children,
+                          const int32_t *next, const int32_t *start,
+                          const int32_t *end, const float *params, int bf16,
+                          uint64_t n_targets, const uint32_t *targets,
+                          float theta, double eps2, double *ax, double *ay,
+                          uint32_t *far_count, uint32_t *pair_count);
+
+/* Round to the nearest BF16 value (ties to even), as the GPU converts. */
+float or_bf16_round(float x);
 
 /* QuadTree.cpp:11-26 fastInvSqrt, exposed for the exhaustive known-answer
  * test: out[i] = fisr(bits-reinterpreted in[i]) for i in [0, count). */

@@ -133,6 +133,21 @@ class Oracle:
         L.or_jacobi_cell.argtypes = [_u64, _f32p]
         L.or_jacobi_cell.restype = _f32
         L.or_scenario.restype = _int
+        L.or_mlp_pairs.argtypes = ([_u64, _f32p, _f32p, _i64] + [_f32p] * 6
+                                   + [_i32p, _i32p, _u64, _u32p, _f32, _int, _i64,
+                                      _u32p, _i32p])
+        L.or_mlp_pairs.restype = _i64
+        L.or_mlp_features.argtypes = ([_f32p] * 7 + [_i32p, _i32p, _u64, _u32p, _i32p,
+                                                     _f64, _f32p, _f32p])
+        L.or_mlp_forward.argtypes = [_f32p, _u64, _f32p, _int, _f32p]
+        L.or_mlp_loss_grad.argtypes = [_f32p, _u64, _f32p, _f32p, _int,
+                                       C.POINTER(C.c_double), _f64p]
+        L.or_mlp_accelerations.argtypes = ([_u64, _f32p, _f32p, _f32p, _i64] + [_f32p] * 6
+                                           + [_i32p] * 4 + [C.c_void_p, _int, _u64, _u32p,
+                                                            _f32, _f64, _f64p, _f64p,
+                                                            C.c_void_p, C.c_void_p])
+        L.or_bf16_round.argtypes = [_f32]
+        L.or_bf16_round.restype = _f32
         self.L = L
 
     def uniform_disc(self, n: int, seed: int = 0) -> Bodies:
@@ -245,6 +260,70 @@ class Oracle:
         self.L.or_exact_forces(b.n, b.px, b.py, b.m, targets.shape[0], targets, eps2,
                                ax, ay)
         return ax, ay
+
+    # ---- learned far-field correction (oracle/nbx_oracle_mlp.cpp) ----
+    MLP_PARAMS = 9154
+
+    def mlp_pairs(self, b: Bodies, t: Tree, targets, theta: float, use_range=True,
+                  cap: int | None = None):
+        """(target, node) pairs the theta walk accepts (inside the trained
+        range when use_range), in walk order per target."""
+        targets = np.ascontiguousarray(targets, np.uint32)
+        cap = cap if cap is not None else 256 * max(1, targets.size)
+        ot = np.zeros(cap, np.uint32)
+        on = np.zeros(cap, np.int32)
+        cnt = self.L.or_mlp_pairs(b.n, b.px, b.py, t.count, t.comX, t.comY, t.mass,
+                                  t.centerX, t.centerY, t.size, t.children, t.next,
+                                  targets.size, targets, theta, int(use_range), cap, ot, on)
+        if cnt > cap:
+            raise RuntimeError("pair capacity exceeded")
+        return ot[:cnt].copy(), on[:cnt].copy()
+
+    def mlp_features(self, b: Bodies, t: Tree, targets, nodes, eps2: float = 1.0):
+        targets = np.ascontiguousarray(targets, np.uint32)
+        nodes = np.ascontiguousarray(nodes, np.int32)
+        k = targets.size
+        feat = np.zeros(k * 10, np.float32)
+        lab = np.zeros(k * 2, np.float32)
+        self.L.or_mlp_features(b.px, b.py, b.m, t.comX, t.comY, t.mass, t.size, t.start,
+                               t.end, k, targets, nodes, eps2, feat, lab)
+        return feat.reshape(k, 10), lab.reshape(k, 2)
+
+    def mlp_forward(self, params, feat, bf16: bool = False):
+        params = np.ascontiguousarray(params, np.float32)
+        feat = np.ascontiguousarray(feat, np.float32)
+        k = feat.shape[0]
+        out = np.zeros(k * 2, np.float32)
+        self.L.or_mlp_forward(params, k, feat.reshape(-1), int(bf16), out)
+        return out.reshape(k, 2)
+
+    def mlp_loss_grad(self, params, feat, lab, bf16: bool = False):
+        params = np.ascontiguousarray(params, np.float32)
+        feat = np.ascontiguousarray(feat, np.float32)
+        lab = np.ascontiguousarray(lab, np.float32)
+        g = np.zeros(self.MLP_PARAMS, np.float64)
+        loss = C.c_double()
+        self.L.or_mlp_loss_grad(params, feat.shape[0], feat.reshape(-1), lab.reshape(-1),
+                                int(bf16), C.byref(loss), g)
+        return float(loss.value), g
+
+    def mlp_accelerations(self, b: Bodies, t: Tree, targets, theta: float, params=None,
+                          bf16: bool = True, eps2: float = 1.0):
+        """Learned-law walk in FP64 for the targets: (ax, ay, far, pairs)."""
+        targets = np.ascontiguousarray(targets, np.uint32)
+        k = targets.size
+        ax, ay = np.zeros(k, np.float64), np.zeros(k, np.float64)
+        far, pairs = np.zeros(k, np.uint32), np.zeros(k, np.uint32)
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        self.L.or_mlp_accelerations(b.n, b.px, b.py, b.m, t.count, t.comX, t.comY, t.mass,
+                                    t.centerX, t.centerY, t.size, t.children, t.next,
+                                    t.start, t.end, None if p is None else p.ctypes.data,
+                                    int(bf16), k, targets, theta, eps2, ax, ay,
+                                    far.ctypes.data, pairs.ctypes.data)
+        return ax, ay, far, pairs
+
+    def bf16_round(self, x: float) -> float:
+        return float(self.L.or_bf16_round(x))
 
     def fisr_bits(self, bits: np.ndarray) -> np.ndarray:
         bits = np.ascontiguousarray(bits, np.uint32)

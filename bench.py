@@ -143,6 +143,110 @@ def force_accuracy(nbx, sim, targets: int, seed: int = 12345) -> dict:
             "reference": "FP64 direct sum m*d/(|d|(|d|^2+eps^2)) over all bodies"}
 
 
+def equal_error(nbx, n: int, steps: int, warmup: int, targets: int,
+                coarse_theta: float = 1.2) -> dict:
+    """The metric's comparison: the learned correction at the coarse theta
+    against plain Barnes-Hut (accurate law) at the theta whose force error
+    against the FP64 direct sum is the same, both on the n-body disc's
+    initial state and both timed as full steps (iterate, collide, build,
+    walk) over steps 1..K from it. The plain theta is the largest one
+    (bisection to 0.005) whose RMS error is at most the learned one's."""
+    import numpy as np
+    params, meta = nbx.load_farfield_params()
+    net = nbx.FarFieldNet(16)
+    net.params = params
+    start = nbx.Simulation(n)
+    init = start.download()
+    start.close()
+    rng = np.random.default_rng(12345)
+    tgt = np.sort(rng.choice(n, size=min(targets, n), replace=False)).astype(np.uint32)
+
+    probe = nbx.Simulation(state=init)
+    probe.theta = 0.5
+    probe.attract()  # key order; later builds of these positions keep it
+    ex, ey = probe.exact_forces(tgt)
+    a2 = float((ex * ex + ey * ey).sum())
+    net.attach(probe)
+
+    def err(law, theta):
+        probe.law, probe.theta = law, theta
+        probe.attract()
+        got = probe.download()
+        dx = got.ax[tgt].astype(np.float64) - ex
+        dy = got.ay[tgt].astype(np.float64) - ey
+        far, pairs = probe.count_interactions()
+        return float(np.sqrt((dx * dx + dy * dy).sum() / a2)), (far + pairs) / n
+
+    learned_rms, learned_ipb = err(nbx.LAW_LEARNED, coarse_theta)
+    # Plain theta: scan down in steps of 0.1, then bisect.
+    hi_t, lo_t = coarse_theta, None
+    t = coarse_theta
+    while t > 0.05:
+        e, _ = err(nbx.LAW_ACCURATE, t)
+        if e <= learned_rms:
+            lo_t = t
+            break
+        hi_t = t
+        t = round(t - 0.1, 4)
+    if lo_t is None:
+        probe.close()
+        net.close()
+        return {"error": "no plain theta >= 0.05 reaches the learned error"}
+    while hi_t - lo_t > 0.005:
+        mid = 0.5 * (hi_t + lo_t)
+        if err(nbx.LAW_ACCURATE, mid)[0] <= learned_rms:
+            lo_t = mid
+        else:
+            hi_t = mid
+    plain_rms, plain_ipb = err(nbx.LAW_ACCURATE, lo_t)
+    ranged_rms = None
+    net0 = nbx.FarFieldNet(16)
+    net0.params = np.zeros_like(params)
+    net0.attach(probe)
+    ranged_rms, _ = err(nbx.LAW_LEARNED, coarse_theta)
+    net0.close()
+    probe.close()
+
+    def timed(law, theta):
+        sim = nbx.Simulation(state=init)
+        sim.law, sim.theta, sim.collisions = law, theta, True
+        net.attach(sim)
+        sim.step(warmup)
+        sim.upload(init)
+        device_sync()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        device_sync()
+        sps = steps / (time.perf_counter() - t0)
+        sim.upload(init)
+        sim.set_profiling(True)
+        sim.step(steps)
+        ph, ns = sim.phase_times()
+        sim.set_profiling(False)
+        sim.close()
+        return sps, {k: round(v / max(ns, 1), 4) for k, v in ph.items()}
+
+    l_sps, l_ph = timed(nbx.LAW_LEARNED, coarse_theta)
+    p_sps, p_ph = timed(nbx.LAW_ACCURATE, lo_t)
+    net.close()
+    out = {
+        "workload": (f"rotating disc, N={n}, initial state; force error on {tgt.size} "
+                     f"seeded targets vs the FP64 direct sum; steps 1..{steps} timed"),
+        "learned": {"theta": coarse_theta, "rms": learned_rms,
+                    "interactions_per_body": learned_ipb, "steps_per_s": round(l_sps, 3),
+                    "phase_ms": l_ph},
+        "plain": {"theta": round(lo_t, 4), "law": "accurate", "rms": plain_rms,
+                  "interactions_per_body": plain_ipb, "steps_per_s": round(p_sps, 3),
+                  "phase_ms": p_ph},
+        "range_rule_only_rms": ranged_rms,
+        "speedup_learned_over_plain": round(l_sps / p_sps, 4),
+        "network": {"pairs": meta.get("pairs"), "epochs": meta.get("epochs"),
+                    "held_out_rms": meta.get("held_out_force_error", {})},
+    }
+    print(f"[bench] equal error: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +258,8 @@ def main() -> int:
                     help="CPU baseline window: steps 1..min(K, this)")
     ap.add_argument("--targets", type=int, default=4096,
                     help="seeded targets of the accuracy check")
+    ap.add_argument("--no-equal-error", action="store_true",
+                    help="skip the learned-vs-plain equal-error comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (PMC scaling)")
@@ -254,6 +360,8 @@ def main() -> int:
         "walk_leaf_util": round(pair1 / (64.0 * max(terms1, 1)), 4),
         "walk_wave_slots": {"visits": visits1, "leaf_terms": terms1},
     }
+    eq = None if args.no_equal_error else equal_error(nbx, args.n, args.steps,
+                                                      args.warmup, args.targets)
     cpu = None if args.no_cpu else cpu_baseline(args.n, args.theta, cpu_steps)
     if cpu is not None:
         cpu["gpu_value_same_window"] = round(value if value_same is None else value_same, 3)
@@ -286,6 +394,7 @@ def main() -> int:
         "tree": {"nodes": nodes, "levels": levels},
         "collisions": info,
         "accuracy": accuracy,
+        "equal_error": eq,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

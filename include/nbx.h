@@ -318,15 +318,16 @@ int nbx_exact_forces_noalloc(nbx_sim *sim, uint64_t n_targets,
  * d = c - x, r = |d|, w = 1/(r^2 + eps^2), q = S/M, k = 8 w:
  *   log2(M)/16, d.x sqrt(w), d.y sqrt(w), s/r, k q_xx, k q_xy, k q_yy,
  *   k d.q.d/r^2 (radial), k (d x q d)/r^2 (mixed), k tr(q) - radial;
- * output: the residual times (r^2 + eps^2) / M * 16. Trained range: s <= 1.2
- * r and the target at least s/4 outside the node's square.
+ * output: the residual in units of M w mu / 2, mu = w tr(q) (monopole
+ * magnitude times quadrupole scale). Trained range: s <= 1.2 r and the
+ * target at least s/4 outside the node's square.
  *
  * Parameters (NBX_MLP_PARAMS floats, row-major [out][in]): W1 (64x10), b1,
  * W2 (64x64), b2, W3 (64x64), b3, W4 (2x64), b4. */
 #define NBX_MLP_PARAMS 9154
 typedef struct nbx_mlp nbx_mlp;
 
-/* A network handle with room for `pair_capacity` training pairs (48 bytes
+/* A network handle with room for `pair_capacity` training pairs (64 bytes
  * each) in device memory. */
 int nbx_mlp_create(uint64_t pair_capacity, nbx_mlp **out);
 int nbx_mlp_destroy(nbx_mlp *mlp);
@@ -344,7 +345,9 @@ int nbx_mlp_clear(nbx_mlp *mlp);
  * listed target body (device index), every node of positive mass that a
  * walk at `theta` accepts inside the trained range (nodes outside it are
  * opened); inputs as the walk computes them, the target residual in FP64
- * over the node's bodies. *added (may be null) gets the number appended;
+ * over the node's bodies, and a loss weight: the squared ratio of the
+ * output unit M w mu to the magnitude of the sum of the target's recorded
+ * monopoles, so the loss measures each target's relative force error. *added (may be null) gets the number appended;
  * NBX_ERR_CAPACITY if the set overflowed (the excess is dropped). At most
  * 2^22 targets per call. */
 int nbx_mlp_generate(nbx_mlp *mlp, nbx_sim *sim, float theta,
@@ -358,22 +361,24 @@ int nbx_mlp_last_pairs(nbx_mlp *mlp, uint64_t cap, uint32_t *target,
                        int32_t *node, uint64_t *count);
 
 /* Replaces the training set with host pairs (inputs count x 10, targets
- * count x 2), or reads pairs [first, first + count) back. */
+ * count x 2, loss weights count; null weights: all 1), or reads pairs
+ * [first, first + count) back (weights may be null). */
 int nbx_mlp_upload(nbx_mlp *mlp, uint64_t count, const float *features,
-                   const float *labels);
+                   const float *labels, const float *weights);
 int nbx_mlp_download(nbx_mlp *mlp, uint64_t first, uint64_t count,
-                     float *features, float *labels);
+                     float *features, float *labels, float *weights);
 
-/* Loss (mean over pairs of the summed squared output error) and its
- * gradient (NBX_MLP_PARAMS floats) over pairs [first, first + count) of the
- * set, without changing the parameters: the training kernels' backward
- * pass, for checks. */
+/* Loss -- the weighted mean squared error sum_k w_k |y_k - t_k|^2 /
+ * (count mean(w)) -- and its gradient (NBX_MLP_PARAMS floats) over pairs
+ * [first, first + count) of the set, without changing the parameters: the
+ * training kernels' backward pass, for checks. */
 int nbx_mlp_gradient(nbx_mlp *mlp, uint64_t first, uint64_t count,
                      float *grad, float *loss);
 
-/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8, learning rate lr) on
- * batches of `batch` pairs taken in the order of a seeded permutation of
- * the set; losses (may be null) gets each step's batch loss. */
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8, learning rate lr) on the
+ * weighted loss of batches of `batch` pairs (weights normalised by the
+ * whole set's mean) taken in the order of a seeded permutation of the set;
+ * losses (may be null) gets each step's batch loss. */
 int nbx_mlp_train(nbx_mlp *mlp, uint32_t steps, uint32_t batch, float lr,
                   uint64_t seed, float *losses);
 

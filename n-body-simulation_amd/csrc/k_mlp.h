@@ -55,7 +55,7 @@ constexpr int kBcB1 = 0, kBcB2 = 64, kBcB3 = 128, kBcB4 = 192, kBcAll = 224;
 constexpr float kMlpMaxRatio = 1.2f;
 constexpr float kMlpBoxMargin = 0.25f;
 constexpr float kMlpMomentScale = 8.0f;
-constexpr float kMlpOutScale = 16.0f;
+constexpr float kMlpOutScale = 0.5f;
 
 // GELU, tanh form: x * sigmoid(A (x + B x^3)).
 constexpr float kGeluA = 1.5957691216057308f;
@@ -392,7 +392,7 @@ __device__ __forceinline__ void backLayer64(const bf16x8 *__restrict__ wt,
 __global__ __launch_bounds__(kMlpTrainBlock) void k_mlp_grad(
     const bf16x8 *__restrict__ pk, const float *__restrict__ bc,
     const float4 *__restrict__ data, uint64_t count, uint64_t mulA,
-    uint64_t addC, uint64_t k0, uint32_t batch, uint32_t per,
+    uint64_t addC, uint64_t k0, uint32_t batch, uint32_t per, float invBW,
     float *__restrict__ partial) {
   __shared__ float acc[kMlpParams + 1];
   __shared__ MlpWaveLds wl[kMlpTrainWaves];
@@ -405,22 +405,22 @@ __global__ __launch_bounds__(kMlpTrainBlock) void k_mlp_grad(
   MlpWaveLds &L = wl[wave];
   const uint32_t first = blockIdx.x * per;
   const uint32_t last = min(batch, first + per);
-  const float dScale = 2.0f / static_cast<float>(batch);
   float lossSum = 0.0f;
   for (uint32_t t = first + wave * kMlpTile; t < last;
        t += kMlpTrainWaves * kMlpTile) {
     const uint32_t pos = t + j;
     const bool valid = pos < last;
     float f[kMlpIn] = {};
-    float t0 = 0.0f, t1 = 0.0f;
+    float t0 = 0.0f, t1 = 0.0f, wt = 0.0f;
     if (valid) {
       const uint64_t idx = (mulA * ((k0 + pos) % count) + addC) % count;
-      const float4 a = data[3 * idx], b = data[3 * idx + 1], c = data[3 * idx + 2];
+      const float4 a = data[4 * idx], b = data[4 * idx + 1], c = data[4 * idx + 2];
       f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
       f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
       f[8] = c.x; f[9] = c.y;
       t0 = c.z;
       t1 = c.w;
+      wt = data[4 * idx + 3].x;
     }
     const bf16x8 x = mlpInput(f, h);
 #pragma unroll
@@ -429,16 +429,18 @@ __global__ __launch_bounds__(kMlpTrainBlock) void k_mlp_grad(
     }
     f32x16 zs[3][2];
     const f32x16 y = mlpForward<true>(pk, bc, x, lane, zs, L.h);
-    // Output delta: 2 (y - t) / batch, lanes of half 0 (their pair's outputs).
+    // Output delta of the weighted loss sum w |y - t|^2 / (batch W):
+    // 2 (y - t) w / (batch W), lanes of half 0 (their pair's outputs).
     float e0 = 0.0f, e1 = 0.0f;
+    const float sc = wt * invBW;
     if (h == 0 && valid) {
       e0 = y[0] - t0;
       e1 = y[1] - t1;
-      lossSum += e0 * e0 + e1 * e1;
+      lossSum += (e0 * e0 + e1 * e1) * sc;
     }
     bf16x8 d4 = {};
-    d4[0] = static_cast<__bf16>(e0 * dScale);
-    d4[1] = static_cast<__bf16>(e1 * dScale);
+    d4[0] = static_cast<__bf16>(2.0f * e0 * sc);
+    d4[1] = static_cast<__bf16>(2.0f * e1 * sc);
     if (h == 0) {
       L.dz[0][j] = d4[0];
       L.dz[1][j] = d4[1];
@@ -490,9 +492,9 @@ __global__ __launch_bounds__(kMlpTrainBlock) void k_mlp_grad(
 // Sums the workgroups' partials in order; with `update` applies one Adam
 // step to the master parameters (m, v moments; bc1 = 1 / (1 - beta1^t),
 // bc2 = 1 / (1 - beta2^t)), else writes the gradient to `grad`. The loss
-// (mean squared error of the batch) goes to *loss.
+// (weighted mean squared error of the batch) goes to *loss.
 __global__ __launch_bounds__(kBlock) void k_mlp_adam(
-    const float *__restrict__ partial, uint32_t groups, uint32_t batch,
+    const float *__restrict__ partial, uint32_t groups,
     int update, float lr, float bc1, float bc2, float *__restrict__ p,
     float *__restrict__ m, float *__restrict__ v, float *__restrict__ grad,
     float *__restrict__ loss) {
@@ -505,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_adam(
     g += partial[static_cast<size_t>(w) * (kMlpParams + 1) + i];
   }
   if (i == kMlpParams) {
-    *loss = g / static_cast<float>(batch);
+    *loss = g;
     return;
   }
   if (!update) {
@@ -518,6 +520,31 @@ __global__ __launch_bounds__(kBlock) void k_mlp_adam(
   m[i] = mi;
   v[i] = vi;
   p[i] -= lr * (mi * bc1) / (sqrtf(vi * bc2) + eps);
+}
+
+// Sum of the loss weights of pairs [first, first + count) (FP64).
+__global__ __launch_bounds__(kBlock) void k_mlp_wsum(
+    const float4 *__restrict__ data, uint64_t first, uint64_t count,
+    double *__restrict__ out) {
+  __shared__ double part[kBlock / kWave];
+  double s = 0.0;
+  for (uint64_t k = blockIdx.x * static_cast<uint64_t>(kBlock) + threadIdx.x; k < count;
+       k += static_cast<uint64_t>(gridDim.x) * kBlock) {
+    s += data[4 * (first + k) + 3].x;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off, kWave);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    part[threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += part[w];
+    atomicAdd(out, t);
+  }
 }
 
 // Network outputs of `count` pairs (inputs, 10 floats each).
@@ -563,15 +590,17 @@ __global__ __launch_bounds__(kBlock) void k_node_learn(
 // One thread per target: the stackless preorder walk (next pointers, as
 // QuadTree.cpp:264-389 walks) at the coarse theta, recording every accepted
 // node of positive mass inside the trained range; accepted nodes outside it
-// are opened instead. Pairs {target, node} go to `pairs` (at most cap; the
-// counter keeps counting past it).
+// are opened instead. Pairs {target body, node, target's list index} go to
+// `pairs` (at most cap; the counter keeps counting past it); norm[t] gets
+// |sum of the recorded nodes' monopoles| (FP64), the scale of the pairs'
+// loss weights.
 __global__ __launch_bounds__(kBlock) void k_mlp_gen_pairs(
     uint32_t nt, const uint32_t *__restrict__ targets,
     const float4 *__restrict__ body, const float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, const float *__restrict__ centerX,
     const float *__restrict__ centerY, const float *__restrict__ size,
-    const Ctl *__restrict__ ctl, float tSq, int2 *__restrict__ pairs,
-    uint32_t cap, uint32_t *__restrict__ counter) {
+    const Ctl *__restrict__ ctl, float tSq, double eps2, int4 *__restrict__ pairs,
+    uint32_t cap, uint32_t *__restrict__ counter, double *__restrict__ norm) {
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= nt) {
     return;
@@ -580,6 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_gen_pairs(
   const float x = body[ti].x, y = body[ti].y;
   const int nodeCount = ctl->nodeCount;
   int node = 0;
+  double ax = 0.0, ay = 0.0;
   for (int guard = 0; guard <= nodeCount; ++guard) {
     const float4 f = nodeF[node];
     const int4 c = nodeI[node];
@@ -594,8 +624,14 @@ __global__ __launch_bounds__(kBlock) void k_mlp_gen_pairs(
       if (far && f.z > 0.0f) {
         const uint32_t slot = atomicAdd(counter, 1u);
         if (slot < cap) {
-          pairs[slot] = make_int2(static_cast<int>(ti), node);
+          pairs[slot] = make_int4(static_cast<int>(ti), node, static_cast<int>(t), 0);
         }
+        const double ddx = static_cast<double>(f.x) - x;
+        const double ddy = static_cast<double>(f.y) - y;
+        const double r2 = ddx * ddx + ddy * ddy;
+        const double fm = static_cast<double>(f.z) / (sqrt(r2) * (r2 + eps2));
+        ax += ddx * fm;
+        ay += ddy * fm;
       }
       if (c.y == 0) {
         break;
@@ -608,24 +644,28 @@ __global__ __launch_bounds__(kBlock) void k_mlp_gen_pairs(
       break;
     }
   }
+  norm[t] = sqrt(ax * ax + ay * ay);
 }
 
-// One wave per pair: inputs (the walk's own arithmetic) and the regression
-// target, the FP64 residual of the node's bodies against its monopole,
-// scaled by (r^2 + eps^2) / M * kMlpOutScale. Written as three float4 per
-// pair: {f0..f3}, {f4..f7}, {f8, f9, t0, t1}.
+// One wave per pair: inputs (the walk's own arithmetic), the regression
+// target -- the FP64 residual of the node's bodies against its monopole, in
+// units of u = M w mu (w = 1 / (r^2 + eps^2), mu = w tr(S/M)) times
+// kMlpOutScale, zero when u = 0 -- and the loss weight (u / norm[target])^2:
+// what an output error costs that target's force, squared and relative.
+// Written as four float4 per pair: {f0..f3}, {f4..f7}, {f8, f9, t0, t1},
+// {weight, 0, 0, 0}.
 __global__ __launch_bounds__(kBlock) void k_mlp_label(
-    uint32_t count, const int2 *__restrict__ pairs,
-    const float4 *__restrict__ body, const float4 *__restrict__ nodeF,
-    const int4 *__restrict__ nodeI, const float4 *__restrict__ nodeL,
-    const float *__restrict__ size, float eSq, double eps2,
-    float4 *__restrict__ out) {
+    uint32_t count, const int4 *__restrict__ pairs,
+    const double *__restrict__ norm, const float4 *__restrict__ body,
+    const float4 *__restrict__ nodeF, const int4 *__restrict__ nodeI,
+    const float4 *__restrict__ nodeL, const float *__restrict__ size,
+    float eSq, double eps2, float4 *__restrict__ out) {
   const uint32_t k = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const uint32_t lane = threadIdx.x & 63;
   if (k >= count) {
     return;
   }
-  const int2 pr = pairs[k];
+  const int4 pr = pairs[k];
   const float4 me = body[pr.x];
   const float4 f = nodeF[pr.y];
   const int4 c = nodeI[pr.y];
@@ -659,12 +699,19 @@ __global__ __launch_bounds__(kBlock) void k_mlp_label(
     const double ddy = static_cast<double>(f.y) - me.y;
     const double r2 = ddx * ddx + ddy * ddy;
     const double fm = static_cast<double>(f.z) / (sqrt(r2) * (r2 + eps2));
-    const double norm = (r2 + eps2) / static_cast<double>(f.z) * kMlpOutScale;
-    const float t0 = static_cast<float>((sx - ddx * fm) * norm);
-    const float t1 = static_cast<float>((sy - ddy * fm) * norm);
-    out[3 * k] = make_float4(ft[0], ft[1], ft[2], ft[3]);
-    out[3 * k + 1] = make_float4(ft[4], ft[5], ft[6], ft[7]);
-    out[3 * k + 2] = make_float4(ft[8], ft[9], t0, t1);
+    // Output unit: monopole magnitude times the quadrupole scale tr(S/M) w.
+    const float4 q = nodeL[pr.y];
+    const double mu = (static_cast<double>(q.x) + q.z) / (r2 + eps2);
+    const double unit = static_cast<double>(f.z) / (r2 + eps2) * mu;
+    const double scale = unit > 0.0 ? kMlpOutScale / unit : 0.0;
+    const float t0 = static_cast<float>((sx - ddx * fm) * scale);
+    const float t1 = static_cast<float>((sy - ddy * fm) * scale);
+    const double at = norm[pr.z];
+    const double share = at > 0.0 ? unit / at : 0.0;
+    out[4 * k] = make_float4(ft[0], ft[1], ft[2], ft[3]);
+    out[4 * k + 1] = make_float4(ft[4], ft[5], ft[6], ft[7]);
+    out[4 * k + 2] = make_float4(ft[8], ft[9], t0, t1);
+    out[4 * k + 3] = make_float4(static_cast<float>(share * share), 0.f, 0.f, 0.f);
   }
 }
 

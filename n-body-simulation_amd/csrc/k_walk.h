@@ -404,8 +404,8 @@ __device__ inline void leafSpread(uint64_t near, uint32_t k, uint32_t logK,
 // A far node inside the trained range adds its monopole at once and queues
 // (features, lane) in this wave's LDS queue; every 32 queued pairs the wave
 // runs the network on the matrix cores (mlpForward, weights staged in LDS)
-// and adds each output, scaled by the node's M / (r^2 + eps^2), to its
-// lane's correction sum. Far nodes outside the range are opened instead.
+// and adds each output, times the node's M w mu / kMlpOutScale (w = 1 /
+// (r^2 + eps^2), mu = w tr(S/M)), to its lane's correction sum. Far nodes outside the range are opened instead.
 constexpr int kMlpQueueCap = 96; // < 32 pending + up to 64 new
 struct MlpQueue {
   bf16x8 feat[kMlpQueueCap][2];
@@ -542,7 +542,8 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
       mlpFeatures(dx, dy, dSq, eSq, geo.z, sm, ft);
       x0 = mlpInput(ft, 0);
       x1 = mlpInput(ft, 1);
-      scale = f.z * w * (1.0f / kMlpOutScale);
+      // Output unit: monopole magnitude times the quadrupole scale.
+      scale = f.z * w * ((sm.x + sm.z) * w) * (1.0f / kMlpOutScale);
       if (COUNT) {
         ++nFar;
       }

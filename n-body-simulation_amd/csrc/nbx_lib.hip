@@ -1706,7 +1706,7 @@ struct nbx_mlp {
   std::vector<void *> allocs;
   uint64_t cap = 0;   // training pairs the set can hold
   uint64_t count = 0; // pairs held
-  float4 *data = nullptr; // 3 float4 per pair: {f0..f3} {f4..f7} {f8 f9 t0 t1}
+  float4 *data = nullptr; // 4 float4 per pair: {f0..f3} {f4..f7} {f8 f9 t0 t1} {w}
   float *params = nullptr, *adamM = nullptr, *adamV = nullptr, *grad = nullptr;
   bf16x8 *pk = nullptr; // packed operands (forward + transposed)
   float *bc = nullptr;  // biases, accumulator order
@@ -1715,7 +1715,9 @@ struct nbx_mlp {
   uint32_t lossCap = 0;
   uint64_t adamStep = 0;
   bool hasParams = false;
-  int2 *pairs = nullptr;    // generation scratch
+  int4 *pairs = nullptr;    // generation scratch: {body, node, target index}
+  double *targetNorm = nullptr; // per target: |monopole sum| of its pairs
+  double *wsum = nullptr;   // loss-weight sum
   uint32_t pairCap = 0;
   uint32_t *pairCount = nullptr;
   uint32_t *targets = nullptr;
@@ -1765,16 +1767,33 @@ uint64_t gcd64(uint64_t a, uint64_t b) {
 }
 
 // One gradient pass over `batch` positions of the permutation (mulA, addC)
-// starting at k0, summed into m->partial.
+// starting at k0, summed into m->partial; wmean normalises the loss weights.
 int mlpGradLaunch(nbx_mlp *m, uint64_t mulA, uint64_t addC, uint64_t k0,
-                  uint32_t batch) {
+                  uint32_t batch, double wmean) {
   const uint32_t per =
       ((batch + kMlpGroups - 1) / kMlpGroups + kMlpTile - 1) / kMlpTile * kMlpTile;
   const uint32_t groups = (batch + per - 1) / per;
+  const float invBW = static_cast<float>(1.0 / (static_cast<double>(batch) * wmean));
   hipLaunchKernelGGL(k_mlp_grad, dim3(groups), dim3(kMlpTrainBlock), 0,
                      m->stream, m->pk, m->bc, m->data, m->count, mulA, addC,
-                     k0, batch, per, m->partial);
+                     k0, batch, per, invBW, m->partial);
   return checkLaunch();
+}
+
+// Mean loss weight of pairs [first, first + count) (a zero mean counts as 1).
+int mlpWeightMean(nbx_mlp *m, uint64_t first, uint64_t count, double *mean) {
+  NBX_HIP(hipMemsetAsync(m->wsum, 0, sizeof(double), m->stream));
+  const uint32_t grid = static_cast<uint32_t>(
+      std::min<uint64_t>(1024, std::max<uint64_t>(1, blocksFor(count))));
+  hipLaunchKernelGGL(k_mlp_wsum, dim3(grid), dim3(kBlock), 0, m->stream, m->data,
+                     first, count, m->wsum);
+  NBX_TRY(checkLaunch());
+  double sum = 0.0;
+  NBX_HIP(hipMemcpyAsync(&sum, m->wsum, sizeof(double), hipMemcpyDeviceToHost,
+                         m->stream));
+  NBX_HIP(hipStreamSynchronize(m->stream));
+  *mean = sum > 0.0 ? sum / static_cast<double>(count) : 1.0;
+  return NBX_OK;
 }
 
 uint32_t mlpGroups(uint32_t batch) {
@@ -1809,7 +1828,7 @@ int nbx_mlp_create(uint64_t pair_capacity, nbx_mlp **out) {
   m->targetCap = kMlpTargetScratch;
   m->lossCap = kMlpLossCap;
   int rc = NBX_OK;
-  if ((rc = mlpAlloc(m, &m->data, 3 * pair_capacity)) ||
+  if ((rc = mlpAlloc(m, &m->data, 4 * pair_capacity)) ||
       (rc = mlpAlloc(m, &m->params, kMlpParams)) ||
       (rc = mlpAlloc(m, &m->adamM, kMlpParams)) ||
       (rc = mlpAlloc(m, &m->adamV, kMlpParams)) ||
@@ -1820,7 +1839,9 @@ int nbx_mlp_create(uint64_t pair_capacity, nbx_mlp **out) {
       (rc = mlpAlloc(m, &m->loss, m->lossCap)) ||
       (rc = mlpAlloc(m, &m->pairs, m->pairCap)) ||
       (rc = mlpAlloc(m, &m->pairCount, 1)) ||
-      (rc = mlpAlloc(m, &m->targets, m->targetCap))) {
+      (rc = mlpAlloc(m, &m->targets, m->targetCap)) ||
+      (rc = mlpAlloc(m, &m->targetNorm, m->targetCap)) ||
+      (rc = mlpAlloc(m, &m->wsum, 1))) {
     return fail(rc);
   }
   *out = m;
@@ -1928,10 +1949,12 @@ int nbx_mlp_generate(nbx_mlp *m, nbx_sim *s, float theta, uint64_t n_targets,
                        s->stream, s->nodeF, s->nodeQ, s->ctl, s->nodeL);
     NBX_TRY(checkLaunch());
   }
+  const float eSq = s->params.epsilon * s->params.epsilon;
   hipLaunchKernelGGL(k_mlp_gen_pairs, dim3(blocksFor(nt)), dim3(kBlock), 0,
                      s->stream, nt, m->targets, s->bodyPack, s->nodeF, s->nodeI,
                      s->centerX, s->centerY, s->size, s->ctl, theta * theta,
-                     m->pairs, m->pairCap, m->pairCount);
+                     static_cast<double>(eSq), m->pairs, m->pairCap, m->pairCount,
+                     m->targetNorm);
   NBX_TRY(checkLaunch());
   uint32_t found = 0;
   NBX_HIP(hipMemcpyAsync(&found, m->pairCount, sizeof(uint32_t),
@@ -1941,11 +1964,10 @@ int nbx_mlp_generate(nbx_mlp *m, nbx_sim *s, float theta, uint64_t n_targets,
   const uint32_t take = static_cast<uint32_t>(
       std::min<uint64_t>(std::min<uint64_t>(found, m->pairCap), room));
   if (take > 0) {
-    const float eSq = s->params.epsilon * s->params.epsilon;
     hipLaunchKernelGGL(k_mlp_label, dim3(blocksFor(static_cast<uint64_t>(take) * 64)),
-                       dim3(kBlock), 0, s->stream, take, m->pairs, s->bodyPack,
-                       s->nodeF, s->nodeI, s->nodeL, s->size, eSq,
-                       static_cast<double>(eSq), m->data + 3 * m->count);
+                       dim3(kBlock), 0, s->stream, take, m->pairs, m->targetNorm,
+                       s->bodyPack, s->nodeF, s->nodeI, s->nodeL, s->size, eSq,
+                       static_cast<double>(eSq), m->data + 4 * m->count);
     NBX_TRY(checkLaunch());
     NBX_HIP(hipStreamSynchronize(s->stream));
   }
@@ -1968,9 +1990,9 @@ int nbx_mlp_last_pairs(nbx_mlp *m, uint64_t cap, uint32_t *target,
     return setError(NBX_ERR_ARG, "null argument");
   }
   const uint64_t k = std::min<uint64_t>(cap, m->lastTake);
-  std::vector<int2> h(k);
+  std::vector<int4> h(k);
   if (k) {
-    NBX_HIP(hipMemcpyAsync(h.data(), m->pairs, k * sizeof(int2),
+    NBX_HIP(hipMemcpyAsync(h.data(), m->pairs, k * sizeof(int4),
                            hipMemcpyDeviceToHost, m->stream));
     NBX_HIP(hipStreamSynchronize(m->stream));
   }
@@ -1983,19 +2005,20 @@ int nbx_mlp_last_pairs(nbx_mlp *m, uint64_t cap, uint32_t *target,
 }
 
 int nbx_mlp_upload(nbx_mlp *m, uint64_t count, const float *features,
-                   const float *labels) {
+                   const float *labels, const float *weights) {
   if (!m || (count && (!features || !labels))) {
     return setError(NBX_ERR_ARG, "null argument");
   }
   if (count > m->cap) {
     return setError(NBX_ERR_ARG, "count above the set's capacity");
   }
-  std::vector<float4> h(3 * count);
+  std::vector<float4> h(4 * count);
   for (uint64_t k = 0; k < count; ++k) {
     const float *f = features + k * kMlpIn;
-    h[3 * k] = make_float4(f[0], f[1], f[2], f[3]);
-    h[3 * k + 1] = make_float4(f[4], f[5], f[6], f[7]);
-    h[3 * k + 2] = make_float4(f[8], f[9], labels[2 * k], labels[2 * k + 1]);
+    h[4 * k] = make_float4(f[0], f[1], f[2], f[3]);
+    h[4 * k + 1] = make_float4(f[4], f[5], f[6], f[7]);
+    h[4 * k + 2] = make_float4(f[8], f[9], labels[2 * k], labels[2 * k + 1]);
+    h[4 * k + 3] = make_float4(weights ? weights[k] : 1.0f, 0.f, 0.f, 0.f);
   }
   if (count) {
     NBX_HIP(hipMemcpyAsync(m->data, h.data(), h.size() * sizeof(float4),
@@ -2007,22 +2030,25 @@ int nbx_mlp_upload(nbx_mlp *m, uint64_t count, const float *features,
 }
 
 int nbx_mlp_download(nbx_mlp *m, uint64_t first, uint64_t count,
-                     float *features, float *labels) {
+                     float *features, float *labels, float *weights) {
   if (!m || (count && (!features || !labels))) {
     return setError(NBX_ERR_ARG, "null argument");
   }
   if (first > m->count || count > m->count - first) {
     return setError(NBX_ERR_ARG, "range outside the training set");
   }
-  std::vector<float4> h(3 * count);
+  std::vector<float4> h(4 * count);
   if (count) {
-    NBX_HIP(hipMemcpyAsync(h.data(), m->data + 3 * first, h.size() * sizeof(float4),
+    NBX_HIP(hipMemcpyAsync(h.data(), m->data + 4 * first, h.size() * sizeof(float4),
                            hipMemcpyDeviceToHost, m->stream));
     NBX_HIP(hipStreamSynchronize(m->stream));
   }
   for (uint64_t k = 0; k < count; ++k) {
     float *f = features + k * kMlpIn;
-    const float4 a = h[3 * k], b = h[3 * k + 1], c = h[3 * k + 2];
+    const float4 a = h[4 * k], b = h[4 * k + 1], c = h[4 * k + 2];
+    if (weights) {
+      weights[k] = h[4 * k + 3].x;
+    }
     f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
     f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
     f[8] = c.x; f[9] = c.y;
@@ -2045,9 +2071,11 @@ int nbx_mlp_gradient(nbx_mlp *m, uint64_t first, uint64_t count, float *grad,
     return setError(NBX_ERR_ARG, "range outside the training set");
   }
   const uint32_t batch = static_cast<uint32_t>(count);
-  NBX_TRY(mlpGradLaunch(m, 1, first, 0, batch));
+  double wmean = 1.0;
+  NBX_TRY(mlpWeightMean(m, first, count, &wmean));
+  NBX_TRY(mlpGradLaunch(m, 1, first, 0, batch, wmean));
   hipLaunchKernelGGL(k_mlp_adam, dim3(blocksFor(kMlpParams + 1)), dim3(kBlock), 0,
-                     m->stream, m->partial, mlpGroups(batch), batch, 0, 0.0f,
+                     m->stream, m->partial, mlpGroups(batch), 0, 0.0f,
                      0.0f, 0.0f, m->params, m->adamM, m->adamV, m->grad, m->loss);
   NBX_TRY(checkLaunch());
   NBX_HIP(hipMemcpyAsync(grad, m->grad, kMlpParams * sizeof(float),
@@ -2083,14 +2111,17 @@ int nbx_mlp_train(nbx_mlp *m, uint32_t steps, uint32_t batch, float lr,
   }
   const uint64_t c = (seed * 0xbf58476d1ce4e5b9ull + 12345) % count;
   const uint32_t groups = mlpGroups(batch);
+  double wmean = 1.0;
+  NBX_TRY(mlpWeightMean(m, 0, count, &wmean));
   for (uint32_t t = 0; t < steps; ++t) {
-    NBX_TRY(mlpGradLaunch(m, a, c, static_cast<uint64_t>(t) * batch % count, batch));
+    NBX_TRY(mlpGradLaunch(m, a, c, static_cast<uint64_t>(t) * batch % count, batch,
+                          wmean));
     ++m->adamStep;
     const double st = static_cast<double>(m->adamStep);
     const float bc1 = static_cast<float>(1.0 / (1.0 - std::pow(0.9, st)));
     const float bc2 = static_cast<float>(1.0 / (1.0 - std::pow(0.999, st)));
     hipLaunchKernelGGL(k_mlp_adam, dim3(blocksFor(kMlpParams + 1)), dim3(kBlock), 0,
-                       m->stream, m->partial, groups, batch, 1, lr, bc1, bc2,
+                       m->stream, m->partial, groups, 1, lr, bc1, bc2,
                        m->params, m->adamM, m->adamV, m->grad, m->loss + t);
     NBX_TRY(checkLaunch());
     NBX_TRY(mlpPack(m));

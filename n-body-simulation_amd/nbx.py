@@ -130,8 +130,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_mlp_clear.argtypes = [_vp]
     L.nbx_mlp_generate.argtypes = [_vp, _vp, C.c_float, C.c_uint64, _u32p,
                                    C.POINTER(C.c_uint64)]
-    L.nbx_mlp_upload.argtypes = [_vp, C.c_uint64, _f32p, _f32p]
-    L.nbx_mlp_download.argtypes = [_vp, C.c_uint64, C.c_uint64, _f32p, _f32p]
+    L.nbx_mlp_upload.argtypes = [_vp, C.c_uint64, _f32p, _f32p, _vp]
+    L.nbx_mlp_download.argtypes = [_vp, C.c_uint64, C.c_uint64, _f32p, _f32p, _vp]
     L.nbx_mlp_gradient.argtypes = [_vp, C.c_uint64, C.c_uint64, _f32p,
                                    C.POINTER(C.c_float)]
     L.nbx_mlp_train.argtypes = [_vp, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
@@ -470,18 +470,23 @@ class FarFieldNet:
                                               C.byref(cnt)))
         return t[:k], nd[:k]
 
-    def upload(self, features, labels) -> None:
+    def upload(self, features, labels, weights=None) -> None:
         f = np.ascontiguousarray(features, np.float32).reshape(-1, 10)
         y = np.ascontiguousarray(labels, np.float32).reshape(-1, 2)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
         self._check(self.L.nbx_mlp_upload(self.h, f.shape[0], f.reshape(-1),
-                                          y.reshape(-1)))
+                                          y.reshape(-1), _ptr(w)))
 
-    def download(self, first: int = 0, count: int | None = None):
+    def download(self, first: int = 0, count: int | None = None, weights: bool = False):
+        """Inputs, targets and (weights=True) loss weights of pairs
+        [first, first + count)."""
         count = self.count - first if count is None else count
         f = np.zeros(count * 10, np.float32)
         y = np.zeros(count * 2, np.float32)
-        self._check(self.L.nbx_mlp_download(self.h, first, count, f, y))
-        return f.reshape(count, 10), y.reshape(count, 2)
+        w = np.zeros(max(count, 1), np.float32)
+        self._check(self.L.nbx_mlp_download(self.h, first, count, f, y, w.ctypes.data))
+        out = (f.reshape(count, 10), y.reshape(count, 2))
+        return out + (w[:count],) if weights else out
 
     def gradient(self, first: int = 0, count: int | None = None):
         count = self.count - first if count is None else count
@@ -516,3 +521,18 @@ def init_mlp_params(seed: int = 0) -> np.ndarray:
                         ((9024, 9152), 64)):
         p[a:b] = rng.uniform(-1.0, 1.0, b - a) / np.sqrt(fan)
     return p
+
+
+WEIGHTS_PATH = os.path.join(HERE, "weights", "farfield_mlp.npz")
+
+
+def load_farfield_params(path: str = WEIGHTS_PATH):
+    """Trained parameters (float32, MLP_PARAMS) and the training record
+    saved beside them by farfield_train.py."""
+    import json
+    with np.load(path, allow_pickle=False) as d:
+        params = np.ascontiguousarray(d["params"], np.float32)
+        meta = json.loads(str(d["meta"]))
+    if params.shape != (MLP_PARAMS,):
+        raise ValueError(f"{path}: expected {MLP_PARAMS} parameters")
+    return params, meta

@@ -165,7 +165,9 @@ void or_exact_forces(uint64_t n, const float *px, const float *py,
  *   f7 = k d^T q d / r^2, f8 = k (d x q d) / r^2, f9 = k tr q - f7
  * (q in the frame of d: radial, mixed, tangential), and the target is the
  * residual R = sum over the node's bodies of the law minus the node's
- * monopole, scaled as R (r^2 + eps^2) / M * OR_MLP_OUT_SCALE. The trained
+ * monopole in units of u = M w mu, mu = w tr(q) (the monopole's magnitude
+ * times the quadrupole scale): R / u * OR_MLP_OUT_SCALE (0 when u = 0, a
+ * node whose mass sits at its centre of mass). The trained
  * range: s^2 <= OR_MLP_MAX_RATIO^2 r^2 and the target at least
  * OR_MLP_BOX_MARGIN * s from the node's square (FP32, as the walk tests it);
  * far nodes outside it are opened instead. */
@@ -176,7 +178,7 @@ void or_exact_forces(uint64_t n, const float *px, const float *py,
 #define OR_MLP_MAX_RATIO 1.2
 #define OR_MLP_BOX_MARGIN 0.25
 #define OR_MLP_MOMENT_SCALE 8.0
-#define OR_MLP_OUT_SCALE 16.0
+#define OR_MLP_OUT_SCALE 0.5
 
 /* Stackless preorder walk (QuadTree.cpp:264-389's order) at `theta` for
  * each target, recording every accepted node of positive mass; with
@@ -192,13 +194,16 @@ int64_t or_mlp_pairs(uint64_t n, const float *px, const float *py,
                      int32_t *out_node);
 
 /* Inputs (count x 10) and regression targets (count x 2) of the pairs,
- * moments and residuals in FP64 from the node's bodies [start, end). */
+ * moments and residuals in FP64 from the node's bodies [start, end), and
+ * (weight may be null) each pair's loss weight (u / |A|)^2, u the output
+ * unit above and A the sum of the monopoles of the listed pairs with the
+ * same target: an output error e costs e u / |A| of that target's force. */
 void or_mlp_features(const float *px, const float *py, const float *m,
                      const float *comX, const float *comY, const float *mass,
                      const float *size, const int32_t *start,
                      const int32_t *end, uint64_t count,
                      const uint32_t *targets, const int32_t *nodes,
-                     double eps2, float *feat, float *label);
+                     double eps2, float *feat, float *label, float *weight);
 
 /* Network outputs (count x 2). bf16 != 0 rounds weights, inputs and hidden
  * activations to BF16 (round to nearest even) and computes in FP32; 0
@@ -206,12 +211,13 @@ void or_mlp_features(const float *px, const float *py, const float *m,
 void or_mlp_forward(const float *params, uint64_t count, const float *feat,
                     int bf16, float *out);
 
-/* Mean over pairs of the summed squared output error, and its gradient with
- * respect to every parameter (OR_MLP_PARAMS doubles). bf16 as above, with
- * the backward operands (output and hidden deltas) rounded to BF16 too. */
+/* Weighted mean squared error, sum_k w_k |y_k - t_k|^2 / (count wnorm)
+ * (weight null: every w_k = 1), and its gradient with respect to every
+ * parameter (OR_MLP_PARAMS doubles). bf16 as above, with the backward
+ * operands (output and hidden deltas) rounded to BF16 too. */
 void or_mlp_loss_grad(const float *params, uint64_t count, const float *feat,
-                      const float *label, int bf16, double *loss,
-                      double *grad);
+                      const float *label, const float *weight, double wnorm,
+                      int bf16, double *loss, double *grad);
 
 /* The learned-law walk for the listed targets, in FP64: accurate law; far
  * nodes inside the trained range add their monopole plus the network's

@@ -8,8 +8,10 @@
 // it, exactTreeLawForces (src/cuda/CheckCuda.cu:456-475). On top of those:
 //   * or_mlp_pairs       the (target, node) pairs a coarse-theta walk accepts
 //                        inside the trained range (the training set's pairs);
-//   * or_mlp_features    the network's 10 inputs and its regression target,
-//                        the residual exact - monopole of that node's bodies;
+//   * or_mlp_features    the network's 10 inputs, its regression target (the
+//                        residual exact - monopole of that node's bodies) and
+//                        the pair's loss weight, the squared share of its
+//                        monopole in the target's monopole sum;
 //   * or_mlp_forward /   the 10 -> 64 -> 64 -> 64 -> 2 GELU network and the
 //     or_mlp_loss_grad   gradient of its mean squared error, either exactly
 //                        (double) or with the BF16 rounding points of the
@@ -30,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <vector>
 
 namespace {
@@ -248,7 +251,20 @@ void or_mlp_features(const float *px, const float *py, const float *m,
                      const float *size, const int32_t *start,
                      const int32_t *end, uint64_t count,
                      const uint32_t *targets, const int32_t *nodes,
-                     double eps2, float *feat, float *label) {
+                     double eps2, float *feat, float *label, float *weight) {
+  // Per target: the monopole sum of its pairs, |A_t|, the scale of the
+  // pairs' loss weights (M / (r^2 + eps^2) / |A_t|)^2.
+  std::map<uint32_t, std::pair<double, double>> acc;
+  for (uint64_t k = 0; k < count; ++k) {
+    const uint32_t t = targets[k];
+    const int32_t nd = nodes[k];
+    const double dx = double(comX[nd]) - px[t], dy = double(comY[nd]) - py[t];
+    const double r2 = dx * dx + dy * dy;
+    const double fm = mass[nd] / (std::sqrt(r2) * (r2 + eps2));
+    auto &a = acc[t];
+    a.first += dx * fm;
+    a.second += dy * fm;
+  }
   for (uint64_t k = 0; k < count; ++k) {
     const uint32_t t = targets[k];
     const int32_t nd = nodes[k];
@@ -270,9 +286,20 @@ void or_mlp_features(const float *px, const float *py, const float *m,
     }
     const double dx = cx - x, dy = cy - y, r2 = dx * dx + dy * dy;
     const double fm = M / (std::sqrt(r2) * (r2 + eps2));
-    const double norm = (r2 + eps2) / M * OR_MLP_OUT_SCALE;
+    // Output scale: the node's monopole magnitude M / (r^2 + eps^2) times
+    // its quadrupole scale mu = tr(S) / (M (r^2 + eps^2)); mu = 0 (all mass
+    // at the centre of mass) has a zero residual, target and weight.
+    const double mu = (q.sxx + q.syy) / M / (r2 + eps2);
+    const double unit = M / (r2 + eps2) * mu;
+    const double norm = unit > 0.0 ? OR_MLP_OUT_SCALE / unit : 0.0;
     label[2 * k] = float((ex - dx * fm) * norm);
     label[2 * k + 1] = float((ey - dy * fm) * norm);
+    if (weight) {
+      const auto &a = acc[t];
+      const double at = std::sqrt(a.first * a.first + a.second * a.second);
+      const double share = at > 0.0 ? unit / at : 0.0;
+      weight[k] = float(share * share);
+    }
   }
 }
 
@@ -298,34 +325,40 @@ void or_mlp_forward(const float *params, uint64_t count, const float *feat,
 }
 
 void or_mlp_loss_grad(const float *params, uint64_t count, const float *feat,
-                      const float *label, int bf16, double *loss,
-                      double *grad) {
+                      const float *label, const float *weight, double wnorm,
+                      int bf16, double *loss, double *grad) {
   std::memset(grad, 0, sizeof(double) * OR_MLP_PARAMS);
   double l = 0;
   if (count == 0) {
     *loss = 0;
     return;
   }
-  auto run = [&](auto &net, auto &a, auto scale) {
+  // Loss sum_k w_k |y_k - t_k|^2 / (count wnorm): pair k's delta scale is
+  // w_k / (count wnorm).
+  const double inv = 1.0 / (double(count) * wnorm);
+  auto run = [&](auto &net, auto &a, auto one) {
+    using T = decltype(one);
+    const T s = T(inv);
     for (uint64_t k = 0; k < count; ++k) {
+      const double w = weight ? weight[k] : 1.0;
       net.forward(feat + k * kIn, a);
       for (int o = 0; o < kOut; ++o) {
         const double e = double(a.y[o]) - label[2 * k + o];
-        l += e * e;
+        l += w * e * e;
       }
-      net.backward(a, label + 2 * k, scale, grad);
+      net.backward(a, label + 2 * k, T(w) * s, grad);
     }
   };
   if (bf16) {
     Net<true> net(params);
     Net<true>::Acts a;
-    run(net, a, 1.0f / float(count));
+    run(net, a, 1.0f);
   } else {
     Net<false> net(params);
     Net<false>::Acts a;
-    run(net, a, 1.0 / double(count));
+    run(net, a, 1.0);
   }
-  *loss = l / double(count);
+  *loss = l * inv;
 }
 
 void or_mlp_accelerations(uint64_t n, const float *px, const float *py,
@@ -372,7 +405,8 @@ void or_mlp_accelerations(uint64_t n, const float *px, const float *py,
             f.resize(f.size() + kIn);
             features(x, yy, comX[node], comY[node], mass[node], size[node], q,
                      eps2, f.data() + f.size() - kIn);
-            scale.push_back(mass[node] / (r2 + eps2) / OR_MLP_OUT_SCALE);
+            const double mu = (q.sxx + q.syy) / mass[node] / (r2 + eps2);
+            scale.push_back(mass[node] / (r2 + eps2) * mu / OR_MLP_OUT_SCALE);
           }
         }
       } else if (children[node] == 0) {

@@ -138,10 +138,10 @@ class Oracle:
                                       _u32p, _i32p])
         L.or_mlp_pairs.restype = _i64
         L.or_mlp_features.argtypes = ([_f32p] * 7 + [_i32p, _i32p, _u64, _u32p, _i32p,
-                                                     _f64, _f32p, _f32p])
+                                                     _f64, _f32p, _f32p, _f32p])
         L.or_mlp_forward.argtypes = [_f32p, _u64, _f32p, _int, _f32p]
-        L.or_mlp_loss_grad.argtypes = [_f32p, _u64, _f32p, _f32p, _int,
-                                       C.POINTER(C.c_double), _f64p]
+        L.or_mlp_loss_grad.argtypes = [_f32p, _u64, _f32p, _f32p, C.c_void_p, _f64,
+                                       _int, C.POINTER(C.c_double), _f64p]
         L.or_mlp_accelerations.argtypes = ([_u64, _f32p, _f32p, _f32p, _i64] + [_f32p] * 6
                                            + [_i32p] * 4 + [C.c_void_p, _int, _u64, _u32p,
                                                             _f32, _f64, _f64p, _f64p,
@@ -279,15 +279,19 @@ class Oracle:
             raise RuntimeError("pair capacity exceeded")
         return ot[:cnt].copy(), on[:cnt].copy()
 
-    def mlp_features(self, b: Bodies, t: Tree, targets, nodes, eps2: float = 1.0):
+    def mlp_features(self, b: Bodies, t: Tree, targets, nodes, eps2: float = 1.0,
+                     weights: bool = False):
+        """Inputs, regression targets and (weights=True) loss weights."""
         targets = np.ascontiguousarray(targets, np.uint32)
         nodes = np.ascontiguousarray(nodes, np.int32)
         k = targets.size
         feat = np.zeros(k * 10, np.float32)
         lab = np.zeros(k * 2, np.float32)
+        wt = np.zeros(max(k, 1), np.float32)
         self.L.or_mlp_features(b.px, b.py, b.m, t.comX, t.comY, t.mass, t.size, t.start,
-                               t.end, k, targets, nodes, eps2, feat, lab)
-        return feat.reshape(k, 10), lab.reshape(k, 2)
+                               t.end, k, targets, nodes, eps2, feat, lab, wt)
+        out = (feat.reshape(k, 10), lab.reshape(k, 2))
+        return out + (wt[:k],) if weights else out
 
     def mlp_forward(self, params, feat, bf16: bool = False):
         params = np.ascontiguousarray(params, np.float32)
@@ -297,14 +301,21 @@ class Oracle:
         self.L.or_mlp_forward(params, k, feat.reshape(-1), int(bf16), out)
         return out.reshape(k, 2)
 
-    def mlp_loss_grad(self, params, feat, lab, bf16: bool = False):
+    def mlp_loss_grad(self, params, feat, lab, bf16: bool = False, weights=None,
+                      wnorm: float | None = None):
+        """Weighted MSE sum w |y - t|^2 / (count wnorm) and its gradient;
+        wnorm defaults to the mean weight (1 without weights)."""
         params = np.ascontiguousarray(params, np.float32)
         feat = np.ascontiguousarray(feat, np.float32)
         lab = np.ascontiguousarray(lab, np.float32)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        if wnorm is None:
+            wnorm = 1.0 if w is None else float(np.mean(w.astype(np.float64)))
         g = np.zeros(self.MLP_PARAMS, np.float64)
         loss = C.c_double()
         self.L.or_mlp_loss_grad(params, feat.shape[0], feat.reshape(-1), lab.reshape(-1),
-                                int(bf16), C.byref(loss), g)
+                                None if w is None else w.ctypes.data, wnorm, int(bf16),
+                                C.byref(loss), g)
         return float(loss.value), g
 
     def mlp_accelerations(self, b: Bodies, t: Tree, targets, theta: float, params=None,

@@ -79,19 +79,22 @@ def test_forward_unit_routing(nbx, oracle):
     assert np.allclose(y, want, rtol=2e-2, atol=1e-3)
 
 
-@pytest.mark.parametrize("count", [32, 777])
-def test_gradient_matches_oracle(nbx, oracle, count):
-    """The training kernels' loss and gradient over a set: against the
-    oracle with the same BF16 rounding points (relative 2e-2 per parameter
-    block, 1e-2 overall) and against the exact gradient (5e-2 overall)."""
+@pytest.mark.parametrize("count,weighted", [(32, False), (777, False), (777, True)])
+def test_gradient_matches_oracle(nbx, oracle, count, weighted):
+    """The training kernels' loss and gradient over a set (unit or random
+    loss weights, normalised by their mean): against the oracle with the
+    same BF16 rounding points (relative 2e-2 per parameter block, 1e-2
+    overall) and against the exact gradient (5e-2 overall)."""
     p = rand_params(nbx, 4, scale=1.5)
     f, t = rand_pairs(count, 5)
+    w = np.random.default_rng(6).exponential(1.0, count).astype(np.float32) if weighted \
+        else None
     with nbx.FarFieldNet(4096) as net:
         net.params = p
-        net.upload(f, t)
+        net.upload(f, t, w)
         loss, g = net.gradient()
-    lo, go = oracle.mlp_loss_grad(p, f, t, bf16=True)
-    le, ge = oracle.mlp_loss_grad(p, f, t, bf16=False)
+    lo, go = oracle.mlp_loss_grad(p, f, t, bf16=True, weights=w)
+    le, ge = oracle.mlp_loss_grad(p, f, t, bf16=False, weights=w)
     assert loss == pytest.approx(lo, rel=2e-3)
     assert loss == pytest.approx(le, rel=3e-2)
     blocks = {"W1": (0, 640), "b1": (640, 704), "W2": (704, 4800), "b2": (4800, 4864),
@@ -135,7 +138,7 @@ def test_generated_pairs_match_oracle(nbx, oracle, disc_tree):
     """Training pairs of a theta = 1.2 walk for 3000 targets: the same
     (target, node) set as the oracle's walk with the range rule, inputs
     within FP32 rounding of the oracle's (FP64 moments), residual targets
-    within FP64 summation-order rounding."""
+    and loss weights within FP32 rounding."""
     b, srt, tree = disc_tree
     targets = np.arange(0, srt.n, 10, dtype=np.uint32)
     with nbx.Simulation(state=b) as sim, nbx.FarFieldNet(1 << 20) as net:
@@ -146,16 +149,20 @@ def test_generated_pairs_match_oracle(nbx, oracle, disc_tree):
         assert np.array_equal(got.id, srt.id)
         added = net.generate(sim, targets, theta=1.2)
         gt, gn = net.last_pairs()
-        f, lab = net.download()
+        f, lab, w = net.download(weights=True)
     assert added == gt.size == f.shape[0]
     ot, on = oracle.mlp_pairs(srt, tree, targets, 1.2)
     key_g = gt.astype(np.int64) * (1 << 32) + gn
     key_o = ot.astype(np.int64) * (1 << 32) + on
     assert np.array_equal(np.sort(key_g), np.sort(key_o))
-    of, olab = oracle.mlp_features(srt, tree, gt, gn)
+    of, olab, ow = oracle.mlp_features(srt, tree, gt, gn, weights=True)
     assert np.abs(f - of).max() <= 2e-5 * max(1.0, np.abs(of).max())
-    assert np.abs(lab - olab).max() <= 1e-6 * max(1.0, np.abs(olab).max())
+    # Targets are in units of the node's quadrupole scale, which the GPU takes
+    # from the build's FP32 moments: FP32 rounding, not FP64 summation order.
+    assert np.abs(lab - olab).max() <= 2e-5 * max(1.0, np.abs(olab).max())
     assert np.abs(olab).max() > 0.1  # the residuals are not trivially small
+    assert np.allclose(w, ow, rtol=1e-5, atol=1e-12)
+    assert np.all(w >= 0) and np.any(w > 0)
 
 
 def test_learned_walk_zero_network_is_the_range_rule(nbx, oracle, disc_tree):
@@ -202,5 +209,84 @@ def test_learned_walk_matches_oracle(nbx, oracle, disc_tree):
     norm = np.sqrt((ax * ax + ay * ay).sum())
     rms = np.sqrt(((gx - ax) ** 2 + (gy - ay) ** 2).sum()) / norm
     corr = np.sqrt(((ax - ax0) ** 2 + (ay - ay0) ** 2).sum()) / norm
-    assert corr > 1e-3          # the network's contribution is visible ...
+    assert corr > 1e-4          # the network's contribution is visible ...
     assert rms <= 0.01 * corr   # ... and reproduced to 1% of its size
+
+
+def _err(nbx, sim, tgt, ex, ey, law, theta):
+    sim.law, sim.theta = law, theta
+    sim.attract()
+    got = sim.download()
+    dx = got.ax[tgt].astype(np.float64) - ex
+    dy = got.ay[tgt].astype(np.float64) - ey
+    return float(np.sqrt((dx * dx + dy * dy).sum() / (ex * ex + ey * ey).sum()))
+
+
+def _held_out(nbx, n, seed, steps):
+    sim = nbx.Simulation(n, seed=seed)
+    sim.theta = 0.5
+    sim.step(steps)
+    rng = np.random.default_rng(99)
+    tgt = np.sort(rng.choice(n, 4096, replace=False)).astype(np.uint32)
+    sim.attract()  # key order, kept by later builds of the same positions
+    ex, ey = sim.exact_forces(tgt)
+    return sim, tgt, ex, ey
+
+
+def test_training_from_scratch_cuts_force_error(nbx):
+    """The whole learned path on the GPU: pairs generated from two discs'
+    coarse walks, a network trained from random weights by the library's
+    kernels, then on a disc it never saw (other seed, size and step) the
+    learned law at theta = 1.2 beats both the same walk with a zero network
+    (the range rule alone) and the plain walk at theta = 1.2."""
+    rng = np.random.default_rng(0)
+    with nbx.FarFieldNet(6_000_000) as net:
+        for n, seed, steps in ((200_000, 0, 0), (200_000, 1, 60)):
+            sim = nbx.Simulation(n, seed=seed)
+            sim.theta = 0.5
+            if steps:
+                sim.step(steps)
+            sim.law, sim.theta = nbx.LAW_ACCURATE_QUAD, 1.2
+            sim.build()
+            net.generate(sim, rng.choice(n, 40000, replace=False).astype(np.uint32), 1.2)
+            sim.close()
+        assert net.count > 2_000_000
+        net.params = nbx.init_mlp_params(3)
+        for k, lr in enumerate((2e-3, 1e-3, 3e-4, 1e-4)):
+            losses = net.train(800, batch=16384, lr=lr, seed=k)
+        assert np.all(np.isfinite(losses))
+        sim, tgt, ex, ey = _held_out(nbx, 150_000, 5, 30)
+        try:
+            net.attach(sim)
+            learned = _err(nbx, sim, tgt, ex, ey, nbx.LAW_LEARNED, 1.2)
+            net.params = np.zeros(P, np.float32)
+            net.attach(sim)
+            ranged = _err(nbx, sim, tgt, ex, ey, nbx.LAW_LEARNED, 1.2)
+            plain = _err(nbx, sim, tgt, ex, ey, nbx.LAW_ACCURATE, 1.2)
+        finally:
+            sim.close()
+    assert learned < 0.8 * ranged, (learned, ranged)
+    assert learned < 0.3 * plain, (learned, plain)
+
+
+def test_shipped_network_gain_at_1m(nbx):
+    """The committed network (farfield_train.py: 50M pairs from 1M-body
+    discs and mergers) on a 1M disc it was not trained on (seed 2, 80 steps):
+    at theta = 1.2 it cuts the error of the range rule alone by more than a
+    third and of the plain walk by more than 5x."""
+    params, meta = nbx.load_farfield_params()
+    assert meta["pairs"] >= 50_000_000
+    sim, tgt, ex, ey = _held_out(nbx, 1_000_000, 2, 80)
+    try:
+        with nbx.FarFieldNet(16) as net:
+            net.params = params
+            net.attach(sim)
+            learned = _err(nbx, sim, tgt, ex, ey, nbx.LAW_LEARNED, 1.2)
+            net.params = np.zeros(P, np.float32)
+            net.attach(sim)
+            ranged = _err(nbx, sim, tgt, ex, ey, nbx.LAW_LEARNED, 1.2)
+        plain = _err(nbx, sim, tgt, ex, ey, nbx.LAW_ACCURATE, 1.2)
+    finally:
+        sim.close()
+    assert learned < 0.66 * ranged, (learned, ranged)
+    assert learned < 0.2 * plain, (learned, plain)

@@ -59,6 +59,31 @@ def test_gradient_matches_finite_differences(oracle):
                 name, int(k), fd, g[k])
 
 
+def test_weighted_gradient_matches_finite_differences(oracle):
+    """The weighted loss sum w |y - t|^2 / (count wnorm): its gradient against
+    central differences, and unit weights with wnorm 1 give the plain MSE."""
+    p = init_params(13)
+    f, t = random_pairs(20, 14)
+    w = np.random.default_rng(15).exponential(1.0, 20).astype(np.float32)
+    l1, g1 = oracle.mlp_loss_grad(p, f, t)
+    l2, g2 = oracle.mlp_loss_grad(p, f, t, weights=np.ones(20, np.float32), wnorm=1.0)
+    assert l1 == l2 and np.array_equal(g1, g2)
+    loss, g = oracle.mlp_loss_grad(p, f, t, weights=w, wnorm=0.7)
+    y = oracle.mlp_forward(p, f).astype(np.float64)
+    assert loss == pytest.approx(float((w * ((y - t) ** 2).sum(1)).sum() / (20 * 0.7)),
+                                 rel=1e-6)
+    rng = np.random.default_rng(16)
+    for k in rng.choice(P, 12, replace=False):
+        h = 1e-2 * max(abs(float(p[k])), 0.05)
+        pp, pm = p.copy(), p.copy()
+        pp[k] = np.float32(p[k] + h)
+        pm[k] = np.float32(p[k] - h)
+        lp, _ = oracle.mlp_loss_grad(pp, f, t, weights=w, wnorm=0.7)
+        lm, _ = oracle.mlp_loss_grad(pm, f, t, weights=w, wnorm=0.7)
+        fd = (lp - lm) / (float(pp[k]) - float(pm[k]))
+        assert abs(fd - g[k]) <= 1e-5 * max(1.0, abs(g[k])) + 2e-4 * abs(g[k])
+
+
 def test_zero_network_zero_gradient_of_weights(oracle):
     """With every weight zero the output is the output bias, the loss is the
     mean squared (b4 - t), and only b4 gets a gradient: 2 mean(b4 - t)."""
@@ -116,8 +141,10 @@ def _node_tree(com, mass, size, start, end):
 def test_residual_closed_forms(oracle):
     """Residual of a node of one body is exactly zero; of two equal bodies at
     x = +-a seen from a target on their axis at distance r (eps = 0) it is
-    m (1/(r-a)^2 + 1/(r+a)^2 - 2/r^2) toward them, scaled by r^2 / M * 16;
-    the features are the hand-computed ones."""
+    m (1/(r-a)^2 + 1/(r+a)^2 - 2/r^2) toward them, in units of
+    u = M/r^2 * a^2/r^2 (monopole times quadrupole scale) times 0.5; the
+    loss weight is (u / monopole)^2 and the features are the hand-computed
+    ones."""
     b = Bodies.empty(2)
     b.px[:] = [0.0, 30.0]
     b.m[:] = [2.0, 1.0]
@@ -131,9 +158,13 @@ def test_residual_closed_forms(oracle):
     b.px[:] = [-a, a, r]
     b.m[:] = [m, m, 1e-6]
     t = _node_tree((0.0, 0.0), 2 * m, 2.0, 0, 2)
-    f, lab = oracle.mlp_features(b, t, [2], [0], eps2=0.0)
+    f, lab, w = oracle.mlp_features(b, t, [2], [0], eps2=0.0, weights=True)
+    unit = 2 * m / r ** 2 * (a * a / r ** 2)
+    assert w[0] == pytest.approx((a * a / r ** 2) ** 2, rel=1e-6)  # only pair
     want = m * (1 / (r + a) ** 2 + 1 / (r - a) ** 2 - 2 / r ** 2)  # toward -x
-    assert lab[0, 0] == pytest.approx(-want * r ** 2 / (2 * m) * 16.0, rel=1e-6)
+    assert lab[0, 0] == pytest.approx(-want / unit * 0.5, rel=1e-6)
+    # The leading (quadrupole) order of that residual: 6 m a^2 / r^4.
+    assert lab[0, 0] == pytest.approx(-6 * m * a * a / r ** 4 / unit * 0.5, rel=2e-3)
     assert lab[0, 1] == 0.0
     # M = 2, d = (-r, 0), S/M = diag(a^2, 0), k = 8 / r^2.
     k = 8.0 / r ** 2
@@ -147,7 +178,7 @@ def test_residual_closed_forms(oracle):
     # Rotated a quarter turn (bodies on y, target on y): radial is S_yy.
     b.px[:], b.py[:] = 0.0, [-a, a, r]
     f, lab = oracle.mlp_features(b, t, [2], [0], eps2=0.0)
-    assert lab[0, 1] == pytest.approx(-want * r ** 2 / (2 * m) * 16.0, rel=1e-6)
+    assert lab[0, 1] == pytest.approx(-want / unit * 0.5, rel=1e-6)
     assert f[0, 6] == pytest.approx(a * a * k, rel=1e-7)
     assert f[0, 7] == pytest.approx(a * a * k, rel=1e-7) and abs(f[0, 9]) < 1e-12
 
@@ -195,8 +226,9 @@ def test_learned_walk_reduces_to_plain_walk(oracle):
 
 def test_learned_walk_adds_the_network_output(oracle):
     """With a network whose only non-zero parameter is the output bias b4,
-    every in-range far node adds b4 * M / (r^2 + eps^2) / 16 to the
-    monopole-only walk."""
+    every in-range far node adds b4 * u / 0.5 to the monopole-only walk,
+    u = M / (r^2 + eps^2) * mu the output unit, mu = tr(S/M) / (r^2 + eps^2)
+    (from the node's inputs: (f4 + f6) / 8)."""
     b = oracle.uniform_disc(2000, seed=4)
     srt, hi, lo, perm, quad = oracle.sorted_state(b)
     t = oracle.build_tree(srt, hi, lo, quad)
@@ -209,10 +241,13 @@ def test_learned_walk_adds_the_network_output(oracle):
     pt, pn = oracle.mlp_pairs(srt, t, targets, 1.2)
     dx = t.comX[pn].astype(np.float64) - srt.px[pt]
     dy = t.comY[pn].astype(np.float64) - srt.py[pt]
-    s = t.mass[pn] / (dx * dx + dy * dy + 1.0) / 16.0
+    f, _ = oracle.mlp_features(srt, t, pt, pn)
+    mu = (f[:, 4].astype(np.float64) + f[:, 6]) / 8.0
+    s = t.mass[pn] / (dx * dx + dy * dy + 1.0) * mu / 0.5
     want_x = np.zeros(2000)
     want_y = np.zeros(2000)
     np.add.at(want_x, pt, 0.5 * s)
     np.add.at(want_y, pt, -0.25 * s)
-    assert np.allclose(ax1 - ax0, want_x[targets], rtol=1e-9, atol=1e-12)
-    assert np.allclose(ay1 - ay0, want_y[targets], rtol=1e-9, atol=1e-12)
+    assert np.allclose(ax1 - ax0, want_x[targets], rtol=1e-5, atol=1e-12)
+    assert np.allclose(ay1 - ay0, want_y[targets], rtol=1e-5, atol=1e-12)
+    assert np.abs(ax1 - ax0).max() > 0

@@ -10,7 +10,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SCRIPTS = ["n-body-simulation_amd/ensemble.py", "n-body-simulation_amd/precision_study.py",
-           "n-body-simulation_amd/snapshot.py", "bench.py"]
+           "n-body-simulation_amd/snapshot.py", "n-body-simulation_amd/farfield_train.py",
+           "bench.py"]
 
 
 @pytest.mark.parametrize("script", SCRIPTS)
@@ -62,3 +63,8 @@ def test_bench_json_line():
         assert k in cb, k
     assert cb["value"] > 0 and cb["cores"] >= 1 and cb["kind"] in ("reference", "port")
     assert cb["steps"] == 2 and cb["cpu_model"] and cb["gpu_value_same_window"] > 0
+    eq = d["equal_error"]
+    assert eq["learned"]["theta"] == 1.2 and eq["plain"]["rms"] <= eq["learned"]["rms"]
+    assert eq["learned"]["steps_per_s"] > 0 and eq["plain"]["steps_per_s"] > 0
+    assert eq["speedup_learned_over_plain"] == pytest.approx(
+        eq["learned"]["steps_per_s"] / eq["plain"]["steps_per_s"], rel=1e-3)

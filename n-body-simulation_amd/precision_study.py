@@ -5,8 +5,10 @@ degrees in-plane; nbx_init_scenario's merger).
 
 * Force error of the compensated walk against the FP64 direct sum
   (nbx_exact_forces, the law m d / (|d| (|d|^2 + eps^2))) on seeded targets
-  at several theta, for both force laws: the norm-weighted RMS
-  sqrt(sum |da|^2 / sum |a|^2), the median and the maximum of |da| / |a|.
+  at several theta, for every force law (reference, accurate, accurate with
+  the quadrupole term, and the learned correction with the shipped network):
+  the norm-weighted RMS sqrt(sum |da|^2 / sum |a|^2), the median and the
+  maximum of |da| / |a|.
 * Equal-error comparison: the quadrupole law (NBX_LAW_ACCURATE_QUAD) at
   --quad-theta against the accurate law at the largest theta of a grid whose
   RMS error is no larger; both timed over --time-steps steps of the full
@@ -83,8 +85,13 @@ def main() -> int:
                                  replace=False)).astype(np.uint32)
 
     errors = {}
+    params, _ = nbx.load_farfield_params()
+    net = nbx.FarFieldNet(16)
+    net.params = params
+    net.attach(sim)
     for law_name, law in (("reference", nbx.LAW_REFERENCE), ("accurate", nbx.LAW_ACCURATE),
-                          ("accurate+quadrupole", nbx.LAW_ACCURATE_QUAD)):
+                          ("accurate+quadrupole", nbx.LAW_ACCURATE_QUAD),
+                          ("learned", nbx.LAW_LEARNED)):
         sim.law = law
         for theta in args.thetas:
             sim.theta = theta
@@ -108,6 +115,7 @@ def main() -> int:
             plain = (theta, e)
             break
     sim.close()
+    net.close()
     equal = {"quad_theta": args.quad_theta, "quad_error": quad_err}
     configs = [("quadrupole", nbx.LAW_ACCURATE_QUAD, args.quad_theta)]
     if plain is not None:

@@ -390,6 +390,46 @@ int nbx_mlp_predict(nbx_mlp *mlp, uint64_t count, const float *features,
  * NBX_LAW_LEARNED (later changes to the network need another call). */
 int nbx_set_mlp(nbx_sim *sim, nbx_mlp *mlp);
 
+/* ---- Ensembles: many independent simulations stepped together (no
+ * reference counterpart; each instance is a CudaSimulation of its own,
+ * src/cuda/CudaSimulation.cuh:25-51, stepped as CudaSimulation::step,
+ * CudaSimulation.cu:51-58). A step of every instance is two dispatches
+ * whatever the number of instances: a persistent kernel in which each
+ * workgroup takes whole instances (largest first) through iterate, collide
+ * and build, and one walk over all instances' bodies; steps replay a
+ * captured HIP graph. Results are bit-identical to stepping each instance
+ * with an nbx_sim of its own (reference or accurate law, compensated
+ * summation, sequential collide). ---- */
+typedef struct nbx_ensemble nbx_ensemble;
+
+/* `count` instances (at most 65536) with room for capacity[k] <= 2^20
+ * bodies each; all memory allocated here. Instances start empty. */
+int nbx_ensemble_create(uint32_t count, const uint64_t *capacity,
+                        nbx_ensemble **out);
+int nbx_ensemble_destroy(nbx_ensemble *ens);
+
+/* Loads instance k: makeScenario's initial condition (NBX_SCENARIO_*, as
+ * nbx_init_scenario) or given bodies (as nbx_upload; ax/ay may be null). */
+int nbx_ensemble_init(nbx_ensemble *ens, uint32_t k, int scenario, uint64_t n,
+                      uint32_t seed);
+int nbx_ensemble_upload(nbx_ensemble *ens, uint32_t k, uint64_t n,
+                        const float *px, const float *py, const float *vx,
+                        const float *vy, const float *ax, const float *ay,
+                        const float *m, const float *r, const uint32_t *id);
+
+/* Instance k's bodies in device order (any pointer may be null). */
+int nbx_ensemble_download(nbx_ensemble *ens, uint32_t k, float *px, float *py,
+                          float *vx, float *vy, float *ax, float *ay, float *m,
+                          float *r, uint32_t *id);
+
+/* Parameters shared by all instances: law NBX_LAW_REFERENCE or
+ * NBX_LAW_ACCURATE, compensated summation, the sequential collide. */
+int nbx_ensemble_set_params(nbx_ensemble *ens, const nbx_params *p);
+
+/* `steps` steps of every instance; returns after they finished, with
+ * NBX_ERR_CAPACITY if an instance overflowed its node or pair storage. */
+int nbx_ensemble_step(nbx_ensemble *ens, uint32_t steps);
+
 #ifdef __cplusplus
 }
 #endif

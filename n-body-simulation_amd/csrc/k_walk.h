@@ -292,8 +292,13 @@ struct RegStack {
   // v_writelane, one per field, with the lane select in M0 (gfx9 reads one
   // SGPR per VALU instruction besides M0); e is computed on the scalar unit.
   __device__ inline void put(int e, int nd, uint64_t m) {
-    const int mlo = static_cast<int>(static_cast<uint32_t>(m));
-    const int mhi = static_cast<int>(static_cast<uint32_t>(m >> 32));
+    // The values are wave-uniform; readfirstlane states it where the
+    // compiler could not prove it (a no-op on values already scalar).
+    nd = __builtin_amdgcn_readfirstlane(nd);
+    e = __builtin_amdgcn_readfirstlane(e);
+    const int mlo = __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(m)));
+    const int mhi =
+        __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(m >> 32)));
     asm volatile(
         "v_writelane_b32 %0, %4, %3\n\t"
         "v_writelane_b32 %1, %5, %3\n\t"

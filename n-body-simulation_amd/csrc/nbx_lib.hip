@@ -10,6 +10,7 @@
 #include "nbx.h"
 
 #include "k_collide.h"
+#include "k_ensemble.h"
 #include "k_jacobi.h"
 #include "k_sort.h"
 #include "k_tree.h"
@@ -2186,6 +2187,411 @@ int nbx_set_mlp(nbx_sim *s, nbx_mlp *m) {
                          hipMemcpyDeviceToDevice, s->stream));
   NBX_HIP(hipStreamSynchronize(s->stream));
   s->mlpLoaded = true;
+  return NBX_OK;
+}
+
+} // extern "C"
+
+// ---- Ensembles: many independent simulations stepped together (two
+//      dispatches a step for all of them, captured in a HIP graph). ----
+
+struct nbx_ensemble {
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  std::vector<EnsInst> inst; // host copy (pointers, sizes; `cur` mirrored)
+  std::vector<uint64_t> cap;
+  EnsInst *dInst = nullptr;
+  uint32_t *dOrder = nullptr, *dWaveInst = nullptr, *dQueue = nullptr;
+  uint32_t totalWaves = 0;
+  nbx_params params{};
+  uint32_t prepareGrid = 0;
+  uint64_t frame = 0;
+  hipGraphExec_t graph = nullptr;
+  bool instDirty = true; // host table changed: upload before the next step
+};
+
+namespace {
+
+template <class T>
+int ensAlloc(nbx_ensemble *e, T **p, uint64_t count) {
+  void *q = nullptr;
+  const size_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+  if (hipMalloc(&q, bytes) != hipSuccess) {
+    return setError(NBX_ERR_HIP, "hipMalloc of " + std::to_string(bytes) +
+                                     " bytes failed");
+  }
+  e->allocs.push_back(q);
+  if (hipMemset(q, 0, bytes) != hipSuccess) {
+    return setError(NBX_ERR_HIP, "hipMemset failed");
+  }
+  *p = static_cast<T *>(q);
+  return NBX_OK;
+}
+
+int ensCheck(nbx_ensemble *e, uint32_t k) {
+  if (!e) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  if (k >= e->inst.size()) {
+    return setError(NBX_ERR_ARG, "instance index out of range");
+  }
+  return NBX_OK;
+}
+
+void ensDropGraph(nbx_ensemble *e) {
+  if (e->graph) {
+    (void)hipGraphExecDestroy(e->graph);
+    e->graph = nullptr;
+  }
+}
+
+// Instance order for the persistent kernel (largest first) and the flat
+// wave table of the walk.
+int ensLayout(nbx_ensemble *e) {
+  const uint32_t count = static_cast<uint32_t>(e->inst.size());
+  std::vector<uint32_t> order(count);
+  for (uint32_t k = 0; k < count; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return e->inst[a].n > e->inst[b].n;
+  });
+  std::vector<uint32_t> waveInst;
+  for (uint32_t k = 0; k < count; ++k) {
+    e->inst[k].waveBase = static_cast<uint32_t>(waveInst.size());
+    const uint32_t waves = (e->inst[k].n + kWave - 1) / kWave;
+    waveInst.insert(waveInst.end(), waves, k);
+  }
+  e->totalWaves = static_cast<uint32_t>(waveInst.size());
+  NBX_HIP(hipMemcpy(e->dOrder, order.data(), count * sizeof(uint32_t),
+                    hipMemcpyHostToDevice));
+  if (!waveInst.empty()) {
+    NBX_HIP(hipMemcpy(e->dWaveInst, waveInst.data(),
+                      waveInst.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  e->instDirty = true;
+  ensDropGraph(e);
+  return NBX_OK;
+}
+
+int ensPushTable(nbx_ensemble *e) {
+  if (e->instDirty) {
+    NBX_HIP(hipMemcpyAsync(e->dInst, e->inst.data(), e->inst.size() * sizeof(EnsInst),
+                           hipMemcpyHostToDevice, e->stream));
+    NBX_HIP(hipStreamSynchronize(e->stream));
+    e->instDirty = false;
+  }
+  return NBX_OK;
+}
+
+// `cur` of every instance as the device has it.
+int ensPullCur(nbx_ensemble *e) {
+  std::vector<EnsInst> d(e->inst.size());
+  NBX_HIP(hipMemcpyAsync(d.data(), e->dInst, d.size() * sizeof(EnsInst),
+                         hipMemcpyDeviceToHost, e->stream));
+  NBX_HIP(hipStreamSynchronize(e->stream));
+  for (size_t k = 0; k < d.size(); ++k) {
+    e->inst[k].cur = d[k].cur;
+  }
+  return NBX_OK;
+}
+
+int ensLaunchStep(nbx_ensemble *e) {
+  const float tSq = e->params.theta * e->params.theta;
+  const float eSq = e->params.epsilon * e->params.epsilon;
+  EnsParams p{e->params.dt, tSq, eSq, e->params.collisions};
+  hipLaunchKernelGGL(k_ens_reset, dim3(1), dim3(1), 0, e->stream, e->dQueue);
+  hipLaunchKernelGGL(k_ens_prepare, dim3(e->prepareGrid), dim3(kEnsBlock), 0,
+                     e->stream, e->dInst, e->dOrder,
+                     static_cast<uint32_t>(e->inst.size()), p, e->dQueue);
+  const uint32_t walkBlocks =
+      (e->totalWaves + kEnsWalkBlock / kWave - 1) / (kEnsWalkBlock / kWave);
+  if (walkBlocks) {
+    if (e->params.law == NBX_LAW_ACCURATE) {
+      hipLaunchKernelGGL(k_ens_walk<kLawAccurate>, dim3(walkBlocks),
+                         dim3(kEnsWalkBlock), 0, e->stream, e->dInst, e->dWaveInst,
+                         e->totalWaves, tSq, eSq);
+    } else {
+      hipLaunchKernelGGL(k_ens_walk<kLawReference>, dim3(walkBlocks),
+                         dim3(kEnsWalkBlock), 0, e->stream, e->dInst, e->dWaveInst,
+                         e->totalWaves, tSq, eSq);
+    }
+  }
+  return checkLaunch();
+}
+
+} // namespace
+
+extern "C" {
+
+int nbx_ensemble_create(uint32_t count, const uint64_t *capacity,
+                        nbx_ensemble **out) {
+  if (!out || (count && !capacity)) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  *out = nullptr;
+  if (count == 0 || count > 65536) {
+    return setError(NBX_ERR_ARG, "instance count must be in [1, 65536]");
+  }
+  for (uint32_t k = 0; k < count; ++k) {
+    if (capacity[k] == 0 || capacity[k] > (1ull << 20)) {
+      return setError(NBX_ERR_ARG, "instance capacity must be in [1, 2^20]");
+    }
+  }
+  nbx_ensemble *e = new nbx_ensemble();
+  auto fail = [&](int rc) {
+    nbx_ensemble_destroy(e);
+    return rc;
+  };
+  nbx_default_params(&e->params);
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return setError(NBX_ERR_HIP, "hipStreamCreateWithFlags failed");
+  }
+  e->inst.resize(count);
+  e->cap.assign(capacity, capacity + count);
+  int rc = NBX_OK;
+#define EA(ptr, cnt)                                                          \
+  if ((rc = ensAlloc(e, &(ptr), (cnt))) != NBX_OK)                            \
+    return fail(rc);
+  for (uint32_t k = 0; k < count; ++k) {
+    EnsInst &I = e->inst[k];
+    const uint64_t n = capacity[k];
+    I.n = 0;
+    I.cur = 0;
+    I.buckets = static_cast<uint32_t>(std::max<uint64_t>(1024, pow2AtLeast(2 * n)));
+    I.pairCap = static_cast<uint32_t>(8 * n + 1024);
+    I.nodeCap = static_cast<int32_t>(8 * n + 8);
+    I.cell = I.invCell = 1.0f;
+    for (int b = 0; b < 2; ++b) {
+      EA(I.px[b], n) EA(I.py[b], n) EA(I.vx[b], n) EA(I.vy[b], n)
+      EA(I.m[b], n) EA(I.r[b], n) EA(I.id[b], n)
+      EA(I.keyHi[b], n) EA(I.keyLo[b], n) EA(I.val[b], n)
+    }
+    EA(I.ax, n) EA(I.ay, n) EA(I.body, n + kBodyPad)
+    EA(I.bucketCount, I.buckets + 1) EA(I.bucketRank, n) EA(I.bucketOf, n)
+    I.bucketStart = I.bucketCount;
+    EA(I.grid, 2 * n) EA(I.largeList, n) EA(I.largeBox, n)
+    EA(I.pairs, I.pairCap) EA(I.seg, I.pairCap) EA(I.parent, n)
+    EA(I.compCount, n + 1) EA(I.compStart, n + 1)
+    EA(I.nodeF, I.nodeCap) EA(I.nodeI, I.nodeCap) EA(I.centerX, I.nodeCap)
+    EA(I.centerY, I.nodeCap) EA(I.size, I.nodeCap) EA(I.wsum, I.nodeCap)
+    EA(I.ctl, 1)
+  }
+  EA(e->dInst, count) EA(e->dOrder, count) EA(e->dQueue, 1)
+  {
+    uint64_t waves = 0;
+    for (uint32_t k = 0; k < count; ++k) waves += (capacity[k] + kWave - 1) / kWave;
+    EA(e->dWaveInst, waves)
+  }
+#undef EA
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+          hipSuccess) {
+    return fail(setError(NBX_ERR_HIP, "device query failed"));
+  }
+  e->prepareGrid = std::min<uint32_t>(count, 2u * static_cast<uint32_t>(cus));
+  if ((rc = ensLayout(e)) != NBX_OK) {
+    return fail(rc);
+  }
+  *out = e;
+  return NBX_OK;
+}
+
+int nbx_ensemble_destroy(nbx_ensemble *e) {
+  if (!e) {
+    return NBX_OK;
+  }
+  if (e->stream) {
+    (void)hipStreamSynchronize(e->stream);
+  }
+  ensDropGraph(e);
+  for (void *p : e->allocs) {
+    (void)hipFree(p);
+  }
+  if (e->stream) {
+    (void)hipStreamDestroy(e->stream);
+  }
+  delete e;
+  return NBX_OK;
+}
+
+int nbx_ensemble_upload(nbx_ensemble *e, uint32_t k, uint64_t n, const float *px,
+                        const float *py, const float *vx, const float *vy,
+                        const float *ax, const float *ay, const float *m,
+                        const float *r, const uint32_t *id) {
+  NBX_TRY(ensCheck(e, k));
+  if (n && (!px || !py || !vx || !vy || !m || !r || !id)) {
+    return setError(NBX_ERR_ARG, "null array");
+  }
+  if (n > e->cap[k]) {
+    return setError(NBX_ERR_ARG, "n exceeds the instance's capacity");
+  }
+  NBX_HIP(hipStreamSynchronize(e->stream));
+  NBX_TRY(ensPullCur(e));
+  EnsInst &I = e->inst[k];
+  I.n = static_cast<uint32_t>(n);
+  I.cur = 0;
+  // The grid cell: 1.01 x the median box width (Collision.cpp:173-183),
+  // fixed by the radii, which never change. Any positive cell gives the
+  // same pair set; this one keeps the stencil small.
+  float cell = 1.0f;
+  if (n > 0) {
+    std::vector<float> radii(r, r + n);
+    std::nth_element(radii.begin(), radii.begin() + n / 2, radii.end());
+    cell = 2.0f * radii[n / 2] * 1.01f;
+  }
+  if (!(cell > 0.0f) || !std::isfinite(cell)) {
+    cell = 1.0f;
+  }
+  I.cell = cell;
+  I.invCell = 1.0f / cell;
+  const size_t fb = n * sizeof(float);
+  NBX_HIP(hipMemcpy(I.px[0], px, fb, hipMemcpyHostToDevice));
+  NBX_HIP(hipMemcpy(I.py[0], py, fb, hipMemcpyHostToDevice));
+  NBX_HIP(hipMemcpy(I.vx[0], vx, fb, hipMemcpyHostToDevice));
+  NBX_HIP(hipMemcpy(I.vy[0], vy, fb, hipMemcpyHostToDevice));
+  NBX_HIP(hipMemcpy(I.m[0], m, fb, hipMemcpyHostToDevice));
+  NBX_HIP(hipMemcpy(I.r[0], r, fb, hipMemcpyHostToDevice));
+  NBX_HIP(hipMemcpy(I.id[0], id, n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (ax) {
+    NBX_HIP(hipMemcpy(I.ax, ax, fb, hipMemcpyHostToDevice));
+  } else {
+    NBX_HIP(hipMemset(I.ax, 0, fb));
+  }
+  if (ay) {
+    NBX_HIP(hipMemcpy(I.ay, ay, fb, hipMemcpyHostToDevice));
+  } else {
+    NBX_HIP(hipMemset(I.ay, 0, fb));
+  }
+  NBX_HIP(hipMemset(I.ctl, 0, sizeof(Ctl)));
+  return ensLayout(e);
+}
+
+int nbx_ensemble_init(nbx_ensemble *e, uint32_t k, int scenario, uint64_t n,
+                      uint32_t seed) {
+  NBX_TRY(ensCheck(e, k));
+  if (n > e->cap[k]) {
+    return setError(NBX_ERR_ARG, "n exceeds the instance's capacity");
+  }
+  std::vector<float> px(n), py(n), vx(n), vy(n), m(n), r(n);
+  std::vector<uint32_t> id(n);
+  switch (scenario) {
+  case NBX_SCENARIO_DISC:
+    nbx_host::uniformDisc(n, seed, px.data(), py.data(), vx.data(), vy.data(),
+                          m.data(), r.data(), id.data());
+    break;
+  case NBX_SCENARIO_TWO_GALAXIES:
+  case NBX_SCENARIO_MERGER:
+    if (n < 2) {
+      return setError(NBX_ERR_ARG, "the two-disc scenarios need n >= 2");
+    }
+    (scenario == NBX_SCENARIO_MERGER ? nbx_host::galaxyMerger : nbx_host::twoGalaxies)(
+        n, seed, px.data(), py.data(), vx.data(), vy.data(), m.data(), r.data(),
+        id.data());
+    break;
+  case NBX_SCENARIO_COLD_SQUARE:
+    nbx_host::coldSquare(n, seed, px.data(), py.data(), vx.data(), vy.data(),
+                         m.data(), r.data(), id.data());
+    break;
+  default:
+    return setError(NBX_ERR_ARG, "unknown scenario");
+  }
+  return nbx_ensemble_upload(e, k, n, px.data(), py.data(), vx.data(), vy.data(),
+                             nullptr, nullptr, m.data(), r.data(), id.data());
+}
+
+int nbx_ensemble_download(nbx_ensemble *e, uint32_t k, float *px, float *py,
+                          float *vx, float *vy, float *ax, float *ay, float *m,
+                          float *r, uint32_t *id) {
+  NBX_TRY(ensCheck(e, k));
+  NBX_HIP(hipStreamSynchronize(e->stream));
+  NBX_TRY(ensPullCur(e));
+  const EnsInst &I = e->inst[k];
+  const int c = static_cast<int>(I.cur);
+  const size_t fb = I.n * sizeof(float);
+  if (I.n == 0) {
+    return NBX_OK;
+  }
+  if (px) NBX_HIP(hipMemcpy(px, I.px[c], fb, hipMemcpyDeviceToHost));
+  if (py) NBX_HIP(hipMemcpy(py, I.py[c], fb, hipMemcpyDeviceToHost));
+  if (vx) NBX_HIP(hipMemcpy(vx, I.vx[c], fb, hipMemcpyDeviceToHost));
+  if (vy) NBX_HIP(hipMemcpy(vy, I.vy[c], fb, hipMemcpyDeviceToHost));
+  if (ax) NBX_HIP(hipMemcpy(ax, I.ax, fb, hipMemcpyDeviceToHost));
+  if (ay) NBX_HIP(hipMemcpy(ay, I.ay, fb, hipMemcpyDeviceToHost));
+  if (m) NBX_HIP(hipMemcpy(m, I.m[c], fb, hipMemcpyDeviceToHost));
+  if (r) NBX_HIP(hipMemcpy(r, I.r[c], fb, hipMemcpyDeviceToHost));
+  if (id) NBX_HIP(hipMemcpy(id, I.id[c], I.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return NBX_OK;
+}
+
+int nbx_ensemble_set_params(nbx_ensemble *e, const nbx_params *p) {
+  if (!e || !p) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  NBX_TRY(validParams(p));
+  if (p->law != NBX_LAW_REFERENCE && p->law != NBX_LAW_ACCURATE) {
+    return setError(NBX_ERR_ARG, "ensembles support the reference and accurate laws");
+  }
+  if (p->summation != NBX_SUM_COMPENSATED ||
+      p->collide_mode != NBX_COLLIDE_SEQUENTIAL) {
+    return setError(NBX_ERR_ARG,
+                    "ensembles run compensated summation and the sequential collide");
+  }
+  if (std::memcmp(&e->params, p, sizeof *p) != 0) {
+    NBX_HIP(hipStreamSynchronize(e->stream));
+    ensDropGraph(e);
+    e->params = *p;
+  }
+  return NBX_OK;
+}
+
+int nbx_ensemble_step(nbx_ensemble *e, uint32_t steps) {
+  if (!e) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  NBX_TRY(ensPushTable(e));
+  if (e->params.use_graph && !e->graph && steps > 0) {
+    hipGraph_t g = nullptr;
+    NBX_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = ensLaunchStep(e);
+    hipError_t ec = hipStreamEndCapture(e->stream, &g);
+    if (rc != NBX_OK || ec != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc != NBX_OK ? rc : setError(NBX_ERR_HIP, "graph capture failed");
+    }
+    ec = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ec != hipSuccess) {
+      e->graph = nullptr;
+      return setError(NBX_ERR_HIP, "graph instantiation failed");
+    }
+  }
+  for (uint32_t s = 0; s < steps; ++s) {
+    if (e->graph) {
+      NBX_HIP(hipGraphLaunch(e->graph, e->stream));
+    } else {
+      NBX_TRY(ensLaunchStep(e));
+    }
+    ++e->frame;
+  }
+  NBX_HIP(hipStreamSynchronize(e->stream));
+  // Device-side failures (capacity) of any instance.
+  std::vector<Ctl> ctl(e->inst.size());
+  for (size_t k = 0; k < e->inst.size(); ++k) {
+    NBX_HIP(hipMemcpyAsync(&ctl[k], e->inst[k].ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
+                           e->stream));
+  }
+  NBX_HIP(hipStreamSynchronize(e->stream));
+  for (size_t k = 0; k < ctl.size(); ++k) {
+    if (ctl[k].status & (kStatusNodeOverflow | kStatusPairOverflow)) {
+      return setError(NBX_ERR_CAPACITY, "instance " + std::to_string(k) +
+                                            " exceeded its node or pair capacity");
+    }
+    if (ctl[k].status & kStatusTreeInconsistent) {
+      return setError(NBX_ERR_STATE, "instance " + std::to_string(k) +
+                                         ": inconsistent tree");
+    }
+  }
   return NBX_OK;
 }
 

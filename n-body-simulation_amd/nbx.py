@@ -42,6 +42,9 @@ EXPORTS = (
     "nbx_mlp_count", "nbx_mlp_clear", "nbx_mlp_generate", "nbx_mlp_upload",
     "nbx_mlp_download", "nbx_mlp_gradient", "nbx_mlp_train", "nbx_mlp_predict",
     "nbx_set_mlp", "nbx_mlp_last_pairs",
+    "nbx_ensemble_create", "nbx_ensemble_destroy", "nbx_ensemble_init",
+    "nbx_ensemble_upload", "nbx_ensemble_download", "nbx_ensemble_set_params",
+    "nbx_ensemble_step",
 )
 
 MLP_PARAMS = 9154
@@ -139,6 +142,13 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_mlp_predict.argtypes = [_vp, C.c_uint64, _f32p, _f32p]
     L.nbx_set_mlp.argtypes = [_vp, _vp]
     L.nbx_mlp_last_pairs.argtypes = [_vp, C.c_uint64, _vp, _vp, C.POINTER(C.c_uint64)]
+    L.nbx_ensemble_create.argtypes = [C.c_uint32, _vp, C.POINTER(_vp)]
+    L.nbx_ensemble_destroy.argtypes = [_vp]
+    L.nbx_ensemble_init.argtypes = [_vp, C.c_uint32, C.c_int32, C.c_uint64, C.c_uint32]
+    L.nbx_ensemble_upload.argtypes = [_vp, C.c_uint32, C.c_uint64] + [_vp] * 9
+    L.nbx_ensemble_download.argtypes = [_vp, C.c_uint32] + [_vp] * 9
+    L.nbx_ensemble_set_params.argtypes = [_vp, C.POINTER(Params)]
+    L.nbx_ensemble_step.argtypes = [_vp, C.c_uint32]
     _lib = L
     return L
 
@@ -536,3 +546,76 @@ def load_farfield_params(path: str = WEIGHTS_PATH):
     if params.shape != (MLP_PARAMS,):
         raise ValueError(f"{path}: expected {MLP_PARAMS} parameters")
     return params, meta
+
+
+class Ensemble:
+    """Many independent simulations stepped together (nbx_ensemble): every
+    instance's step in two dispatches, replayed as a HIP graph. Instances are
+    loaded with init() (makeScenario's initial conditions) or upload(); the
+    attributes dt, theta, epsilon, collisions and law apply to all."""
+
+    def __init__(self, capacities):
+        self.L = load_library()
+        caps = np.ascontiguousarray(capacities, np.uint64)
+        h = _vp()
+        self._check(self.L.nbx_ensemble_create(caps.size, caps.ctypes.data, C.byref(h)))
+        self.h = h
+        self.count = int(caps.size)
+        self.sizes = [0] * self.count
+        p = Params()
+        self.L.nbx_default_params(C.byref(p))
+        self.dt, self.theta, self.epsilon = p.dt, p.theta, p.epsilon
+        self.collisions, self.law, self.use_graph = True, LAW_REFERENCE, True
+
+    def _check(self, rc: int) -> None:
+        if rc != NBX_OK:
+            raise NbxError(rc, self.L.nbx_last_error().decode())
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.L.nbx_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init(self, k: int, n: int, scenario: int = SCENARIO_DISC, seed: int = 0) -> None:
+        self._check(self.L.nbx_ensemble_init(self.h, k, scenario, n, seed))
+        self.sizes[k] = int(n)
+
+    def upload(self, k: int, s) -> None:
+        arr = lambda a, t: None if a is None else np.ascontiguousarray(a, t)
+        n = int(np.asarray(s.px).shape[0])
+        f = [arr(getattr(s, x), np.float32) for x in ("px", "py", "vx", "vy")]
+        ax = arr(getattr(s, "ax", None), np.float32)
+        ay = arr(getattr(s, "ay", None), np.float32)
+        m, r, ids = arr(s.m, np.float32), arr(s.r, np.float32), arr(s.id, np.uint32)
+        self._check(self.L.nbx_ensemble_upload(self.h, k, n, *(_ptr(a) for a in f),
+                                               _ptr(ax), _ptr(ay), _ptr(m), _ptr(r),
+                                               _ptr(ids)))
+        self.sizes[k] = n
+
+    def download(self, k: int) -> State:
+        n = self.sizes[k]
+        z = lambda t=np.float32: np.zeros(n, t)
+        st = State(z(), z(), z(), z(), z(), z(), z(), z(), z(np.uint32))
+        self._check(self.L.nbx_ensemble_download(
+            self.h, k, *(_ptr(getattr(st, f)) for f in
+                         ("px", "py", "vx", "vy", "ax", "ay", "m", "r", "id"))))
+        return st
+
+    def step(self, steps: int = 1) -> None:
+        p = Params(self.dt, self.theta, self.epsilon, int(bool(self.collisions)),
+                   int(self.law), SUM_COMPENSATED, int(bool(self.use_graph)),
+                   COLLIDE_SEQUENTIAL)
+        self._check(self.L.nbx_ensemble_set_params(self.h, C.byref(p)))
+        self._check(self.L.nbx_ensemble_step(self.h, steps))

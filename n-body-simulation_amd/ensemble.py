@@ -1,27 +1,30 @@
 #!/usr/bin/env python3
-"""Batched ensemble on one GPU: many independent simulations stepped side by
-side, against the same simulations run one at a time.
+"""Batched ensemble on one GPU: many independent simulations stepped
+together, against the same simulations run one at a time.
 
 Workload (defaults): 256 instances, N log-uniform in [2000, 64000] (seeded),
 alternately the rotating disc (uniformDisc, seed = instance index) and the
 cold square (makeScenario's ColdSquare: uniform in [-L, L]^2, at rest, equal
 masses), theta = 0.5, collisions on, 500 steps each.
 
-Each instance is its own handle with its own HIP stream. "One at a time"
-steps each instance to the end (nbx_step, one host sync per instance) before
-the next; "together" queues every instance's steps in rounds of --chunk
-steps with nbx_step_async and syncs all handles at the end, so the GPU runs
-the instances' step graphs side by side (as many at once as it has hardware
-queues for the process). Both runs start from the same initial states and
-must end in bit-identical states, which the script checks. The handles are
-created with NBX_CREATE_NO_GRID_BARRIERS: concurrent handles must not use
-the kernels that wait on a grid barrier (those need all of their workgroups
-resident), and nbx_step_async refuses handles that would.
+Three runs from the same initial states, which must end in bit-identical
+states (checked):
+  * one at a time: each instance its own handle, stepped to the end
+    (nbx_step) before the next;
+  * concurrent handles: every handle's steps queued with nbx_step_async in
+    rounds of --chunk steps, each on its own stream (handles created with
+    NBX_CREATE_NO_GRID_BARRIERS), so the GPU runs as many instances side by
+    side as it has hardware queues for the process;
+  * ensemble: one nbx_ensemble holding every instance, a step of all of
+    them two dispatches (a persistent kernel taking whole instances through
+    iterate/collide/build, and one walk over all bodies), replayed as a
+    captured HIP graph.
 
     python n-body-simulation_amd/ensemble.py [--instances 256] [--steps 500]
         [--nmin 2000] [--nmax 64000] [--theta 0.5] [--chunk 10] [--seed 7]
 
-Prints one JSON object: aggregate bodies*steps/s of both runs and their ratio.
+Prints one JSON object: aggregate bodies*steps/s of every run and the
+ratios.
 """
 from __future__ import annotations
 
@@ -65,9 +68,10 @@ def main() -> int:
 
     nbx = load_nbx()
     ns = sizes(args.instances, args.nmin, args.nmax, args.seed)
+    kinds = [nbx.SCENARIO_DISC if i % 2 == 0 else nbx.SCENARIO_COLD_SQUARE
+             for i in range(len(ns))]
     sims, starts = [], []
-    for i, n in enumerate(ns):
-        scenario = nbx.SCENARIO_DISC if i % 2 == 0 else nbx.SCENARIO_COLD_SQUARE
+    for i, (n, scenario) in enumerate(zip(ns, kinds)):
         sim = nbx.Simulation(n, seed=i, scenario=scenario, no_grid_barriers=True)
         sim.theta = args.theta
         sim.collisions = True
@@ -77,25 +81,26 @@ def main() -> int:
     print(f"[ensemble] {len(sims)} instances, {bodies} bodies, {args.steps} steps",
           file=sys.stderr, flush=True)
 
-    # Warm-up (graph capture) outside both timed runs.
+    # Warm-up (graph capture) outside every timed run.
     for sim in sims:
         sim.step(1)
 
+    def reload():
+        for sim, st in zip(sims, starts):
+            sim.upload(st)
+            sim.sync()
+
     # One at a time.
-    for sim, st in zip(sims, starts):
-        sim.upload(st)
-        sim.sync()
+    reload()
     t0 = time.perf_counter()
     for sim in sims:
         sim.step(args.steps)
     one_s = time.perf_counter() - t0
-    ref = [sim.download_by_id() for sim in sims]
+    ref = [sim.download() for sim in sims]
     print(f"[ensemble] one at a time: {one_s:.3f} s", file=sys.stderr, flush=True)
 
-    # Together.
-    for sim, st in zip(sims, starts):
-        sim.upload(st)
-        sim.sync()
+    # Concurrent handles.
+    reload()
     t0 = time.perf_counter()
     done = 0
     while done < args.steps:
@@ -105,17 +110,36 @@ def main() -> int:
         done += k
     for sim in sims:
         sim.sync()
-    together_s = time.perf_counter() - t0
-    print(f"[ensemble] together: {together_s:.3f} s", file=sys.stderr, flush=True)
+    conc_s = time.perf_counter() - t0
+    print(f"[ensemble] concurrent handles: {conc_s:.3f} s", file=sys.stderr, flush=True)
 
-    identical = True
-    for sim, want in zip(sims, ref):
-        got = sim.download_by_id()
-        for f in ("px", "py", "vx", "vy"):
-            if not np.array_equal(got[f].view(np.uint32), want[f].view(np.uint32)):
-                identical = False
+    def same(a, b):
+        return all(np.array_equal(getattr(a, f).view(np.uint32), getattr(b, f).view(np.uint32))
+                   for f in ("px", "py", "vx", "vy", "ax", "ay")) and \
+            np.array_equal(a.id, b.id)
+
+    identical = all(same(sim.download(), want) for sim, want in zip(sims, ref))
     for sim in sims:
         sim.close()
+
+    # The ensemble path.
+    ens = nbx.Ensemble(ns)
+    ens.theta = args.theta
+    ens.collisions = True
+    for i, (n, scenario) in enumerate(zip(ns, kinds)):
+        ens.init(i, n, scenario=scenario, seed=i)
+    ens.step(1)  # graph capture
+    for i, st in enumerate(starts):
+        ens.upload(i, st)
+    ens.step(1)  # capture again after the reload (tables changed)
+    for i, st in enumerate(starts):
+        ens.upload(i, st)
+    t0 = time.perf_counter()
+    ens.step(args.steps)
+    ens_s = time.perf_counter() - t0
+    print(f"[ensemble] ensemble: {ens_s:.3f} s", file=sys.stderr, flush=True)
+    identical = identical and all(same(ens.download(i), want) for i, want in enumerate(ref))
+    ens.close()
 
     work = bodies * args.steps
     out = {
@@ -125,10 +149,13 @@ def main() -> int:
         "bodies": bodies,
         "one_at_a_time": {"seconds": round(one_s, 3),
                           "bodies_steps_per_s": work / one_s},
-        "together": {"seconds": round(together_s, 3),
-                     "bodies_steps_per_s": work / together_s,
-                     "chunk": args.chunk},
-        "speedup": one_s / together_s,
+        "concurrent_handles": {"seconds": round(conc_s, 3),
+                               "bodies_steps_per_s": work / conc_s,
+                               "chunk": args.chunk},
+        "ensemble": {"seconds": round(ens_s, 3), "bodies_steps_per_s": work / ens_s,
+                     "dispatches_per_step": 2},
+        "speedup_ensemble_over_one_at_a_time": one_s / ens_s,
+        "speedup_concurrent_over_one_at_a_time": one_s / conc_s,
         "identical_states": identical,
     }
     print(json.dumps(out), flush=True)

@@ -198,6 +198,29 @@ int nbx_sync(nbx_sim *sim);
  * with NBX_ERR_STATE. */
 int nbx_step_async(nbx_sim *sim, uint32_t steps);
 
+/* ---- Launch shape ----
+ * The reference launches its force walk with one fixed block size
+ * (forceKernel<<<blocksFor(n), kBlock>>>, src/cuda/CudaQuadTree.cu:511); the
+ * best workgroup size of this walk depends on n and theta, so it is a
+ * per-handle setting. */
+
+/* Lanes per workgroup of the compensated force walk (the reference,
+ * accurate and quadrupole laws): 64, 128 (the default) or 256. Every size
+ * gives bit-identical forces; only speed differs. The reference-summation
+ * walk, the counting walk and the learned law keep their own sizes. */
+int nbx_set_walk_block(nbx_sim *sim, uint32_t lanes);
+int nbx_get_walk_block(nbx_sim *sim, uint32_t *lanes);
+
+/* Times `reps` (1..1000) force walks of the current tree with each
+ * workgroup size, keeps the fastest (as nbx_set_walk_block would) and
+ * returns it in *best; `ms` (may be null) receives each size's mean walk
+ * time in milliseconds, in the order 64, 128, 256. With no tree built it
+ * builds one first, which reorders the bodies as nbx_build does; the
+ * accelerations are recomputed, bit-identical to the last force pass on
+ * that tree. NBX_ERR_STATE under the reference summation or the learned
+ * law, whose walks the setting does not reach. */
+int nbx_tune_walk(nbx_sim *sim, uint32_t reps, uint32_t *best, float *ms);
+
 /* ---- Inspection, for checks (CudaQuadTree::download / nodeCount /
  *      levelCount, src/cuda/CudaQuadTree.cuh:42-63). ---- */
 
@@ -429,6 +452,16 @@ int nbx_ensemble_set_params(nbx_ensemble *ens, const nbx_params *p);
 /* `steps` steps of every instance; returns after they finished, with
  * NBX_ERR_CAPACITY if an instance overflowed its node or pair storage. */
 int nbx_ensemble_step(nbx_ensemble *ens, uint32_t steps);
+
+/* Launch shape of nbx_ensemble_step: `walk_lanes` lanes per workgroup of
+ * the walk (64, 128 or 256; the default 128) and `prepare_groups`
+ * workgroups of the per-instance kernel (1..count; the default is twice
+ * the device's compute units, at most count). 0 keeps a setting as it is.
+ * Results are bit-identical for every choice; the tuner times them. */
+int nbx_ensemble_set_launch(nbx_ensemble *ens, uint32_t walk_lanes,
+                            uint32_t prepare_groups);
+int nbx_ensemble_get_launch(nbx_ensemble *ens, uint32_t *walk_lanes,
+                            uint32_t *prepare_groups);
 
 #ifdef __cplusplus
 }

@@ -660,19 +660,20 @@ __global__ void k_ens_reset(uint32_t *__restrict__ queue) { *queue = 0; }
 // The walk of every instance in one grid: wave w of the grid takes the
 // instance whose waves [waveBase, next waveBase) hold it (waveInst maps a
 // wave to its instance).
+// BLOCK: lanes per workgroup (64, 128 or 256; nbx_ensemble_set_launch).
 constexpr int kEnsWalkBlock = 128;
-template <int LAW>
-__global__ __launch_bounds__(kEnsWalkBlock) void k_ens_walk(
+template <int LAW, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_ens_walk(
     const EnsInst *__restrict__ inst, const uint32_t *__restrict__ waveInst,
     uint32_t totalWaves, float tSq, float eSq) {
-  __shared__ float2 slots[kEnsWalkBlock / kWave][kWave];
-  __shared__ int spillNode[kEnsWalkBlock / kWave][kStackDepth - kWave];
-  __shared__ uint64_t spillMask[kEnsWalkBlock / kWave][kStackDepth - kWave];
+  __shared__ float2 slots[BLOCK / kWave][kWave];
+  __shared__ int spillNode[BLOCK / kWave][kStackDepth - kWave];
+  __shared__ uint64_t spillMask[BLOCK / kWave][kStackDepth - kWave];
   const uint32_t w = threadIdx.x >> 6;
   // Wave-uniform (scalar) from here on: the instance's pointers and node
   // records are scalar loads, as in the single-simulation walk.
   const uint32_t gw = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(
-      static_cast<int>(xcdBlock() * (kEnsWalkBlock / kWave) + w)));
+      static_cast<int>(xcdBlock() * (BLOCK / kWave) + w)));
   if (gw >= totalWaves) {
     return;
   }

@@ -781,8 +781,10 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
   }
 }
 
-template <int LAW, bool COUNT, bool CAP, bool MASS>
-__global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
+// BLOCK: lanes per workgroup (64, 128 or 256), the autotuner's choice
+// (nbx_tune_walk); every size gives the same results.
+template <int LAW, bool COUNT, bool CAP, bool MASS, int BLOCK = kWalkBlock>
+__global__ __launch_bounds__(BLOCK) void k_walk_fast(
     uint32_t n, const float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, const float4 *__restrict__ nodeQ,
     const float4 *__restrict__ body,
@@ -791,12 +793,12 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fast(
   if (barrierStalled(ctl)) {
     return;
   }
-  __shared__ float2 slots[kWalkBlock / kWave][kWave];
-  __shared__ int spillNode[kWalkBlock / kWave][kStackDepth - kWave];
-  __shared__ uint64_t spillMask[kWalkBlock / kWave][kStackDepth - kWave];
+  __shared__ float2 slots[BLOCK / kWave][kWave];
+  __shared__ int spillNode[BLOCK / kWave][kStackDepth - kWave];
+  __shared__ uint64_t spillMask[BLOCK / kWave][kStackDepth - kWave];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t base =
-      xcdBlock() * kWalkBlock + (threadIdx.x & ~(kWave - 1));
+      xcdBlock() * BLOCK + (threadIdx.x & ~(kWave - 1));
   if (base >= n ||
       (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent))) {
     return;

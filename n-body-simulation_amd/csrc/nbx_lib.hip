@@ -168,6 +168,7 @@ struct nbx_sim {
   bool treeBuilt = false;
   bool momentsBuilt = false; // the last build computed nodeQ
   bool learnBuilt = false;   // ... and nodeL
+  uint32_t walkBlock = kWalkBlock; // walk workgroup size (nbx_tune_walk)
   bool collided = false;
 
   Ctl *ctl = nullptr;
@@ -581,7 +582,6 @@ int launchForces(nbx_sim *s, bool count) {
   const float tSq = s->params.theta * s->params.theta;
   const float eSq = s->params.epsilon * s->params.epsilon;
   const uint32_t grid = blocksFor(n);
-  const uint32_t walkGrid = blocksFor(n, kWalkBlock);
   const bool ref = s->params.summation == NBX_SUM_REFERENCE;
   const bool acc = s->params.law == NBX_LAW_ACCURATE;
   const bool quad = s->params.law == NBX_LAW_ACCURATE_QUAD;
@@ -598,16 +598,30 @@ int launchForces(nbx_sim *s, bool count) {
   // The node-mass test is needed only when some body mass is not positive
   // and finite (and always by the counting walk, which counts far nodes of
   // positive mass).
+#define FAST_B(LAWV, CNT, CAPV, MASSV, BL)                                    \
+  LAUNCH((k_walk_fast<LAWV, CNT, CAPV, MASSV, BL>), blocksFor(n, BL), BL, n,  \
+         s->nodeF, s->nodeI, s->nodeQ, s->bodyPack, tSq, eSq, s->ax, s->ay,   \
+         s->ctl)
+  // Workgroup size: the handle's (nbx_set_walk_block, nbx_tune_walk) for
+  // the force walk; the counting walk keeps the default.
+#define FAST_MB(LAWV, CNT, CAPV, MASSV)                                       \
+  do {                                                                        \
+    if constexpr (CNT) {                                                      \
+      FAST_B(LAWV, CNT, CAPV, MASSV, kWalkBlock);                             \
+    } else if (s->walkBlock == 64) {                                          \
+      FAST_B(LAWV, CNT, CAPV, MASSV, 64);                                     \
+    } else if (s->walkBlock == 256) {                                         \
+      FAST_B(LAWV, CNT, CAPV, MASSV, 256);                                    \
+    } else {                                                                  \
+      FAST_B(LAWV, CNT, CAPV, MASSV, 128);                                    \
+    }                                                                         \
+  } while (0)
 #define FAST_M(LAWV, CNT, CAPV)                                               \
   do {                                                                        \
     if (!(CNT) && s->massPositive) {                                          \
-      LAUNCH((k_walk_fast<LAWV, CNT, CAPV, false>), walkGrid, kWalkBlock, n,  \
-             s->nodeF, s->nodeI, s->nodeQ, s->bodyPack, tSq, eSq, s->ax,     \
-             s->ay, s->ctl);                                                  \
+      FAST_MB(LAWV, CNT, CAPV, false);                                        \
     } else {                                                                  \
-      LAUNCH((k_walk_fast<LAWV, CNT, CAPV, true>), walkGrid, kWalkBlock, n,   \
-             s->nodeF, s->nodeI, s->nodeQ, s->bodyPack, tSq, eSq, s->ax,     \
-             s->ay, s->ctl);                                                  \
+      FAST_MB(LAWV, CNT, CAPV, true);                                         \
     }                                                                         \
   } while (0)
 #define FAST(LAWV, CNT)                                                       \
@@ -649,6 +663,8 @@ int launchForces(nbx_sim *s, bool count) {
 #undef LEARNED
 #undef FAST
 #undef FAST_M
+#undef FAST_MB
+#undef FAST_B
 #undef WALK
   return NBX_OK;
 }
@@ -1227,6 +1243,103 @@ int nbx_get_params(nbx_sim *s, nbx_params *p) {
   }
   *p = s->params;
   return NBX_OK;
+}
+
+int nbx_set_walk_block(nbx_sim *s, uint32_t lanes) {
+  if (!s) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  if (lanes != 64 && lanes != 128 && lanes != 256) {
+    return setError(NBX_ERR_ARG, "walk workgroup size must be 64, 128 or 256");
+  }
+  if (lanes != s->walkBlock) {
+    NBX_HIP(hipStreamSynchronize(s->stream));
+    dropGraphs(s);
+    s->walkBlock = lanes;
+  }
+  return NBX_OK;
+}
+
+int nbx_get_walk_block(nbx_sim *s, uint32_t *lanes) {
+  if (!s || !lanes) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  *lanes = s->walkBlock;
+  return NBX_OK;
+}
+
+int nbx_tune_walk(nbx_sim *s, uint32_t reps, uint32_t *best, float *ms) {
+  if (!s || !best) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (reps == 0 || reps > 1000) {
+    return setError(NBX_ERR_ARG, "reps must be in 1..1000");
+  }
+  if (s->params.summation == NBX_SUM_REFERENCE ||
+      s->params.law == NBX_LAW_LEARNED) {
+    return setError(NBX_ERR_STATE, "the walk workgroup size applies to the "
+                                   "compensated walk of the closed-form laws");
+  }
+  static constexpr uint32_t kCand[3] = {64, 128, 256};
+  if (s->n == 0) {
+    *best = s->walkBlock;
+    if (ms) {
+      std::fill(ms, ms + 3, 0.0f);
+    }
+    return NBX_OK;
+  }
+  if (!s->treeBuilt) {
+    NBX_TRY(launchBuild(s));
+  }
+  hipEvent_t ev[2]{};
+  NBX_HIP(hipEventCreate(&ev[0]));
+  if (hipEventCreate(&ev[1]) != hipSuccess) {
+    (void)hipEventDestroy(ev[0]);
+    return setError(NBX_ERR_HIP, "hipEventCreate failed");
+  }
+  const uint32_t prev = s->walkBlock;
+  float t[3] = {0, 0, 0};
+  int rc = NBX_OK;
+  for (int i = 0; i < 3 && rc == NBX_OK; ++i) {
+    s->walkBlock = kCand[i];
+    rc = launchForces(s, false); // warm-up: code load, caches
+    for (uint32_t r = 0; r < reps && rc == NBX_OK; ++r) {
+      if (r == 0 && hipEventRecord(ev[0], s->stream) != hipSuccess) {
+        rc = setError(NBX_ERR_HIP, "hipEventRecord failed");
+        break;
+      }
+      rc = launchForces(s, false);
+    }
+    if (rc == NBX_OK) {
+      if (hipEventRecord(ev[1], s->stream) != hipSuccess ||
+          hipEventSynchronize(ev[1]) != hipSuccess ||
+          hipEventElapsedTime(&t[i], ev[0], ev[1]) != hipSuccess) {
+        rc = setError(NBX_ERR_HIP, "timing the walk failed");
+      }
+      t[i] /= static_cast<float>(reps);
+    }
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc != NBX_OK) {
+    s->walkBlock = prev;
+    return rc;
+  }
+  int pick = 0;
+  for (int i = 1; i < 3; ++i) {
+    if (t[i] < t[pick]) {
+      pick = i;
+    }
+  }
+  s->walkBlock = kCand[pick];
+  if (s->walkBlock != prev) {
+    dropGraphs(s);
+  }
+  *best = s->walkBlock;
+  if (ms) {
+    std::copy(t, t + 3, ms);
+  }
+  return syncAndCheck(s);
 }
 
 int nbx_step(nbx_sim *s, uint32_t steps) {
@@ -2204,7 +2317,8 @@ struct nbx_ensemble {
   uint32_t *dOrder = nullptr, *dWaveInst = nullptr, *dQueue = nullptr;
   uint32_t totalWaves = 0;
   nbx_params params{};
-  uint32_t prepareGrid = 0;
+  uint32_t prepareGrid = 0;           // workgroups of k_ens_prepare
+  uint32_t walkBlock = kEnsWalkBlock; // lanes per workgroup of k_ens_walk
   uint64_t frame = 0;
   hipGraphExec_t graph = nullptr;
   bool instDirty = true; // host table changed: upload before the next step
@@ -2302,18 +2416,27 @@ int ensLaunchStep(nbx_ensemble *e) {
   hipLaunchKernelGGL(k_ens_prepare, dim3(e->prepareGrid), dim3(kEnsBlock), 0,
                      e->stream, e->dInst, e->dOrder,
                      static_cast<uint32_t>(e->inst.size()), p, e->dQueue);
-  const uint32_t walkBlocks =
-      (e->totalWaves + kEnsWalkBlock / kWave - 1) / (kEnsWalkBlock / kWave);
-  if (walkBlocks) {
-    if (e->params.law == NBX_LAW_ACCURATE) {
-      hipLaunchKernelGGL(k_ens_walk<kLawAccurate>, dim3(walkBlocks),
-                         dim3(kEnsWalkBlock), 0, e->stream, e->dInst, e->dWaveInst,
-                         e->totalWaves, tSq, eSq);
-    } else {
-      hipLaunchKernelGGL(k_ens_walk<kLawReference>, dim3(walkBlocks),
-                         dim3(kEnsWalkBlock), 0, e->stream, e->dInst, e->dWaveInst,
-                         e->totalWaves, tSq, eSq);
+  const bool acc = e->params.law == NBX_LAW_ACCURATE;
+  auto walk = [&](auto block) {
+    constexpr int kB = decltype(block)::value;
+    const uint32_t blocks = (e->totalWaves + kB / kWave - 1) / (kB / kWave);
+    if (!blocks) {
+      return;
     }
+    if (acc) {
+      hipLaunchKernelGGL((k_ens_walk<kLawAccurate, kB>), dim3(blocks), dim3(kB), 0,
+                         e->stream, e->dInst, e->dWaveInst, e->totalWaves, tSq, eSq);
+    } else {
+      hipLaunchKernelGGL((k_ens_walk<kLawReference, kB>), dim3(blocks), dim3(kB), 0,
+                         e->stream, e->dInst, e->dWaveInst, e->totalWaves, tSq, eSq);
+    }
+  };
+  if (e->walkBlock == 64) {
+    walk(std::integral_constant<int, 64>{});
+  } else if (e->walkBlock == 256) {
+    walk(std::integral_constant<int, 256>{});
+  } else {
+    walk(std::integral_constant<int, 128>{});
   }
   return checkLaunch();
 }
@@ -2542,6 +2665,38 @@ int nbx_ensemble_set_params(nbx_ensemble *e, const nbx_params *p) {
     ensDropGraph(e);
     e->params = *p;
   }
+  return NBX_OK;
+}
+
+int nbx_ensemble_set_launch(nbx_ensemble *e, uint32_t walkLanes,
+                            uint32_t prepareGroups) {
+  if (!e) {
+    return setError(NBX_ERR_ARG, "null handle");
+  }
+  if (walkLanes != 0 && walkLanes != 64 && walkLanes != 128 && walkLanes != 256) {
+    return setError(NBX_ERR_ARG, "walk workgroup size must be 64, 128 or 256");
+  }
+  if (prepareGroups > e->inst.size()) {
+    return setError(NBX_ERR_ARG, "more per-instance workgroups than instances");
+  }
+  const uint32_t wl = walkLanes ? walkLanes : e->walkBlock;
+  const uint32_t pg = prepareGroups ? prepareGroups : e->prepareGrid;
+  if (wl != e->walkBlock || pg != e->prepareGrid) {
+    NBX_HIP(hipStreamSynchronize(e->stream));
+    ensDropGraph(e);
+    e->walkBlock = wl;
+    e->prepareGrid = pg;
+  }
+  return NBX_OK;
+}
+
+int nbx_ensemble_get_launch(nbx_ensemble *e, uint32_t *walkLanes,
+                            uint32_t *prepareGroups) {
+  if (!e || !walkLanes || !prepareGroups) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  *walkLanes = e->walkBlock;
+  *prepareGroups = e->prepareGrid;
   return NBX_OK;
 }
 

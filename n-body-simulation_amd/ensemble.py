@@ -22,6 +22,10 @@ states (checked):
 
     python n-body-simulation_amd/ensemble.py [--instances 256] [--steps 500]
         [--nmin 2000] [--nmax 64000] [--theta 0.5] [--chunk 10] [--seed 7]
+        [--tune]
+
+--tune applies the launch-shape autotuner (tuning.py, cached per shape)
+to every handle and to the ensemble before anything is timed.
 
 Prints one JSON object: aggregate bodies*steps/s of every run and the
 ratios.
@@ -41,12 +45,16 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def load_nbx():
-    spec = importlib.util.spec_from_file_location("nbx", os.path.join(HERE, "nbx.py"))
+def load_module(name: str, file: str):
+    spec = importlib.util.spec_from_file_location(name, os.path.join(HERE, file))
     mod = importlib.util.module_from_spec(spec)
-    sys.modules["nbx"] = mod
+    sys.modules[name] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def load_nbx():
+    return load_module("nbx", "nbx.py")
 
 
 def sizes(count: int, nmin: int, nmax: int, seed: int) -> list[int]:
@@ -64,7 +72,10 @@ def main() -> int:
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--chunk", type=int, default=10)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--tune", action="store_true")
     args = ap.parse_args()
+    tuning = load_module("nbx_tuning", "tuning.py") if args.tune else None
+    tuned = {}
 
     nbx = load_nbx()
     ns = sizes(args.instances, args.nmin, args.nmax, args.seed)
@@ -80,6 +91,15 @@ def main() -> int:
     bodies = sum(ns)
     print(f"[ensemble] {len(sims)} instances, {bodies} bodies, {args.steps} steps",
           file=sys.stderr, flush=True)
+
+    if tuning is not None:
+        blocks = {}
+        for sim in sims:
+            e = tuning.tune_simulation(sim)
+            blocks[e["walk_block"]] = blocks.get(e["walk_block"], 0) + 1
+        tuned["walk_blocks"] = {str(b): c for b, c in sorted(blocks.items())}
+        print(f"[ensemble] tuned walk sizes {tuned['walk_blocks']}", file=sys.stderr,
+              flush=True)
 
     # Warm-up (graph capture) outside every timed run.
     for sim in sims:
@@ -128,6 +148,12 @@ def main() -> int:
     ens.collisions = True
     for i, (n, scenario) in enumerate(zip(ns, kinds)):
         ens.init(i, n, scenario=scenario, seed=i)
+    if tuning is not None:
+        e = tuning.tune_ensemble(ens)
+        tuned["ensemble"] = {"walk_lanes": e["walk_lanes"],
+                             "prepare_groups": e["prepare_groups"]}
+        print(f"[ensemble] tuned ensemble launch {tuned['ensemble']}", file=sys.stderr,
+              flush=True)
     ens.step(1)  # graph capture
     for i, st in enumerate(starts):
         ens.upload(i, st)
@@ -137,6 +163,7 @@ def main() -> int:
     t0 = time.perf_counter()
     ens.step(args.steps)
     ens_s = time.perf_counter() - t0
+    launch = ens.launch
     print(f"[ensemble] ensemble: {ens_s:.3f} s", file=sys.stderr, flush=True)
     identical = identical and all(same(ens.download(i), want) for i, want in enumerate(ref))
     ens.close()
@@ -153,7 +180,8 @@ def main() -> int:
                                "bodies_steps_per_s": work / conc_s,
                                "chunk": args.chunk},
         "ensemble": {"seconds": round(ens_s, 3), "bodies_steps_per_s": work / ens_s,
-                     "dispatches_per_step": 2},
+                     "dispatches_per_step": 2, "launch": list(launch)},
+        "tuned": tuned or None,
         "speedup_ensemble_over_one_at_a_time": one_s / ens_s,
         "speedup_concurrent_over_one_at_a_time": one_s / conc_s,
         "identical_states": identical,

@@ -44,8 +44,11 @@ EXPORTS = (
     "nbx_set_mlp", "nbx_mlp_last_pairs",
     "nbx_ensemble_create", "nbx_ensemble_destroy", "nbx_ensemble_init",
     "nbx_ensemble_upload", "nbx_ensemble_download", "nbx_ensemble_set_params",
-    "nbx_ensemble_step",
+    "nbx_ensemble_step", "nbx_set_walk_block", "nbx_get_walk_block", "nbx_tune_walk",
+    "nbx_ensemble_set_launch", "nbx_ensemble_get_launch",
 )
+
+WALK_BLOCKS = (64, 128, 256)
 
 MLP_PARAMS = 9154
 
@@ -149,6 +152,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_ensemble_download.argtypes = [_vp, C.c_uint32] + [_vp] * 9
     L.nbx_ensemble_set_params.argtypes = [_vp, C.POINTER(Params)]
     L.nbx_ensemble_step.argtypes = [_vp, C.c_uint32]
+    L.nbx_set_walk_block.argtypes = [_vp, C.c_uint32]
+    L.nbx_get_walk_block.argtypes = [_vp, C.POINTER(C.c_uint32)]
+    L.nbx_tune_walk.argtypes = [_vp, C.c_uint32, C.POINTER(C.c_uint32), _f32p]
+    L.nbx_ensemble_set_launch.argtypes = [_vp, C.c_uint32, C.c_uint32]
+    L.nbx_ensemble_get_launch.argtypes = [_vp, C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_uint32)]
     _lib = L
     return L
 
@@ -286,6 +295,27 @@ class Simulation:
     def step(self, steps: int = 1) -> None:
         self._push_params()
         self._check(self.L.nbx_step(self.h, steps))
+
+    # -- launch shape --
+    @property
+    def walk_block(self) -> int:
+        """Lanes per workgroup of the compensated force walk (64, 128, 256)."""
+        v = C.c_uint32()
+        self._check(self.L.nbx_get_walk_block(self.h, C.byref(v)))
+        return int(v.value)
+
+    @walk_block.setter
+    def walk_block(self, lanes: int) -> None:
+        self._check(self.L.nbx_set_walk_block(self.h, int(lanes)))
+
+    def tune_walk(self, reps: int = 10) -> tuple[int, dict]:
+        """Times `reps` walks of the current tree per workgroup size, keeps
+        the fastest; returns it and {size: mean ms}."""
+        self._push_params()
+        best = C.c_uint32()
+        ms = np.zeros(3, np.float32)
+        self._check(self.L.nbx_tune_walk(self.h, reps, C.byref(best), ms))
+        return int(best.value), {b: float(t) for b, t in zip(WALK_BLOCKS, ms)}
 
     def step_async(self, steps: int = 1) -> None:
         """Queues steps without waiting; errors surface at sync()."""
@@ -566,6 +596,7 @@ class Ensemble:
         self.L.nbx_default_params(C.byref(p))
         self.dt, self.theta, self.epsilon = p.dt, p.theta, p.epsilon
         self.collisions, self.law, self.use_graph = True, LAW_REFERENCE, True
+        self.default_launch = self.launch
 
     def _check(self, rc: int) -> None:
         if rc != NBX_OK:
@@ -619,3 +650,14 @@ class Ensemble:
                    COLLIDE_SEQUENTIAL)
         self._check(self.L.nbx_ensemble_set_params(self.h, C.byref(p)))
         self._check(self.L.nbx_ensemble_step(self.h, steps))
+
+    @property
+    def launch(self) -> tuple[int, int]:
+        """(walk lanes per workgroup, workgroups of the per-instance kernel)."""
+        w, g = C.c_uint32(), C.c_uint32()
+        self._check(self.L.nbx_ensemble_get_launch(self.h, C.byref(w), C.byref(g)))
+        return int(w.value), int(g.value)
+
+    def set_launch(self, walk_lanes: int = 0, prepare_groups: int = 0) -> None:
+        """0 keeps a setting; results are identical for every choice."""
+        self._check(self.L.nbx_ensemble_set_launch(self.h, walk_lanes, prepare_groups))

@@ -31,3 +31,9 @@ def reference():
 def nbx():
     import __graft_entry__ as ge
     return ge.load_nbx()
+
+
+@pytest.fixture(scope="session")
+def tuning():
+    import __graft_entry__ as ge
+    return ge.load_tuning()

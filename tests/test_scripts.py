@@ -11,7 +11,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SCRIPTS = ["n-body-simulation_amd/ensemble.py", "n-body-simulation_amd/precision_study.py",
            "n-body-simulation_amd/snapshot.py", "n-body-simulation_amd/farfield_train.py",
-           "bench.py"]
+           "n-body-simulation_amd/tuning.py", "bench.py"]
 
 
 @pytest.mark.parametrize("script", SCRIPTS)

@@ -23,10 +23,15 @@ Fixtures (all float32 bit patterns preserved):
   scenario_k1_n1501_s2.npz, scenario_k2_n1200_s9.npz
       makeScenario(TwoGalaxies, 1501, 2) and makeScenario(ColdSquare, 1200,
       9) (src/viewer/Scenarios.cpp:29-62), frame 0.
+  nbs2_ref_n500_t6_e3.bin
+      the file the reference's own CLI (src/Main.cpp, built by
+      oracle/Makefile as _ref/sim_parity) writes for `500 6 3 <path>`:
+      NBS2 header, mass, radius and frames 0, 3 and 6, byte for byte.
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [fixture-name ...]   (default: all)
 """
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -62,7 +67,12 @@ def descent_keys(px, py):
     return key
 
 
+ONLY = ()  # fixture names to write (empty: all)
+
+
 def save(name, **arrays):
+    if ONLY and name not in ONLY:
+        return
     path = os.path.join(HERE, name)
     np.savez_compressed(path, **arrays)
     print(f"wrote {name} ({os.path.getsize(path)} bytes)")
@@ -72,7 +82,19 @@ def bodies_dict(b: Bodies, prefix=""):
     return {prefix + f: getattr(b, f) for f in FIELDS}
 
 
+def write_nbs2():
+    name = "nbs2_ref_n500_t6_e3.bin"
+    if ONLY and name not in ONLY:
+        return
+    sim = os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle", "_ref",
+                       "sim_parity")
+    path = os.path.join(HERE, name)
+    subprocess.run([sim, "500", "6", "3", path], check=True, capture_output=True)
+    print(f"wrote {name} ({os.path.getsize(path)} bytes)")
+
+
 def main():
+    write_nbs2()
     ref = Reference()
     for n, seed in ((2000, 0), (777, 5)):
         save(f"disc_n{n}_s{seed}.npz", n=n, seed=seed,
@@ -107,4 +129,5 @@ def main():
 
 
 if __name__ == "__main__":
+    ONLY = tuple(sys.argv[1:])
     main()

@@ -140,6 +140,29 @@ def test_cli_writes_nbs2(tmp_path, oracle):
     assert np.median(d / np.maximum(np.hypot(o.px, o.py), 1)) < 1e-5
 
 
+@pytest.mark.parametrize("summation", ["reference", "compensated"])
+def test_cli_nbs2_bytes_equal_reference_file(tmp_path, summation):
+    """`nbx_sim 500 6 3 out.bin` against the file the reference's own CLI
+    wrote for the same arguments (tests/golden/nbs2_ref_n500_t6_e3.bin):
+    header, mass, radius and frame 0 byte for byte; frames 3 and 6 within
+    the whole-step tolerance (leaf order within a partition differs)."""
+    out = tmp_path / "run.bin"
+    r = subprocess.run([CLI, "500", "6", "3", str(out)], capture_output=True, text=True,
+                       timeout=120, env=dict(os.environ, NBX_SUM=summation))
+    assert r.returncode == 0, r.stderr
+    got = out.read_bytes()
+    want = open(os.path.join(GOLDEN, "nbs2_ref_n500_t6_e3.bin"), "rb").read()
+    assert len(got) == len(want)
+    head = 28 + 8 * 500 + 16 * 500
+    assert got[:head] == want[:head]
+    g, w = nbs2().read(str(out)), nbs2().read(os.path.join(GOLDEN, "nbs2_ref_n500_t6_e3.bin"))
+    for frame in (1, 2):
+        d = np.hypot(g.pos[frame, :, 0] - w.pos[frame, :, 0],
+                     g.pos[frame, :, 1] - w.pos[frame, :, 1]) / \
+            np.maximum(np.hypot(w.pos[frame, :, 0], w.pos[frame, :, 1]), 1)
+        assert np.median(d) < 1e-5 and d.max() < 1e-2, frame
+
+
 def test_cli_scenario(tmp_path, oracle):
     out = tmp_path / "sq.bin"
     env = dict(os.environ, NBX_SCENARIO="cold-square", NBX_SEED="9")

@@ -281,6 +281,21 @@ int nbx_count_interactions(nbx_sim *sim, uint64_t *far_nodes,
                            uint64_t *body_pairs, uint64_t *wave_visits,
                            uint64_t *wave_leaf_terms);
 
+/* The compensated walk's work profile: runs the counting walk (as
+ * nbx_count_interactions) and returns wave-level totals, each counted once
+ * per wave whatever the number of lanes taking part:
+ *   [0] sibling blocks popped        [1] lanes entering them (sum of masks)
+ *   [2] node visits with a far lane  [3] far lanes at those visits
+ *   [4] leaf visits, plain loop      [5] leaf records in plain loops
+ *   [6] leaf visits, spread          [7] spread term evaluations
+ *   [8] spread butterfly steps       [9] traversal stack pushes
+ *  [10] empty nodes (no bodies) visited
+ * Fills min(count, NBX_WALK_PROFILE_FIELDS) entries of out. NBX_ERR_STATE
+ * under the reference summation or the learned law. (No reference
+ * counterpart: instrumentation.) */
+#define NBX_WALK_PROFILE_FIELDS 11
+int nbx_walk_profile(nbx_sim *sim, uint64_t *out, uint32_t count);
+
 /* ---- Timing (the per-phase event bench of src/cuda/CheckCuda.cu:677-712).
  * With profiling on, nbx_step / phase calls launch eagerly and record HIP
  * events on the handle's stream around each phase. ---- */

@@ -362,6 +362,63 @@ __device__ inline void quadTerm(const float4 &sm, float dx, float dy,
   qy = fmaf(h1, sdy, fmaf(c, dy, qy));
 }
 
+// k = the lanes of `near` (non-zero), logK = ceil(log2 k), on the scalar
+// unit (the compiler would test k > 1 as a 64-bit vector compare).
+__device__ inline void spreadShape(uint64_t near, uint32_t &k, uint32_t &logK) {
+  const uint64_t m = uniform64(near);
+  int kk, lk, t;
+  asm("s_bcnt1_i32_b64 %0, %3\n\t"
+      "s_add_i32 %2, %0, -1\n\t"
+      "s_flbit_i32_b32 %2, %2\n\t"
+      "s_sub_i32 %1, 32, %2\n\t"
+      "s_cmp_gt_u32 %0, 1\n\t"
+      "s_cselect_b32 %1, %1, 0"
+      : "=&s"(kk), "=&s"(lk), "=&s"(t)
+      : "s"(m)
+      : "scc");
+  (void)t;
+  k = static_cast<uint32_t>(kk);
+  logK = static_cast<uint32_t>(lk);
+}
+
+// Whether spreading a leaf over the wave takes fewer instructions than the
+// plain loop (about 18 per record there; 16 + 6 per butterfly step + 20 per
+// spread evaluation): 16 + 6 (6 - logK) + 20 iters < 18 cnt. Evaluated with
+// scalar instructions (the compiler would fold the multiply-add into a
+// vector mad24).
+__device__ inline bool spreadPays(uint32_t logK, int iters, int cnt) {
+  const int lk = __builtin_amdgcn_readfirstlane(static_cast<int>(logK));
+  const int it = __builtin_amdgcn_readfirstlane(iters);
+  const int c = __builtin_amdgcn_readfirstlane(cnt);
+  int r, t, u;
+  asm("s_sub_i32 %1, 6, %4\n\t"
+      "s_mul_i32 %1, %1, 6\n\t"
+      "s_mul_i32 %2, %5, 20\n\t"
+      "s_add_i32 %1, %1, %2\n\t"
+      "s_add_i32 %1, %1, 16\n\t"
+      "s_mul_i32 %2, %6, 18\n\t"
+      "s_cmp_lt_i32 %1, %2\n\t"
+      "s_cselect_b32 %0, 1, 0"
+      : "=s"(r), "=&s"(t), "=&s"(u)
+      : "0"(0), "s"(lk), "s"(it), "s"(c)
+      : "scc");
+  (void)t;
+  (void)u;
+  return r != 0;
+}
+
+// px += px[lane ^ 2^B], py likewise, when B >= logK (B < 5: ds_swizzle in
+// bit-mask mode, and_mask 0x1f, xor_mask 2^B, within each 32-lane half).
+template <int B>
+__device__ inline void swizzleAdd(uint32_t logK, float &px, float &py) {
+  static_assert(B >= 0 && B < 5, "swizzle xor within 32 lanes");
+  constexpr int kPattern = 0x1f | ((1 << B) << 10);
+  if (static_cast<uint32_t>(B) >= logK) {
+    px += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(px), kPattern));
+    py += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(py), kPattern));
+  }
+}
+
 // A leaf's bodies for the lanes of `near` when few lanes need them. The
 // (lane, record) pairs are spread over the wave: with k near lanes and K the
 // smallest power of two >= k, lane slot r = lane % K takes the r-th near
@@ -377,8 +434,7 @@ __device__ inline void leafSpread(uint64_t near, uint32_t k, uint32_t logK,
                                   float2 *__restrict__ slot, float &qx,
                                   float &qy) {
   const uint32_t lane = threadIdx.x & (kWave - 1);
-  const uint32_t rank =
-      static_cast<uint32_t>(__popcll(near & laneMaskLt()));
+  const uint32_t rank = rankInMask(near);
   const bool isNear = __builtin_amdgcn_inverse_ballot_w64(near);
   if (isNear) {
     slot[rank] = make_float2(x, y);
@@ -393,12 +449,21 @@ __device__ inline void leafSpread(uint64_t near, uint32_t k, uint32_t logK,
       leafTerm<LAW, CAP>(body[s + j], tp.x, tp.y, eSq, px, py);
     }
   }
-  for (uint32_t off = 1u << logK; off < kWave; off <<= 1) {
-    px += __shfl_xor(px, off, kWave);
-    py += __shfl_xor(py, off, kWave);
+  // Butterfly over the group bits: ds_swizzle (xor within 32 lanes, no
+  // address arithmetic) for bits 0..4, ds_bpermute for bit 5.
+  swizzleAdd<0>(logK, px, py);
+  swizzleAdd<1>(logK, px, py);
+  swizzleAdd<2>(logK, px, py);
+  swizzleAdd<3>(logK, px, py);
+  swizzleAdd<4>(logK, px, py);
+  if (logK <= 5) {
+    const int a = static_cast<int>((lane ^ 32u) << 2);
+    px += __int_as_float(__builtin_amdgcn_ds_bpermute(a, __float_as_int(px)));
+    py += __int_as_float(__builtin_amdgcn_ds_bpermute(a, __float_as_int(py)));
   }
-  const float ax = __shfl(px, static_cast<int>(rank), kWave);
-  const float ay = __shfl(py, static_cast<int>(rank), kWave);
+  const int a = static_cast<int>(rank << 2);
+  const float ax = __int_as_float(__builtin_amdgcn_ds_bpermute(a, __float_as_int(px)));
+  const float ay = __int_as_float(__builtin_amdgcn_ds_bpermute(a, __float_as_int(py)));
   if (isNear) {
     qx += ax;
     qy += ay;
@@ -507,7 +572,8 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
                                      uint32_t &nPair, uint32_t &nLeafTerm,
                                      const float4 &geo = float4{},
                                      const LearnedArgs &la = LearnedArgs{},
-                                     int *qn = nullptr) {
+                                     int *qn = nullptr,
+                                     uint32_t *ws = nullptr) {
   const float dx = f.x - x;
   const float dy = f.y - y;
   const float dSq = dx * dx + dy * dy;
@@ -521,6 +587,17 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
   uint64_t farMask = (!MASS || (mb > 0 && mb <= 0x7f800000))
                          ? (mask & ~near)
                          : 0ull;
+  if constexpr (COUNT) {
+    // Wave-level counts, outside the lane branches below (every lane holds
+    // the same copy).
+    if (ws && farMask) {
+      ws[2] += 1;
+      ws[3] += static_cast<uint32_t>(__popcll(farMask));
+    }
+    if (ws && c.x == 0 && c.w == c.z) {
+      ws[10] += 1;
+    }
+  }
   if constexpr (LAW == kLawLearned) {
     // Only nodes of positive mass take the network; far lanes outside the
     // trained range open the node instead (geo = {square centre, size}).
@@ -580,11 +657,16 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
     // Spread the pairs over the wave when that takes fewer instructions
     // (about 18 per record in the loop below; 16 + 6 per butterfly step +
     // 20 per spread evaluation).
-    const uint32_t k = static_cast<uint32_t>(__popcll(near));
-    const uint32_t logK = k <= 1 ? 0u : 32u - __clz(k - 1);
+    uint32_t k, logK;
+    spreadShape(near, k, logK);
     const int iters = (cnt + (kWave >> logK) - 1) >> (6 - logK);
-    if (16 + 6 * static_cast<int>(6 - logK) + 20 * iters < 18 * cnt) {
+    if (spreadPays(logK, iters, cnt)) {
       if (COUNT) {
+        if (ws) {
+          ws[6] += 1;
+          ws[7] += static_cast<uint32_t>(iters);
+          ws[8] += 6u - logK;
+        }
         nLeafTerm += static_cast<uint32_t>(iters);
         if (__builtin_amdgcn_inverse_ballot_w64(near)) {
           nPair += static_cast<uint32_t>(cnt);
@@ -595,6 +677,10 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
       return 0;
     }
     if (COUNT) {
+      if (ws) {
+        ws[4] += 1;
+        ws[5] += static_cast<uint32_t>(cnt);
+      }
       nLeafTerm += static_cast<uint32_t>(cnt);
     }
     if (__builtin_amdgcn_inverse_ballot_w64(near)) {
@@ -648,6 +734,7 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
   float sx = 0.0f, sy = 0.0f, cx = 0.0f, cy = 0.0f; // compensated total
   float qx = 0.0f, qy = 0.0f;                       // partial since flush
   uint32_t nFar = 0, nPair = 0, nLeafTerm = 0, nVisit = 0;
+  uint32_t ws[kWalkStats] = {}; // COUNT only: wave-level work profile
   const int nodeCount = ctl->nodeCount;
   const uint32_t cbLimit =
       nodeCount >= 5 ? static_cast<uint32_t>(nodeCount - 4) : 0u;
@@ -667,7 +754,7 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
     const uint64_t open =
         visitNode<LAW, COUNT, CAP, MASS>(nodeF[0], nodeI[0], q0, all, x, y, tSq, eSq,
                                    body, slot, qx, qy, nFar, nPair, nLeafTerm,
-                                   g0, la, &qn);
+                                   g0, la, &qn, COUNT ? ws : nullptr);
     nVisit += 1;
     if (open) {
       st.put(0, nodeI[0].x, open);
@@ -721,10 +808,13 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
     for (int q = 0; q < 4; ++q) {
       open[q] = visitNode<LAW, COUNT, CAP, MASS>(bf[q], bi[q], bq[q], mask, x, y, tSq,
                                            eSq, body, slot, qx, qy, nFar, nPair,
-                                           nLeafTerm, bg[q], la, &qn);
+                                           nLeafTerm, bg[q], la, &qn,
+                                           COUNT ? ws : nullptr);
     }
     if (COUNT) {
       nVisit += 4;
+      ws[0] += 1;
+      ws[1] += static_cast<uint32_t>(__popcll(mask));
     }
     // Push in reverse so the first child's subtree is walked next.
 #pragma unroll
@@ -739,6 +829,9 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
         }
         st.put(sp, bi[q].x, open[q]);
         ++sp;
+        if (COUNT) {
+          ws[9] += 1;
+        }
       }
     }
     if ((blocks & 1) == 0) {
@@ -777,6 +870,10 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
       atomicAdd(&ctl->bodyPairCount, p64);
       atomicAdd(&ctl->waveVisits, static_cast<unsigned long long>(nVisit));
       atomicAdd(&ctl->waveLeafTerms, static_cast<unsigned long long>(nLeafTerm));
+#pragma unroll
+      for (int k = 0; k < kWalkStats; ++k) {
+        atomicAdd(&ctl->walkStats[k], static_cast<unsigned long long>(ws[k]));
+      }
     }
   }
 }

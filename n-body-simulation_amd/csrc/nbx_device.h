@@ -42,6 +42,12 @@ enum : uint32_t {
 // Per-handle control block in device memory: everything the host would
 // otherwise have to read back between kernels (counts, sizes, the root quad)
 // lives here, so a step needs no host round trip.
+// Work profile of the counting walk (wave-level totals, nbx_walk_profile):
+// blocks popped, lanes entering them, visits with a far lane, far lanes,
+// plain leaf visits, plain leaf records, spread leaf visits, spread term
+// evaluations, spread butterfly steps, stack pushes, empty nodes visited.
+constexpr int kWalkStats = 11;
+
 struct Ctl {
   uint32_t bbox[4];  // minX, maxX, minY, maxY as order-preserving bits
   float quad[4];     // root quad: cx, cy, size, unused
@@ -61,6 +67,7 @@ struct Ctl {
   uint32_t pad[2];
   unsigned long long farCount, bodyPairCount;
   unsigned long long waveVisits, waveLeafTerms; // counting walk only
+  unsigned long long walkStats[kWalkStats];     // counting walk only
 };
 
 // A one-launch kernel's grid barrier timed out earlier in this step: the
@@ -94,6 +101,13 @@ __device__ inline uint32_t xcdBlock() {
   const uint32_t x = b % kXcds, k = b / kXcds;
   const uint32_t q = g / kXcds, r = g % kXcds;
   return x * q + min(x, r) + k;
+}
+
+// Number of lanes of `m` below this lane (v_mbcnt: two instructions).
+__device__ inline uint32_t rankInMask(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(
+      static_cast<uint32_t>(m >> 32),
+      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
 }
 
 __device__ inline uint64_t laneMaskLt() {

@@ -259,6 +259,9 @@ __global__ void k_zero_counts(Ctl *__restrict__ ctl) {
   ctl->bodyPairCount = 0;
   ctl->waveVisits = 0;
   ctl->waveLeafTerms = 0;
+  for (int k = 0; k < kWalkStats; ++k) {
+    ctl->walkStats[k] = 0;
+  }
 }
 
 int launchIterate(nbx_sim *s) {
@@ -1609,6 +1612,26 @@ int nbx_count_interactions(nbx_sim *s, uint64_t *far_nodes,
   if (body_pairs) *body_pairs = c.bodyPairCount;
   if (wave_visits) *wave_visits = c.waveVisits;
   if (wave_leaf_terms) *wave_leaf_terms = c.waveLeafTerms;
+  return NBX_OK;
+}
+
+int nbx_walk_profile(nbx_sim *s, uint64_t *out, uint32_t count) {
+  if (!s || (count && !out)) {
+    return setError(NBX_ERR_ARG, "null argument");
+  }
+  if (s->params.summation == NBX_SUM_REFERENCE || s->params.law == NBX_LAW_LEARNED) {
+    return setError(NBX_ERR_STATE, "the work profile covers the compensated walk "
+                                   "of the closed-form laws");
+  }
+  hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(1), 0, s->stream, s->ctl);
+  NBX_TRY(checkLaunch());
+  NBX_TRY(launchForces(s, true));
+  NBX_TRY(syncAndCheck(s));
+  Ctl c{};
+  NBX_TRY(readCtl(s, &c));
+  for (uint32_t k = 0; k < count && k < static_cast<uint32_t>(kWalkStats); ++k) {
+    out[k] = c.walkStats[k];
+  }
   return NBX_OK;
 }
 

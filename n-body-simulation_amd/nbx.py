@@ -45,8 +45,12 @@ EXPORTS = (
     "nbx_ensemble_create", "nbx_ensemble_destroy", "nbx_ensemble_init",
     "nbx_ensemble_upload", "nbx_ensemble_download", "nbx_ensemble_set_params",
     "nbx_ensemble_step", "nbx_set_walk_block", "nbx_get_walk_block", "nbx_tune_walk",
-    "nbx_ensemble_set_launch", "nbx_ensemble_get_launch",
+    "nbx_ensemble_set_launch", "nbx_ensemble_get_launch", "nbx_walk_profile",
 )
+
+WALK_PROFILE = ("blocks", "block_lanes", "far_visits", "far_lanes", "leaf_plain",
+                "leaf_plain_records", "leaf_spread", "spread_terms", "spread_steps",
+                "pushes", "empty_nodes")
 
 WALK_BLOCKS = (64, 128, 256)
 
@@ -156,6 +160,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     L.nbx_get_walk_block.argtypes = [_vp, C.POINTER(C.c_uint32)]
     L.nbx_tune_walk.argtypes = [_vp, C.c_uint32, C.POINTER(C.c_uint32), _f32p]
     L.nbx_ensemble_set_launch.argtypes = [_vp, C.c_uint32, C.c_uint32]
+    L.nbx_walk_profile.argtypes = [_vp, _vp, C.c_uint32]
     L.nbx_ensemble_get_launch.argtypes = [_vp, C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint32)]
     _lib = L
@@ -408,6 +413,13 @@ class Simulation:
         self._check(self.L.nbx_count_interactions(self.h, *(C.byref(x) for x in v)))
         out = tuple(int(x.value) for x in v)
         return out if detail else out[:2]
+
+    def walk_profile(self) -> dict:
+        """The compensated walk's wave-level work profile (WALK_PROFILE)."""
+        self._push_params()
+        out = np.zeros(len(WALK_PROFILE), np.uint64)
+        self._check(self.L.nbx_walk_profile(self.h, out.ctypes.data, out.size))
+        return {k: int(v) for k, v in zip(WALK_PROFILE, out)}
 
     def set_profiling(self, on: bool) -> None:
         self._check(self.L.nbx_set_profiling(self.h, int(bool(on))))

@@ -50,16 +50,31 @@ __device__ inline int32_t cellCoord(float v, float invCell) {
   return static_cast<int32_t>(c);
 }
 
-// Cell -> bucket, row-major: the low colBits bits of the column, then the
-// row, folded into the power-of-two bucket count. The three cells
-// (cx - 1 .. cx + 1, cy) of a stencil row are consecutive buckets (unless the
-// row wraps), so their entries form one contiguous range of the grid; cells
-// that share a bucket are told apart by their exact coordinates.
-__device__ inline uint32_t rowBucket(int32_t cx, int32_t cy, uint32_t colBits,
-                                     uint32_t mask) {
-  const uint32_t colMask = (1u << colBits) - 1u;
-  return ((static_cast<uint32_t>(cy) << colBits) +
-          (static_cast<uint32_t>(cx) & colMask)) &
+// Cell -> bucket: cells are grouped in tiles of 8 x 8; a tile's 64 cells
+// take 64 consecutive buckets, row by row, and tiles follow each other in
+// Z order of their coordinates (low bits), folded into the power-of-two
+// bucket count. Grid order is then spatially coherent at tile scale (the
+// lanes of a wave of the stencil search cover one tile, and the pairs of a
+// clump leave the search together), and the three cells of a stencil row
+// are consecutive buckets unless the row crosses a tile edge (then two
+// ranges). Cells that share a bucket are told apart by their exact
+// coordinates.
+constexpr int kTileBits = 3;
+__device__ inline uint32_t spreadBits16(uint32_t v) {
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+__device__ inline uint32_t tileBucket(int32_t cx, int32_t cy, uint32_t mask) {
+  constexpr int32_t kIn = (1 << kTileBits) - 1;
+  const uint32_t z = spreadBits16(static_cast<uint32_t>(cx >> kTileBits)) |
+                     (spreadBits16(static_cast<uint32_t>(cy >> kTileBits)) << 1);
+  return ((z << (2 * kTileBits)) |
+          (static_cast<uint32_t>(cy & kIn) << kTileBits) |
+          static_cast<uint32_t>(cx & kIn)) &
          mask;
 }
 
@@ -273,6 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_size(
     ctl->cell = cell;
     ctl->invCell = 1.0f / cell;
     ctl->largeCount = 0;
+    ctl->prevPairs = ctl->pairCount;
     ctl->pairCount = 0;
     ctl->compCount = 0;
     ctl->compPairs = 0;
@@ -289,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_size(
 // Large/small split, grid cells, bucket counts; also seeds the union-find.
 __global__ __launch_bounds__(kBlock) void k_cells(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t colBits,
+    const float *__restrict__ r, Ctl *__restrict__ ctl,
     uint32_t mask, uint32_t *__restrict__ bucketOf,
     uint32_t *__restrict__ bucketRank,
     uint32_t *__restrict__ bucketCount, int32_t *__restrict__ largeList,
@@ -310,20 +326,19 @@ __global__ __launch_bounds__(kBlock) void k_cells(
   }
   const int32_t cx = cellCoord(b.minX, invCell);
   const int32_t cy = cellCoord(b.minY, invCell);
-  const uint32_t h = rowBucket(cx, cy, colBits, mask);
+  const uint32_t h = tileBucket(cx, cy, mask);
   bucketOf[i] = h;
   bucketRank[i] = atomicAdd(&bucketCount[h], 1u);
 }
 
 // Counting-sort scatter of small boxes into their buckets, at the rank
-// each body drew in k_cells. Clears the bucket counts (all of them were
-// read by the scan) for the next step.
+// each body drew in k_cells (the scan already cleared the bucket counts for
+// the next step).
 __global__ __launch_bounds__(kBlock) void k_grid_scatter(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
     const float *__restrict__ r, const Ctl *__restrict__ ctl,
     const uint32_t *__restrict__ bucketOf,
     const uint32_t *__restrict__ bucketRank,
-    uint32_t *__restrict__ bucketCount,
     const uint32_t *__restrict__ bucketStart, float4 *__restrict__ grid) {
   if (barrierStalled(ctl)) {
     return;
@@ -337,7 +352,6 @@ __global__ __launch_bounds__(kBlock) void k_grid_scatter(
     return;
   }
   const uint32_t slot = bucketStart[h] + bucketRank[i];
-  bucketCount[h] = 0;
   const float invCell = ctl->invCell;
   const Box b = bodyBox(px[i], py[i], r[i]);
   grid[2 * slot] = make_float4(b.minX, b.maxX, b.minY, b.maxY);
@@ -417,14 +431,22 @@ __device__ inline void stageFlush(PairStageT<CAP> &st, Ctl *__restrict__ ctl,
   }
 }
 
-// One thread per body, in device order (the previous build's key order, so a
-// wave covers a compact patch of space): a small body's earlier partners
-// among small boxes of the 3x3 neighbouring cells, and every body's pairs
-// with large boxes.
+// The stencil search: a small box's earlier partners among small boxes of
+// the 3x3 neighbouring cells, and every box's pairs with large boxes. Two
+// orders of work, chosen by the previous collide's pair count:
+//  * few contacts (fewer pairs than half the bodies): one thread per grid
+//    entry, in grid order (thread o < n - large takes the small box in grid
+//    slot o, the rest the large boxes). A wave then covers one tile of
+//    cells, its stencils share records, and each lane's own box, cell and
+//    index come from its one 32-byte record;
+//  * many contacts: one thread per body, in device order (the previous
+//    build's key order), so that the pairs of a clump leave the search
+//    together -- the component kernels after it run on that locality.
+// The pair set is the same either way.
 __global__ __launch_bounds__(kFindBlock) void k_find_small(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t colBits,
-    uint32_t mask, const uint32_t *__restrict__ bucketOf,
+    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
+    const uint32_t *__restrict__ bucketOf,
     const uint32_t *__restrict__ bucketStart,
     const float4 *__restrict__ grid,
     const int32_t *__restrict__ largeList,
@@ -435,23 +457,44 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
   }
   __shared__ PairStageT<kFindStage> st;
   stageInit(st);
-  const uint32_t i = xcdBlock() * kFindBlock + threadIdx.x;
-  if (i < n) {
+  const uint32_t o = xcdBlock() * kFindBlock + threadIdx.x;
+  if (o < n) {
     const uint32_t large = ctl->largeCount;
-    const bool small = bucketOf[i] != 0xffffffffu;
-    const float invCell = ctl->invCell;
-    const Box me = bodyBox(px[i], py[i], r[i]);
-    const int self = static_cast<int>(i);
+    const bool gridOrder = ctl->prevPairs < n / 2;
+    bool small;
+    Box me;
+    int self;
+    int32_t mx = 0, my = 0;
+    if (gridOrder) {
+      const uint32_t nSmall = n - min(large, n);
+      small = o < nSmall;
+      if (small) {
+        const float4 rb = grid[2 * o], rc = grid[2 * o + 1];
+        me = Box{rb.x, rb.y, rb.z, rb.w};
+        mx = __float_as_int(rc.x);
+        my = __float_as_int(rc.y);
+        self = __float_as_int(rc.z);
+      } else {
+        const float4 lb = largeBox[o - nSmall];
+        me = Box{lb.x, lb.y, lb.z, lb.w};
+        self = largeList[o - nSmall];
+      }
+    } else {
+      small = bucketOf[o] != 0xffffffffu;
+      me = bodyBox(px[o], py[o], r[o]);
+      self = static_cast<int>(o);
+      if (small) {
+        const float invCell = ctl->invCell;
+        mx = cellCoord(me.minX, invCell);
+        my = cellCoord(me.minY, invCell);
+      }
+    }
     if (small) {
-      const int32_t mx = cellCoord(me.minX, invCell);
-      const int32_t my = cellCoord(me.minY, invCell);
-      // Per stencil row, the bucket range of its three cells -- one
-      // contiguous range unless the row wraps around the bucket columns
-      // (then three single-bucket ranges) -- all requested up front.
-      const uint32_t colMask = (1u << colBits) - 1u;
-      const uint32_t col = static_cast<uint32_t>(mx) & colMask;
-      const bool wraps = col == 0 || col == colMask;
-      // Box and cell of an entry share one 32-byte record: both are
+      // Per stencil row, the bucket ranges of its three cells: one
+      // contiguous range, or two where the row crosses a tile edge (the
+      // cell left of a tile's first column, or right of its last, lies in
+      // the neighbouring tile) -- six ranges at most, all requested up
+      // front. Box and cell of an entry share one 32-byte record: both are
       // requested before either is tested.
       auto test = [&](const float4 &ob, const float4 &ocf, int32_t cy) {
         const int32_t ox = __float_as_int(ocf.x), oy = __float_as_int(ocf.y);
@@ -464,71 +507,74 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
           stagePair(st, ctl, pairs, cap, parent, self, other, me.minX, ob.x);
         }
       };
-      if (!wraps) {
-        uint32_t lo[3], hi[3];
+      constexpr int32_t kIn = (1 << kTileBits) - 1;
+      const int32_t ix = mx & kIn;
+      // Cells of the first range: all three, or the one (two) left of the
+      // tile edge.
+      const int32_t lenA = ix == 0 ? 1 : (ix == kIn ? 2 : 3);
+      uint32_t lo[6], hi[6];
 #pragma unroll
-        for (int row = 0; row < 3; ++row) {
-          const uint32_t h = rowBucket(mx, my + row - 1, colBits, mask);
-          lo[row] = bucketStart[h - 1];
-          hi[row] = bucketStart[h + 2];
-        }
-        // The three rows' entries as one list, four entries per iteration
-        // (so a wave runs max-over-lanes of the total, not of each row, and
-        // four records are in flight per lane; a lane past its end reloads
-        // its first entry of the iteration and skips the test).
-        const uint32_t n0 = hi[0] - lo[0], n01 = n0 + (hi[1] - lo[1]);
-        const uint32_t total = n01 + (hi[2] - lo[2]);
-        auto at = [&](uint32_t q, int32_t &cy) {
-          cy = my + (q < n0 ? -1 : (q < n01 ? 0 : 1));
-          return q < n0 ? lo[0] + q : (q < n01 ? lo[1] + q - n0 : lo[2] + q - n01);
-        };
-        for (uint32_t q = 0; q < total; q += 4) {
-          int32_t cy0, cy1, cy2, cy3;
-          const uint32_t o0 = at(q, cy0);
-          const uint32_t o1 = q + 1 < total ? at(q + 1, cy1) : o0;
-          const uint32_t o2 = q + 2 < total ? at(q + 2, cy2) : o0;
-          const uint32_t o3 = q + 3 < total ? at(q + 3, cy3) : o0;
-          const float4 ob0 = grid[2 * o0], oc0 = grid[2 * o0 + 1];
-          const float4 ob1 = grid[2 * o1], oc1 = grid[2 * o1 + 1];
-          const float4 ob2 = grid[2 * o2], oc2 = grid[2 * o2 + 1];
-          const float4 ob3 = grid[2 * o3], oc3 = grid[2 * o3 + 1];
-          asm volatile("" ::"v"(ob0.x), "v"(ob0.y), "v"(ob0.z), "v"(ob0.w),
-                       "v"(oc0.x), "v"(oc0.y), "v"(oc0.z), "v"(ob1.x),
-                       "v"(ob1.y), "v"(ob1.z), "v"(ob1.w), "v"(oc1.x),
-                       "v"(oc1.y), "v"(oc1.z), "v"(ob2.x), "v"(ob2.y),
-                       "v"(ob2.z), "v"(ob2.w), "v"(oc2.x), "v"(oc2.y),
-                       "v"(oc2.z), "v"(ob3.x), "v"(ob3.y), "v"(ob3.z),
-                       "v"(ob3.w), "v"(oc3.x), "v"(oc3.y), "v"(oc3.z));
-          test(ob0, oc0, cy0);
-          if (q + 1 < total) {
-            test(ob1, oc1, cy1);
-          }
-          if (q + 2 < total) {
-            test(ob2, oc2, cy2);
-          }
-          if (q + 3 < total) {
-            test(ob3, oc3, cy3);
+      for (int row = 0; row < 3; ++row) {
+        const int32_t cy = my + row - 1;
+        const uint32_t a = tileBucket(mx - 1, cy, mask);
+        const uint32_t b = tileBucket(mx - 1 + lenA, cy, mask);
+        lo[2 * row] = bucketStart[a];
+        hi[2 * row] = bucketStart[a + lenA];
+        lo[2 * row + 1] = bucketStart[b];
+        hi[2 * row + 1] = bucketStart[b + (3 - lenA)];
+      }
+      // The six ranges' entries as one list, four entries per iteration
+      // (so a wave runs max-over-lanes of the total, not of each range, and
+      // four records are in flight per lane; a lane past its end reloads its
+      // first entry of the iteration and skips the test).
+      uint32_t end[6];
+      uint32_t acc = 0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        acc += hi[k] - lo[k];
+        end[k] = acc;
+      }
+      const uint32_t total = acc;
+      auto at = [&](uint32_t q, int32_t &cy) {
+        uint32_t start = 0, o = lo[0] + q;
+        int k = 0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+          if (q >= end[j]) {
+            start = end[j];
+            k = j + 1;
+            o = lo[j + 1] + (q - start);
           }
         }
-      } else {
-        uint32_t lo[9], hi[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          const uint32_t h =
-              rowBucket(mx + (k % 3) - 1, my + (k / 3) - 1, colBits, mask);
-          lo[k] = bucketStart[h];
-          hi[k] = bucketStart[h + 1];
+        cy = my + (k >> 1) - 1;
+        return o;
+      };
+      for (uint32_t q = 0; q < total; q += 4) {
+        int32_t cy0, cy1, cy2, cy3;
+        const uint32_t o0 = at(q, cy0);
+        const uint32_t o1 = q + 1 < total ? at(q + 1, cy1) : o0;
+        const uint32_t o2 = q + 2 < total ? at(q + 2, cy2) : o0;
+        const uint32_t o3 = q + 3 < total ? at(q + 3, cy3) : o0;
+        const float4 ob0 = grid[2 * o0], oc0 = grid[2 * o0 + 1];
+        const float4 ob1 = grid[2 * o1], oc1 = grid[2 * o1 + 1];
+        const float4 ob2 = grid[2 * o2], oc2 = grid[2 * o2 + 1];
+        const float4 ob3 = grid[2 * o3], oc3 = grid[2 * o3 + 1];
+        asm volatile("" ::"v"(ob0.x), "v"(ob0.y), "v"(ob0.z), "v"(ob0.w),
+                     "v"(oc0.x), "v"(oc0.y), "v"(oc0.z), "v"(ob1.x),
+                     "v"(ob1.y), "v"(ob1.z), "v"(ob1.w), "v"(oc1.x),
+                     "v"(oc1.y), "v"(oc1.z), "v"(ob2.x), "v"(ob2.y),
+                     "v"(ob2.z), "v"(ob2.w), "v"(oc2.x), "v"(oc2.y),
+                     "v"(oc2.z), "v"(ob3.x), "v"(ob3.y), "v"(ob3.z),
+                     "v"(ob3.w), "v"(oc3.x), "v"(oc3.y), "v"(oc3.z));
+        test(ob0, oc0, cy0);
+        if (q + 1 < total) {
+          test(ob1, oc1, cy1);
         }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          const int32_t cy = my + k / 3 - 1;
-          for (uint32_t o = lo[k]; o < hi[k]; ++o) {
-            const float4 ob = grid[2 * o];
-            const float4 ocf = grid[2 * o + 1];
-            asm volatile("" ::"v"(ob.x), "v"(ob.y), "v"(ob.z), "v"(ob.w),
-                         "v"(ocf.x), "v"(ocf.y), "v"(ocf.z));
-            test(ob, ocf, cy);
-          }
+        if (q + 2 < total) {
+          test(ob2, oc2, cy2);
+        }
+        if (q + 3 < total) {
+          test(ob3, oc3, cy3);
         }
       }
     }

@@ -97,9 +97,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(
 }
 
 // Blocked per-thread scan of a tile: thread t owns values
-// [base + t*kScanItems, ...), so the output order is the input order.
+// [base + t*kScanItems, ...), so the output order is the input order. The
+// input (the grid's bucket counts) is cleared once read.
 __global__ __launch_bounds__(kBlock) void k_scan_down(
-    uint32_t count, const uint32_t *__restrict__ in,
+    uint32_t count, uint32_t *__restrict__ in,
     const uint32_t *__restrict__ tileSums, uint32_t *__restrict__ out) {
   __shared__ uint32_t scratch[kBlock / kWave];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(
   if (full) {
     // Four 16-byte loads (the buffers are 16-byte aligned, base is a
     // multiple of 16 values).
-    const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+    uint4 *p = reinterpret_cast<uint4 *>(in + base);
 #pragma unroll
     for (int k = 0; k < kScanItems / 4; ++k) {
       const uint4 q = p[k];
@@ -117,10 +118,17 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(
       v[4 * k + 2] = q.z;
       v[4 * k + 3] = q.w;
     }
+#pragma unroll
+    for (int k = 0; k < kScanItems / 4; ++k) {
+      p[k] = make_uint4(0u, 0u, 0u, 0u);
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
       v[k] = base + k < count ? in[base + k] : 0u;
+      if (base + k < count) {
+        in[base + k] = 0u;
+      }
     }
   }
   uint32_t s = 0;
@@ -608,10 +616,11 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
 // ITEMS consecutive values (16-byte loads). The chunk is reduced and its
 // total published; after one grid barrier each workgroup adds the totals of
 // the workgroups before it and writes its chunk's prefixes from registers.
-// Replaces the reduce / tile scan / down-sweep launches.
+// Replaces the reduce / tile scan / down-sweep launches. The input (the
+// grid's bucket counts) is cleared once read, ready for the next count.
 template <int ITEMS>
 __global__ __launch_bounds__(kPsBlock) void k_scan_onelaunch(
-    uint32_t count, uint32_t chunk, const uint32_t *__restrict__ in,
+    uint32_t count, uint32_t chunk, uint32_t *__restrict__ in,
     uint32_t *__restrict__ out, uint32_t *tileSums, uint32_t *bar,
     uint32_t spinLimit, Ctl *ctl) {
   static_assert(ITEMS % 4 == 0, "16-byte loads");
@@ -624,7 +633,7 @@ __global__ __launch_bounds__(kPsBlock) void k_scan_onelaunch(
   const bool full = base + ITEMS <= hi;
   uint32_t v[ITEMS];
   if (full) {
-    const uint4 *p = reinterpret_cast<const uint4 *>(in + base);
+    uint4 *p = reinterpret_cast<uint4 *>(in + base);
 #pragma unroll
     for (int k = 0; k < ITEMS / 4; ++k) {
       const uint4 q = p[k];
@@ -633,10 +642,17 @@ __global__ __launch_bounds__(kPsBlock) void k_scan_onelaunch(
       v[4 * k + 2] = q.z;
       v[4 * k + 3] = q.w;
     }
+#pragma unroll
+    for (int k = 0; k < ITEMS / 4; ++k) {
+      p[k] = make_uint4(0u, 0u, 0u, 0u);
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       v[k] = base + k < hi ? in[base + k] : 0u;
+      if (base + k < hi) {
+        in[base + k] = 0u;
+      }
     }
   }
   uint32_t sum = 0;

@@ -64,7 +64,8 @@ struct Ctl {
   uint32_t status;
   uint32_t waveCount[kWaveTiers]; // components queued per k_resolve_wave tier
   uint32_t waveNext[kWaveTiers];  // next queue entry per tier
-  uint32_t pad[2];
+  uint32_t prevPairs; // contact pairs the previous collide found
+  uint32_t pad;
   unsigned long long farCount, bodyPairCount;
   unsigned long long waveVisits, waveLeafTerms; // counting walk only
   unsigned long long walkStats[kWalkStats];     // counting walk only

@@ -275,9 +275,10 @@ int launchIterate(nbx_sim *s) {
   return NBX_OK;
 }
 
-// Exclusive scan of `count` words: one launch of k_scan_onelaunch where its
-// grid fits the CUs, else reduce / tile scan / down-sweep launches.
-int launchScan(nbx_sim *s, uint32_t count, const uint32_t *in, uint32_t *out) {
+// Exclusive scan of `count` words, clearing the input: one launch of
+// k_scan_onelaunch where its grid fits the CUs, else reduce / tile scan /
+// down-sweep launches.
+int launchScan(nbx_sim *s, uint32_t count, uint32_t *in, uint32_t *out) {
   if (count == 0) {
     return NBX_OK;
   }
@@ -432,21 +433,18 @@ int launchCollide(nbx_sim *s) {
   const uint32_t buckets =
       static_cast<uint32_t>(std::max<uint64_t>(1024, pow2AtLeast(2ull * n)));
   const uint32_t mask = buckets - 1;
-  // Bucket columns: half the bucket bits (rounded up), rows the rest.
-  const uint32_t colBits = (static_cast<uint32_t>(__builtin_ctz(buckets)) + 1) / 2;
   const uint32_t pairGrid = std::min<uint32_t>(blocksFor(s->pairCap), 2048);
 
   LAUNCH(k_cell_size, 1, kBlock, n, px, py, r, s->ctl);
-  LAUNCH(k_cells, nb, kBlock, n, px, py, r, s->ctl, colBits, mask, s->bucketOf,
+  LAUNCH(k_cells, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
          s->bucketRank, s->bucketCount, s->largeList, s->largeBox,
          s->ufParent);
   NBX_TRY(launchScan(s, buckets + 1, s->bucketCount, s->bucketStart));
   LAUNCH(k_grid_scatter, nb, kBlock, n, px, py, r, s->ctl, s->bucketOf,
-         s->bucketRank, s->bucketCount, s->bucketStart, s->grid);
+         s->bucketRank, s->bucketStart, s->grid);
   LAUNCH(k_find_small, blocksFor(n, kFindBlock), kFindBlock, n, px, py, r,
-         s->ctl, colBits, mask, s->bucketOf, s->bucketStart, s->grid,
-         s->largeList, s->largeBox, s->pairs,
-         s->pairCap, s->ufParent);
+         s->ctl, mask, s->bucketOf, s->bucketStart, s->grid, s->largeList,
+         s->largeBox, s->pairs, s->pairCap, s->ufParent);
   LAUNCH(k_union, pairGrid, kBlock, s->ctl, static_cast<uint32_t>(n),
          s->pairCap, s->pairs, s->ufParent);
   LAUNCH(k_comp_count, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,

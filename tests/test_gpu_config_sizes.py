@@ -98,7 +98,10 @@ def test_late_run_1m_collide_bit_exact(nbx, oracle):
     """The 1M disc after 700 GPU steps (theta 0.5, collisions on), when the
     contact components have grown to hundreds of pairs: the next iterate and
     collide on the GPU (union-find components, tiered topological replay)
-    equal the oracle's sequential replay in pairLess order, bit for bit."""
+    equal the oracle's sequential replay in pairLess order, bit for bit --
+    with the stencil search in grid order (a fresh handle: no previous
+    pairs) and in body order (the same state again after a collide that
+    found more pairs than half the bodies)."""
     with nbx.Simulation(1_000_000) as sim:
         sim.theta = 0.5
         sim.step(700)
@@ -106,12 +109,15 @@ def test_late_run_1m_collide_bit_exact(nbx, oracle):
     want = state.copy()
     oracle.iterate(want)
     later, earlier, _ = oracle.collide(want)
+    assert later.size > state.px.size // 2
     with nbx.Simulation(state=state) as sim:
-        sim.iterate()
-        sim.collide()
-        info = sim.collide_info()
-        got = sim.download()
-    assert info["pairs"] == later.size
-    assert info["largest"] >= 100, info
-    for f in ("px", "py", "vx", "vy"):
-        assert_bits_equal(getattr(got, f), getattr(want, f), f)
+        for search in ("grid order", "body order"):
+            sim.upload(state)
+            sim.iterate()
+            sim.collide()
+            info = sim.collide_info()
+            got = sim.download()
+            assert info["pairs"] == later.size, search
+            assert info["largest"] >= 100, info
+            for f in ("px", "py", "vx", "vy"):
+                assert_bits_equal(getattr(got, f), getattr(want, f), f"{f} ({search})")

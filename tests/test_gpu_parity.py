@@ -184,6 +184,11 @@ def test_collide_600k_bit_exact(nbx, oracle, monkeypatch, mode):
 
 
 def check_collide(nbx, oracle, s):
+    """One collide on a fresh handle, checked against the oracle's; then the
+    same state collided again on that handle, which must give the same
+    bodies (the stencil search runs in grid order after a collide with
+    fewer pairs than half the bodies, in body order after one with more:
+    states with many contacts check both)."""
     want = s.copy()
     later, earlier, cell = oracle.collide(want)
     with new_sim(nbx, s) as sim:
@@ -191,6 +196,11 @@ def check_collide(nbx, oracle, s):
         got = sim.download()
         info = sim.collide_info()
         glater, gearlier, gcomp = sim.pairs()
+        sim.upload(s)
+        sim.collide()
+        again = sim.download()
+    for f in ("px", "py", "vx", "vy"):
+        assert_bits_equal(getattr(again, f), getattr(want, f), f"{f} (second collide)")
     # The grid cell is a free parameter of the broadphase (the pair set does
     # not depend on it): the library samples the widths, the oracle takes
     # the exact median as the reference does.

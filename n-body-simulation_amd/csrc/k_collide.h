@@ -169,137 +169,35 @@ __device__ inline void resolvePair(float *__restrict__ px,
 // Grid cell size. The reference takes 1.01 x the median box width
 // (Collision.cpp:163-183). The pair set does not depend on the cell (every
 // overlapping pair is found for any positive cell: small boxes through the
-// 3x3 stencil, boxes wider than a cell directly), only the work does, so
-// the cell here is 1.01 x the 5/8 quantile of a sample of at most
-// kCellSample box widths. The quantile sits above
-// the median so that sampling error cannot turn half the bodies into large
-// boxes when most widths are equal. One workgroup holds the sample in
-// registers and selects the quantile exactly by radix selection on the
-// float bit patterns (11, 11 and 10 bits, LDS histograms); it also resets
-// the collide counters.
-constexpr int kCellSample = 2048;
-constexpr int kCellPer = kCellSample / kBlock; // sample values per thread
-constexpr int kCellBins = 2048;
+// 3x3 stencil, boxes wider than a cell directly), only the work does. A
+// box's width is 2r up to the rounding of x - r and x + r, and radii never
+// change, so the cell is set once per upload on the host: 1.01 x the 5/8
+// quantile of 2r (above the median, so that when most widths are equal the
+// rounding of x -/+ r cannot turn half the bodies into large boxes).
 
-__global__ __launch_bounds__(kBlock) void k_cell_size(
-    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ r, Ctl *__restrict__ ctl) {
-  __shared__ uint32_t hist[kCellBins];
-  __shared__ uint32_t scratch[kBlock / kWave];
-  __shared__ uint32_t sel[2]; // decided bit prefix, rank still to skip
-  // All bodies when n <= kCellSample, else 64 runs of 32 consecutive bodies
-  // spread evenly over the array (consecutive records share cache lines; a
-  // single workgroup cannot afford one line per sample).
-  const uint32_t s = min<uint32_t>(n, kCellSample);
-  const uint32_t runStride = n <= kCellSample ? 0u : n / 64;
-  uint32_t v[kCellPer];
-  // Every load of the thread is issued before any is used (index 0 stands in
-  // for padding), so the sample costs one memory round trip.
-  float sx[kCellPer], sy[kCellPer], sr[kCellPer];
-#pragma unroll
-  for (int u = 0; u < kCellPer; ++u) {
-    const uint32_t k = threadIdx.x + u * kBlock;
-    const uint32_t i =
-        k >= s ? 0u : (runStride ? (k >> 5) * runStride + (k & 31) : k);
-    sx[u] = px[i];
-    sy[u] = py[i];
-    sr[u] = r[i];
+// Resets the collide's counters (one thread; the step runs it inside the
+// iterate kernel, a collide on its own as k_collide_reset). The previous
+// collide's pair count was built by atomics in an earlier launch, possibly
+// an earlier graph replayed back to back with this one: it is read, and its
+// copy for the stencil search written, at agent scope (a plain load here
+// has returned a stale zero from a cached line).
+__device__ inline void resetCollideCounters(Ctl *__restrict__ ctl) {
+  ctl->largeCount = 0;
+  storeAgent(&ctl->prevPairs, loadAgent(&ctl->pairCount));
+  ctl->pairCount = 0;
+  ctl->compCount = 0;
+  ctl->compPairs = 0;
+  ctl->bigCount = 0;
+  for (int t = 0; t < kWaveTiers; ++t) {
+    ctl->waveCount[t] = 0;
+    ctl->waveNext[t] = 0;
   }
-  static_assert(kCellPer == 8, "the register fence below names 8 samples");
-  asm volatile("" ::"v"(sx[0]), "v"(sx[1]), "v"(sx[2]), "v"(sx[3]),
-               "v"(sx[4]), "v"(sx[5]), "v"(sx[6]), "v"(sx[7]), "v"(sy[0]),
-               "v"(sy[1]), "v"(sy[2]), "v"(sy[3]), "v"(sy[4]), "v"(sy[5]),
-               "v"(sy[6]), "v"(sy[7]), "v"(sr[0]), "v"(sr[1]), "v"(sr[2]),
-               "v"(sr[3]), "v"(sr[4]), "v"(sr[5]), "v"(sr[6]), "v"(sr[7]));
-#pragma unroll
-  for (int u = 0; u < kCellPer; ++u) {
-    const uint32_t k = threadIdx.x + u * kBlock;
-    const Box b = bodyBox(sx[u], sy[u], sr[u]);
-    const float wx = b.maxX - b.minX, wy = b.maxY - b.minY;
-    const float w = (wx < wy) ? wy : wx;
-    // +inf: padding and NaN widths sort last.
-    v[u] = (k >= s || w != w) ? 0x7f800000u
-                              : (w > 0.0f ? __float_as_uint(w) : 0u);
-  }
-  if (threadIdx.x == 0) {
-    sel[0] = 0;
-    sel[1] = (s * 5) / 8;
-  }
-  constexpr int kShift[3] = {21, 10, 0};
-  constexpr int kBits[3] = {11, 11, 10};
-#pragma unroll
-  for (int pass = 0; pass < 3; ++pass) {
-    for (int b = threadIdx.x; b < kCellBins; b += kBlock) {
-      hist[b] = 0;
-    }
-    __syncthreads();
-    const uint32_t prefix = sel[0], rank = sel[1];
-    const uint32_t done = pass == 0 ? 0u : (~0u << (kShift[pass] + kBits[pass]));
-    const uint32_t digitMask = (1u << kBits[pass]) - 1;
-    // Most widths are equal: runs of one digit are counted in registers
-    // and added with one LDS atomic.
-    uint32_t last = 0, run = 0;
-#pragma unroll
-    for (int u = 0; u < kCellPer; ++u) {
-      if ((v[u] & done) == prefix) {
-        const uint32_t d = (v[u] >> kShift[pass]) & digitMask;
-        if (run != 0 && d != last) {
-          atomicAdd(&hist[last], run);
-          run = 0;
-        }
-        last = d;
-        ++run;
-      }
-    }
-    if (run != 0) {
-      atomicAdd(&hist[last], run);
-    }
-    __syncthreads();
-    constexpr int per = kCellBins / kBlock;
-    uint32_t c[per], own = 0;
-#pragma unroll
-    for (int q = 0; q < per; ++q) {
-      c[q] = hist[threadIdx.x * per + q];
-      own += c[q];
-    }
-    uint32_t total;
-    uint32_t before = blockExclusiveScan(own, scratch, &total);
-    if (before <= rank && rank < before + own) {
-      uint32_t bin = threadIdx.x * per;
-#pragma unroll
-      for (int q = 0; q < per; ++q) {
-        if (rank >= before + c[q]) {
-          before += c[q];
-          ++bin;
-        } else {
-          break;
-        }
-      }
-      sel[0] = prefix | (bin << kShift[pass]);
-      sel[1] = rank - before;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    float cell = __uint_as_float(sel[0]) * 1.01f;
-    if (!(cell > 0.0f)) {
-      cell = 1.0f; // Collision.cpp:178-180
-    }
-    ctl->cell = cell;
-    ctl->invCell = 1.0f / cell;
-    ctl->largeCount = 0;
-    ctl->prevPairs = ctl->pairCount;
-    ctl->pairCount = 0;
-    ctl->compCount = 0;
-    ctl->compPairs = 0;
-    ctl->bigCount = 0;
-    for (int t = 0; t < kWaveTiers; ++t) {
-      ctl->waveCount[t] = 0;
-      ctl->waveNext[t] = 0;
-    }
-    ctl->largest = 0;
-    ctl->status &= ~kStatusPairOverflow;
-  }
+  ctl->largest = 0;
+  ctl->status &= ~kStatusPairOverflow;
+}
+
+__global__ void k_collide_reset(Ctl *__restrict__ ctl) {
+  resetCollideCounters(ctl);
 }
 
 // Large/small split, grid cells, bucket counts; also seeds the union-find.
@@ -460,7 +358,7 @@ __global__ __launch_bounds__(kFindBlock) void k_find_small(
   const uint32_t o = xcdBlock() * kFindBlock + threadIdx.x;
   if (o < n) {
     const uint32_t large = ctl->largeCount;
-    const bool gridOrder = ctl->prevPairs < n / 2;
+    const bool gridOrder = loadAgent(&ctl->prevPairs) < n / 2;
     bool small;
     Box me;
     int self;

@@ -335,10 +335,67 @@ __global__ __launch_bounds__(kBlock) void k_lcp_count(
   }
 }
 
+// Level bookkeeping (QuadTree.cpp:188-204) for all depths at once, the
+// capacity check and the root node; split counts read at agent scope.
+__device__ inline void levelFinalize(uint32_t n, int64_t capacity,
+                                     Ctl *__restrict__ ctl,
+                                     float4 *__restrict__ nodeF,
+                                     int4 *__restrict__ nodeI,
+                                     float *__restrict__ centerX,
+                                     float *__restrict__ centerY,
+                                     float *__restrict__ size,
+                                     int32_t *__restrict__ nodeOfSplit) {
+  int32_t splits[kMaxDepth];
+  for (int d = 0; d < kMaxDepth; ++d) {
+    splits[d] = static_cast<int32_t>(
+        loadAgent(reinterpret_cast<const uint32_t *>(&ctl->splits[d])));
+  }
+  int32_t base = 0;
+  for (int d = 0; d < kMaxDepth; ++d) {
+    ctl->splitBase[d] = base;
+    base += splits[d];
+  }
+  ctl->splitBase[kMaxDepth] = base;
+  ctl->totalSplits = base;
+  int64_t start = 1;
+  ctl->levelStart[0] = 0;
+  ctl->levelStart[1] = 1;
+  int levels = 1;
+  for (int d = 1; d <= kMaxDepth; ++d) {
+    const int64_t cnt = 4ll * splits[d - 1];
+    if (cnt > 0) {
+      levels = d + 1;
+    }
+    start += cnt;
+    ctl->levelStart[d + 1] = static_cast<int32_t>(start < 0x7fffffff ? start : 0x7fffffff);
+  }
+  ctl->levels = levels;
+  if (start > capacity) {
+    ctl->nodeCount = 0;
+    ctl->status |= kStatusNodeOverflow;
+    return;
+  }
+  ctl->nodeCount = static_cast<int32_t>(start);
+  const float s = ctl->quad[2];
+  nodeI[0] = make_int4(splits[0] > 0 ? 1 : 0, 0, 0, static_cast<int>(n));
+  nodeF[0] = make_float4(0.0f, 0.0f, 0.0f, s * s);
+  centerX[0] = ctl->quad[0];
+  centerY[0] = ctl->quad[1];
+  size[0] = s;
+  nodeOfSplit[0] = 0;
+}
+
 // Block d: exclusive scan over tiles of depth d's counts (in place) and the
-// depth total.
+// depth total; the last block to finish then does the level bookkeeping
+// (levelFinalize), so the build needs no single-thread launch. Totals are
+// stored at agent scope and waited for before the block arrives at the
+// counter, which the last arrival resets for the next build.
 __global__ __launch_bounds__(1024) void k_level_scan(
-    uint32_t tiles, uint32_t *__restrict__ tileCount, Ctl *__restrict__ ctl) {
+    uint32_t tiles, uint32_t *__restrict__ tileCount, Ctl *__restrict__ ctl,
+    uint32_t n, int64_t capacity, float4 *__restrict__ nodeF,
+    int4 *__restrict__ nodeI, float *__restrict__ centerX,
+    float *__restrict__ centerY, float *__restrict__ size,
+    int32_t *__restrict__ nodeOfSplit) {
   if (barrierStalled(ctl)) {
     return;
   }
@@ -357,56 +414,16 @@ __global__ __launch_bounds__(1024) void k_level_scan(
     carry += total;
   }
   if (threadIdx.x == 0) {
-    ctl->splits[d] = static_cast<int32_t>(carry);
-  }
-}
-
-// Level bookkeeping (QuadTree.cpp:188-204) for all depths at once, the
-// capacity check and the root node.
-__global__ void k_level_finalize(uint32_t n, int64_t capacity,
-                                 Ctl *__restrict__ ctl,
-                                 float4 *__restrict__ nodeF,
-                                 int4 *__restrict__ nodeI,
-                                 float *__restrict__ centerX,
-                                 float *__restrict__ centerY,
-                                 float *__restrict__ size,
-                                 int32_t *__restrict__ nodeOfSplit) {
-  if (barrierStalled(ctl)) {
-    return;
-  }
-  int32_t base = 0;
-  for (int d = 0; d < kMaxDepth; ++d) {
-    ctl->splitBase[d] = base;
-    base += ctl->splits[d];
-  }
-  ctl->splitBase[kMaxDepth] = base;
-  ctl->totalSplits = base;
-  int64_t start = 1;
-  ctl->levelStart[0] = 0;
-  ctl->levelStart[1] = 1;
-  int levels = 1;
-  for (int d = 1; d <= kMaxDepth; ++d) {
-    const int64_t cnt = 4ll * ctl->splits[d - 1];
-    if (cnt > 0) {
-      levels = d + 1;
+    storeAgent(reinterpret_cast<uint32_t *>(&ctl->splits[d]), carry);
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0); // the total lands before arriving
+    const uint32_t arrived = __hip_atomic_fetch_add(
+        &ctl->levelDone, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == gridDim.x - 1) {
+      storeAgent(&ctl->levelDone, 0u);
+      levelFinalize(n, capacity, ctl, nodeF, nodeI, centerX, centerY, size,
+                    nodeOfSplit);
     }
-    start += cnt;
-    ctl->levelStart[d + 1] = static_cast<int32_t>(start < 0x7fffffff ? start : 0x7fffffff);
   }
-  ctl->levels = levels;
-  if (start > capacity) {
-    ctl->nodeCount = 0;
-    ctl->status |= kStatusNodeOverflow;
-    return;
-  }
-  ctl->nodeCount = static_cast<int32_t>(start);
-  const float s = ctl->quad[2];
-  nodeI[0] = make_int4(ctl->splits[0] > 0 ? 1 : 0, 0, 0, static_cast<int>(n));
-  nodeF[0] = make_float4(0.0f, 0.0f, 0.0f, s * s);
-  centerX[0] = ctl->quad[0];
-  centerY[0] = ctl->quad[1];
-  size[0] = s;
-  nodeOfSplit[0] = 0;
 }
 
 // Writes every splitting run start, ranked within its depth, into the split

@@ -65,7 +65,7 @@ struct Ctl {
   uint32_t waveCount[kWaveTiers]; // components queued per k_resolve_wave tier
   uint32_t waveNext[kWaveTiers];  // next queue entry per tier
   uint32_t prevPairs; // contact pairs the previous collide found
-  uint32_t pad;
+  uint32_t levelDone; // k_level_scan blocks finished (0 at rest)
   unsigned long long farCount, bodyPairCount;
   unsigned long long waveVisits, waveLeafTerms; // counting walk only
   unsigned long long walkStats[kWalkStats];     // counting walk only

@@ -112,6 +112,7 @@ struct nbx_sim {
   double massAbsSum = 0; // sum of |m| of the bodies held (masses never change)
   bool massPositive = false; // every body mass in (0, FLT_MAX]
   float jacobiCell = 1.0f; // reference GPU collide cell (radii never change)
+  float gridCell[2] = {1.0f, 1.0f}; // sequential collide cell and inverse
   JacAcc *jacPartials = nullptr;
   uint32_t jacMaxLarge = 0;
   bool jacobiCollided = false; // last collide ran in the Jacobi mode
@@ -232,16 +233,22 @@ int checkLaunch() {
     NBX_TRY(checkLaunch());                                                   \
   } while (0)
 
-int launchIterate(nbx_sim *s);
-int launchCollide(nbx_sim *s);
+int launchIterate(nbx_sim *s, bool resetCollide = false);
+int launchCollide(nbx_sim *s, bool countersReset = false);
 int launchBuild(nbx_sim *s);
 int launchForces(nbx_sim *s, bool count);
 
+// ctl non-null: also resets the collide's counters (the step's collide
+// follows).
 __global__ __launch_bounds__(kBlock) void k_iterate(
     uint32_t n, float dt, float *__restrict__ px, float *__restrict__ py,
     float *__restrict__ vx, float *__restrict__ vy,
-    const float *__restrict__ ax, const float *__restrict__ ay) {
+    const float *__restrict__ ax, const float *__restrict__ ay,
+    Ctl *__restrict__ ctl) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (ctl && i == 0) {
+    resetCollideCounters(ctl);
+  }
   if (i >= n) {
     return;
   }
@@ -264,14 +271,16 @@ __global__ void k_zero_counts(Ctl *__restrict__ ctl) {
   }
 }
 
-int launchIterate(nbx_sim *s) {
+// resetCollide: the step's iterate, whose collide then skips its own
+// counter reset.
+int launchIterate(nbx_sim *s, bool resetCollide) {
   const uint32_t n = static_cast<uint32_t>(s->n);
   if (n == 0) {
     return NBX_OK;
   }
   const int c = s->cur;
   LAUNCH(k_iterate, blocksFor(n), kBlock, n, s->params.dt, s->px[c], s->py[c],
-         s->vx[c], s->vy[c], s->ax, s->ay);
+         s->vx[c], s->vy[c], s->ax, s->ay, resetCollide ? s->ctl : nullptr);
   return NBX_OK;
 }
 
@@ -417,7 +426,8 @@ int launchCollideJacobi(nbx_sim *s) {
   return NBX_OK;
 }
 
-int launchCollide(nbx_sim *s) {
+// countersReset: the step's iterate has reset the collide's counters.
+int launchCollide(nbx_sim *s, bool countersReset) {
   const uint32_t n = static_cast<uint32_t>(s->n);
   if (n < 2) {
     return NBX_OK;
@@ -435,7 +445,9 @@ int launchCollide(nbx_sim *s) {
   const uint32_t mask = buckets - 1;
   const uint32_t pairGrid = std::min<uint32_t>(blocksFor(s->pairCap), 2048);
 
-  LAUNCH(k_cell_size, 1, kBlock, n, px, py, r, s->ctl);
+  if (!countersReset) {
+    LAUNCH(k_collide_reset, 1, 1, s->ctl);
+  }
   LAUNCH(k_cells, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
          s->bucketRank, s->bucketCount, s->largeList, s->largeBox,
          s->ufParent);
@@ -528,9 +540,9 @@ int launchBuild(nbx_sim *s) {
   const uint32_t tiles256 = nb;
   LAUNCH(k_lcp_count, tiles256, kBlock, n, hi, s->keyLo, perm, tiles256,
          s->tileCount);
-  LAUNCH(k_level_scan, kMaxDepth, 1024, tiles256, s->tileCount, s->ctl);
-  LAUNCH(k_level_finalize, 1, 1, n, s->nodeCap, s->ctl, s->nodeF, s->nodeI,
-         s->centerX, s->centerY, s->size, s->nodeOfSplit);
+  LAUNCH(k_level_scan, kMaxDepth, 1024, tiles256, s->tileCount, s->ctl, n,
+         s->nodeCap, s->nodeF, s->nodeI, s->centerX, s->centerY, s->size,
+         s->nodeOfSplit);
   LAUNCH(k_emit, tiles256, kBlock, n, hi, s->keyLo, perm, tiles256,
          s->tileCount, s->ctl, s->splitList);
   const uint32_t nodeGrid = std::max<uint32_t>(64, std::min<uint32_t>(nb, 2048));
@@ -705,12 +717,14 @@ int collectEvents(nbx_sim *s) {
   return NBX_OK;
 }
 
+// A phase of a step: the iterate resets the counters of the collide that
+// follows it.
 int launchPhase(nbx_sim *s, int phase) {
   switch (phase) {
   case NBX_PHASE_ITERATE:
-    return launchIterate(s);
+    return launchIterate(s, s->params.collisions != 0);
   case NBX_PHASE_COLLIDE:
-    return s->params.collisions ? launchCollide(s) : NBX_OK;
+    return s->params.collisions ? launchCollide(s, true) : NBX_OK;
   case NBX_PHASE_BUILD:
     return launchBuild(s);
   default:
@@ -1081,14 +1095,29 @@ int nbx_upload(nbx_sim *s, uint64_t n, const float *px, const float *py,
   }
   // CudaCollider's cell (CudaCollision.cu:197-209): 2 x median radius x 1.01.
   s->jacobiCell = 1.0f;
+  // The sequential collide's grid cell: 1.01 x the 5/8 quantile of the box
+  // widths 2r (a performance choice only: the pair set does not depend on
+  // it), and 1 where that is not positive (Collision.cpp:178-180).
+  float gridCell = 1.0f;
   if (n > 0) {
     std::vector<float> radii(r, r + n);
     std::nth_element(radii.begin(), radii.begin() + n / 2, radii.end());
     s->jacobiCell = 2.0f * radii[n / 2] * 1.01f;
+    const size_t q = std::min<size_t>(n - 1, (5 * n) / 8);
+    std::nth_element(radii.begin(), radii.begin() + q, radii.end());
+    gridCell = 2.0f * radii[q] * 1.01f;
   }
   if (!(s->jacobiCell > 0.0f)) {
     s->jacobiCell = 1.0f;
   }
+  if (!(gridCell > 0.0f) || !(gridCell <= FLT_MAX)) {
+    gridCell = 1.0f;
+  }
+  s->gridCell[0] = gridCell;
+  s->gridCell[1] = 1.0f / gridCell;
+  NBX_HIP(hipMemcpyAsync(reinterpret_cast<char *>(s->ctl) + offsetof(Ctl, cell),
+                         s->gridCell, sizeof s->gridCell, hipMemcpyHostToDevice,
+                         s->stream));
   s->jacobiCollided = false;
   NBX_TRY(copyIn(s, s->px[0], px, n));
   NBX_TRY(copyIn(s, s->py[0], py, n));

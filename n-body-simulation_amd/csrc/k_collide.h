@@ -111,8 +111,15 @@ __device__ inline void resolvePair(float *__restrict__ px,
   }
   const float wx = v2x - v1x, wy = v2y - v1y;
   float dDotV = dx * wx + dy * wy;
-  const float weight1 = m2 / (m1 + m2);
-  const float weight2 = m1 / (m1 + m2);
+  // m2 / (m1 + m2) and m1 / (m1 + m2); for equal positive masses whose sum
+  // is finite both quotients are exactly 0.5 (the sum is 2m exactly), so the
+  // two divisions are skipped -- the common case, and a uniform branch when
+  // every lane's pair has equal masses.
+  float weight1 = 0.5f, weight2 = 0.5f;
+  if (!(m1 == m2 && m1 > 0.0f && m1 <= 0x1p126f)) {
+    weight1 = m2 / (m1 + m2);
+    weight2 = m1 / (m1 + m2);
+  }
   const bool dNonZero = !(dx == 0.0f && dy == 0.0f);
   if (dDotV >= 0.0f && dNonZero) {
     const float mag = sqrtf(dx * dx + dy * dy);

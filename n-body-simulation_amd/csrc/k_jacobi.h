@@ -317,22 +317,85 @@ __global__ __launch_bounds__(kBlock) void k_jac_small(
   jacWrite(a, sa, acc, out, contacts, asLater);
 }
 
-// Sum of a 128-lane workgroup's values with the reference's tree shape
-// (blockSum, CudaCollision.cu:62-78).
-template <typename T>
-__device__ inline T jacBlockSum(T v, T *sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int stride = kJacLanes / 2; stride > 0; stride >>= 1) {
-    if (t < stride) {
-      sh[t] += sh[t + stride];
-    }
-    __syncthreads();
+// Sum of a 128-lane workgroup's accumulators with the reference's tree
+// shape (blockSum, CudaCollision.cu:62-78: stride 64, 32, ..., 1, lane t
+// adding lane t + stride), all ten fields at once. The stride-64 step is one
+// LDS exchange (the second wave hands its values to the first); strides 32
+// to 1 stay inside the first wave: ds_bpermute for 32 (across the wave's
+// halves), ds_swizzle xor 16..1 (within 32 lanes, no address arithmetic).
+// For t < stride, t ^ stride = t + stride, so every step adds the same two
+// partial sums the reference adds and the total is bit-identical. One
+// barrier per call; `buf` alternates between two 64-entry areas by call
+// parity, so the next call's writes cannot reach an area still being read.
+// The total is valid in lane 0 (the reference also uses only thread 0's).
+template <int B>
+__device__ inline float jacSwz(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x1f | ((1 << B) << 10)));
+}
+template <int B> __device__ inline int jacSwz(int v) {
+  return __builtin_amdgcn_ds_swizzle(v, 0x1f | ((1 << B) << 10));
+}
+__device__ inline void jacAddInto(JacAcc &s, const JacAcc &o) {
+  s.dpx += o.dpx;
+  s.dpy += o.dpy;
+  s.dvx += o.dvx;
+  s.dvy += o.dvy;
+  s.npx += o.npx;
+  s.npy += o.npy;
+  s.nvx += o.nvx;
+  s.nvy += o.nvy;
+  s.contacts += o.contacts;
+  s.asLater += o.asLater;
+}
+template <int B> __device__ inline void jacSwzStep(JacAcc &s) {
+  JacAcc o;
+  o.dpx = jacSwz<B>(s.dpx);
+  o.dpy = jacSwz<B>(s.dpy);
+  o.dvx = jacSwz<B>(s.dvx);
+  o.dvy = jacSwz<B>(s.dvy);
+  o.npx = jacSwz<B>(s.npx);
+  o.npy = jacSwz<B>(s.npy);
+  o.nvx = jacSwz<B>(s.nvx);
+  o.nvy = jacSwz<B>(s.nvy);
+  o.contacts = jacSwz<B>(s.contacts);
+  o.asLater = jacSwz<B>(s.asLater);
+  jacAddInto(s, o);
+}
+__device__ inline JacAcc jacBlockSum(const JacAcc &v, JacAcc *__restrict__ buf) {
+  static_assert(kJacLanes == 2 * kWave, "two waves per workgroup");
+  const int t = static_cast<int>(threadIdx.x);
+  const int lane = t & (kWave - 1);
+  if (t >= kWave) {
+    buf[lane] = v;
   }
-  const T total = sh[0];
   __syncthreads();
-  return total;
+  JacAcc s = v;
+  if (t < kWave) {
+    jacAddInto(s, buf[lane]); // stride 64
+    const int a = (lane ^ 32) << 2;
+    auto perm = [&](float x) {
+      return __int_as_float(__builtin_amdgcn_ds_bpermute(a, __float_as_int(x)));
+    };
+    JacAcc o;
+    o.dpx = perm(s.dpx);
+    o.dpy = perm(s.dpy);
+    o.dvx = perm(s.dvx);
+    o.dvy = perm(s.dvy);
+    o.npx = perm(s.npx);
+    o.npy = perm(s.npy);
+    o.nvx = perm(s.nvx);
+    o.nvy = perm(s.nvy);
+    o.contacts = __builtin_amdgcn_ds_bpermute(a, s.contacts);
+    o.asLater = __builtin_amdgcn_ds_bpermute(a, s.asLater);
+    jacAddInto(s, o); // stride 32
+    jacSwzStep<4>(s);  // 16
+    jacSwzStep<3>(s);  // 8
+    jacSwzStep<2>(s);  // 4
+    jacSwzStep<1>(s);  // 2
+    jacSwzStep<0>(s);  // 1
+  }
+  return s;
 }
 
 // largeBodyPartialKernel (CudaCollision.cu:133-163): workgroup (c, y) sums
@@ -346,8 +409,7 @@ __global__ __launch_bounds__(kJacLanes) void k_jac_large_partial(
   if (barrierStalled(ctl)) {
     return;
   }
-  __shared__ float shF[kJacLanes];
-  __shared__ int shI[kJacLanes];
+  __shared__ JacAcc sh[2][kWave];
   const uint32_t large = ctl->largeCount;
   if (large > maxLarge) {
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
@@ -358,7 +420,8 @@ __global__ __launch_bounds__(kJacLanes) void k_jac_large_partial(
   const int nn = static_cast<int>(n);
   const int c = static_cast<int>(blockIdx.x);
   const int begin = c * kJacChunk, end = min(nn, begin + kJacChunk);
-  for (uint32_t l = blockIdx.y; l < large; l += gridDim.y) {
+  int parity = 0;
+  for (uint32_t l = blockIdx.y; l < large; l += gridDim.y, parity ^= 1) {
     const int a = static_cast<int>(indexSorted[n - large + l]);
     const JacBody sa = jacLoad(a, px, py, vx, vy, m, r);
     const Box boxA = bodyBox(sa.px, sa.py, sa.r);
@@ -369,17 +432,7 @@ __global__ __launch_bounds__(kJacLanes) void k_jac_large_partial(
         jacPair(boxA, a, sa, b, px, py, vx, vy, m, r, mine);
       }
     }
-    JacAcc t;
-    t.dpx = jacBlockSum(mine.dpx, shF);
-    t.dpy = jacBlockSum(mine.dpy, shF);
-    t.dvx = jacBlockSum(mine.dvx, shF);
-    t.dvy = jacBlockSum(mine.dvy, shF);
-    t.npx = jacBlockSum(mine.npx, shF);
-    t.npy = jacBlockSum(mine.npy, shF);
-    t.nvx = jacBlockSum(mine.nvx, shF);
-    t.nvy = jacBlockSum(mine.nvy, shF);
-    t.contacts = jacBlockSum(mine.contacts, shI);
-    t.asLater = jacBlockSum(mine.asLater, shI);
+    const JacAcc t = jacBlockSum(mine, sh[parity]);
     if (threadIdx.x == 0) {
       partials[static_cast<size_t>(l) * chunks + c] = t;
     }

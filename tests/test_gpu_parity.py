@@ -702,9 +702,10 @@ def test_interaction_counts(nbx, oracle, states):
     assert gf == int(far.sum()) and gp == int(pairs.sum())
 
 
-def mixed_state(n=9000, seed=5):
-    """Several large boxes, coincident bodies (some at rest relative to each
-    other) and a zero radius, spread over three 4096-body chunks."""
+def mixed_state(n=9000, seed=5, large=40):
+    """Several large boxes (`large`), coincident bodies (some at rest
+    relative to each other) and a zero radius, spread over three 4096-body
+    chunks."""
     from oracle_lib import Bodies
     rng = np.random.default_rng(seed)
     b = Bodies.empty(n)
@@ -714,8 +715,8 @@ def mixed_state(n=9000, seed=5):
     b.vy[:] = rng.normal(0, 1, n).astype(np.float32)
     b.m[:] = 1.0
     b.r[:] = 0.5
-    b.r[:40] = rng.uniform(1.5, 6.0, 40).astype(np.float32)
-    b.m[:40] = b.r[:40] ** 3
+    b.r[:large] = rng.uniform(1.5, 6.0, large).astype(np.float32)
+    b.m[:large] = b.r[:large] ** 3
     b.px[50:60] = b.px[49]
     b.py[50:60] = b.py[49]
     b.vx[55:60] = b.vx[49]
@@ -743,10 +744,15 @@ def test_collide_many_large_boxes(nbx, oracle):
     assert any(a in big and b in big for a, b in zip(later.tolist(), earlier.tolist()))
 
 
-@pytest.mark.parametrize("name", ["collide10k_0", "collide20k_60", "mixed9k"])
+@pytest.mark.parametrize("name", ["collide10k_0", "collide20k_60", "mixed9k", "mixed9k_200"])
 def test_collide_jacobi_bit_exact(nbx, oracle, states, name):
-    """The reference GPU collide semantics (CudaCollision.cu), bit for bit."""
-    s = mixed_state() if name == "mixed9k" else states[name]
+    """The reference GPU collide semantics (CudaCollision.cu), bit for bit;
+    mixed9k_200 has 200 large boxes, so each workgroup of the large-box
+    partial sums reduces several boxes in turn."""
+    if name.startswith("mixed9k"):
+        s = mixed_state(large=200 if name.endswith("_200") else 40)
+    else:
+        s = states[name]
     want = s.copy()
     contacts, later, large = oracle.collide_jacobi(want)
     with new_sim(nbx, s) as sim:

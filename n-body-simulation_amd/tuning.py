@@ -69,17 +69,21 @@ def device_name() -> str:
     "unknown" where none can be read."""
     global _device
     if _device is None:
-        try:
-            hip = C.CDLL("libamdhip64.so")
+        _device = "unknown"
+        rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+        # By soname first: once libnbx is loaded this finds the same runtime.
+        for name in ("libamdhip64.so.7", "libamdhip64.so",
+                     os.path.join(rocm, "lib", "libamdhip64.so")):
+            try:
+                hip = C.CDLL(name)
+            except OSError:
+                continue
             dev = C.c_int()
             buf = C.create_string_buffer(256)
             if hip.hipGetDevice(C.byref(dev)) == 0 and \
                     hip.hipDeviceGetName(buf, 256, dev.value) == 0:
                 _device = buf.value.decode(errors="replace") or "unknown"
-            else:
-                _device = "unknown"
-        except OSError:
-            _device = "unknown"
+            break
     return _device
 
 

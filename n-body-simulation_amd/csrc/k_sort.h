@@ -377,6 +377,161 @@ __device__ inline void gridBarrier(uint32_t *__restrict__ bar, uint32_t blocks,
   __syncthreads();
 }
 
+// Largest bucket the MSD path sorts in LDS: 8 keys a lane (a top-digit
+// bucket of a uniform 1M-body disc holds about 5100).
+constexpr uint32_t kMsdCap = 8 * kPsBlock;
+constexpr int kMsdItems = kMsdCap / kPsBlock;
+
+// One stable counting pass over the first `cnt` keys in LDS, by digit
+// (key >> shift) & 255, from (srcK, srcV) to (dstK, dstV): `it` keys a
+// lane at positions wave * 64 * it + j * 64 + lane (j < it), ranked among
+// equal digits per wave (ballots), then across waves (wh: per-wave digit
+// counts; dstart: digit starts).
+__device__ inline void countingPassLds(const uint32_t *__restrict__ srcK,
+                                       const uint32_t *__restrict__ srcV,
+                                       uint32_t *__restrict__ dstK,
+                                       uint32_t *__restrict__ dstV, int it,
+                                       uint32_t cnt, int shift,
+                                       uint32_t (*wh)[256],
+                                       uint32_t *__restrict__ dstart,
+                                       uint32_t *__restrict__ scratch) {
+  const uint32_t t = threadIdx.x, lane = t & (kWave - 1), wave = t >> 6;
+  const uint32_t first = wave * kWave * static_cast<uint32_t>(it) + lane;
+  for (uint32_t i = t; i < kPsWaves * 256; i += kPsBlock) {
+    (&wh[0][0])[i] = 0;
+  }
+  __syncthreads();
+  uint32_t rank[kMsdItems], dg[kMsdItems];
+#pragma unroll
+  for (int j = 0; j < kMsdItems; ++j) {
+    const uint32_t p = first + j * kWave;
+    dg[j] = (j < it && p < cnt) ? (srcK[p] >> shift) & 255u : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kMsdItems; ++j) {
+    rank[j] = 0;
+    if (j < it) {
+      const uint32_t p = first + j * kWave;
+      const bool valid = p < cnt;
+      const uint32_t dig = dg[j];
+      const uint64_t peers = matchDigit8(dig, valid);
+      const uint32_t below = static_cast<uint32_t>(__popcll(peers & laneMaskLt()));
+      const uint32_t base = wh[wave][dig];
+      rank[j] = base + below;
+      if (valid && below == 0) {
+        wh[wave][dig] = base + static_cast<uint32_t>(__popcll(peers));
+      }
+    }
+  }
+  __syncthreads();
+  // Per digit, the exclusive prefix of the 16 waves' counts: lane t takes
+  // digit t / 4 and waves 4 (t % 4) .. 4 (t % 4) + 3, the four quarters are
+  // scanned across four neighbouring lanes, and the digit totals (on the
+  // quarter-3 lanes) are scanned across the workgroup in lane order, which
+  // is digit order.
+  {
+    const uint32_t d = t >> 2, q = t & 3u;
+    const uint32_t c0 = wh[4 * q][d], c1 = wh[4 * q + 1][d];
+    const uint32_t c2 = wh[4 * q + 2][d], c3 = wh[4 * q + 3][d];
+    const uint32_t sum = c0 + c1 + c2 + c3;
+    uint32_t x = sum;
+    uint32_t y = __shfl_up(x, 1, 4);
+    x += q >= 1 ? y : 0u;
+    y = __shfl_up(x, 2, 4);
+    x += q >= 2 ? y : 0u;
+    const uint32_t ex = x - sum;
+    wh[4 * q][d] = ex;
+    wh[4 * q + 1][d] = ex + c0;
+    wh[4 * q + 2][d] = ex + c0 + c1;
+    wh[4 * q + 3][d] = ex + c0 + c1 + c2;
+    uint32_t total;
+    const uint32_t pre =
+        blockExclusiveScan<kPsBlock>(q == 3 ? x : 0u, scratch, &total);
+    if (q == 0) {
+      dstart[d] = pre;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kMsdItems; ++j) {
+    const uint32_t p = first + j * kWave;
+    if (j < it && p < cnt) {
+      const uint32_t k = srcK[p];
+      const uint32_t dig = (k >> shift) & 255u;
+      const uint32_t pos = dstart[dig] + wh[wave][dig] + rank[j];
+      dstK[pos] = k;
+      dstV[pos] = srcV[p];
+    }
+  }
+  __syncthreads();
+}
+
+// After the MSD pass: bucket d (the keys whose top digit is d) lies at
+// [bstart[d], bstart[d] + bsize[d]) of (bucketKeys, bucketVals); each is
+// sorted stably by the three lower digits in LDS (two buffers of kMsdCap
+// keys and payloads, sk and sv holding 2 kMsdCap words each) and written to
+// (outKeys, outVals) at the same place. The non-empty buckets go to the
+// workgroups in digit order, one each for workgroups 0..G-1, then snaking
+// back from G-1 (a uniform disc fills about 224 of the 256 top cells, so at
+// 1M bodies with G = 245 every workgroup sorts at most one). Every address
+// is bounds-checked (a stalled barrier leaves values in range).
+__device__ inline void msdBuckets(uint32_t n, uint32_t grid,
+                                  const uint32_t *bucketKeys,
+                                  const uint32_t *bucketVals,
+                                  uint32_t *outKeys, uint32_t *outVals,
+                                  const uint32_t *__restrict__ bsize,
+                                  const uint32_t *__restrict__ bstart,
+                                  uint32_t *__restrict__ sk,
+                                  uint32_t *__restrict__ sv,
+                                  uint32_t (*wh)[256],
+                                  uint32_t *__restrict__ dstart,
+                                  uint32_t *__restrict__ list,
+                                  uint32_t *__restrict__ scratch) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t b = blockIdx.x;
+  // The non-empty buckets in digit order (in `list`): one each for
+  // workgroups 0..G-1, then snaking back from G-1.
+  __shared__ uint32_t nonEmpty;
+  {
+    const bool nz = t < 256 && bsize[t] > 0;
+    uint32_t total;
+    const uint32_t r = blockExclusiveScan<kPsBlock>(nz ? 1u : 0u, scratch, &total);
+    if (nz) {
+      list[r] = t;
+    }
+    if (t == 0) {
+      nonEmpty = total;
+    }
+  }
+  __syncthreads();
+  const uint32_t count = nonEmpty;
+  uint32_t *k0 = sk, *k1 = sk + kMsdCap, *v0 = sv, *v1 = sv + kMsdCap;
+  for (uint32_t round = 0; round * grid < count; ++round) {
+    const uint32_t slot = round * grid + ((round & 1u) ? grid - 1u - b : b);
+    if (slot >= count) {
+      continue;
+    }
+    const uint32_t bk = list[slot];
+    const uint32_t cnt = min(bsize[bk], kMsdCap);
+    const uint32_t st = min(bstart[bk], n);
+    const uint32_t len = min(cnt, n - st);
+    const int it = static_cast<int>((len + kPsBlock - 1) / kPsBlock);
+    for (uint32_t p = t; p < len; p += kPsBlock) {
+      k0[p] = loadAgent(bucketKeys + st + p);
+      v0[p] = loadAgent(bucketVals + st + p);
+    }
+    __syncthreads();
+    countingPassLds(k0, v0, k1, v1, it, len, 0, wh, dstart, scratch);
+    countingPassLds(k1, v1, k0, v0, it, len, 8, wh, dstart, scratch);
+    countingPassLds(k0, v0, k1, v1, it, len, 16, wh, dstart, scratch);
+    for (uint32_t p = t; p < len; p += kPsBlock) {
+      outKeys[st + p] = k1[p];
+      outVals[st + p] = v1[p];
+    }
+    __syncthreads();
+  }
+}
+
 // Sorts (keys0, vals0) by key, stably, through (keys1, vals1); the result
 // ends in (keys0, vals0). table holds 256 * gridDim.x words, rowTotal 256;
 // chunk <= ITEMS * 1024. Global writes are bounds-checked, so even a stalled
@@ -396,12 +551,14 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
     uint32_t *bar, uint32_t spinLimit, Ctl *ctl, const float *px,
     const float *py, uint32_t *parts, uint32_t *keyLo) {
   constexpr int kTile = ITEMS * kPsBlock;
-  __shared__ uint32_t sk[kTile];
-  __shared__ uint32_t sv[kTile];
+  constexpr int kStage = (BUILD && 2 * kMsdCap > kTile) ? 2 * kMsdCap : kTile;
+  __shared__ uint32_t sk[kStage];
+  __shared__ uint32_t sv[kStage];
   __shared__ uint32_t wh[kPsWaves][256]; // per wave: digit counts, then offsets
   __shared__ uint32_t dstart[256], goff[256];
   __shared__ uint32_t scratch[kPsWaves];
   __shared__ uint4 red[kPsWaves];
+  __shared__ uint32_t bsize[BUILD ? 256 : 1], bstart[BUILD ? 256 : 1];
 
   const uint32_t t = threadIdx.x;
   const uint32_t b = blockIdx.x, grid = gridDim.x;
@@ -480,12 +637,19 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
     }
   }
 
-  for (int pass = 0; pass < 4; ++pass) {
-    const int shift = 8 * pass;
+  // BUILD: pass -1 distributes the keys by their top digit (MSD) into
+  // buckets in keys1/vals1, and every workgroup then sorts whole buckets by
+  // the three lower digits in LDS (msdBuckets) -- four grid barriers instead
+  // of twelve. When a bucket is larger than the LDS holds, every workgroup
+  // sees it after the row scans and the four LSD passes run instead, from
+  // the keys still in registers.
+  for (int pass = BUILD ? -1 : 0; pass < 4; ++pass) {
+    const bool msd = pass < 0;
+    const int shift = msd ? 24 : 8 * pass;
     const uint32_t *kin = (pass & 1) ? keys1 : keys0;
     const uint32_t *vin = (pass & 1) ? vals1 : vals0;
-    uint32_t *kout = (pass & 1) ? keys0 : keys1;
-    uint32_t *vout = (pass & 1) ? vals0 : vals1;
+    uint32_t *kout = (msd || (pass & 1) == 0) ? keys1 : keys0;
+    uint32_t *vout = (msd || (pass & 1) == 0) ? vals1 : vals0;
 
     // A. load, rank, stage, publish.
     for (uint32_t i = t; i < kPsWaves * 256; i += kPsBlock) {
@@ -592,6 +756,30 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
       if (t < 256) {
         goff[t] = base + own;
       }
+      if (msd) {
+        if (t < 256) {
+          bsize[t] = v;
+          bstart[t] = base;
+        }
+        // The largest bucket, the same in every workgroup.
+        uint32_t mx = v;
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+          mx = max(mx, static_cast<uint32_t>(__shfl_xor(mx, off, kWave)));
+        }
+        if (lane == 0) {
+          red[wave].x = mx;
+        }
+        __syncthreads();
+        mx = 0;
+        for (uint32_t w = 0; w < kPsWaves; ++w) {
+          mx = max(mx, red[w].x);
+        }
+        __syncthreads();
+        if (mx > kMsdCap) {
+          continue; // LSD passes 0..3 from the registers
+        }
+      }
     }
     __syncthreads();
     for (uint32_t p = t; p < cnt; p += kPsBlock) {
@@ -605,6 +793,11 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
     }
     if (pass < 3) {
       gridBarrier(bar, grid, status, spinLimit);
+    }
+    if (msd) {
+      msdBuckets(n, grid, keys1, vals1, keys0, vals0, bsize, bstart, sk, sv,
+                 wh, dstart, goff, scratch);
+      return;
     }
   }
 }

@@ -884,23 +884,17 @@ __global__ __launch_bounds__(kBlock) void k_subtree(
   }
 }
 
-// One workgroup: every branch not yet summed (the nodes above the subtree
-// roots, plus any subtree too large for k_subtree's list). That set is
-// closed upward, so it is found top-down from the root, then summed
-// bottom-up.
-__global__ __launch_bounds__(1024) void k_upward_top(
-    const Ctl *__restrict__ ctl, float4 *__restrict__ nodeF,
-    const int4 *__restrict__ nodeI, float2 *__restrict__ wsum,
-    const uint8_t *__restrict__ done, int32_t *__restrict__ big,
-    float4 *__restrict__ nodeQ) {
-  if (barrierStalled(ctl)) {
-    return;
-  }
-  __shared__ int levelOff[kMaxDepth + 3];
-  __shared__ int appended;
-  if (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent)) {
-    return;
-  }
+// The top set through global memory (any size): found top-down into `big`
+// level by level, then summed bottom-up, one memory round trip per step.
+__device__ inline void upwardTopGlobal(const Ctl *__restrict__ ctl,
+                                       float4 *__restrict__ nodeF,
+                                       const int4 *__restrict__ nodeI,
+                                       float2 *__restrict__ wsum,
+                                       const uint8_t *__restrict__ done,
+                                       int32_t *__restrict__ big,
+                                       float4 *__restrict__ nodeQ,
+                                       int *__restrict__ levelOff,
+                                       int &appended) {
   const int tid = threadIdx.x;
   if (tid == 0) {
     const bool rootOpen = ctl->nodeCount > 0 && nodeI[0].x != 0 && !done[0];
@@ -942,6 +936,166 @@ __global__ __launch_bounds__(1024) void k_upward_top(
       sumBranch(x, nodeI[x].x, nodeF, wsum, nodeQ);
     }
     __syncthreads();
+  }
+}
+
+// One workgroup: every branch not yet summed (the nodes above the subtree
+// roots, plus any subtree too large for k_subtree's list). That set is
+// closed upward. When it lies among the first kTopPrefetch nodes and has at
+// most one node per lane (the usual case: a few hundred nodes in the top
+// levels), it is found and summed in LDS: the children index and done flag
+// of those nodes are fetched at once, the set is found top-down in LDS (lane
+// t owns set member t and keeps its children's set slots in registers),
+// each member's finished children are fetched at once into its lane's
+// registers, the levels are summed bottom-up through LDS, and every result
+// is stored at the end -- a handful of memory round trips instead of a few
+// per level. Otherwise upwardTopGlobal. Each value is sumBranch's, in its
+// association.
+constexpr int kTopBlock = 1024;
+constexpr int kTopPrefetch = 4096;
+__global__ __launch_bounds__(kTopBlock) void k_upward_top(
+    const Ctl *__restrict__ ctl, float4 *__restrict__ nodeF,
+    const int4 *__restrict__ nodeI, float2 *__restrict__ wsum,
+    const uint8_t *__restrict__ done, int32_t *__restrict__ big,
+    float4 *__restrict__ nodeQ) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
+  __shared__ int levelOff[kMaxDepth + 3];
+  __shared__ int appended;
+  __shared__ int overflow;
+  __shared__ int32_t pch[kTopPrefetch]; // children index, -1 once summed
+  __shared__ int32_t setNode[kTopBlock];
+  __shared__ float vwx[kTopBlock], vwy[kTopBlock], vm[kTopBlock];
+  __shared__ float vcx[kTopBlock], vcy[kTopBlock];
+  __shared__ float vqx[kTopBlock], vqy[kTopBlock], vqz[kTopBlock];
+  if (ctl->status & (kStatusNodeOverflow | kStatusTreeInconsistent)) {
+    return;
+  }
+  const int tid = threadIdx.x;
+  const int count = ctl->nodeCount;
+  const int pre = min(count, kTopPrefetch);
+  // 1. Children index of the first nodes; -1 marks a summed node (a leaf,
+  //    or a branch done by k_subtree), which the set never enters.
+  for (int x = tid; x < pre; x += kTopBlock) {
+    const int c = nodeI[x].x;
+    pch[x] = (c == 0 || done[x]) ? -1 : c;
+  }
+  if (tid == 0) {
+    // Lane 0 wrote pch[0] itself above.
+    overflow = 0;
+    levelOff[0] = 0;
+    levelOff[1] = (pre > 0 && pch[0] >= 0) ? 1 : 0;
+    setNode[0] = 0;
+  }
+  __syncthreads();
+  // 2. The set, top-down: lane t holds member t and its children's slots.
+  int child = 0;            // first child of my member
+  int slot[4] = {-1, -1, -1, -1};
+  int myLevel = -1, levels = 0;
+  for (int d = 0; d <= kMaxDepth; ++d) {
+    const int lo = levelOff[d], hi = levelOff[d + 1];
+    if (lo == hi || overflow) {
+      break;
+    }
+    levels = d + 1;
+    if (tid == 0) {
+      appended = 0;
+    }
+    __syncthreads();
+    if (tid >= lo && tid < hi) {
+      myLevel = d;
+      child = pch[setNode[tid]];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int y = child + q;
+        if (y >= pre) {
+          overflow = 1;
+        } else if (pch[y] >= 0) {
+          const int k = hi + atomicAdd(&appended, 1);
+          if (k < kTopBlock) {
+            setNode[k] = y;
+            slot[q] = k;
+          } else {
+            overflow = 1;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      levelOff[d + 2] = min(hi + appended, kTopBlock);
+    }
+    __syncthreads();
+  }
+  if (overflow) {
+    upwardTopGlobal(ctl, nodeF, nodeI, wsum, done, big, nodeQ, levelOff,
+                    appended);
+    return;
+  }
+  // 3. My member's finished children, fetched at once.
+  float2 w[4] = {};
+  float4 f[4] = {}, k4[4] = {};
+  if (myLevel >= 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (slot[q] < 0) {
+        w[q] = wsum[child + q];
+        f[q] = nodeF[child + q];
+        if (nodeQ) {
+          k4[q] = nodeQ[child + q];
+        }
+      }
+    }
+  }
+  // 4. Bottom-up through LDS; each member as sumBranch / writeSum.
+  for (int d = levels - 1; d >= 0; --d) {
+    if (myLevel == d) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = slot[q];
+        if (k >= 0) {
+          w[q] = make_float2(vwx[k], vwy[k]);
+          f[q] = make_float4(vcx[k], vcy[k], vm[k], 0.f);
+          k4[q] = make_float4(vqx[k], vqy[k], vqz[k], 0.f);
+        }
+      }
+      const Sum3 v{w[0].x + w[1].x + w[2].x + w[3].x,
+                   w[0].y + w[1].y + w[2].y + w[3].y,
+                   f[0].z + f[1].z + f[2].z + f[3].z};
+      const float dv = (v.m < FLT_MIN) ? FLT_MIN : v.m;
+      const float cx = v.x / dv, cy = v.y / dv;
+      vwx[tid] = v.x;
+      vwy[tid] = v.y;
+      vm[tid] = v.m;
+      vcx[tid] = cx;
+      vcy[tid] = cy;
+      if (nodeQ) {
+        float sxx = 0.0f, sxy = 0.0f, syy = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float dx = f[q].x - cx, dy = f[q].y - cy;
+          sxx += k4[q].x + f[q].z * dx * dx;
+          sxy += k4[q].y + f[q].z * dx * dy;
+          syy += k4[q].z + f[q].z * dy * dy;
+        }
+        vqx[tid] = sxx;
+        vqy[tid] = sxy;
+        vqz[tid] = syy;
+      }
+    }
+    __syncthreads();
+  }
+  // 5. Results out.
+  if (myLevel >= 0) {
+    const int x = setNode[tid];
+    wsum[x] = make_float2(vwx[tid], vwy[tid]);
+    nodeF[x].x = vcx[tid];
+    nodeF[x].y = vcy[tid];
+    nodeF[x].z = vm[tid];
+    if (nodeQ) {
+      nodeQ[x] = make_float4(vqx[tid], vqy[tid], vqz[tid], vqx[tid] + vqz[tid]);
+    }
   }
 }
 

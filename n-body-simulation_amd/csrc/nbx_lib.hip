@@ -560,7 +560,7 @@ int launchBuild(nbx_sim *s) {
   LAUNCH(k_subtree, std::min<uint32_t>(1024, std::max<uint32_t>(1, n / 1024)),
          kBlock, s->ctl, s->subtreeRoots, s->nodeF, s->nodeI, s->wsum,
          s->nodeDone, nodeQ);
-  LAUNCH(k_upward_top, 1, 1024, s->ctl, s->nodeF, s->nodeI, s->wsum,
+  LAUNCH(k_upward_top, 1, kTopBlock, s->ctl, s->nodeF, s->nodeI, s->wsum,
          s->nodeDone, s->bigNodes, nodeQ);
   if (learned) {
     LAUNCH(k_node_learn, blocksFor(s->nodeCap), kBlock, s->nodeF, s->nodeQ,

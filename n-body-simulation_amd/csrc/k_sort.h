@@ -643,6 +643,7 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
   // of twelve. When a bucket is larger than the LDS holds, every workgroup
   // sees it after the row scans and the four LSD passes run instead, from
   // the keys still in registers.
+  bool msdDone = false;
   for (int pass = BUILD ? -1 : 0; pass < 4; ++pass) {
     const bool msd = pass < 0;
     const int shift = msd ? 24 : 8 * pass;
@@ -795,10 +796,13 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
       gridBarrier(bar, grid, status, spinLimit);
     }
     if (msd) {
-      msdBuckets(n, grid, keys1, vals1, keys0, vals0, bsize, bstart, sk, sv,
-                 wh, dstart, goff, scratch);
-      return;
+      msdDone = true;
+      break;
     }
+  }
+  if (BUILD && msdDone) {
+    msdBuckets(n, grid, keys1, vals1, keys0, vals0, bsize, bstart, sk, sv,
+               wh, dstart, goff, scratch);
   }
 }
 

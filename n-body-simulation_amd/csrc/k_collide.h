@@ -889,8 +889,9 @@ __device__ __forceinline__ void resolveComponent(int start, int size,
 }
 
 // One lane per component of up to kSmallComp pairs: pairLess order, then the
-// sequential replay. Larger components are queued by size for
-// k_resolve_wave's tiers, beyond those for k_resolve_big.
+// sequential replay. Larger components are queued by size for the wave
+// tiers (k_resolve_wave, tier 3 in k_resolve_large), beyond those for the
+// workgroup path (k_resolve_large).
 __global__ __launch_bounds__(kBlock) void k_resolve_small(
     Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
     int4 *__restrict__ segPairs, int4 *__restrict__ tierList,
@@ -1004,7 +1005,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
 // that body, the bodies are staged in LDS and the rounds replay the pairs.
 // The pairs are written back in the order they were replayed. A component
 // with more than BMAX bodies goes on to the next tier's queue, one with a
-// body of more than kMaxDegree pairs to k_resolve_big.
+// body of more than kMaxDegree pairs to the workgroup path
+// (resolveBigComponent, in k_resolve_large).
 constexpr int kMaxDegree = 16;
 constexpr uint32_t kEmptyKey = 0xffffffffu;
 constexpr uint16_t kNoPair = 0xffffu;
@@ -1149,9 +1151,19 @@ __device__ __forceinline__ void resolveWaveComponent(
     int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
     float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
     float *__restrict__ vy, const float *__restrict__ m,
-    const float *__restrict__ r) {
+    const float *__restrict__ r, int *localBig = nullptr) {
   using L = WaveLds<PMAX, BMAX>;
   using W = WaveCtx<PMAX, BMAX, G>;
+  // A component the wave path cannot hold goes to the next tier, or to the
+  // workgroup path: appended to bigList, or (localBig) handed back to the
+  // caller, which runs it at once.
+  auto toBig = [&]() {
+    if (localBig) {
+      *localBig = info.z;
+    } else {
+      bigList[atomicAdd(&ctl->bigCount, 1u)] = info.z;
+    }
+  };
   uint32_t *pairSlot = lds + L::kPairSlot;
   uint16_t *succ = reinterpret_cast<uint16_t *>(lds + L::kSucc);
   uint32_t *indeg = lds + L::kIndeg;
@@ -1226,7 +1238,7 @@ __device__ __forceinline__ void resolveWaveComponent(
         tierList[static_cast<size_t>(tier + 1) * listCap +
                  atomicAdd(&ctl->waveCount[tier + 1], 1u)] = info;
       } else {
-        bigList[atomicAdd(&ctl->bigCount, 1u)] = info.z;
+        toBig();
       }
     }
     return;
@@ -1297,7 +1309,7 @@ __device__ __forceinline__ void resolveWaveComponent(
   }
   if (maxDeg > static_cast<uint32_t>(kMaxDegree)) {
     if (lane == 0) {
-      bigList[atomicAdd(&ctl->bigCount, 1u)] = info.z;
+      toBig();
     }
     waveSync();
     return;
@@ -1563,18 +1575,15 @@ __device__ inline int mergeSplit(const int4 *__restrict__ a, int na,
   return lo;
 }
 
-__global__ __launch_bounds__(kBlock) void k_resolve_big(
-    Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
-    const int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
-    int4 *__restrict__ tmpPairs, int32_t *__restrict__ bodySlot,
-    float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
-    float *__restrict__ vy, const float *__restrict__ m,
-    const float *__restrict__ r) {
-  if (barrierStalled(ctl)) {
-    return;
-  }
-  __shared__ __align__(16) uint32_t lds[kResolveLds];
-  __shared__ int nBodies;
+// One contact component by the whole workgroup (info = its compInfo
+// entry): pairLess order, then the replay (the two steps described above).
+// lds: kResolveLds words.
+__device__ inline void resolveBigComponent(
+    int4 info, uint32_t *__restrict__ lds, int &nBodies,
+    int4 *__restrict__ segPairs, int4 *__restrict__ tmpPairs,
+    int32_t *__restrict__ bodySlot, float *__restrict__ px,
+    float *__restrict__ py, float *__restrict__ vx, float *__restrict__ vy,
+    const float *__restrict__ m, const float *__restrict__ r) {
   // Sort area.
   int4 *sp = reinterpret_cast<int4 *>(lds);
   int16_t *order = reinterpret_cast<int16_t *>(lds + 4 * kBigCompMax);
@@ -1585,128 +1594,178 @@ __global__ __launch_bounds__(kBlock) void k_resolve_big(
   float *lr = lm + kReplayBodies;
   int32_t *bodyIdx = reinterpret_cast<int32_t *>(lr + kReplayBodies);
   int2 *chunk = reinterpret_cast<int2 *>(bodyIdx + kReplayBodies);
+  const int start = info.y, size = info.z;
+  int4 *seg = segPairs + start;
 
-  const uint32_t big = ctl->bigCount;
-  for (uint32_t g = blockIdx.x; g < big; g += gridDim.x) {
-    const int4 info = compInfo[bigList[g]];
-    const int start = info.y, size = info.z;
-    int4 *seg = segPairs + start;
-
-    // 1. pairLess order.
-    if (size <= kBigCompMax) {
-      sortRunInLds(seg, size, sp, order);
-    } else {
-      for (int c0 = 0; c0 < size; c0 += kBigCompMax) {
-        sortRunInLds(seg + c0, min(kBigCompMax, size - c0), sp, order);
-      }
-      int4 *src = seg, *dst = tmpPairs + start;
-      for (int w = kBigCompMax; w < size; w <<= 1) {
-        for (int base = 0; base < size; base += 2 * w) {
-          const int na = min(w, size - base);
-          const int nb = max(0, min(w, size - base - na));
-          const int4 *a = src + base, *b = src + base + na;
-          const int total = na + nb;
-          for (int o = threadIdx.x * kMergeItems; o < total;
-               o += kBlock * kMergeItems) {
-            int ia = mergeSplit(a, na, b, nb, o);
-            int ib = o - ia;
-            const int cnt = min(kMergeItems, total - o);
-            for (int q = 0; q < cnt; ++q) {
-              const bool takeA =
-                  ib >= nb || (ia < na && !pairLess(b[ib], a[ia]));
-              dst[base + o + q] = takeA ? a[ia++] : b[ib++];
-            }
-          }
-        }
-        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-        __syncthreads();
-        int4 *t = src;
-        src = dst;
-        dst = t;
-      }
-      if (src != seg) {
-        for (int k = threadIdx.x; k < size; k += kBlock) {
-          seg[k] = src[k];
-        }
-        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-        __syncthreads();
-      }
+  // 1. pairLess order.
+  if (size <= kBigCompMax) {
+    sortRunInLds(seg, size, sp, order);
+  } else {
+    for (int c0 = 0; c0 < size; c0 += kBigCompMax) {
+      sortRunInLds(seg + c0, min(kBigCompMax, size - c0), sp, order);
     }
-
-    // 2. Local slots for the component's bodies.
-    if (threadIdx.x == 0) {
-      nBodies = 0;
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < size; k += kBlock) {
-      const int4 q = seg[k];
-      const int ends[2] = {q.x, q.y};
-      for (int e = 0; e < 2; ++e) {
-        if (atomicCAS(&bodySlot[ends[e]], -1, -2) == -1) {
-          const int slot = atomicAdd(&nBodies, 1);
-          storeAgent(reinterpret_cast<uint32_t *>(&bodySlot[ends[e]]),
-                     static_cast<uint32_t>(slot));
-          if (slot < kReplayBodies) {
-            bodyIdx[slot] = ends[e];
+    int4 *src = seg, *dst = tmpPairs + start;
+    for (int w = kBigCompMax; w < size; w <<= 1) {
+      for (int base = 0; base < size; base += 2 * w) {
+        const int na = min(w, size - base);
+        const int nb = max(0, min(w, size - base - na));
+        const int4 *a = src + base, *b = src + base + na;
+        const int total = na + nb;
+        for (int o = threadIdx.x * kMergeItems; o < total;
+             o += kBlock * kMergeItems) {
+          int ia = mergeSplit(a, na, b, nb, o);
+          int ib = o - ia;
+          const int cnt = min(kMergeItems, total - o);
+          for (int q = 0; q < cnt; ++q) {
+            const bool takeA =
+                ib >= nb || (ia < na && !pairLess(b[ib], a[ia]));
+            dst[base + o + q] = takeA ? a[ia++] : b[ib++];
           }
         }
       }
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      __syncthreads();
+      int4 *t = src;
+      src = dst;
+      dst = t;
     }
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-    __syncthreads();
-    const int nb = nBodies;
-
-    // 3. Replay in pairLess order.
-    if (nb <= kReplayBodies) {
-      for (int s = threadIdx.x; s < nb; s += kBlock) {
-        const int i = bodyIdx[s];
-        lx[s] = px[i];
-        ly[s] = py[i];
-        lvx[s] = vx[i];
-        lvy[s] = vy[i];
-        lm[s] = m[i];
-        lr[s] = r[i];
+    if (src != seg) {
+      for (int k = threadIdx.x; k < size; k += kBlock) {
+        seg[k] = src[k];
       }
-      for (int c0 = 0; c0 < size; c0 += kReplayChunk) {
-        const int cnt = min(kReplayChunk, size - c0);
-        __syncthreads();
-        for (int k = threadIdx.x; k < cnt; k += kBlock) {
-          const int4 q = seg[c0 + k];
-          chunk[k] = make_int2(
-              static_cast<int>(loadAgent(reinterpret_cast<const uint32_t *>(&bodySlot[q.x]))),
-              static_cast<int>(loadAgent(reinterpret_cast<const uint32_t *>(&bodySlot[q.y]))));
+      __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+      __syncthreads();
+    }
+  }
+
+  // 2. Local slots for the component's bodies.
+  if (threadIdx.x == 0) {
+    nBodies = 0;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < size; k += kBlock) {
+    const int4 q = seg[k];
+    const int ends[2] = {q.x, q.y};
+    for (int e = 0; e < 2; ++e) {
+      if (atomicCAS(&bodySlot[ends[e]], -1, -2) == -1) {
+        const int slot = atomicAdd(&nBodies, 1);
+        storeAgent(reinterpret_cast<uint32_t *>(&bodySlot[ends[e]]),
+                   static_cast<uint32_t>(slot));
+        if (slot < kReplayBodies) {
+          bodyIdx[slot] = ends[e];
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-          for (int k = 0; k < cnt; ++k) {
-            resolvePair(lx, ly, lvx, lvy, lm, lr, chunk[k].x, chunk[k].y);
-          }
-        }
+      }
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  __syncthreads();
+  const int nb = nBodies;
+
+  // 3. Replay in pairLess order.
+  if (nb <= kReplayBodies) {
+    for (int s = threadIdx.x; s < nb; s += kBlock) {
+      const int i = bodyIdx[s];
+      lx[s] = px[i];
+      ly[s] = py[i];
+      lvx[s] = vx[i];
+      lvy[s] = vy[i];
+      lm[s] = m[i];
+      lr[s] = r[i];
+    }
+    for (int c0 = 0; c0 < size; c0 += kReplayChunk) {
+      const int cnt = min(kReplayChunk, size - c0);
+      __syncthreads();
+      for (int k = threadIdx.x; k < cnt; k += kBlock) {
+        const int4 q = seg[c0 + k];
+        chunk[k] = make_int2(
+            static_cast<int>(loadAgent(reinterpret_cast<const uint32_t *>(&bodySlot[q.x]))),
+            static_cast<int>(loadAgent(reinterpret_cast<const uint32_t *>(&bodySlot[q.y]))));
       }
       __syncthreads();
-      for (int s = threadIdx.x; s < nb; s += kBlock) {
-        const int i = bodyIdx[s];
-        px[i] = lx[s];
-        py[i] = ly[s];
-        vx[i] = lvx[s];
-        vy[i] = lvy[s];
-        bodySlot[i] = -1;
-      }
-    } else {
       if (threadIdx.x == 0) {
-        for (int k = 0; k < size; ++k) {
-          const int4 q = seg[k];
-          resolvePair(px, py, vx, vy, m, r, q.x, q.y);
+        for (int k = 0; k < cnt; ++k) {
+          resolvePair(lx, ly, lvx, lvy, lm, lr, chunk[k].x, chunk[k].y);
         }
       }
-      for (int k = threadIdx.x; k < size; k += kBlock) {
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < nb; s += kBlock) {
+      const int i = bodyIdx[s];
+      px[i] = lx[s];
+      py[i] = ly[s];
+      vx[i] = lvx[s];
+      vy[i] = lvy[s];
+      bodySlot[i] = -1;
+    }
+  } else {
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < size; ++k) {
         const int4 q = seg[k];
-        bodySlot[q.x] = -1;
-        bodySlot[q.y] = -1;
+        resolvePair(px, py, vx, vy, m, r, q.x, q.y);
       }
     }
-    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-    __syncthreads();
+    for (int k = threadIdx.x; k < size; k += kBlock) {
+      const int4 q = seg[k];
+      bodySlot[q.x] = -1;
+      bodySlot[q.y] = -1;
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  __syncthreads();
+}
+
+// The largest contact components, in one launch: first tier 3 of the wave
+// path (1025..4096 pairs; wave 0 of each workgroup, 64 lanes a component),
+// where a component that the tier cannot hold is run by the workgroup path
+// at once; then the components earlier kernels sent to the workgroup path
+// (bigList, complete when this kernel starts). LDS holds either layout.
+constexpr int kLargeLds = WaveLds<4096, 2560>::kWords > kResolveLds
+                              ? WaveLds<4096, 2560>::kWords
+                              : kResolveLds;
+__global__ __launch_bounds__(kBlock) void k_resolve_large(
+    Ctl *__restrict__ ctl, int4 *__restrict__ tierList, uint32_t listCap,
+    const int4 *__restrict__ compInfo, int32_t *__restrict__ bigList,
+    int4 *__restrict__ segPairs, int4 *__restrict__ tmpPairs,
+    int32_t *__restrict__ bodySlot, float *__restrict__ px,
+    float *__restrict__ py, float *__restrict__ vx, float *__restrict__ vy,
+    const float *__restrict__ m, const float *__restrict__ r) {
+  if (barrierStalled(ctl)) {
+    return;
+  }
+  __shared__ __align__(16) uint32_t lds[kLargeLds];
+  __shared__ int nBodies;
+  __shared__ int local;
+  constexpr int kTier = kWaveTiers - 1;
+  static_assert(waveTierPairs(kTier) == 4096, "tier 3 is 1025..4096 pairs");
+  using W = WaveCtx<4096, 2560, kWave>;
+  static_assert(W::kPf == 0, "tier 3 loads its pairs itself");
+  if (!(ctl->status & kStatusPairOverflow)) {
+    const uint32_t count = ctl->waveCount[kTier];
+    const int4 *list = tierList + static_cast<size_t>(kTier) * listCap;
+    for (uint32_t g = blockIdx.x; g < count; g += gridDim.x) {
+      if (threadIdx.x == 0) {
+        local = -1;
+      }
+      __syncthreads();
+      if (threadIdx.x < kWave) {
+        typename W::Pairs none{};
+        resolveWaveComponent<4096, 2560, kWave>(
+            lds, ctl, kTier, list[g], none, tierList, listCap, bigList,
+            segPairs, px, py, vx, vy, m, r, &local);
+      }
+      __syncthreads();
+      const int c = local;
+      if (c >= 0) {
+        resolveBigComponent(compInfo[c], lds, nBodies, segPairs, tmpPairs,
+                            bodySlot, px, py, vx, vy, m, r);
+      }
+      __syncthreads();
+    }
+  }
+  const uint32_t big = ctl->bigCount;
+  for (uint32_t g = blockIdx.x; g < big; g += gridDim.x) {
+    resolveBigComponent(compInfo[bigList[g]], lds, nBodies, segPairs,
+                        tmpPairs, bodySlot, px, py, vx, vy, m, r);
   }
 }
 

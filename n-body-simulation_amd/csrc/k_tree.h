@@ -118,27 +118,84 @@ __global__ __launch_bounds__(kBlock) void k_keys(
   vals[i] = i;
 }
 
+// Stable bitonic sort, in LDS, of perm[start, start + len) by low key word
+// (len <= kDeepMax), by the whole workgroup.
+__device__ inline void deepSortRun(int start, int len,
+                                   const uint32_t *__restrict__ keyLo,
+                                   uint32_t *__restrict__ perm,
+                                   uint32_t *__restrict__ sk,
+                                   uint32_t *__restrict__ sv,
+                                   int16_t *__restrict__ order) {
+  int p2 = 1;
+  while (p2 < len) {
+    p2 <<= 1;
+  }
+  for (int k = threadIdx.x; k < p2; k += kBlock) {
+    if (k < len) {
+      sv[k] = perm[start + k];
+      sk[k] = keyLo[sv[k]];
+    }
+    order[k] = static_cast<int16_t>(k);
+  }
+  __syncthreads();
+  for (int size = 2; size <= p2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = threadIdx.x; k < p2; k += kBlock) {
+        const int partner = k ^ stride;
+        if (partner > k) {
+          const bool up = (k & size) == 0;
+          const int a = order[k], c = order[partner];
+          // (key, position) order; padding sorts last.
+          bool aGreater;
+          if (a >= len || c >= len) {
+            aGreater = a > c;
+          } else {
+            aGreater = sk[a] > sk[c] || (sk[a] == sk[c] && a > c);
+          }
+          if (aGreater == up) {
+            order[k] = static_cast<int16_t>(c);
+            order[partner] = static_cast<int16_t>(a);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = threadIdx.x; k < len; k += kBlock) {
+    perm[start + k] = sv[order[k]];
+  }
+  __syncthreads();
+}
+
 // Runs of bodies with equal high key words are put in low-word order, so the
 // bodies end in full-key order (equal full keys keep their order, as the
 // oracle's stable sort keeps them): a run of up to kLeafCapacity bodies by
 // the lane at its start (insertion sort), a longer run -- whose order
-// decides nodes below depth 16 -- queued for k_deep_sort.
-__global__ __launch_bounds__(kBlock) void k_deep_detect(
+// decides nodes below depth 16 -- by the whole workgroup whose range holds
+// its start, in LDS, after the lanes' runs. Runs are disjoint and only a
+// run's start acts on it, so no two lanes or workgroups touch the same
+// entries.
+__global__ __launch_bounds__(kBlock) void k_deep(
     uint32_t n, const uint32_t *__restrict__ hi,
     const uint32_t *__restrict__ keyLo, uint32_t *__restrict__ perm,
-    Ctl *__restrict__ ctl, int2 *__restrict__ deepList) {
+    Ctl *__restrict__ ctl) {
   if (barrierStalled(ctl)) {
     return;
   }
+  __shared__ int2 longRuns[kBlock];
+  __shared__ int nLong;
+  __shared__ uint32_t sk[kDeepMax];
+  __shared__ uint32_t sv[kDeepMax];
+  __shared__ int16_t order[kDeepMax];
+  if (threadIdx.x == 0) {
+    nLong = 0;
+  }
+  __syncthreads();
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j + 1 >= n) {
-    return;
-  }
-  const uint32_t h = hi[j];
-  if ((j > 0 && hi[j - 1] == h) || hi[j + 1] != h) {
-    return;
-  }
-  if (j + kLeafCapacity < n && hi[j + kLeafCapacity] == h) {
+  const uint32_t h = j + 1 < n ? hi[j] : 0u;
+  const bool start =
+      j + 1 < n && !(j > 0 && hi[j - 1] == h) && hi[j + 1] == h;
+  if (start && j + kLeafCapacity < n && hi[j + kLeafCapacity] == h) {
     uint32_t a = j + kLeafCapacity + 1, b = n;
     while (a < b) {
       const uint32_t mid = (a + b) >> 1;
@@ -148,90 +205,39 @@ __global__ __launch_bounds__(kBlock) void k_deep_detect(
         b = mid;
       }
     }
-    const int slot = atomicAdd(&ctl->deepCount, 1);
-    deepList[slot] = make_int2(static_cast<int>(j), static_cast<int>(a - j));
-    return;
-  }
-  uint32_t p[kLeafCapacity], l[kLeafCapacity];
-  int len = 0;
-  while (len < kLeafCapacity && j + len < n && hi[j + len] == h) {
-    const uint32_t v = perm[j + len];
-    const uint32_t key = keyLo[v];
-    int q = len;
-    while (q > 0 && key < l[q - 1]) {
-      p[q] = p[q - 1];
-      l[q] = l[q - 1];
-      --q;
+    longRuns[atomicAdd(&nLong, 1)] =
+        make_int2(static_cast<int>(j), static_cast<int>(a - j));
+  } else if (start) {
+    uint32_t p[kLeafCapacity], l[kLeafCapacity];
+    int len = 0;
+    while (len < kLeafCapacity && j + len < n && hi[j + len] == h) {
+      const uint32_t v = perm[j + len];
+      const uint32_t key = keyLo[v];
+      int q = len;
+      while (q > 0 && key < l[q - 1]) {
+        p[q] = p[q - 1];
+        l[q] = l[q - 1];
+        --q;
+      }
+      p[q] = v;
+      l[q] = key;
+      ++len;
     }
-    p[q] = v;
-    l[q] = key;
-    ++len;
+    for (int k = 0; k < len; ++k) {
+      perm[j + k] = p[k];
+    }
   }
-  for (int k = 0; k < len; ++k) {
-    perm[j + k] = p[k];
-  }
-}
-
-// One workgroup per deep run: stable bitonic sort of its permutation
-// entries by low key word, in LDS.
-__global__ __launch_bounds__(kBlock) void k_deep_sort(
-    Ctl *__restrict__ ctl, const int2 *__restrict__ deepList,
-    const uint32_t *__restrict__ keyLo, uint32_t *__restrict__ perm) {
-  if (barrierStalled(ctl)) {
-    return;
-  }
-  __shared__ uint32_t sk[kDeepMax];
-  __shared__ uint32_t sv[kDeepMax];
-  __shared__ int16_t order[kDeepMax];
-  const int count = ctl->deepCount;
-  for (int g = blockIdx.x; g < count; g += gridDim.x) {
-    const int2 seg = deepList[g];
-    const int start = seg.x, len = seg.y;
-    if (len > kDeepMax) {
+  __syncthreads();
+  const int runs = nLong;
+  for (int g = 0; g < runs; ++g) {
+    const int2 seg = longRuns[g];
+    if (seg.y > kDeepMax) {
       if (threadIdx.x == 0) {
         atomicOr(&ctl->status, kStatusDeepTooLarge);
       }
       continue;
     }
-    int p2 = 1;
-    while (p2 < len) {
-      p2 <<= 1;
-    }
-    for (int k = threadIdx.x; k < p2; k += kBlock) {
-      if (k < len) {
-        sv[k] = perm[start + k];
-        sk[k] = keyLo[sv[k]];
-      }
-      order[k] = static_cast<int16_t>(k);
-    }
-    __syncthreads();
-    for (int size = 2; size <= p2; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int k = threadIdx.x; k < p2; k += kBlock) {
-          const int partner = k ^ stride;
-          if (partner > k) {
-            const bool up = (k & size) == 0;
-            const int a = order[k], c = order[partner];
-            // (key, position) order; padding sorts last.
-            bool aGreater;
-            if (a >= len || c >= len) {
-              aGreater = a > c;
-            } else {
-              aGreater = sk[a] > sk[c] || (sk[a] == sk[c] && a > c);
-            }
-            if (aGreater == up) {
-              order[k] = static_cast<int16_t>(c);
-              order[partner] = static_cast<int16_t>(a);
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    for (int k = threadIdx.x; k < len; k += kBlock) {
-      perm[start + k] = sv[order[k]];
-    }
-    __syncthreads();
+    deepSortRun(seg.x, seg.y, keyLo, perm, sk, sv, order);
   }
 }
 

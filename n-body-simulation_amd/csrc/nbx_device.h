@@ -59,7 +59,7 @@ struct Ctl {
   int32_t splits[kMaxDepth];         // splitting nodes at each depth
   int32_t levelStart[kMaxDepth + 2]; // first node index at each depth
   int32_t splitBase[kMaxDepth + 1];  // offset of each depth in the split list
-  int32_t totalSplits, nodeCount, levels, deepCount;
+  int32_t totalSplits, nodeCount, levels;
   int32_t subtreeRoots;
   uint32_t status;
   uint32_t waveCount[kWaveTiers]; // components queued per k_resolve_wave tier
@@ -184,7 +184,6 @@ __device__ inline void publishRoot(Ctl *ctl, uint4 b, float qcx, float qcy,
   ctl->quad[0] = qcx;
   ctl->quad[1] = qcy;
   ctl->quad[2] = qs;
-  ctl->deepCount = 0;
   ctl->subtreeRoots = 0;
   atomicAnd(&ctl->status, ~(kStatusNodeOverflow | kStatusDeepTooLarge |
                             kStatusTreeInconsistent));

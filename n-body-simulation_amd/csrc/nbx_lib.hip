@@ -148,7 +148,6 @@ struct nbx_sim {
   uint32_t *sortBarrier = nullptr; // grid barrier words of k_sort_onelaunch
   uint32_t sortGrid = 0; // its maximum workgroups; 0: three launches a pass
   uint32_t barrierSpins = kBarrierSpins; // poll bound of the grid barriers
-  int2 *deepList = nullptr;
   uint32_t *tileCount = nullptr;
   int64_t nodeCap = 0;
   int32_t *splitList = nullptr, *nodeOfSplit = nullptr;
@@ -487,7 +486,7 @@ int launchCollide(nbx_sim *s, bool countersReset) {
          s->pairComp, s->compCount, s->compStart, s->segPairs);
   LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->segPairs,
          s->tierList, s->tierCap, s->bigList, px, py, vx, vy, m, r);
-  // Components of 17..4096 pairs, in tiers by size (4, 2, 1, 1 a wave); the
+  // Components of 17..1024 pairs, in tiers by size (4, 2, 1 a wave); the
   // grids are the LDS residency of each tier on 256 CUs (a workgroup
   // beyond the queued count returns at once).
   static_assert(waveTierPairs(0) == 64 && waveTierPairs(1) == 256 &&
@@ -500,10 +499,10 @@ int launchCollide(nbx_sim *s, bool countersReset) {
          s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
   LAUNCH((k_resolve_wave<1024, 640, 64>), 256 * 4, kWave, s->ctl, 2,
          s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
-  LAUNCH((k_resolve_wave<4096, 2560, 64>), 256, kWave, s->ctl, 3,
-         s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
-  LAUNCH(k_resolve_big, 256, kBlock, s->ctl, s->compInfo, s->bigList,
-         s->segPairs, s->pairs, s->bodySlot, px, py, vx, vy, m, r);
+  // Tier 3 (1025..4096 pairs) and the workgroup path in one launch.
+  LAUNCH(k_resolve_large, 256, kBlock, s->ctl, s->tierList, s->tierCap,
+         s->compInfo, s->bigList, s->segPairs, s->pairs, s->bodySlot, px, py,
+         vx, vy, m, r);
   s->collided = true;
   return NBX_OK;
 }
@@ -531,8 +530,7 @@ int launchBuild(nbx_sim *s) {
   }
   const uint32_t *hi = s->keyHi[0];
   uint32_t *perm = s->vals[0];
-  LAUNCH(k_deep_detect, nb, kBlock, n, hi, s->keyLo, perm, s->ctl, s->deepList);
-  LAUNCH(k_deep_sort, 64, kBlock, s->ctl, s->deepList, s->keyLo, perm);
+  LAUNCH(k_deep, nb, kBlock, n, hi, s->keyLo, perm, s->ctl);
   LAUNCH(k_permute, nb, kBlock, n, perm, s->px[c], s->py[c], s->vx[c],
          s->vy[c], s->m[c], s->r[c], s->id[c], s->px[o], s->py[o], s->vx[o],
          s->vy[o], s->m[o], s->r[o], s->id[o], s->bodyPack);
@@ -1017,7 +1015,6 @@ int nbx_create_ex(uint64_t capacity, uint32_t flags, nbx_sim **out) {
     false)
   A(s->sortRowTotal, 256, false)
   A(s->sortBarrier, kBarrierWords, true)
-  A(s->deepList, n / (kLeafCapacity + 1) + 1, false)
   A(s->tileCount, static_cast<uint64_t>(kMaxDepth) * blocksFor(n), false)
   s->nodeCap = static_cast<int64_t>(8 * n + 8);
   A(s->splitList, s->nodeCap / 4 + 1, false)

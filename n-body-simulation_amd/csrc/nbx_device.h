@@ -192,29 +192,30 @@ __device__ inline void publishRoot(Ctl *ctl, uint4 b, float qcx, float qcy,
 // Float-descent quadrant key of a body (QuadTree.cpp:57-90 partitions,
 // Quad::intoQuadrant QuadTree.h:14-25): digit 2*(y >= cy) + (x >= cx) at
 // each of 32 depths, the centre moved as the reference moves it; depths
-// 0-15 in `hi`, 16-31 in `lo`, two bits each from the top.
+// 0-15 in `hi`, 16-31 in `lo`, two bits each from the top. The reference's
+// move (b - 0.5) * s is exactly +-(0.5 * s) (a scaling by 0.5, and IEEE
+// rounding is symmetric in sign), so the centre takes a select of that
+// half-size and its negation: the same bits in fewer instructions.
+__device__ inline uint32_t descentWord(float x, float y, float &cx, float &cy,
+                                       float &s) {
+  uint32_t key = 0;
+#pragma unroll 4
+  for (int d = 0; d < 16; ++d) {
+    const bool bx = !(x < cx);
+    const bool by = !(y < cy);
+    key = (key << 2) | (by ? 2u : 0u) | (bx ? 1u : 0u);
+    s *= 0.5f;
+    const float h = 0.5f * s;
+    cx += bx ? h : -h;
+    cy += by ? h : -h;
+  }
+  return key;
+}
+
 __device__ inline void descentKeys(float x, float y, float cx, float cy,
                                    float s, uint32_t &hi, uint32_t &lo) {
-  hi = 0;
-  lo = 0;
-#pragma unroll 4
-  for (int d = 0; d < 16; ++d) {
-    const int bx = (x < cx) ? 0 : 1;
-    const int by = (y < cy) ? 0 : 1;
-    hi |= static_cast<uint32_t>(2 * by + bx) << (30 - 2 * d);
-    s *= 0.5f;
-    cx += (static_cast<float>(bx) - 0.5f) * s;
-    cy += (static_cast<float>(by) - 0.5f) * s;
-  }
-#pragma unroll 4
-  for (int d = 0; d < 16; ++d) {
-    const int bx = (x < cx) ? 0 : 1;
-    const int by = (y < cy) ? 0 : 1;
-    lo |= static_cast<uint32_t>(2 * by + bx) << (30 - 2 * d);
-    s *= 0.5f;
-    cx += (static_cast<float>(bx) - 0.5f) * s;
-    cy += (static_cast<float>(by) - 0.5f) * s;
-  }
+  hi = descentWord(x, y, cx, cy, s);
+  lo = descentWord(x, y, cx, cy, s);
 }
 
 // QuadTree.cpp:11-26 fastInvSqrt: bit-level guess + one Newton step.

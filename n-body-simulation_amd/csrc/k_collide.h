@@ -901,8 +901,14 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
   if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
     return;
   }
+  // The grid covers the bodies (the most components there can be); a
+  // workgroup past the components has nothing to queue or replay.
+  const uint32_t comps = ctl->compCount;
+  if (blockIdx.x * kBlock >= comps) {
+    return;
+  }
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  const bool live = j < ctl->compCount;
+  const bool live = j < comps;
   int4 info = make_int4(0, 0, 0, 0);
   if (live) {
     info = compInfo[j];

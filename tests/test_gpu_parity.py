@@ -333,6 +333,45 @@ def test_deep_cluster_below_depth_16(nbx, oracle):
     assert_bits_equal(got.ay, wy, "ay")
 
 
+@pytest.mark.parametrize("n,clump", [(200_000, 6000), (20_000, 10_000)])
+def test_build_with_dense_clump(nbx, oracle, n, clump):
+    """A uniform square with a dense clump: (200k, 6000) puts the clump's
+    ancestors (more than 4096 bodies each) beyond the first 4096 nodes, so
+    the upward pass's top set takes its level-by-level form through global
+    memory; (20k, 10000) puts more than 8192 keys under one top digit, so the
+    sort runs its four passes across the grid instead of the LDS buckets.
+    Keys, permutation and every node field bit-exact against the oracle."""
+    from oracle_lib import Bodies
+    rng = np.random.default_rng(11)
+    b = Bodies.empty(n)
+    b.px[:] = rng.uniform(-1e4, 1e4, n).astype(np.float32)
+    b.py[:] = rng.uniform(-1e4, 1e4, n).astype(np.float32)
+    b.px[:clump] = (2345.5 + rng.uniform(0, 6.0, clump)).astype(np.float32)
+    b.py[:clump] = (-1234.25 + rng.uniform(0, 6.0, clump)).astype(np.float32)
+    b.m[:] = 1.0
+    b.r[:] = 1e-4
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    srt, hi, lo, perm, quad = oracle.sorted_state(b)
+    tree = oracle.build_tree(srt, hi, lo, quad)
+    top = np.bincount(hi >> 24, minlength=256).max()
+    if n == 20_000:
+        assert top > 8192, "the clump must overflow one top-digit bucket"
+    else:
+        assert top <= 8192
+        deep = np.flatnonzero((tree.end - tree.start > 4096) & (tree.children != 0))
+        assert deep.max() >= 4096, "a top-set node must lie beyond the first 4096"
+    with new_sim(nbx, b, theta=0.5) as sim:
+        sim.build()
+        ghi, glo, gperm = sim.keys()
+        gt = sim.tree()
+    assert np.array_equal(ghi, hi) and np.array_equal(glo, lo), "keys"
+    assert np.array_equal(gperm, perm), "permutation"
+    for k in ("children", "next", "start", "end"):
+        assert np.array_equal(gt[k], getattr(tree, k)), k
+    for k in ("comX", "comY", "mass", "size", "centerX", "centerY"):
+        assert_bits_equal(gt[k], getattr(tree, k), k)
+
+
 def test_deep_tree_compensated_walk(nbx, oracle):
     """A tree deeper than 21 levels: the compensated walk's traversal stack
     outgrows its 64 register lanes and spills to LDS. Forces within the

@@ -224,14 +224,6 @@ constexpr int kWalkBlock = NBX_WALK_BLOCK; // lanes per workgroup of k_walk_fast
 constexpr int kBodyPad = 4;
 constexpr int kStackDepth = 128; // >= 3 * kMaxDepth + 4 entries per wave
 
-// fastInvSqrt (QuadTree.cpp:11-26) with its Newton step as one fused
-// multiply-add: the same approximation, rounded once less.
-__device__ inline float fisrFused(float x) {
-  const uint32_t i = 0x5f3759dfu - (__float_as_uint(x) >> 1);
-  const float y = __uint_as_float(i);
-  return y * fmaf(-(0.5f * x) * y, y, 1.5f);
-}
-
 // A 64-bit value every lane holds (read from one LDS address), moved to
 // scalar registers. readfirstlane works on int: the halves go through
 // uint32_t so the low word is not sign-extended.
@@ -317,11 +309,26 @@ struct RegStack {
 
 // CAP = false drops the min(., FLT_MAX) guard; the host selects it only when
 // no factor can overflow (see capFree in the launch code).
+// Reference law: m fastInvSqrt(d^2) / (d^2 + eps^2) (QuadTree.cpp:11-26,
+// :276-292), with the Newton step y (1.5 - 0.5 d^2 y y) as one fused
+// multiply-add (one rounding less than the reference), and computed as
+// m (2 fisr) (w / 2): the doubled Newton factor 2b = 3 - (d^2 y) y is b
+// doubled exactly (IEEE rounding commutes with scaling by 2) and saves the
+// product -0.5 d^2; w / 2 = rcp(2 d^2 + 2 eps^2) is w halved exactly. The
+// factors of 2 cancel exactly in the products, so the factor equals
+// m * y (1.5 - 0.5 d^2 y y) * w with that fused step, bit for bit, for
+// arguments away from overflow and the subnormal range.
 template <int LAW, bool CAP>
 __device__ inline float fastFactor(float mass, float dSq, float eSq) {
-  const float inv =
-      LAW == kLawReference ? fisrFused(dSq) : __builtin_amdgcn_rsqf(dSq);
-  const float f = mass * inv * __builtin_amdgcn_rcpf(dSq + eSq);
+  float f;
+  if constexpr (LAW == kLawReference) {
+    const float y = __uint_as_float(0x5f3759dfu - (__float_as_uint(dSq) >> 1));
+    const float inv2 = y * fmaf(-(dSq * y), y, 3.0f);
+    const float w2 = __builtin_amdgcn_rcpf(fmaf(dSq, 2.0f, 2.0f * eSq));
+    f = mass * inv2 * w2;
+  } else {
+    f = mass * __builtin_amdgcn_rsqf(dSq) * __builtin_amdgcn_rcpf(dSq + eSq);
+  }
   return CAP ? capMax(f) : f;
 }
 

@@ -568,7 +568,7 @@ __device__ __forceinline__ void mlpEnqueue(const LearnedArgs &la, int &qn,
 // only. The opening test is rounded exactly as the reference rounds it
 // (no contraction): the decisions, and so the interaction set, are the
 // reference's.
-template <int LAW, bool COUNT, bool CAP, bool MASS>
+template <int LAW, bool COUNT, bool CAP, bool MASS, bool POW2 = false>
 __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
                                      const float4 &sm, uint64_t mask, float x,
                                      float y,
@@ -580,11 +580,22 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
                                      const float4 &geo = float4{},
                                      const LearnedArgs &la = LearnedArgs{},
                                      int *qn = nullptr,
-                                     uint32_t *ws = nullptr) {
+                                     uint32_t *ws = nullptr,
+                                     int tShift = 0) {
   const float dx = f.x - x;
   const float dy = f.y - y;
   const float dSq = dx * dx + dy * dy;
-  uint64_t near = mask & ~__builtin_amdgcn_ballot_w64(f.w < dSq * tSq);
+  // The reference's test size^2 < d^2 theta^2. POW2 (theta^2 a power of
+  // two, node sizes in the normal range): size^2 / theta^2 < d^2, the same
+  // decision, both sides scaled exactly -- the scaling an exponent add on
+  // the scalar unit (tShift), in place of a vector multiply per lane.
+  bool farTest;
+  if constexpr (POW2) {
+    farTest = __int_as_float(__float_as_int(f.w) + tShift) < dSq;
+  } else {
+    farTest = f.w < dSq * tSq;
+  }
+  uint64_t near = mask & ~__builtin_amdgcn_ballot_w64(farTest);
   // f.z > 0 on the scalar unit: positive floats, +inf included, are exactly
   // the bit patterns in (0, 0x7f800000]. MASS = false (every body mass
   // positive and finite) skips the test: a non-empty node's mass is then in
@@ -720,7 +731,8 @@ __device__ inline uint64_t visitNode(const float4 &f, const int4 &c,
 
 // One wave's walk: 64 consecutive bodies from `base`, with stack type STACK
 // (RegStack for trees of at most kRegStackLevels levels, else LaneStack).
-template <int LAW, bool COUNT, bool CAP, bool MASS, class STACK>
+template <int LAW, bool COUNT, bool CAP, bool MASS, class STACK,
+          bool POW2 = false>
 __device__ inline void walkWave(uint32_t n, uint32_t base,
                                 const float4 *__restrict__ nodeF,
                                 const int4 *__restrict__ nodeI,
@@ -729,7 +741,8 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
                                 float eSq, float *__restrict__ ax,
                                 float *__restrict__ ay, Ctl *__restrict__ ctl,
                                 STACK &st, float2 *__restrict__ slot,
-                                const LearnedArgs &la = LearnedArgs{}) {
+                                const LearnedArgs &la = LearnedArgs{},
+                                int tShift = 0) {
   constexpr bool kSpill = sizeof(STACK) > sizeof(RegStack);
   constexpr bool kMoments = LAW == kLawQuadrupole || LAW == kLawLearned;
   int qn = 0; // learned law: queued pairs
@@ -759,9 +772,9 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
       g0 = make_float4(la.centerX[0], la.centerY[0], la.size[0], 0.0f);
     }
     const uint64_t open =
-        visitNode<LAW, COUNT, CAP, MASS>(nodeF[0], nodeI[0], q0, all, x, y, tSq, eSq,
+        visitNode<LAW, COUNT, CAP, MASS, POW2>(nodeF[0], nodeI[0], q0, all, x, y, tSq, eSq,
                                    body, slot, qx, qy, nFar, nPair, nLeafTerm,
-                                   g0, la, &qn, COUNT ? ws : nullptr);
+                                   g0, la, &qn, COUNT ? ws : nullptr, tShift);
     nVisit += 1;
     if (open) {
       st.put(0, nodeI[0].x, open);
@@ -813,10 +826,10 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
     uint64_t open[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      open[q] = visitNode<LAW, COUNT, CAP, MASS>(bf[q], bi[q], bq[q], mask, x, y, tSq,
+      open[q] = visitNode<LAW, COUNT, CAP, MASS, POW2>(bf[q], bi[q], bq[q], mask, x, y, tSq,
                                            eSq, body, slot, qx, qy, nFar, nPair,
                                            nLeafTerm, bg[q], la, &qn,
-                                           COUNT ? ws : nullptr);
+                                           COUNT ? ws : nullptr, tShift);
     }
     if (COUNT) {
       nVisit += 4;
@@ -887,13 +900,17 @@ __device__ inline void walkWave(uint32_t n, uint32_t base,
 
 // BLOCK: lanes per workgroup (64, 128 or 256), the autotuner's choice
 // (nbx_tune_walk); every size gives the same results.
+// tShift >= 0: theta^2 = 2^-(tShift >> 23) (a power of two), and the
+// reference-law force walk compares size^2 / theta^2 with d^2 when the root
+// size puts every node's size^2 and its scaled value in the normal range
+// (root size in [2^-31, 2^60]: depth-32 sizes squared >= 2^-126).
 template <int LAW, bool COUNT, bool CAP, bool MASS, int BLOCK = kWalkBlock>
 __global__ __launch_bounds__(BLOCK) void k_walk_fast(
     uint32_t n, const float4 *__restrict__ nodeF,
     const int4 *__restrict__ nodeI, const float4 *__restrict__ nodeQ,
     const float4 *__restrict__ body,
     float tSq, float eSq, float *__restrict__ ax, float *__restrict__ ay,
-    Ctl *__restrict__ ctl) {
+    Ctl *__restrict__ ctl, int tShift = -1) {
   if (barrierStalled(ctl)) {
     return;
   }
@@ -909,6 +926,15 @@ __global__ __launch_bounds__(BLOCK) void k_walk_fast(
   }
   if (ctl->levels <= kRegStackLevels) {
     RegStack st;
+    if constexpr (LAW == kLawReference && !COUNT) {
+      const float rootSize = ctl->quad[2];
+      if (tShift >= 0 && rootSize >= 0x1p-31f && rootSize <= 0x1p60f) {
+        walkWave<LAW, COUNT, CAP, MASS, RegStack, true>(
+            n, base, nodeF, nodeI, nodeQ, body, tSq, eSq, ax, ay, ctl, st,
+            slots[w], LearnedArgs{}, tShift);
+        return;
+      }
+    }
     walkWave<LAW, COUNT, CAP, MASS>(n, base, nodeF, nodeI, nodeQ, body, tSq, eSq, ax, ay,
                               ctl, st, slots[w]);
   } else {

@@ -609,10 +609,20 @@ int launchForces(nbx_sim *s, bool count) {
   // The node-mass test is needed only when some body mass is not positive
   // and finite (and always by the counting walk, which counts far nodes of
   // positive mass).
+  // theta^2 a power of two: its exponent shift for the scaled opening test
+  // of the reference-law walk (-1: not a power of two).
+  int tShift = -1;
+  {
+    int e = 0;
+    const float mnt = std::frexp(tSq, &e);
+    if (tSq > 0.0f && mnt == 0.5f && e - 1 <= 0 && e - 1 >= -60) {
+      tShift = (1 - e) << 23;
+    }
+  }
 #define FAST_B(LAWV, CNT, CAPV, MASSV, BL)                                    \
   LAUNCH((k_walk_fast<LAWV, CNT, CAPV, MASSV, BL>), blocksFor(n, BL), BL, n,  \
          s->nodeF, s->nodeI, s->nodeQ, s->bodyPack, tSq, eSq, s->ax, s->ay,   \
-         s->ctl)
+         s->ctl, tShift)
   // Workgroup size: the handle's (nbx_set_walk_block, nbx_tune_walk) for
   // the force walk; the counting walk keeps the default.
 #define FAST_MB(LAWV, CNT, CAPV, MASSV)                                       \

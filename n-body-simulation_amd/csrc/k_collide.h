@@ -195,9 +195,8 @@ __device__ inline void resetCollideCounters(Ctl *__restrict__ ctl) {
   ctl->compCount = 0;
   ctl->compPairs = 0;
   ctl->bigCount = 0;
-  for (int t = 0; t < kWaveTiers; ++t) {
+  for (int t = 0; t < kTierQueues * kWaveTiers; ++t) {
     ctl->waveCount[t] = 0;
-    ctl->waveNext[t] = 0;
   }
   ctl->largest = 0;
   ctl->status &= ~kStatusPairOverflow;
@@ -914,20 +913,30 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
     info = compInfo[j];
   }
   const int start = info.y, size = info.z;
-  // Queue class: 0..kWaveTiers-1 a wave tier, kWaveTiers the workgroup
-  // path, -1 none. One atomic per class and wave.
+  // Queue class: kTierQueues t + k for a component of wave tier t in the
+  // tier's k-th size class counted from the largest (the tier's pair range
+  // (lo, P] cut in kTierQueues equal parts), kTierQueues kWaveTiers the
+  // workgroup path, -1 none.
   int cls = -1;
   if (size > kSmallComp) {
-    cls = 0;
-    while (cls < kWaveTiers && size > waveTierPairs(cls)) {
-      ++cls;
+    int t = 0;
+    while (t < kWaveTiers && size > waveTierPairs(t)) {
+      ++t;
+    }
+    if (t < kWaveTiers) {
+      const int lo = t == 0 ? kSmallComp : waveTierPairs(t - 1);
+      const int hi = waveTierPairs(t);
+      const int k = (size - lo - 1) * kTierQueues / (hi - lo);
+      cls = kTierQueues * t + (kTierQueues - 1 - k);
+    } else {
+      cls = kTierQueues * kWaveTiers;
     }
   }
   // Queue slots: the workgroup's count per class in LDS, then one atomic
   // per class and workgroup, the classes' atomics issued together (a wave
   // waiting on each class's contended counter in turn cost more than the
   // replays).
-  constexpr int kClasses = kWaveTiers + 1;
+  constexpr int kClasses = kTierQueues * kWaveTiers + 1;
   constexpr int kWaves = kBlock / kWave;
   __shared__ uint32_t waveCnt[kWaves][kClasses];
   __shared__ uint32_t waveBase[kWaves][kClasses];
@@ -954,7 +963,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
     }
     uint32_t base = 0;
     if (total > 0) {
-      base = atomicAdd(c < kWaveTiers ? &ctl->waveCount[c] : &ctl->bigCount, total);
+      base = atomicAdd(c < kTierQueues * kWaveTiers ? &ctl->waveCount[c] : &ctl->bigCount, total);
     }
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
@@ -966,7 +975,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
   if (cls >= 0) {
     const uint32_t at = waveBase[wave][cls] +
                         static_cast<uint32_t>(__popcll(mine & laneMaskLt()));
-    if (cls < kWaveTiers) {
+    if (cls < kTierQueues * kWaveTiers) {
       tierList[static_cast<size_t>(cls) * listCap + at] =
           make_int4(start, size, static_cast<int>(j), 0);
     } else {
@@ -1241,8 +1250,10 @@ __device__ __forceinline__ void resolveWaveComponent(
   if (nb > BMAX) {
     if (lane == 0) {
       if (tier + 1 < kWaveTiers) {
-        tierList[static_cast<size_t>(tier + 1) * listCap +
-                 atomicAdd(&ctl->waveCount[tier + 1], 1u)] = info;
+        // At most a quarter of the next tier's pairs: its smallest class.
+        const int q = kTierQueues * (tier + 1) + kTierQueues - 1;
+        tierList[static_cast<size_t>(q) * listCap +
+                 atomicAdd(&ctl->waveCount[q], 1u)] = info;
       } else {
         toBig();
       }
@@ -1444,6 +1455,34 @@ __device__ __forceinline__ void resolveWaveComponent(
   waveSync();
 }
 
+// A tier's queues read as one list: entry g of the concatenation (largest
+// size class first) at its place in tierList, relative to the tier's first
+// queue (queue k starts k listCap further).
+struct TierQueues {
+  uint32_t end[kTierQueues]; // running totals
+  uint32_t count;
+  __device__ TierQueues(const Ctl *__restrict__ ctl, int tier) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < kTierQueues; ++k) {
+      run += ctl->waveCount[kTierQueues * tier + k];
+      end[k] = run;
+    }
+    count = run;
+  }
+  __device__ inline uint32_t slot(uint32_t g, uint32_t listCap) const {
+    uint32_t k = 0, before = 0;
+#pragma unroll
+    for (int q = 0; q + 1 < kTierQueues; ++q) {
+      if (g >= end[q]) {
+        k = q + 1;
+        before = end[q];
+      }
+    }
+    return k * listCap + (g - before);
+  }
+};
+
 // Components are taken in a fixed stride (no shared counter), G lanes each
 // (a wave resolves 64 / G components side by side); the next component's
 // queue entry, and for the small tiers its pairs, are loaded while the
@@ -1461,7 +1500,9 @@ __global__ __launch_bounds__(kWave) void k_resolve_wave(
   if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
     return;
   }
-  const uint32_t count = ctl->waveCount[tier];
+  // The tier's queues as one list, the larger components first.
+  TierQueues tq(ctl, tier);
+  const uint32_t count = tq.count;
   if (blockIdx.x * kGroups >= count) {
     return;
   }
@@ -1470,13 +1511,13 @@ __global__ __launch_bounds__(kWave) void k_resolve_wave(
   const int lane = static_cast<int>(threadIdx.x) % G;
   const uint32_t stride = gridDim.x * kGroups;
   const uint32_t first = blockIdx.x * kGroups + static_cast<uint32_t>(group);
-  const int4 *list = tierList + static_cast<size_t>(tier) * listCap;
+  const int4 *list = tierList + static_cast<size_t>(kTierQueues * tier) * listCap;
   constexpr int kP = W::kPf > 0 ? W::kPf : 1;
   const int4 none = make_int4(0, 0, -1, 0);
   // Loads from a clamped index and selects values (a select of pointers
   // would turn the load into a flat one through a private copy of `none`).
   auto entry = [&](uint32_t g) {
-    const int4 e = list[min(g, count - 1)];
+    const int4 e = list[tq.slot(min(g, count - 1), listCap)];
     return g < count ? e : none;
   };
   int4 info0 = entry(first);
@@ -1746,8 +1787,9 @@ __global__ __launch_bounds__(kBlock) void k_resolve_large(
   using W = WaveCtx<4096, 2560, kWave>;
   static_assert(W::kPf == 0, "tier 3 loads its pairs itself");
   if (!(ctl->status & kStatusPairOverflow)) {
-    const uint32_t count = ctl->waveCount[kTier];
-    const int4 *list = tierList + static_cast<size_t>(kTier) * listCap;
+    TierQueues tq(ctl, kTier);
+    const uint32_t count = tq.count;
+    const int4 *list = tierList + static_cast<size_t>(kTierQueues * kTier) * listCap;
     for (uint32_t g = blockIdx.x; g < count; g += gridDim.x) {
       if (threadIdx.x == 0) {
         local = -1;
@@ -1756,7 +1798,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve_large(
       if (threadIdx.x < kWave) {
         typename W::Pairs none{};
         resolveWaveComponent<4096, 2560, kWave>(
-            lds, ctl, kTier, list[g], none, tierList, listCap, bigList,
+            lds, ctl, kTier, list[tq.slot(g, listCap)], none, tierList,
+            listCap, bigList,
             segPairs, px, py, vx, vy, m, r, &local);
       }
       __syncthreads();

@@ -23,6 +23,7 @@ constexpr int kBigCompMax = 4096;  // pairs per component sorted in LDS at once
 // Contact components of kSmallComp+1 .. 4096 pairs are replayed by one wave
 // each, in tiers by pair count (k_resolve_wave).
 constexpr int kWaveTiers = 4;
+constexpr int kTierQueues = 4; // size classes (queues) per wave tier
 __host__ __device__ constexpr int waveTierPairs(int tier) {
   return tier == 0 ? 64 : tier == 1 ? 256 : tier == 2 ? 1024 : 4096;
 }
@@ -62,8 +63,10 @@ struct Ctl {
   int32_t totalSplits, nodeCount, levels;
   int32_t subtreeRoots;
   uint32_t status;
-  uint32_t waveCount[kWaveTiers]; // components queued per k_resolve_wave tier
-  uint32_t waveNext[kWaveTiers];  // next queue entry per tier
+  // Components queued per wave tier, in kTierQueues queues a tier by size
+  // (queue kTierQueues t + k holds the k-th largest size class), so that a
+  // tier's kernel starts its largest components first.
+  uint32_t waveCount[kTierQueues * kWaveTiers];
   uint32_t prevPairs; // contact pairs the previous collide found
   uint32_t levelDone; // k_level_scan blocks finished (0 at rest)
   unsigned long long farCount, bodyPairCount;

@@ -138,7 +138,7 @@ struct nbx_sim {
   uint32_t *compCount = nullptr, *compStart = nullptr;
   int4 *compInfo = nullptr;
   int32_t *bigList = nullptr;
-  int4 *tierList = nullptr; // kWaveTiers queues of tierCap {start, size, component}
+  int4 *tierList = nullptr; // kTierQueues x kWaveTiers queues of tierCap {start, size, component}
   uint32_t tierCap = 0;
   int32_t *bodySlot = nullptr; // per body: slot in a big component's replay, -1 at rest
 
@@ -995,7 +995,7 @@ int nbx_create_ex(uint64_t capacity, uint32_t flags, nbx_sim **out) {
   A(s->bigList, n, false)
   // A queued component has more than kSmallComp pairs.
   s->tierCap = s->pairCap / (kSmallComp + 1) + 1;
-  A(s->tierList, static_cast<uint64_t>(kWaveTiers) * s->tierCap, false)
+  A(s->tierList, static_cast<uint64_t>(kTierQueues * kWaveTiers) * s->tierCap, false)
   if ((rc = devAlloc(s, &s->bodySlot, n, false)) != NBX_OK ||
       hipMemset(s->bodySlot, 0xff, n * sizeof(int32_t)) != hipSuccess) {
     return fail(rc != NBX_OK ? rc : setError(NBX_ERR_HIP, "hipMemset failed"));

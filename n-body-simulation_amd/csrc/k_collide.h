@@ -985,13 +985,14 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
   if (!live || size > kSmallComp) {
     return;
   }
-  static_assert(kSmallComp == 16, "size classes below cover 16 pairs");
+  static_assert(kSmallComp == 4 || kSmallComp == 8 || kSmallComp == 16,
+                "size classes below cover the one-lane path");
   if (size == 1) {
     const int4 q = segPairs[start];
     resolvePair(px, py, vx, vy, m, r, q.x, q.y);
-  } else if (size <= 4) {
+  } else if (kSmallComp == 4 || size <= 4) {
     resolveComponent<4>(start, size, segPairs, px, py, vx, vy, m, r);
-  } else if (size <= 8) {
+  } else if (kSmallComp == 8 || size <= 8) {
     resolveComponent<8>(start, size, segPairs, px, py, vx, vy, m, r);
   } else {
     resolveComponent<16>(start, size, segPairs, px, py, vx, vy, m, r);

@@ -18,7 +18,7 @@ constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kLeafCapacity = 16;  // Simulation.cpp:9 leafCapacity
 constexpr int kMaxDepth = 32;      // key depth: 2 words x 16 levels
 constexpr int kDeepMax = 4096;     // bodies per equal-hi-key run sortable in LDS
-constexpr int kSmallComp = 16;     // contact components replayed by one lane
+constexpr int kSmallComp = 8;      // contact components replayed by one lane
 constexpr int kBigCompMax = 4096;  // pairs per component sorted in LDS at once
 // Contact components of kSmallComp+1 .. 4096 pairs are replayed by one wave
 // each, in tiers by pair count (k_resolve_wave).

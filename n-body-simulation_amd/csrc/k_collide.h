@@ -89,28 +89,31 @@ __device__ inline bool pairLess(const int4 &p, const int4 &q) {
   return a < b || (a == b && p.y < q.y);
 }
 
-// Collision.cpp:16-84 resolve(bodies, i, j), on the SoA arrays.
-__device__ inline void resolvePair(float *__restrict__ px,
-                                   float *__restrict__ py,
-                                   float *__restrict__ vx,
-                                   float *__restrict__ vy,
-                                   const float *__restrict__ m,
-                                   const float *__restrict__ r, int i, int j) {
-  // Every input is loaded up front (one memory round trip, not two); the
-  // arithmetic is the reference's, in its order.
-  float p1x = px[i], p1y = py[i];
-  float p2x = px[j], p2y = py[j];
-  const float r1 = r[i], r2 = r[j];
-  const float v1x = vx[i], v1y = vy[i];
-  const float v2x = vx[j], v2y = vy[j];
-  const float m1 = m[i], m2 = m[j];
-  float dx = p2x - p1x, dy = p2y - p1y;
+// The two bodies of a pair as resolve() reads and writes them.
+struct PairBodies {
+  float p1x, p1y, v1x, v1y, p2x, p2y, v2x, v2y;
+};
+
+// Collision.cpp:16-84 resolve() on one pair's values, in place; false (and
+// nothing changed) when the bodies do not overlap. The reference's three
+// contact cases -- moving apart (positions pushed to contact along d),
+// nearly equal velocities (placed rr apart along d, or along x when d = 0),
+// approaching (moved back to the contact time, velocities reflected) --
+// each take one square root and one division first; here they share one
+// of each, on operands selected per case, so lanes of a wave in different
+// cases do not run three branches one after the other. Every value is the
+// reference's operation on the reference's operands.
+__device__ inline bool resolveCore(PairBodies &s, float m1, float m2, float r1,
+                                   float r2) {
+  const float dx = s.p2x - s.p1x, dy = s.p2y - s.p1y;
   const float rr = r1 + r2;
-  if (dx * dx + dy * dy > rr * rr) {
-    return;
+  const float dSq = dx * dx + dy * dy;
+  const float rSq = rr * rr;
+  if (dSq > rSq) {
+    return false;
   }
-  const float wx = v2x - v1x, wy = v2y - v1y;
-  float dDotV = dx * wx + dy * wy;
+  const float wx = s.v2x - s.v1x, wy = s.v2y - s.v1y;
+  const float dDotV = dx * wx + dy * wy;
   // m2 / (m1 + m2) and m1 / (m1 + m2); for equal positive masses whose sum
   // is finite both quotients are exactly 0.5 (the sum is 2m exactly), so the
   // two divisions are skipped -- the common case, and a uniform branch when
@@ -121,56 +124,66 @@ __device__ inline void resolvePair(float *__restrict__ px,
     weight2 = m1 / (m1 + m2);
   }
   const bool dNonZero = !(dx == 0.0f && dy == 0.0f);
-  if (dDotV >= 0.0f && dNonZero) {
-    const float mag = sqrtf(dx * dx + dy * dy);
-    const float s = rr / mag - 1.0f;
-    const float tx = dx * s, ty = dy * s;
-    px[i] = p1x - weight1 * tx;
-    py[i] = p1y - weight1 * ty;
-    px[j] = p2x + weight2 * tx;
-    py[j] = p2y + weight2 * ty;
-    return;
-  }
-  const float vSqEarly = wx * wx + wy * wy;
-  if (vSqEarly < 1e-12f) {
-    float dirx = 1.0f, diry = 0.0f;
-    if (dNonZero) {
-      const float inv = 1.0f / sqrtf(dx * dx + dy * dy);
-      dirx = dx * inv;
-      diry = dy * inv;
-    }
-    const float tx = dirx * rr, ty = diry * rr;
-    px[i] = p1x - weight1 * tx;
-    py[i] = p1y - weight1 * ty;
-    px[j] = p2x + weight2 * tx;
-    py[j] = p2y + weight2 * ty;
-    return;
-  }
+  const bool apart = dDotV >= 0.0f && dNonZero;
   const float vSq = wx * wx + wy * wy;
-  float dSq = dx * dx + dy * dy;
-  const float rSq = rr * rr;
+  const bool approach = !apart && !(vSq < 1e-12f);
+  // apart: rr / |d|; nearly equal velocities: 1 / |d|; approaching:
+  // (d.w + sqrt(max(disc, 0))) / |w|^2, the time back to contact.
   const float disc = dDotV * dDotV - vSq * (dSq - rSq);
-  const float t = (dDotV + sqrtf(disc < 0.0f ? 0.0f : disc)) / vSq;
-  p1x = p1x - v1x * t;
-  p1y = p1y - v1y * t;
-  p2x = p2x - v2x * t;
-  p2y = p2y - v2y * t;
-  dx = p2x - p1x;
-  dy = p2y - p1y;
-  dDotV = dx * wx + dy * wy;
-  dSq = dx * dx + dy * dy;
-  const float s = 1.5f * dDotV / dSq;
-  const float tx = dx * s, ty = dy * s;
-  const float n1x = v1x + tx * weight1, n1y = v1y + ty * weight1;
-  const float n2x = v2x - tx * weight2, n2y = v2y - ty * weight2;
-  vx[i] = n1x;
-  vy[i] = n1y;
-  vx[j] = n2x;
-  vy[j] = n2y;
-  px[i] = p1x + n1x * t;
-  py[i] = p1y + n1y * t;
-  px[j] = p2x + n2x * t;
-  py[j] = p2y + n2y * t;
+  const float root = sqrtf(approach ? (disc < 0.0f ? 0.0f : disc) : dSq);
+  const float q = (apart ? rr : (approach ? dDotV + root : 1.0f)) / (approach ? vSq : root);
+  if (!approach) {
+    const float sc = q - 1.0f;
+    const float dirx = dNonZero ? dx * q : 1.0f, diry = dNonZero ? dy * q : 0.0f;
+    const float tx = apart ? dx * sc : dirx * rr;
+    const float ty = apart ? dy * sc : diry * rr;
+    s.p1x = s.p1x - weight1 * tx;
+    s.p1y = s.p1y - weight1 * ty;
+    s.p2x = s.p2x + weight2 * tx;
+    s.p2y = s.p2y + weight2 * ty;
+    return true;
+  }
+  const float t = q;
+  const float a1x = s.p1x - s.v1x * t, a1y = s.p1y - s.v1y * t;
+  const float a2x = s.p2x - s.v2x * t, a2y = s.p2y - s.v2y * t;
+  const float ex = a2x - a1x, ey = a2y - a1y;
+  const float eDotV = ex * wx + ey * wy;
+  const float eSq = ex * ex + ey * ey;
+  const float sc = 1.5f * eDotV / eSq;
+  const float tx = ex * sc, ty = ey * sc;
+  const float n1x = s.v1x + tx * weight1, n1y = s.v1y + ty * weight1;
+  const float n2x = s.v2x - tx * weight2, n2y = s.v2y - ty * weight2;
+  s.v1x = n1x;
+  s.v1y = n1y;
+  s.v2x = n2x;
+  s.v2y = n2y;
+  s.p1x = a1x + n1x * t;
+  s.p1y = a1y + n1y * t;
+  s.p2x = a2x + n2x * t;
+  s.p2y = a2y + n2y * t;
+  return true;
+}
+
+// Collision.cpp:16-84 resolve(bodies, i, j), on the SoA arrays: every input
+// loaded up front (one memory round trip, not two), resolveCore, and the
+// pair's eight values written back when the bodies overlapped.
+__device__ inline void resolvePair(float *__restrict__ px,
+                                   float *__restrict__ py,
+                                   float *__restrict__ vx,
+                                   float *__restrict__ vy,
+                                   const float *__restrict__ m,
+                                   const float *__restrict__ r, int i, int j) {
+  PairBodies s{px[i], py[i], vx[i], vy[i], px[j], py[j], vx[j], vy[j]};
+  if (resolveCore(s, m[i], m[j], r[i], r[j])) {
+    px[i] = s.p1x;
+    py[i] = s.p1y;
+    vx[i] = s.v1x;
+    vy[i] = s.v1y;
+    px[j] = s.p2x;
+    py[j] = s.p2y;
+    vx[j] = s.v2x;
+    vy[j] = s.v2y;
+  }
 }
 
 // Grid cell size. The reference takes 1.01 x the median box width
@@ -1025,16 +1038,20 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
 // (resolveBigComponent, in k_resolve_large).
 constexpr int kMaxDegree = 16;
 constexpr uint32_t kEmptyKey = 0xffffffffu;
-constexpr uint16_t kNoPair = 0xffffu;
 
 template <int PMAX, int BMAX> struct WaveLds {
   static_assert(PMAX < 65535 && BMAX <= 32768 && BMAX % 2 == 0, "16-bit fields");
   static_assert(2 * BMAX >= PMAX + 2, "the body table always has a free entry");
+  static_assert(PMAX >= BMAX && (PMAX & (PMAX - 1)) == 0,
+                "the ready ring (P / 2 records) holds B / 2, a power of two");
   static constexpr int kHash = 2 * BMAX;            // table entries
-  static constexpr int kPairSlot = 0;               // u32[P]: slot A | slot B << 16
-  static constexpr int kSucc = PMAX;                // u16[2P]: next pair on A, on B
+  static constexpr int kInfo = 0;                   // u64[P] pair records: slot A |
+                                                    // slot B << 16 | next pair on A
+                                                    // << 32 | next pair on B << 48
   static constexpr int kIndeg = 2 * PMAX;           // u32[P]: earlier pairs not done
-  static constexpr int kInc = 3 * PMAX;             // u16[2P]: lists, then frontiers
+  static constexpr int kInc = 3 * PMAX;             // u16[2P]: lists, then the ready
+                                                    // ring (u64[P / 2] records)
+  static constexpr uint32_t kRing = PMAX / 2;
   static constexpr int kBodyIdx = 4 * PMAX;         // i32[B]
   static constexpr int kMinX = 4 * PMAX + BMAX;     // u32[B]: bits of the box minX
   static constexpr int kU = 4 * PMAX + 2 * BMAX;    // 6B words, in turn: table
@@ -1096,16 +1113,18 @@ __device__ inline void sortIncident(uint64_t (&key)[N], uint32_t (&pr)[N]) {
 }
 
 // Sorts body s's list (d <= N entries at inc[b0..]) into pairLess order and
-// links each pair to the next one on s; every pair after the first gets one
-// more earlier pair to wait for.
+// links each pair to the next one on s -- the last one to itself -- and
+// every pair after the first gets one more earlier pair to wait for.
 template <int N>
 __device__ inline void linkBody(int s, uint32_t b0, uint32_t d,
                                 const uint16_t *__restrict__ inc,
-                                const uint32_t *__restrict__ pairSlot,
+                                uint32_t *__restrict__ info,
                                 const uint32_t *__restrict__ minX,
                                 const int32_t *__restrict__ bodyIdx,
-                                uint16_t *__restrict__ succ,
                                 uint32_t *__restrict__ indeg) {
+  // info[2k]: the pair's slots; 16-bit halves 4k + 2 and 4k + 3: its next
+  // pair on its first and on its second body.
+  uint16_t *succ = reinterpret_cast<uint16_t *>(info);
   uint64_t key[N];
   uint32_t pr[N];
 #pragma unroll
@@ -1114,7 +1133,7 @@ __device__ inline void linkBody(int s, uint32_t b0, uint32_t d,
     pr[i] = 0;
     if (static_cast<uint32_t>(i) < d) {
       const uint32_t k = inc[b0 + i];
-      const uint32_t sl = pairSlot[k];
+      const uint32_t sl = info[2 * k];
       const uint32_t a = sl & 0xffffu, b = sl >> 16;
       const uint32_t o = a == static_cast<uint32_t>(s) ? b : a;
       key[i] = beforeKey(minX[o], bodyIdx[o]);
@@ -1126,10 +1145,9 @@ __device__ inline void linkBody(int s, uint32_t b0, uint32_t d,
   for (int i = 0; i < N; ++i) {
     if (static_cast<uint32_t>(i) < d) {
       const uint32_t k = pr[i] & 0xffffu, side = pr[i] >> 16;
-      const uint16_t next =
-          static_cast<uint32_t>(i + 1) < d ? static_cast<uint16_t>(pr[i + 1 < N ? i + 1 : i] & 0xffffu)
-                                           : kNoPair;
-      succ[2 * k + side] = next;
+      const uint16_t next = static_cast<uint16_t>(
+          static_cast<uint32_t>(i + 1) < d ? pr[i + 1 < N ? i + 1 : i] & 0xffffu : k);
+      succ[4 * k + 2 + side] = next;
       if (i > 0) {
         atomicAdd(&indeg[k], 1u);
       }
@@ -1180,8 +1198,8 @@ __device__ __forceinline__ void resolveWaveComponent(
       bigList[atomicAdd(&ctl->bigCount, 1u)] = info.z;
     }
   };
-  uint32_t *pairSlot = lds + L::kPairSlot;
-  uint16_t *succ = reinterpret_cast<uint16_t *>(lds + L::kSucc);
+  uint32_t *slots = lds + L::kInfo; // the slots of pair k at slots[2k]
+  const uint64_t *rec = reinterpret_cast<const uint64_t *>(lds + L::kInfo);
   uint32_t *indeg = lds + L::kIndeg;
   uint16_t *inc = reinterpret_cast<uint16_t *>(lds + L::kInc);
   int32_t *bodyIdx = reinterpret_cast<int32_t *>(lds + L::kBodyIdx);
@@ -1202,7 +1220,8 @@ __device__ __forceinline__ void resolveWaveComponent(
   const int size = info.y; // kSmallComp < size <= PMAX
   int4 *seg = segPairs + info.x;
 
-  // 1. A dense slot per body; pairSlot holds table positions meanwhile.
+  // 1. A dense slot per body; the records' slot words hold table
+  //    positions meanwhile.
   for (int t = lane; t < L::kHash; t += G) {
     hkey[t] = kEmptyKey;
   }
@@ -1214,7 +1233,7 @@ __device__ __forceinline__ void resolveWaveComponent(
     if (k < size) {
       w0 = hashInsert(hkey, L::kHash, static_cast<uint32_t>(q.x), h0);
       w1 = hashInsert(hkey, L::kHash, static_cast<uint32_t>(q.y), h1);
-      pairSlot[k] = h0 | (h1 << 16);
+      slots[2 * k] = h0 | (h1 << 16);
     }
     const uint64_t b0 = gballot(w0), b1 = gballot(w1);
     const uint32_t n0 = static_cast<uint32_t>(__popcll(b0));
@@ -1279,8 +1298,8 @@ __device__ __forceinline__ void resolveWaveComponent(
     }
   }
   for (int k = lane; k < size; k += G) {
-    const uint32_t h = pairSlot[k];
-    pairSlot[k] = static_cast<uint32_t>(hslot[h & 0xffffu]) |
+    const uint32_t h = slots[2 * k];
+    slots[2 * k] = static_cast<uint32_t>(hslot[h & 0xffffu]) |
                   (static_cast<uint32_t>(hslot[h >> 16]) << 16);
   }
   waveSync(); // the table is dead from here on
@@ -1294,7 +1313,7 @@ __device__ __forceinline__ void resolveWaveComponent(
   }
   waveSync();
   for (int k = lane; k < size; k += G) {
-    const uint32_t sl = pairSlot[k];
+    const uint32_t sl = slots[2 * k];
     atomicAdd(&start[sl & 0xffffu], 1u);
     atomicAdd(&start[sl >> 16], 1u);
   }
@@ -1334,7 +1353,7 @@ __device__ __forceinline__ void resolveWaveComponent(
   }
   waveSync();
   for (int k = lane; k < size; k += G) {
-    const uint32_t sl = pairSlot[k];
+    const uint32_t sl = slots[2 * k];
     inc[atomicAdd(&cursor[sl & 0xffffu], 1u)] = static_cast<uint16_t>(k);
     inc[atomicAdd(&cursor[sl >> 16], 1u)] = static_cast<uint16_t>(k);
   }
@@ -1344,11 +1363,11 @@ __device__ __forceinline__ void resolveWaveComponent(
   for (int t = lane; t < static_cast<int>(nb); t += G) {
     const uint32_t b0 = start[t], d = start[t + 1] - b0;
     if (d <= 4) {
-      linkBody<4>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+      linkBody<4>(t, b0, d, inc, slots, minX, bodyIdx, indeg);
     } else if (d <= 8) {
-      linkBody<8>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+      linkBody<8>(t, b0, d, inc, slots, minX, bodyIdx, indeg);
     } else {
-      linkBody<kMaxDegree>(t, b0, d, inc, pairSlot, minX, bodyIdx, succ, indeg);
+      linkBody<kMaxDegree>(t, b0, d, inc, slots, minX, bodyIdx, indeg);
     }
   }
   waveSync(); // the lists are dead from here on
@@ -1378,70 +1397,98 @@ __device__ __forceinline__ void resolveWaveComponent(
       lr[t] = r[i];
     }
   }
-  uint16_t *front = inc;
-  uint32_t cnt = 0;
+  // The ready pairs' records go to a ring (at most B / 2 pairs are ready
+  // at once: they share no body), the first round's to its start.
+  uint64_t *ring = reinterpret_cast<uint64_t *>(inc);
+  constexpr uint32_t kRingMask = L::kRing - 1;
+  uint32_t head = 0, tail = 0;
   for (int k0 = 0; k0 < size; k0 += G) {
     const int k = k0 + lane;
     const bool ready = k < size && indeg[k] == 0u;
     const uint64_t bm = gballot(ready);
     if (ready) {
-      front[cnt + static_cast<uint32_t>(__popcll(bm & lt))] = static_cast<uint16_t>(k);
+      ring[(tail + static_cast<uint32_t>(__popcll(bm & lt))) & kRingMask] = rec[k];
     }
-    cnt += static_cast<uint32_t>(__popcll(bm));
+    tail += static_cast<uint32_t>(__popcll(bm));
   }
   waveSync();
 
-  // 5. Rounds. Pairs are written back in the order they are replayed. A
-  //    pair's successors are released before its resolve (they run in the
-  //    next round, after it); the next frontier is placed by ballots.
+  // 5. Rounds: every lane with a ready pair resolves it, decrements its
+  //    successors' counts and reads their records in the same LDS round
+  //    trip as the bodies; a successor whose count reaches zero is the
+  //    lane's next pair (ready: its earlier pairs are all done), a second
+  //    one goes to the ring, and lanes without a pair take one from the
+  //    ring. Pairs are written back in the order they are replayed.
   uint32_t done = 0;
-  int cur = 0;
-  while (cnt > 0) {
-    const uint16_t *f = front + cur * PMAX;
-    uint16_t *nf = front + (cur ^ 1) * PMAX;
-    uint32_t next = 0;
-    for (uint32_t base = 0; base < cnt; base += G) {
-      const uint32_t idx = base + static_cast<uint32_t>(lane);
-      const bool act = idx < cnt;
-      uint32_t nx0 = kNoPair, nx1 = kNoPair, r0 = 0, r1 = 0;
-      if (act) {
-        const uint32_t k = f[idx];
-        const uint32_t sl = pairSlot[k];
-        nx0 = succ[2 * k];
-        nx1 = succ[2 * k + 1];
-        // Both decrements issued back to back, results used after the
-        // resolve (their latency overlaps it); a missing successor counts
-        // down this pair's own entry instead, which is never read again.
-        r0 = atomicSub(&indeg[nx0 != kNoPair ? nx0 : k], 1u);
-        r1 = atomicSub(&indeg[nx1 != kNoPair ? nx1 : k], 1u);
-        const int a = static_cast<int>(sl & 0xffffu), b = static_cast<int>(sl >> 16);
-        resolvePair(lx, ly, lvx, lvy, lm, lr, a, b);
-        if (done + idx < static_cast<uint32_t>(size)) {
-          seg[done + idx] = make_int4(bodyIdx[a], bodyIdx[b], static_cast<int>(minX[a]),
-                                      static_cast<int>(minX[b]));
-        }
+  uint64_t cur = 0;
+  bool has = false;
+  while (true) {
+    if (head != tail) {
+      const uint64_t need = gballot(!has);
+      const uint32_t avail = tail - head;
+      const uint32_t rank = static_cast<uint32_t>(__popcll(need & lt));
+      if (!has && rank < avail) {
+        cur = ring[(head + rank) & kRingMask];
+        has = true;
       }
-      const bool go0 = nx0 != kNoPair && r0 == 1u;
-      const bool go1 = nx1 != kNoPair && r1 == 1u;
-      const uint64_t m0 = gballot(go0), m1 = gballot(go1);
-      const uint32_t c0 = static_cast<uint32_t>(__popcll(m0));
-      if (go0) {
-        const uint32_t pos = next + static_cast<uint32_t>(__popcll(m0 & lt));
-        if (pos < PMAX) {
-          nf[pos] = static_cast<uint16_t>(nx0);
-        }
-      }
-      if (go1) {
-        const uint32_t pos = next + c0 + static_cast<uint32_t>(__popcll(m1 & lt));
-        if (pos < PMAX) {
-          nf[pos] = static_cast<uint16_t>(nx1);
-        }
-      }
-      next += c0 + static_cast<uint32_t>(__popcll(m1));
+      head += min(avail, static_cast<uint32_t>(__popcll(need)));
     }
-    done += cnt;
-    cnt = min(next, static_cast<uint32_t>(PMAX));
-    cur ^= 1;
+    const uint64_t act = gballot(has);
+    if (act == 0) {
+      break;
+    }
+    const uint32_t sl = static_cast<uint32_t>(cur);
+    const uint32_t nx0 = static_cast<uint32_t>(cur >> 32) & 0xffffu;
+    const uint32_t nx1 = static_cast<uint32_t>(cur >> 48);
+    uint32_t r0 = 0, r1 = 0;
+    uint64_t i0 = 0, i1 = 0;
+    if (has) {
+      // A pair's last link on a body points to itself: its own count is
+      // zero (it is ready) and nothing else counts it down any more, so the
+      // decrement returns 0 or ~0, never 1, and releases nothing.
+      r0 = atomicSub(&indeg[nx0], 1u);
+      r1 = atomicSub(&indeg[nx1], 1u);
+      i0 = rec[nx0];
+      i1 = rec[nx1];
+      const int a = static_cast<int>(sl & 0xffffu), b = static_cast<int>(sl >> 16);
+      PairBodies s{lx[a], ly[a], lvx[a], lvy[a], lx[b], ly[b], lvx[b], lvy[b]};
+      const float m1 = lm[a], m2 = lm[b], r1f = lr[a], r2f = lr[b];
+      const int4 out = make_int4(bodyIdx[a], bodyIdx[b], static_cast<int>(minX[a]),
+                                 static_cast<int>(minX[b]));
+      // Every load of the pair and its successors in one LDS round trip:
+      // the operands are consumed here, so none is sunk into a branch.
+      asm volatile("" ::"v"(s.p1x), "v"(s.p1y), "v"(s.v1x), "v"(s.v1y), "v"(s.p2x),
+                   "v"(s.p2y), "v"(s.v2x), "v"(s.v2y), "v"(m1), "v"(m2), "v"(r1f),
+                   "v"(r2f), "v"(out.x), "v"(out.y), "v"(out.z), "v"(out.w), "v"(r0),
+                   "v"(r1), "v"(i0), "v"(i1));
+      if (resolveCore(s, m1, m2, r1f, r2f)) {
+        lx[a] = s.p1x;
+        ly[a] = s.p1y;
+        lvx[a] = s.v1x;
+        lvy[a] = s.v1y;
+        lx[b] = s.p2x;
+        ly[b] = s.p2y;
+        lvx[b] = s.v2x;
+        lvy[b] = s.v2y;
+      }
+      const uint32_t at = done + static_cast<uint32_t>(__popcll(act & lt));
+      if (at < static_cast<uint32_t>(size)) {
+        seg[at] = out;
+      }
+    }
+    done += static_cast<uint32_t>(__popcll(act));
+    const bool go0 = has && r0 == 1u;
+    const bool go1 = has && r1 == 1u;
+    has = go0 || go1;
+    cur = go0 ? i0 : i1;
+    const bool push = go0 && go1;
+    const uint64_t pm = gballot(push);
+    if (pm != 0) {
+      if (push) {
+        ring[(tail + static_cast<uint32_t>(__popcll(pm & lt))) & kRingMask] = i1;
+      }
+      tail += static_cast<uint32_t>(__popcll(pm));
+    }
     waveSync();
   }
 

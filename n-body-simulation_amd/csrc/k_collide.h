@@ -211,6 +211,9 @@ __device__ inline void resetCollideCounters(Ctl *__restrict__ ctl) {
   for (int t = 0; t < kTierQueues * kWaveTiers; ++t) {
     ctl->waveCount[t] = 0;
   }
+  for (int t = 0; t < kWaveTiers; ++t) {
+    ctl->tierNext[t] = 0;
+  }
   ctl->largest = 0;
   ctl->status &= ~kStatusPairOverflow;
 }
@@ -902,7 +905,7 @@ __device__ __forceinline__ void resolveComponent(int start, int size,
 
 // One lane per component of up to kSmallComp pairs: pairLess order, then the
 // sequential replay. Larger components are queued by size for the wave
-// tiers (k_resolve_wave, tier 3 in k_resolve_large), beyond those for the
+// tiers (tiers 0..2 in k_resolve_tiers, tier 3 in k_resolve_large), beyond those for the
 // workgroup path (k_resolve_large).
 __global__ __launch_bounds__(kBlock) void k_resolve_small(
     Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
@@ -1270,8 +1273,9 @@ __device__ __forceinline__ void resolveWaveComponent(
   if (nb > BMAX) {
     if (lane == 0) {
       if (tier + 1 < kWaveTiers) {
-        // At most a quarter of the next tier's pairs: its smallest class.
-        const int q = kTierQueues * (tier + 1) + kTierQueues - 1;
+        // Tier 3's smallest class: k_resolve_large takes its queue after
+        // every smaller tier is done.
+        const int q = kTierQueues * (kWaveTiers - 1) + kTierQueues - 1;
         tierList[static_cast<size_t>(q) * listCap +
                  atomicAdd(&ctl->waveCount[q], 1u)] = info;
       } else {
@@ -1531,36 +1535,45 @@ struct TierQueues {
   }
 };
 
-// Components are taken in a fixed stride (no shared counter), G lanes each
-// (a wave resolves 64 / G components side by side); the next component's
-// queue entry, and for the small tiers its pairs, are loaded while the
-// current one is resolved.
+// One wave's share of a tier: the wave takes 64 / G consecutive queue
+// entries at a time (the larger components come first), G lanes a
+// component -- the first `waves` waves (those resident from the start)
+// take their first batch by their index, every later batch comes from the
+// tier's counter (so that a queue one residency covers takes no atomic at
+// all); the next entries and, for the small tiers, their pairs are loaded
+// while the current ones are resolved.
 template <int PMAX, int BMAX, int G>
-__global__ __launch_bounds__(kWave) void k_resolve_wave(
-    Ctl *__restrict__ ctl, int tier, int4 *__restrict__ tierList,
-    uint32_t listCap, int32_t *__restrict__ bigList,
-    int4 *__restrict__ segPairs, float *__restrict__ px,
-    float *__restrict__ py, float *__restrict__ vx, float *__restrict__ vy,
-    const float *__restrict__ m, const float *__restrict__ r) {
+__device__ __forceinline__ void drainTier(
+    Ctl *__restrict__ ctl, int tier, uint32_t waveId, uint32_t waves,
+    uint32_t *__restrict__ lds,
+    int4 *__restrict__ tierList, uint32_t listCap,
+    int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
+    float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
+    float *__restrict__ vy, const float *__restrict__ m,
+    const float *__restrict__ r) {
   using L = WaveLds<PMAX, BMAX>;
   using W = WaveCtx<PMAX, BMAX, G>;
-  constexpr int kGroups = kWave / G;
-  if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
-    return;
-  }
-  // The tier's queues as one list, the larger components first.
-  TierQueues tq(ctl, tier);
+  constexpr uint32_t kGroups = kWave / G;
+  const TierQueues tq(ctl, tier);
   const uint32_t count = tq.count;
-  if (blockIdx.x * kGroups >= count) {
+  if (count == 0) {
     return;
   }
-  __shared__ __align__(16) uint32_t lds[L::kWords * kGroups];
-  const int group = static_cast<int>(threadIdx.x) / G;
+  const int group = static_cast<int>(threadIdx.x % kWave) / G;
   const int lane = static_cast<int>(threadIdx.x) % G;
-  const uint32_t stride = gridDim.x * kGroups;
-  const uint32_t first = blockIdx.x * kGroups + static_cast<uint32_t>(group);
   const int4 *list = tierList + static_cast<size_t>(kTierQueues * tier) * listCap;
-  constexpr int kP = W::kPf > 0 ? W::kPf : 1;
+  uint32_t *next = &ctl->tierNext[tier];
+  const uint32_t firstRound = waves * kGroups;
+  auto grab = [&]() {
+    if (firstRound >= count) {
+      return count;
+    }
+    uint32_t g = 0;
+    if (threadIdx.x % kWave == 0) {
+      g = atomicAdd(next, kGroups);
+    }
+    return firstRound + static_cast<uint32_t>(__shfl(static_cast<int>(g), 0));
+  };
   const int4 none = make_int4(0, 0, -1, 0);
   // Loads from a clamped index and selects values (a select of pointers
   // would turn the load into a flat one through a private copy of `none`).
@@ -1568,28 +1581,89 @@ __global__ __launch_bounds__(kWave) void k_resolve_wave(
     const int4 e = list[tq.slot(min(g, count - 1), listCap)];
     return g < count ? e : none;
   };
-  int4 info0 = entry(first);
-  int4 info1 = entry(first + stride);
-  typename W::Pairs q0, q1;
-#pragma unroll
-  for (int j = 0; j < kP; ++j) {
-    const int k = lane + j * G;
-    q0.v[j] = (W::kPf > 0 && k < info0.y) ? segPairs[info0.x + k] : make_int4(0, 0, 0, 0);
-  }
-  for (uint32_t g = first; g < count; g += stride) {
-    const int4 info2 = entry(g + 2 * stride);
+  constexpr int kP = W::kPf > 0 ? W::kPf : 1;
+  auto pairs = [&](typename W::Pairs &q, const int4 &e) {
 #pragma unroll
     for (int j = 0; j < kP; ++j) {
       const int k = lane + j * G;
-      q1.v[j] = (W::kPf > 0 && k < info1.y) ? segPairs[info1.x + k] : make_int4(0, 0, 0, 0);
+      q.v[j] = (W::kPf > 0 && k < e.y) ? segPairs[e.x + k] : make_int4(0, 0, 0, 0);
     }
-    resolveWaveComponent<PMAX, BMAX, G>(lds + group * L::kWords, ctl, tier,
-                                        info0, q0, tierList, listCap, bigList,
-                                        segPairs, px, py, vx, vy, m, r);
+  };
+  uint32_t g0 = waveId < waves ? waveId * kGroups : grab();
+  if (g0 >= count) {
+    return;
+  }
+  uint32_t g1 = grab();
+  int4 info0 = entry(g0 + group), info1 = entry(g1 + group);
+  typename W::Pairs q0, q1;
+  pairs(q0, info0);
+  while (g0 < count) {
+    const uint32_t g2 = g1 < count ? grab() : count;
+    const int4 info2 = entry(g2 + group);
+    pairs(q1, info1);
+    if (g0 + group < count) {
+      resolveWaveComponent<PMAX, BMAX, G>(lds + group * L::kWords, ctl, tier,
+                                          info0, q0, tierList, listCap,
+                                          bigList, segPairs, px, py, vx, vy, m,
+                                          r);
+    }
+    g0 = g1;
+    g1 = g2;
     info0 = info1;
     info1 = info2;
     q0 = q1;
   }
+}
+
+// Tiers 0..2 of the wave path (17..1024 pairs) in one launch, so that the
+// long chains of tier 2 run beside tiers 1 and 0 rather than after them.
+// A workgroup is two waves over one LDS area of a tier-2 component: the
+// first workgroups run tier 2 (one component each, on wave 0, the larger
+// first); in every other workgroup each wave holds half of the area and
+// drains tier 1 (two components side by side), then tier 0 (four). The grid
+// is more than the `resident` workgroups the chip holds at once, so the
+// slots tier 2 frees are taken up; a workgroup that finds the queues empty
+// returns at once.
+constexpr int kTierWaveWords = 2 * WaveLds<256, 192>::kWords;
+static_assert(4 * WaveLds<64, 64>::kWords <= kTierWaveWords &&
+                  WaveLds<1024, 640>::kWords <= 2 * kTierWaveWords,
+              "a wave's half holds tier 1 or 0, the whole area tier 2");
+__global__ __launch_bounds__(2 * kWave) void k_resolve_tiers(
+    Ctl *__restrict__ ctl, int4 *__restrict__ tierList, uint32_t listCap,
+    int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
+    float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
+    float *__restrict__ vy, const float *__restrict__ m,
+    const float *__restrict__ r, uint32_t resident) {
+  static_assert(waveTierPairs(0) == 64 && waveTierPairs(1) == 256 &&
+                    waveTierPairs(2) == 1024 && kWaveTiers == 4,
+                "tier shapes below");
+  if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
+    return;
+  }
+  __shared__ __align__(16) uint32_t lds[2 * kTierWaveWords];
+  const uint32_t wave = threadIdx.x / kWave;
+  // Tier 2 on the first workgroups, at most a quarter of the grid.
+  const TierQueues tq2(ctl, 2);
+  const uint32_t groups2 = min(tq2.count, gridDim.x / 4);
+  if (blockIdx.x < groups2) {
+    if (wave == 0) {
+      const int4 *list = tierList + static_cast<size_t>(kTierQueues * 2) * listCap;
+      typename WaveCtx<1024, 640, kWave>::Pairs none{};
+      for (uint32_t g = blockIdx.x; g < tq2.count; g += groups2) {
+        resolveWaveComponent<1024, 640, kWave>(lds, ctl, 2, list[tq2.slot(g, listCap)],
+                                               none, tierList, listCap, bigList,
+                                               segPairs, px, py, vx, vy, m, r);
+      }
+    }
+    return;
+  }
+  uint32_t *half = lds + wave * kTierWaveWords;
+  const uint32_t waveId = (blockIdx.x - groups2) * 2 + wave;
+  const uint32_t waves = (resident - groups2) * 2;
+  drainTier<256, 192, 32>(ctl, 1, waveId, waves, half, tierList, listCap,
+                          bigList, segPairs, px, py, vx, vy, m, r);
+  drainTier<64, 64, 16>(ctl, 0, waveId, waves, half, tierList, listCap,
+                        bigList, segPairs, px, py, vx, vy, m, r);
 }
 
 // ---- contact components of more than 4096 pairs, or that a wave tier

@@ -209,6 +209,9 @@ __global__ __launch_bounds__(kBlock) void k_jac_keys(
     for (int t = 0; t < kTierQueues * kWaveTiers; ++t) {
       ctl->waveCount[t] = 0;
     }
+    for (int t = 0; t < kWaveTiers; ++t) {
+      ctl->tierNext[t] = 0;
+    }
     ctl->largest = 0;
     ctl->status &= ~kStatusJacobiTooManyLarge;
   }

@@ -67,6 +67,7 @@ struct Ctl {
   // (queue kTierQueues t + k holds the k-th largest size class), so that a
   // tier's kernel starts its largest components first.
   uint32_t waveCount[kTierQueues * kWaveTiers];
+  uint32_t tierNext[kWaveTiers];                // entries of a tier taken so far
   uint32_t prevPairs; // contact pairs the previous collide found
   uint32_t levelDone; // k_level_scan blocks finished (0 at rest)
   unsigned long long farCount, bodyPairCount;

@@ -486,19 +486,11 @@ int launchCollide(nbx_sim *s, bool countersReset) {
          s->pairComp, s->compCount, s->compStart, s->segPairs);
   LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->segPairs,
          s->tierList, s->tierCap, s->bigList, px, py, vx, vy, m, r);
-  // Components of 17..1024 pairs, in tiers by size (4, 2, 1 a wave); the
-  // grids are the LDS residency of each tier on 256 CUs (a workgroup
-  // beyond the queued count returns at once).
-  static_assert(waveTierPairs(0) == 64 && waveTierPairs(1) == 256 &&
-                    waveTierPairs(2) == 1024 && waveTierPairs(3) == 4096 &&
-                    kWaveTiers == 4,
-                "tier launches below");
-  LAUNCH((k_resolve_wave<64, 64, 16>), 256 * 12, kWave, s->ctl, 0,
-         s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
-  LAUNCH((k_resolve_wave<256, 192, 32>), 256 * 8, kWave, s->ctl, 1,
-         s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
-  LAUNCH((k_resolve_wave<1024, 640, 64>), 256 * 4, kWave, s->ctl, 2,
-         s->tierList, s->tierCap, s->bigList, s->segPairs, px, py, vx, vy, m, r);
+  // Components of 9..1024 pairs, tiers 0..2 in one launch: four
+  // workgroups of two waves are resident on each of the 256 CUs (LDS), and
+  // one more a CU takes up the slots that tier 2's workgroups free.
+  LAUNCH(k_resolve_tiers, 256 * 5, 2 * kWave, s->ctl, s->tierList, s->tierCap,
+         s->bigList, s->segPairs, px, py, vx, vy, m, r, 256u * 4);
   // Tier 3 (1025..4096 pairs) and the workgroup path in one launch.
   LAUNCH(k_resolve_large, 256, kBlock, s->ctl, s->tierList, s->tierCap,
          s->compInfo, s->bigList, s->segPairs, s->pairs, s->bodySlot, px, py,

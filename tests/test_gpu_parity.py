@@ -544,6 +544,40 @@ def test_contact_component_tiers(nbx, oracle):
     assert info["largest"] == sizes.max()
 
 
+def test_many_components_per_tier(nbx, oracle):
+    """400 packed squares of side 12 (~500 pairs each: more components of
+    257..1024 pairs than the replay launch gives them workgroups, so each
+    such workgroup takes several in turn), beside squares of side 7 and 4
+    (tiers of 65..256 and 9..64 pairs), bit-exact against the sequential
+    replay."""
+    from oracle_lib import Bodies
+    rng = np.random.default_rng(17)
+    xs, ys = [], []
+    k = 0
+    for side, count in ((12, 400), (7, 300), (4, 300)):
+        g = np.stack(np.meshgrid(np.arange(side), np.arange(side)), -1).reshape(-1, 2)
+        for _ in range(count):
+            xs.append(g[:, 0] * 1.9 + rng.uniform(-0.08, 0.08, g.shape[0]) + 40.0 * (k % 40))
+            ys.append(g[:, 1] * 1.9 + rng.uniform(-0.08, 0.08, g.shape[0]) + 40.0 * (k // 40))
+            k += 1
+    px, py = np.concatenate(xs), np.concatenate(ys)
+    n = px.shape[0]
+    order = rng.permutation(n)
+    b = Bodies.empty(n)
+    b.px[:] = px[order].astype(np.float32)
+    b.py[:] = py[order].astype(np.float32)
+    b.vx[:] = rng.standard_normal(n).astype(np.float32)
+    b.vy[:] = rng.standard_normal(n).astype(np.float32)
+    b.m[:] = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    b.r[:] = 1.0
+    b.id[:] = np.arange(n, dtype=np.uint32)
+    info, comp = check_collide(nbx, oracle, b)
+    sizes = np.unique(comp, return_counts=True)[1]
+    assert np.sum((sizes > 256) & (sizes <= 1024)) > 320
+    assert np.any((sizes > 64) & (sizes <= 256)) and np.any((sizes > 8) & (sizes <= 64))
+    assert info["largest"] == sizes.max()
+
+
 def test_contact_chains_numbered_along(nbx, oracle):
     """Straight chains whose bodies are numbered along their length, the
     order a tree build gives a line of bodies, so the search's hooks (each

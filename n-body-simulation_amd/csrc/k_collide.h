@@ -208,8 +208,6 @@ __device__ inline void resetCollideCounters(Ctl *__restrict__ ctl) {
   ctl->compCount = 0;
   ctl->compPairs = 0;
   ctl->bigCount = 0;
-  ctl->smallCount = 0;
-  ctl->smallNext = 0;
   for (int t = 0; t < kTierQueues * kWaveTiers; ++t) {
     ctl->waveCount[t] = 0;
   }
@@ -905,19 +903,21 @@ __device__ __forceinline__ void resolveComponent(int start, int size,
   }
 }
 
-// Queues every component by size: up to kSmallComp pairs on the small list
-// (replayed one lane a component, in pairLess order, by k_resolve_tiers),
-// larger ones for the wave tiers (tiers 0..2 in k_resolve_tiers, tier 3 in
-// k_resolve_large), beyond those for the workgroup path (k_resolve_large).
+// One lane per component of up to kSmallComp pairs: pairLess order, then the
+// sequential replay. Larger components are queued by size for the wave
+// tiers (tiers 0..2 in k_resolve_tiers, tier 3 in k_resolve_large), beyond those for the
+// workgroup path (k_resolve_large).
 __global__ __launch_bounds__(kBlock) void k_resolve_small(
     Ctl *__restrict__ ctl, const int4 *__restrict__ compInfo,
-    int4 *__restrict__ tierList, uint32_t listCap,
-    int32_t *__restrict__ bigList, int4 *__restrict__ smallList) {
+    int4 *__restrict__ segPairs, int4 *__restrict__ tierList,
+    uint32_t listCap, int32_t *__restrict__ bigList, float *__restrict__ px,
+    float *__restrict__ py, float *__restrict__ vx, float *__restrict__ vy,
+    const float *__restrict__ m, const float *__restrict__ r) {
   if (barrierStalled(ctl) || (ctl->status & kStatusPairOverflow)) {
     return;
   }
   // The grid covers the bodies (the most components there can be); a
-  // workgroup past the components has nothing to queue.
+  // workgroup past the components has nothing to queue or replay.
   const uint32_t comps = ctl->compCount;
   if (blockIdx.x * kBlock >= comps) {
     return;
@@ -932,9 +932,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
   // Queue class: kTierQueues t + k for a component of wave tier t in the
   // tier's k-th size class counted from the largest (the tier's pair range
   // (lo, P] cut in kTierQueues equal parts), kTierQueues kWaveTiers the
-  // workgroup path, kSmallClass the small list, -1 none.
-  constexpr int kSmallClass = kTierQueues * kWaveTiers + 1;
-  int cls = live && size > 0 ? kSmallClass : -1;
+  // workgroup path, -1 none.
+  int cls = -1;
   if (size > kSmallComp) {
     int t = 0;
     while (t < kWaveTiers && size > waveTierPairs(t)) {
@@ -953,7 +952,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
   // per class and workgroup, the classes' atomics issued together (a wave
   // waiting on each class's contended counter in turn cost more than the
   // replays).
-  constexpr int kClasses = kSmallClass + 1;
+  constexpr int kClasses = kTierQueues * kWaveTiers + 1;
   constexpr int kWaves = kBlock / kWave;
   __shared__ uint32_t waveCnt[kWaves][kClasses];
   __shared__ uint32_t waveBase[kWaves][kClasses];
@@ -980,10 +979,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
     }
     uint32_t base = 0;
     if (total > 0) {
-      base = atomicAdd(c < kTierQueues * kWaveTiers
-                           ? &ctl->waveCount[c]
-                           : (c == kSmallClass ? &ctl->smallCount : &ctl->bigCount),
-                       total);
+      base = atomicAdd(c < kTierQueues * kWaveTiers ? &ctl->waveCount[c] : &ctl->bigCount, total);
     }
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
@@ -998,55 +994,24 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
     if (cls < kTierQueues * kWaveTiers) {
       tierList[static_cast<size_t>(cls) * listCap + at] =
           make_int4(start, size, static_cast<int>(j), 0);
-    } else if (cls == kSmallClass) {
-      smallList[at] = make_int4(start, size, static_cast<int>(j), 0);
     } else {
       bigList[at] = static_cast<int32_t>(j);
     }
   }
-}
-
-// One lane per queued component of up to kSmallComp pairs (pairLess order,
-// then the sequential replay), 64 at a time: the first `waves` waves take
-// their first batch by `waveId`, every later batch comes from a counter.
-__device__ inline void drainSmall(Ctl *__restrict__ ctl, uint32_t waveId,
-                                  uint32_t waves,
-                                  const int4 *__restrict__ smallList,
-                                  int4 *__restrict__ segPairs,
-                                  float *__restrict__ px, float *__restrict__ py,
-                                  float *__restrict__ vx, float *__restrict__ vy,
-                                  const float *__restrict__ m,
-                                  const float *__restrict__ r) {
+  if (!live || size > kSmallComp) {
+    return;
+  }
   static_assert(kSmallComp == 4 || kSmallComp == 8 || kSmallComp == 16,
                 "size classes below cover the one-lane path");
-  const uint32_t count = ctl->smallCount;
-  const uint32_t firstRound = waves * kWave;
-  const uint32_t lane = threadIdx.x % kWave;
-  auto grab = [&]() {
-    if (firstRound >= count) {
-      return count;
-    }
-    uint32_t g = 0;
-    if (lane == 0) {
-      g = atomicAdd(&ctl->smallNext, kWave);
-    }
-    return firstRound + static_cast<uint32_t>(__shfl(static_cast<int>(g), 0));
-  };
-  for (uint32_t g = waveId < waves ? waveId * kWave : grab(); g < count; g = grab()) {
-    if (g + lane < count) {
-      const int4 info = smallList[g + lane];
-      const int start = info.x, size = info.y;
-      if (size == 1) {
-        const int4 q = segPairs[start];
-        resolvePair(px, py, vx, vy, m, r, q.x, q.y);
-      } else if (kSmallComp == 4 || size <= 4) {
-        resolveComponent<4>(start, size, segPairs, px, py, vx, vy, m, r);
-      } else if (kSmallComp == 8 || size <= 8) {
-        resolveComponent<8>(start, size, segPairs, px, py, vx, vy, m, r);
-      } else {
-        resolveComponent<16>(start, size, segPairs, px, py, vx, vy, m, r);
-      }
-    }
+  if (size == 1) {
+    const int4 q = segPairs[start];
+    resolvePair(px, py, vx, vy, m, r, q.x, q.y);
+  } else if (kSmallComp == 4 || size <= 4) {
+    resolveComponent<4>(start, size, segPairs, px, py, vx, vy, m, r);
+  } else if (kSmallComp == 8 || size <= 8) {
+    resolveComponent<8>(start, size, segPairs, px, py, vx, vy, m, r);
+  } else {
+    resolveComponent<16>(start, size, segPairs, px, py, vx, vy, m, r);
   }
 }
 
@@ -1668,8 +1633,7 @@ __global__ __launch_bounds__(2 * kWave) void k_resolve_tiers(
     int32_t *__restrict__ bigList, int4 *__restrict__ segPairs,
     float *__restrict__ px, float *__restrict__ py, float *__restrict__ vx,
     float *__restrict__ vy, const float *__restrict__ m,
-    const float *__restrict__ r, uint32_t resident,
-    const int4 *__restrict__ smallList) {
+    const float *__restrict__ r, uint32_t resident) {
   static_assert(waveTierPairs(0) == 64 && waveTierPairs(1) == 256 &&
                     waveTierPairs(2) == 1024 && kWaveTiers == 4,
                 "tier shapes below");
@@ -1681,14 +1645,8 @@ __global__ __launch_bounds__(2 * kWave) void k_resolve_tiers(
   // Tier 2 on the first workgroups, at most a quarter of the grid.
   const TierQueues tq2(ctl, 2);
   const uint32_t groups2 = min(tq2.count, gridDim.x / 4);
-  // The small list's first batches: wave 1 of each tier-2 workgroup, then
-  // both waves of every other resident workgroup.
-  const uint32_t smallWaves = 2 * resident - groups2;
   if (blockIdx.x < groups2) {
-    if (wave == 1) {
-      drainSmall(ctl, blockIdx.x, smallWaves, smallList, segPairs, px, py, vx,
-                 vy, m, r);
-    } else {
+    if (wave == 0) {
       const int4 *list = tierList + static_cast<size_t>(kTierQueues * 2) * listCap;
       typename WaveCtx<1024, 640, kWave>::Pairs none{};
       for (uint32_t g = blockIdx.x; g < tq2.count; g += groups2) {
@@ -1696,16 +1654,12 @@ __global__ __launch_bounds__(2 * kWave) void k_resolve_tiers(
                                                none, tierList, listCap, bigList,
                                                segPairs, px, py, vx, vy, m, r);
       }
-      drainSmall(ctl, smallWaves, smallWaves, smallList, segPairs, px, py, vx,
-                 vy, m, r);
     }
     return;
   }
   uint32_t *half = lds + wave * kTierWaveWords;
   const uint32_t waveId = (blockIdx.x - groups2) * 2 + wave;
   const uint32_t waves = (resident - groups2) * 2;
-  drainSmall(ctl, groups2 + waveId, smallWaves, smallList, segPairs, px, py,
-             vx, vy, m, r);
   drainTier<256, 192, 32>(ctl, 1, waveId, waves, half, tierList, listCap,
                           bigList, segPairs, px, py, vx, vy, m, r);
   drainTier<64, 64, 16>(ctl, 0, waveId, waves, half, tierList, listCap,

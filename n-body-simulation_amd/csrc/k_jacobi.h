@@ -206,8 +206,6 @@ __global__ __launch_bounds__(kBlock) void k_jac_keys(
     ctl->compCount = 0;
     ctl->compPairs = 0;
     ctl->bigCount = 0;
-    ctl->smallCount = 0;
-    ctl->smallNext = 0;
     for (int t = 0; t < kTierQueues * kWaveTiers; ++t) {
       ctl->waveCount[t] = 0;
     }

@@ -56,7 +56,6 @@ struct Ctl {
   float cell, invCell;
   uint32_t largeCount, pairCount;
   uint32_t compCount, compPairs, bigCount, largest;
-  uint32_t smallCount, smallNext; // components of <= kSmallComp pairs: queued, taken
   // tree
   int32_t splits[kMaxDepth];         // splitting nodes at each depth
   int32_t levelStart[kMaxDepth + 2]; // first node index at each depth

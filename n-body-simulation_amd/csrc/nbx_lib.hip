@@ -136,7 +136,7 @@ struct nbx_sim {
   int4 *pairs = nullptr, *segPairs = nullptr;
   int32_t *pairComp = nullptr;
   uint32_t *compCount = nullptr, *compStart = nullptr;
-  int4 *compInfo = nullptr, *smallList = nullptr;
+  int4 *compInfo = nullptr;
   int32_t *bigList = nullptr;
   int4 *tierList = nullptr; // kTierQueues x kWaveTiers queues of tierCap {start, size, component}
   uint32_t tierCap = 0;
@@ -484,13 +484,13 @@ int launchCollide(nbx_sim *s, bool countersReset) {
   }
   LAUNCH(k_comp_scatter, pairGrid, kBlock, s->ctl, s->pairCap, s->pairs,
          s->pairComp, s->compCount, s->compStart, s->segPairs);
-  LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->tierList,
-         s->tierCap, s->bigList, s->smallList);
+  LAUNCH(k_resolve_small, nb, kBlock, s->ctl, s->compInfo, s->segPairs,
+         s->tierList, s->tierCap, s->bigList, px, py, vx, vy, m, r);
   // Components of 9..1024 pairs, tiers 0..2 in one launch: four
   // workgroups of two waves are resident on each of the 256 CUs (LDS), and
   // one more a CU takes up the slots that tier 2's workgroups free.
   LAUNCH(k_resolve_tiers, 256 * 5, 2 * kWave, s->ctl, s->tierList, s->tierCap,
-         s->bigList, s->segPairs, px, py, vx, vy, m, r, 256u * 4, s->smallList);
+         s->bigList, s->segPairs, px, py, vx, vy, m, r, 256u * 4);
   // Tier 3 (1025..4096 pairs) and the workgroup path in one launch.
   LAUNCH(k_resolve_large, 256, kBlock, s->ctl, s->tierList, s->tierCap,
          s->compInfo, s->bigList, s->segPairs, s->pairs, s->bodySlot, px, py,
@@ -984,7 +984,6 @@ int nbx_create_ex(uint64_t capacity, uint32_t flags, nbx_sim **out) {
   A(s->pairs, s->pairCap, false) A(s->segPairs, s->pairCap, false)
   A(s->pairComp, s->pairCap, false)
   A(s->compCount, n, true) A(s->compStart, n, false) A(s->compInfo, n, false)
-  A(s->smallList, n, false)
   A(s->bigList, n, false)
   // A queued component has more than kSmallComp pairs.
   s->tierCap = s->pairCap / (kSmallComp + 1) + 1;

@@ -1015,7 +1015,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
   }
 }
 
-// ---- contact components of kSmallComp+1 .. 4096 pairs: one wave each ----
+// ---- contact components of kSmallComp+1 .. 4096 pairs: 16-64 lanes each ----
 //
 // resolve() touches only its two bodies, so the sequential replay of a
 // component is fixed by the order of the pairs on each body: any order that
@@ -1029,16 +1029,17 @@ __global__ __launch_bounds__(kBlock) void k_resolve_small(
 // parallel -- as many rounds as the longest chain (~0.3 x the pairs in the
 // packed clumps a long collisional run builds) instead of one step a pair.
 //
-// One 64-lane workgroup per component, all in its own LDS region so that
-// many components are in flight on each CU (the rounds are latency-bound):
-// an open-addressing table gives the bodies dense slots, counting sort
-// builds each body's list of pairs, one lane a body sorts its list (at most
-// kMaxDegree pairs, in registers) and links each pair to the next pair on
-// that body, the bodies are staged in LDS and the rounds replay the pairs.
-// The pairs are written back in the order they were replayed. A component
-// with more than BMAX bodies goes on to the next tier's queue, one with a
-// body of more than kMaxDegree pairs to the workgroup path
-// (resolveBigComponent, in k_resolve_large).
+// A group of 16, 32 or 64 lanes per component (by tier), all state in the
+// group's own LDS region so that many components are in flight on each CU
+// (the rounds are latency-bound): an open-addressing table gives the bodies
+// dense slots, counting sort builds each body's list of pairs, one lane a
+// body sorts its list (at most kMaxDegree pairs, in registers) and links
+// each pair to the next pair on that body, the bodies are staged in LDS and
+// the rounds replay the pairs. The pairs are written back in the order they
+// were replayed. A component with more than BMAX bodies goes to tier 3's
+// queue (from tier 3: the workgroup path), one with a body of more than
+// kMaxDegree pairs to the workgroup path (resolveBigComponent, in
+// k_resolve_large).
 constexpr int kMaxDegree = 16;
 constexpr uint32_t kEmptyKey = 0xffffffffu;
 

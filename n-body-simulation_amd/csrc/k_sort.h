@@ -541,15 +541,17 @@ __device__ inline void msdBuckets(uint32_t n, uint32_t grid,
 // (what k_bbox and k_keys do in two launches): each workgroup reduces its
 // chunk's bounding box and publishes it to parts[b]; after a grid barrier
 // every workgroup reduces all G partials to the root quad, computes its
-// bodies' float-descent keys (low words to keyLo, indices as payloads) and
-// ranks the high words of pass 0 straight from registers. Workgroup 0
-// publishes the root quad and clears the build's counters.
+// bodies' high key words by the float descent (indices as payloads) and
+// ranks them straight from registers. Workgroup 0 publishes the root quad
+// and clears the build's counters. The low words (depths 16..31) are
+// computed only where they are read: by k_deep, for the runs of equal high
+// words.
 template <int ITEMS, bool BUILD>
 __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
     uint32_t n, uint32_t chunk, uint32_t *keys0, uint32_t *vals0,
     uint32_t *keys1, uint32_t *vals1, uint32_t *table, uint32_t *rowTotal,
     uint32_t *bar, uint32_t spinLimit, Ctl *ctl, const float *px,
-    const float *py, uint32_t *parts, uint32_t *keyLo) {
+    const float *py, uint32_t *parts) {
   constexpr int kTile = ITEMS * kPsBlock;
   constexpr int kStage = (BUILD && 2 * kMsdCap > kTile) ? 2 * kMsdCap : kTile;
   __shared__ uint32_t sk[kStage];
@@ -627,10 +629,10 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t p = first + j * kWave;
-      uint32_t h = 0, l = 0;
+      uint32_t h = 0;
       if (p < cnt) {
-        descentKeys(px[lo + p], py[lo + p], qcx, qcy, qs, h, l);
-        keyLo[lo + p] = l;
+        float dcx = qcx, dcy = qcy, ds = qs;
+        h = descentWord(px[lo + p], py[lo + p], dcx, dcy, ds);
       }
       key[j] = h;
       val[j] = lo + p;

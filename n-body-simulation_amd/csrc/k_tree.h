@@ -87,6 +87,32 @@ __global__ __launch_bounds__(kBlock) void k_bbox(
 // Stage 2 of the bounding box runs in every workgroup here (the partials
 // are a few KB in L2), so no atomics and no reset are needed. Thread 0 also
 // publishes the root quad and clears the build's counters.
+// Body v's low key word (depths 16..31 of the float descent from the root
+// quad), stored to keyLo: the one-launch sort computes only the high words,
+// and the low words are read only within runs of equal high words.
+__device__ inline uint32_t lowKeyWord(const Ctl *__restrict__ ctl,
+                                      const float *__restrict__ px,
+                                      const float *__restrict__ py,
+                                      uint32_t *__restrict__ keyLo, uint32_t v) {
+  float cx = ctl->quad[0], cy = ctl->quad[1], s = ctl->quad[2];
+  const float x = px[v], y = py[v];
+  (void)descentWord(x, y, cx, cy, s); // depths 0..15: the high word
+  const uint32_t l = descentWord(x, y, cx, cy, s);
+  keyLo[v] = l;
+  return l;
+}
+
+// Every body's low key word (the keys download; the step computes them only
+// for runs of equal high words).
+__global__ __launch_bounds__(kBlock) void k_key_low(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    const Ctl *__restrict__ ctl, uint32_t *__restrict__ keyLo) {
+  const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+  if (v < n) {
+    (void)lowKeyWord(ctl, px, py, keyLo, v);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_keys(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
     const uint4 *__restrict__ parts, uint32_t nparts, Ctl *__restrict__ ctl,
@@ -120,8 +146,10 @@ __global__ __launch_bounds__(kBlock) void k_keys(
 
 // Stable bitonic sort, in LDS, of perm[start, start + len) by low key word
 // (len <= kDeepMax), by the whole workgroup.
-__device__ inline void deepSortRun(int start, int len,
-                                   const uint32_t *__restrict__ keyLo,
+__device__ inline void deepSortRun(int start, int len, const Ctl *__restrict__ ctl,
+                                   const float *__restrict__ px,
+                                   const float *__restrict__ py,
+                                   uint32_t *__restrict__ keyLo,
                                    uint32_t *__restrict__ perm,
                                    uint32_t *__restrict__ sk,
                                    uint32_t *__restrict__ sv,
@@ -133,7 +161,7 @@ __device__ inline void deepSortRun(int start, int len,
   for (int k = threadIdx.x; k < p2; k += kBlock) {
     if (k < len) {
       sv[k] = perm[start + k];
-      sk[k] = keyLo[sv[k]];
+      sk[k] = lowKeyWord(ctl, px, py, keyLo, sv[k]);
     }
     order[k] = static_cast<int16_t>(k);
   }
@@ -176,9 +204,9 @@ __device__ inline void deepSortRun(int start, int len,
 // run's start acts on it, so no two lanes or workgroups touch the same
 // entries.
 __global__ __launch_bounds__(kBlock) void k_deep(
-    uint32_t n, const uint32_t *__restrict__ hi,
-    const uint32_t *__restrict__ keyLo, uint32_t *__restrict__ perm,
-    Ctl *__restrict__ ctl) {
+    uint32_t n, const uint32_t *__restrict__ hi, uint32_t *__restrict__ keyLo,
+    uint32_t *__restrict__ perm, Ctl *__restrict__ ctl,
+    const float *__restrict__ px, const float *__restrict__ py) {
   if (barrierStalled(ctl)) {
     return;
   }
@@ -212,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_deep(
     int len = 0;
     while (len < kLeafCapacity && j + len < n && hi[j + len] == h) {
       const uint32_t v = perm[j + len];
-      const uint32_t key = keyLo[v];
+      const uint32_t key = lowKeyWord(ctl, px, py, keyLo, v);
       int q = len;
       while (q > 0 && key < l[q - 1]) {
         p[q] = p[q - 1];
@@ -237,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_deep(
       }
       continue;
     }
-    deepSortRun(seg.x, seg.y, keyLo, perm, sk, sv, order);
+    deepSortRun(seg.x, seg.y, ctl, px, py, keyLo, perm, sk, sv, order);
   }
 }
 

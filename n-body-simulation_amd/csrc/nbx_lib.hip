@@ -145,6 +145,7 @@ struct nbx_sim {
   // tree
   uint32_t *keyHi[2]{}, *vals[2]{};
   uint32_t *keyLo = nullptr, *sortHist = nullptr, *sortRowTotal = nullptr;
+  int keySource = 0; // buffer whose bodies the last build keyed
   uint32_t *sortBarrier = nullptr; // grid barrier words of k_sort_onelaunch
   uint32_t sortGrid = 0; // its maximum workgroups; 0: three launches a pass
   uint32_t barrierSpins = kBarrierSpins; // poll bound of the grid barriers
@@ -337,15 +338,13 @@ bool launchOneLaunchSort(nbx_sim *s, uint32_t n, uint32_t *const keys[2],
                            dim3(kPsBlock), 0, s->stream, n, chunk, keys[0],   \
                            vals[0], keys[1], vals[1], s->sortHist,            \
                            s->sortRowTotal, s->sortBarrier, s->barrierSpins,  \
-                           s->ctl, px, py,                                    \
-                           parts, s->keyLo);                                  \
+                           s->ctl, px, py, parts);                            \
       } else {                                                                \
         hipLaunchKernelGGL((k_sort_onelaunch<ITEMS, false>), dim3(g),         \
                            dim3(kPsBlock), 0, s->stream, n, chunk, keys[0],   \
                            vals[0], keys[1], vals[1], s->sortHist,            \
                            s->sortRowTotal, s->sortBarrier, s->barrierSpins,  \
-                           s->ctl, px, py,                                    \
-                           parts, s->keyLo);                                  \
+                           s->ctl, px, py, parts);                            \
       }                                                                       \
       *rc = checkLaunch();                                                    \
       return true;                                                            \
@@ -522,7 +521,8 @@ int launchBuild(nbx_sim *s) {
   }
   const uint32_t *hi = s->keyHi[0];
   uint32_t *perm = s->vals[0];
-  LAUNCH(k_deep, nb, kBlock, n, hi, s->keyLo, perm, s->ctl);
+  LAUNCH(k_deep, nb, kBlock, n, hi, s->keyLo, perm, s->ctl, s->px[c], s->py[c]);
+  s->keySource = c;
   LAUNCH(k_permute, nb, kBlock, n, perm, s->px[c], s->py[c], s->vx[c],
          s->vy[c], s->m[c], s->r[c], s->id[c], s->px[o], s->py[o], s->vx[o],
          s->vy[o], s->m[o], s->r[o], s->id[o], s->bodyPack);
@@ -1527,6 +1527,13 @@ int nbx_download_keys(nbx_sim *s, uint32_t *hi, uint32_t *lo, uint32_t *perm) {
   }
   const uint64_t n = s->n;
   std::vector<uint32_t> p(n), l(n);
+  if (lo && n > 0) {
+    // The low words of the bodies the last build keyed (the step computes
+    // them only where the high words tie).
+    const int c = s->keySource;
+    LAUNCH(k_key_low, blocksFor(static_cast<uint32_t>(n)), kBlock,
+           static_cast<uint32_t>(n), s->px[c], s->py[c], s->ctl, s->keyLo);
+  }
   NBX_TRY(copyOut(s, hi, s->keyHi[0], n));
   NBX_TRY(copyOut(s, p.data(), s->vals[0], n));
   NBX_TRY(copyOut(s, l.data(), s->keyLo, n));

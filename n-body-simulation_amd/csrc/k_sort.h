@@ -475,6 +475,11 @@ __device__ inline void countingPassLds(const uint32_t *__restrict__ srcK,
 // back from G-1 (a uniform disc fills about 224 of the 256 top cells, so at
 // 1M bodies with G = 245 every workgroup sorts at most one). Every address
 // is bounds-checked (a stalled barrier leaves values in range).
+// A body the sort places this many positions or more from its old place
+// counts as moved far; when many do (the first build from an unsorted
+// order), k_permute gathers whole records staged by k_deep instead of seven
+// scattered words a body.
+constexpr uint32_t kFarMove = 1024;
 __device__ inline void msdBuckets(uint32_t n, uint32_t grid,
                                   const uint32_t *bucketKeys,
                                   const uint32_t *bucketVals,
@@ -486,9 +491,12 @@ __device__ inline void msdBuckets(uint32_t n, uint32_t grid,
                                   uint32_t (*wh)[256],
                                   uint32_t *__restrict__ dstart,
                                   uint32_t *__restrict__ list,
-                                  uint32_t *__restrict__ scratch) {
+                                  uint32_t *__restrict__ scratch,
+                                  uint32_t *__restrict__ farMoves) {
   const uint32_t t = threadIdx.x;
   const uint32_t b = blockIdx.x;
+  uint32_t far = 0; // bodies this workgroup placed kFarMove or more from
+                    // where they were (an order far from the last one's)
   // The non-empty buckets in digit order (in `list`): one each for
   // workgroups 0..G-1, then snaking back from G-1.
   __shared__ uint32_t nonEmpty;
@@ -527,8 +535,15 @@ __device__ inline void msdBuckets(uint32_t n, uint32_t grid,
     for (uint32_t p = t; p < len; p += kPsBlock) {
       outKeys[st + p] = k1[p];
       outVals[st + p] = v1[p];
+      const uint32_t to = st + p, from = v1[p];
+      far += (from > to ? from - to : to - from) >= kFarMove ? 1u : 0u;
     }
     __syncthreads();
+  }
+  uint32_t total;
+  (void)blockExclusiveScan<kPsBlock>(far, scratch, &total);
+  if (t == 0 && total > 0) {
+    atomicAdd(farMoves, total);
   }
 }
 
@@ -804,7 +819,7 @@ __global__ __launch_bounds__(kPsBlock) void k_sort_onelaunch(
   }
   if (BUILD && msdDone) {
     msdBuckets(n, grid, keys1, vals1, keys0, vals0, bsize, bstart, sk, sv,
-               wh, dstart, goff, scratch);
+               wh, dstart, goff, scratch, &ctl->farMoves);
   }
 }
 

@@ -203,12 +203,30 @@ __device__ inline void deepSortRun(int start, int len, const Ctl *__restrict__ c
 // its start, in LDS, after the lanes' runs. Runs are disjoint and only a
 // run's start acts on it, so no two lanes or workgroups touch the same
 // entries.
+// Whether k_permute gathers staged records: the sort moved more than an
+// eighth of the bodies far (see kFarMove).
+__device__ inline bool permuteStaged(const Ctl *__restrict__ ctl, uint32_t n) {
+  return ctl->farMoves > n / 8;
+}
+
+// Also stages every body's state as one 32-byte record in its current order
+// when the permute will gather those (permuteStaged).
 __global__ __launch_bounds__(kBlock) void k_deep(
     uint32_t n, const uint32_t *__restrict__ hi, uint32_t *__restrict__ keyLo,
     uint32_t *__restrict__ perm, Ctl *__restrict__ ctl,
-    const float *__restrict__ px, const float *__restrict__ py) {
+    const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ vx, const float *__restrict__ vy,
+    const float *__restrict__ m, const float *__restrict__ r,
+    const uint32_t *__restrict__ id, float4 *__restrict__ stage) {
   if (barrierStalled(ctl)) {
     return;
+  }
+  {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n && permuteStaged(ctl, n)) {
+      stage[2 * i] = make_float4(px[i], py[i], vx[i], vy[i]);
+      stage[2 * i + 1] = make_float4(m[i], r[i], __uint_as_float(id[i]), 0.0f);
+    }
   }
   __shared__ int2 longRuns[kBlock];
   __shared__ int nLong;
@@ -280,20 +298,42 @@ __global__ __launch_bounds__(kBlock) void k_permute(
     const uint32_t *__restrict__ id, float *__restrict__ opx,
     float *__restrict__ opy, float *__restrict__ ovx, float *__restrict__ ovy,
     float *__restrict__ om, float *__restrict__ orr,
-    uint32_t *__restrict__ oid, float4 *__restrict__ pack) {
+    uint32_t *__restrict__ oid, float4 *__restrict__ pack,
+    const Ctl *__restrict__ ctl, const float4 *__restrict__ stage) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) {
     return;
   }
   const uint32_t s = perm[j];
-  const float x = px[s], y = py[s], mass = m[s];
+  float x, y, u, w, mass, rad;
+  uint32_t ident;
+  if (permuteStaged(ctl, n)) {
+    // One 32-byte record a body (k_deep staged them) instead of seven
+    // scattered words.
+    const float4 a = stage[2 * s], b = stage[2 * s + 1];
+    x = a.x;
+    y = a.y;
+    u = a.z;
+    w = a.w;
+    mass = b.x;
+    rad = b.y;
+    ident = __float_as_uint(b.z);
+  } else {
+    x = px[s];
+    y = py[s];
+    u = vx[s];
+    w = vy[s];
+    mass = m[s];
+    rad = r[s];
+    ident = id[s];
+  }
   opx[j] = x;
   opy[j] = y;
-  ovx[j] = vx[s];
-  ovy[j] = vy[s];
+  ovx[j] = u;
+  ovy[j] = w;
   om[j] = mass;
-  orr[j] = r[s];
-  oid[j] = id[s];
+  orr[j] = rad;
+  oid[j] = ident;
   pack[j] = make_float4(x, y, mass, 0.0f);
 }
 

@@ -56,6 +56,7 @@ struct Ctl {
   float cell, invCell;
   uint32_t largeCount, pairCount;
   uint32_t compCount, compPairs, bigCount, largest;
+  uint32_t farMoves; // bodies the build's sort moved far (k_deep, k_permute)
   // tree
   int32_t splits[kMaxDepth];         // splitting nodes at each depth
   int32_t levelStart[kMaxDepth + 2]; // first node index at each depth
@@ -189,6 +190,7 @@ __device__ inline void publishRoot(Ctl *ctl, uint4 b, float qcx, float qcy,
   ctl->quad[1] = qcy;
   ctl->quad[2] = qs;
   ctl->subtreeRoots = 0;
+  atomicExch(&ctl->farMoves, 0u); // counted by atomics after grid barriers
   atomicAnd(&ctl->status, ~(kStatusNodeOverflow | kStatusDeepTooLarge |
                             kStatusTreeInconsistent));
 }

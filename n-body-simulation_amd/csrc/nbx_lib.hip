@@ -521,11 +521,14 @@ int launchBuild(nbx_sim *s) {
   }
   const uint32_t *hi = s->keyHi[0];
   uint32_t *perm = s->vals[0];
-  LAUNCH(k_deep, nb, kBlock, n, hi, s->keyLo, perm, s->ctl, s->px[c], s->py[c]);
+  // The collide's grid records are free during the build: the permute's
+  // staged records (2 float4 a body) when the sort moved many bodies far.
+  LAUNCH(k_deep, nb, kBlock, n, hi, s->keyLo, perm, s->ctl, s->px[c], s->py[c],
+         s->vx[c], s->vy[c], s->m[c], s->r[c], s->id[c], s->grid);
   s->keySource = c;
   LAUNCH(k_permute, nb, kBlock, n, perm, s->px[c], s->py[c], s->vx[c],
          s->vy[c], s->m[c], s->r[c], s->id[c], s->px[o], s->py[o], s->vx[o],
-         s->vy[o], s->m[o], s->r[o], s->id[o], s->bodyPack);
+         s->vy[o], s->m[o], s->r[o], s->id[o], s->bodyPack, s->ctl, s->grid);
   s->cur = o;
   const uint32_t tiles256 = nb;
   LAUNCH(k_lcp_count, tiles256, kBlock, n, hi, s->keyLo, perm, tiles256,

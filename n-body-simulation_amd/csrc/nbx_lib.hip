@@ -145,7 +145,8 @@ struct nbx_sim {
   // tree
   uint32_t *keyHi[2]{}, *vals[2]{};
   uint32_t *keyLo = nullptr, *sortHist = nullptr, *sortRowTotal = nullptr;
-  int keySource = 0; // buffer whose bodies the last build keyed
+  int keySource = 0;      // buffer whose bodies the last build keyed
+  bool keyLowFilled = false; // every low key word of the last build stored
   uint32_t *sortBarrier = nullptr; // grid barrier words of k_sort_onelaunch
   uint32_t sortGrid = 0; // its maximum workgroups; 0: three launches a pass
   uint32_t barrierSpins = kBarrierSpins; // poll bound of the grid barriers
@@ -236,6 +237,7 @@ int checkLaunch() {
 int launchIterate(nbx_sim *s, bool resetCollide = false);
 int launchCollide(nbx_sim *s, bool countersReset = false);
 int launchBuild(nbx_sim *s);
+int fillKeyLow(nbx_sim *s);
 int launchForces(nbx_sim *s, bool count);
 
 // ctl non-null: also resets the collide's counters (the step's collide
@@ -523,6 +525,7 @@ int launchBuild(nbx_sim *s) {
   uint32_t *perm = s->vals[0];
   // The collide's grid records are free during the build: the permute's
   // staged records (2 float4 a body) when the sort moved many bodies far.
+  s->keyLowFilled = false;
   LAUNCH(k_deep, nb, kBlock, n, hi, s->keyLo, perm, s->ctl, s->px[c], s->py[c],
          s->vx[c], s->vy[c], s->m[c], s->r[c], s->id[c], s->grid);
   s->keySource = c;
@@ -785,6 +788,7 @@ int graphStep(nbx_sim *s) {
     if (!s->graph[c]) {
       NBX_TRY(capture(s, -1, c, &s->graph[c]));
     }
+    s->keyLowFilled = false; // the step's build
     NBX_HIP(hipGraphLaunch(s->graph[c], s->stream));
   } else {
     for (int p = 0; p < NBX_PHASES; ++p) {
@@ -795,6 +799,7 @@ int graphStep(nbx_sim *s) {
       if (!s->phaseGraph[p][c]) {
         NBX_TRY(capture(s, p, c, &s->phaseGraph[p][c]));
       }
+      s->keyLowFilled = false;
       NBX_HIP(hipGraphLaunch(s->phaseGraph[p][c], s->stream));
     }
     NBX_TRY(record(s));
@@ -902,6 +907,20 @@ int validParams(const nbx_params *p) {
       p->collide_mode != NBX_COLLIDE_JACOBI) {
     return setError(NBX_ERR_ARG, "unknown collide mode");
   }
+  return NBX_OK;
+}
+
+// Every low key word of the last build (the build stores them only where
+// high words tie), from the bodies it keyed: before the download and before
+// an upload overwrites those bodies.
+int fillKeyLow(nbx_sim *s) {
+  if (!s->treeBuilt || s->keyLowFilled || s->n == 0) {
+    return NBX_OK;
+  }
+  const uint32_t n = static_cast<uint32_t>(s->n);
+  LAUNCH(k_key_low, blocksFor(n), kBlock, n, s->px[s->keySource],
+         s->py[s->keySource], s->ctl, s->keyLo);
+  s->keyLowFilled = true;
   return NBX_OK;
 }
 
@@ -1087,6 +1106,8 @@ int nbx_upload(nbx_sim *s, uint64_t n, const float *px, const float *py,
   }
   NBX_HIP(hipStreamSynchronize(s->stream));
   dropGraphs(s);
+  NBX_TRY(fillKeyLow(s));
+  NBX_HIP(hipStreamSynchronize(s->stream));
   s->cur = 0;
   s->n = n;
   s->massAbsSum = 0;
@@ -1530,12 +1551,8 @@ int nbx_download_keys(nbx_sim *s, uint32_t *hi, uint32_t *lo, uint32_t *perm) {
   }
   const uint64_t n = s->n;
   std::vector<uint32_t> p(n), l(n);
-  if (lo && n > 0) {
-    // The low words of the bodies the last build keyed (the step computes
-    // them only where the high words tie).
-    const int c = s->keySource;
-    LAUNCH(k_key_low, blocksFor(static_cast<uint32_t>(n)), kBlock,
-           static_cast<uint32_t>(n), s->px[c], s->py[c], s->ctl, s->keyLo);
+  if (lo) {
+    NBX_TRY(fillKeyLow(s));
   }
   NBX_TRY(copyOut(s, hi, s->keyHi[0], n));
   NBX_TRY(copyOut(s, p.data(), s->vals[0], n));

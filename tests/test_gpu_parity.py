@@ -111,6 +111,27 @@ def test_build_bit_exact(nbx, oracle, states, name):
     assert np.array_equal(got.id, srt.id)
 
 
+def test_keys_after_upload_are_the_last_builds(nbx, oracle):
+    """The build stores the low key words only where high words tie and the
+    keys download fills in the rest from the bodies the build keyed; an
+    upload after the build (into those bodies) keeps the download the last
+    build's, and a later build replaces it."""
+    a = oracle.uniform_disc(20000)
+    b = a.copy()
+    b.px[:] = (b.px * np.float32(1.5)).astype(np.float32)
+    _, hi, lo, perm, _ = oracle.sorted_state(a)
+    with new_sim(nbx, a) as sim:
+        sim.build()
+        sim.upload(b)
+        ghi, glo, gperm = sim.keys()
+        assert np.array_equal(ghi, hi) and np.array_equal(gperm, perm)
+        assert np.array_equal(glo, lo), "low key words of the last build"
+        sim.build()
+        _, lo_b, perm_b = sim.keys()
+    _, _, want_lo, want_perm, _ = oracle.sorted_state(b)
+    assert np.array_equal(perm_b, want_perm) and np.array_equal(lo_b, want_lo)
+
+
 @pytest.mark.parametrize("mode", ["one-launch", "multi"])
 @pytest.mark.parametrize("n", [4097, 300_000, 700_000, 1_100_000, 2_500_000])
 def test_sort_paths_bit_exact(nbx, oracle, monkeypatch, mode, n):

@@ -543,6 +543,23 @@ __global__ __launch_bounds__(kBlock) void k_emit(
   }
 }
 
+// The level tables (split list and node offsets per depth) staged in LDS by
+// every workgroup of the kernels that look depths up per lane: a lookup is
+// then a few LDS reads instead of a chain of dependent global loads.
+struct LevelTables {
+  int32_t splitBase[kMaxDepth + 1];
+  int32_t levelStart[kMaxDepth + 2];
+};
+__device__ inline void stageLevels(const Ctl *__restrict__ ctl, LevelTables &lt) {
+  for (int t = threadIdx.x; t < kMaxDepth + 2; t += blockDim.x) {
+    if (t <= kMaxDepth) {
+      lt.splitBase[t] = ctl->splitBase[t];
+    }
+    lt.levelStart[t] = ctl->levelStart[t];
+  }
+  __syncthreads();
+}
+
 // Creates the 4 children of every splitting node (QuadTree::subdivide,
 // QuadTree.cpp:55-91): body ranges split at the digit boundaries, child quads
 // by intoQuadrant, and each child's own `children` slot if it splits.
@@ -560,20 +577,25 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
     return;
   }
   const int total = ctl->totalSplits;
+  if (static_cast<int>(blockIdx.x * kBlock) >= total) {
+    return;
+  }
+  __shared__ LevelTables lt;
+  stageLevels(ctl, lt);
   for (int s = blockIdx.x * kBlock + threadIdx.x; s < total;
        s += gridDim.x * kBlock) {
     int d = 0;
-    while (s >= ctl->splitBase[d + 1]) {
+    while (s >= lt.splitBase[d + 1]) {
       ++d;
     }
-    const int rank = s - ctl->splitBase[d];
+    const int rank = s - lt.splitBase[d];
     const int j = splitList[s];
     int bound[5];
     bound[0] = j;
     // First depth-(d+1) splitting run at or after j: this node's splitting
     // children are the next entries of that level's (sorted) list.
-    const int lvBase = ctl->splitBase[d + 1];
-    const int lvEnd = d + 1 < kMaxDepth ? ctl->splitBase[d + 2] : lvBase;
+    const int lvBase = lt.splitBase[d + 1];
+    const int lvEnd = d + 1 < kMaxDepth ? lt.splitBase[d + 2] : lvBase;
     int firstChildSplit = lvBase;
     if (d < 16) {
       // The run is every body whose high key word starts with j's d-digit
@@ -590,11 +612,16 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
         key[q] = base + (static_cast<uint64_t>(q + 1) << (sh - 2));
       }
       key[3] = base + (1ull << sh);
+      // Runs of one depth are disjoint and listed in key order, so this run
+      // ends at or before the next depth-d split's first body.
+      const int runEnd = s + 1 < lt.splitBase[d + 1]
+                             ? splitList[s + 1]
+                             : static_cast<int>(n);
       int a[5], b[5];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         a[q] = j;
-        b[q] = static_cast<int>(n);
+        b[q] = runEnd;
       }
       a[4] = lvBase;
       b[4] = lvEnd;
@@ -686,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
       cy += (static_cast<float>(q >> 1) - 0.5f) * sz;
     }
     (void)e;
-    const int c = ctl->levelStart[d + 1] + 4 * rank;
+    const int c = lt.levelStart[d + 1] + 4 * rank;
     int nextSplit = firstChildSplit;
     const float childSize = sz * 0.5f;
     for (int q = 0; q < 4; ++q) {
@@ -698,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
         // split list.
         const int a = nextSplit++;
         if (a < lvEnd && splitList[a] == cs) {
-          grand = ctl->levelStart[d + 2] + 4 * (a - lvBase);
+          grand = lt.levelStart[d + 2] + 4 * (a - lvBase);
           nodeOfSplit[a] = ch;
         } else {
           atomicOr(&const_cast<Ctl *>(ctl)->status, kStatusTreeInconsistent);
@@ -716,9 +743,9 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
 }
 
 // Depth of node x (x >= 1): largest D with levelStart[D] <= x.
-__device__ inline int nodeDepth(const Ctl *__restrict__ ctl, int x) {
+__device__ inline int nodeDepth(const LevelTables &lt, int x) {
   int d = 1;
-  while (d < kMaxDepth && ctl->levelStart[d + 1] <= x) {
+  while (d < kMaxDepth && lt.levelStart[d + 1] <= x) {
     ++d;
   }
   return d;
@@ -828,11 +855,11 @@ __device__ inline void sumBranch(int x, int c, float4 *__restrict__ nodeF,
 
 // Parent of node x >= 1: the splitting node of depth D-1 whose children
 // block holds x.
-__device__ inline int parentOf(const Ctl *__restrict__ ctl,
-                               const int32_t *__restrict__ nodeOfSplit,
-                               int x) {
-  const int D = nodeDepth(ctl, x);
-  return nodeOfSplit[ctl->splitBase[D - 1] + (x - ctl->levelStart[D]) / 4];
+// The parent of node x at depth D >= 1.
+__device__ inline int parentAt(const LevelTables &lt,
+                               const int32_t *__restrict__ nodeOfSplit, int x,
+                               int D) {
+  return nodeOfSplit[lt.splitBase[D - 1] + (x - lt.levelStart[D]) / 4];
 }
 
 __global__ __launch_bounds__(kBlock) void k_node_finish(
@@ -849,13 +876,21 @@ __global__ __launch_bounds__(kBlock) void k_node_finish(
     return;
   }
   const int count = ctl->nodeCount;
+  if (static_cast<int>(blockIdx.x * kBlock) >= count) {
+    return;
+  }
+  __shared__ LevelTables lt;
+  stageLevels(ctl, lt);
   for (int x = blockIdx.x * kBlock + threadIdx.x; x < count;
        x += gridDim.x * kBlock) {
+    const int D = x == 0 ? 0 : nodeDepth(lt, x);
     // `next` (QuadTree.cpp:87-90): child q < 3 points at q + 1; the last
-    // child inherits its parent's next.
-    int y = x;
+    // child inherits its parent's next (the parent of a depth-d node is at
+    // depth d - 1).
+    int y = x, dy = D;
     for (int guard = 0; y != 0 && ((y - 1) & 3) == 3; ++guard) {
-      y = parentOf(ctl, nodeOfSplit, y);
+      y = dy >= 1 ? parentAt(lt, nodeOfSplit, y, dy) : -1;
+      --dy;
       if (guard > kMaxDepth || y < 0 || y >= count) {
         atomicOr(&ctl->status, kStatusTreeInconsistent);
         y = 0;
@@ -876,7 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_node_finish(
     if (ni.w - ni.z <= kSubtreeBodies) {
       bool root = x == 0;
       if (!root) {
-        const int4 pi = nodeI[parentOf(ctl, nodeOfSplit, x)];
+        const int4 pi = nodeI[parentAt(lt, nodeOfSplit, x, D)];
         root = pi.w - pi.z > kSubtreeBodies;
       }
       if (root) {

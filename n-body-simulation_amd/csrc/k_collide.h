@@ -222,23 +222,22 @@ __global__ void k_collide_reset(Ctl *__restrict__ ctl) {
   resetCollideCounters(ctl);
 }
 
-// Large/small split, grid cells, bucket counts; also seeds the union-find.
-__global__ __launch_bounds__(kBlock) void k_cells(
-    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ r, Ctl *__restrict__ ctl,
-    uint32_t mask, uint32_t *__restrict__ bucketOf,
-    uint32_t *__restrict__ bucketRank,
-    uint32_t *__restrict__ bucketCount, int32_t *__restrict__ largeList,
-    float4 *__restrict__ largeBox, int32_t *__restrict__ ufParent) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) {
-    return;
-  }
+// Large/small split, grid cell and bucket rank of body i at (x, y); also
+// seeds the union-find. Large bodies are counted in *largeCounter.
+__device__ inline void cellOf(uint32_t i, float x, float y, float rad,
+                              const Ctl *__restrict__ ctl,
+                              uint32_t *__restrict__ largeCounter,
+                              uint32_t mask, uint32_t *__restrict__ bucketOf,
+                              uint32_t *__restrict__ bucketRank,
+                              uint32_t *__restrict__ bucketCount,
+                              int32_t *__restrict__ largeList,
+                              float4 *__restrict__ largeBox,
+                              int32_t *__restrict__ ufParent) {
   ufParent[i] = static_cast<int32_t>(i);
   const float cell = ctl->cell, invCell = ctl->invCell;
-  const Box b = bodyBox(px[i], py[i], r[i]);
+  const Box b = bodyBox(x, y, rad);
   if (b.maxX - b.minX > cell || b.maxY - b.minY > cell) {
-    const uint32_t slot = atomicAdd(&ctl->largeCount, 1u);
+    const uint32_t slot = atomicAdd(largeCounter, 1u);
     largeList[slot] = static_cast<int32_t>(i);
     largeBox[slot] = make_float4(b.minX, b.maxX, b.minY, b.maxY);
     bucketOf[i] = 0xffffffffu;
@@ -251,15 +250,37 @@ __global__ __launch_bounds__(kBlock) void k_cells(
   bucketRank[i] = atomicAdd(&bucketCount[h], 1u);
 }
 
+__global__ __launch_bounds__(kBlock) void k_cells(
+    uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
+    const float *__restrict__ r, Ctl *__restrict__ ctl,
+    uint32_t mask, uint32_t *__restrict__ bucketOf,
+    uint32_t *__restrict__ bucketRank,
+    uint32_t *__restrict__ bucketCount, int32_t *__restrict__ largeList,
+    float4 *__restrict__ largeBox, int32_t *__restrict__ ufParent) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) {
+    return;
+  }
+  cellOf(i, px[i], py[i], r[i], ctl, &ctl->largeCount, mask, bucketOf,
+         bucketRank, bucketCount, largeList, largeBox, ufParent);
+}
+
 // Counting-sort scatter of small boxes into their buckets, at the rank
-// each body drew in k_cells (the scan already cleared the bucket counts for
-// the next step).
+// each body drew in k_cells or k_iterate_cells (the scan already cleared the
+// bucket counts for the next step). takeFused: the cells came from
+// k_iterate_cells, whose large-body count moves to largeCount here (before
+// k_find_small reads it) and is cleared for the next step.
 __global__ __launch_bounds__(kBlock) void k_grid_scatter(
     uint32_t n, const float *__restrict__ px, const float *__restrict__ py,
-    const float *__restrict__ r, const Ctl *__restrict__ ctl,
+    const float *__restrict__ r, Ctl *__restrict__ ctl,
     const uint32_t *__restrict__ bucketOf,
     const uint32_t *__restrict__ bucketRank,
-    const uint32_t *__restrict__ bucketStart, float4 *__restrict__ grid) {
+    const uint32_t *__restrict__ bucketStart, float4 *__restrict__ grid,
+    uint32_t takeFused) {
+  if (takeFused && blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->largeCount = ctl->largeFused;
+    ctl->largeFused = 0;
+  }
   if (barrierStalled(ctl)) {
     return;
   }

@@ -57,6 +57,8 @@ struct Ctl {
   uint32_t largeCount, pairCount;
   uint32_t compCount, compPairs, bigCount, largest;
   uint32_t farMoves; // bodies the build's sort moved far (k_deep, k_permute)
+  uint32_t largeFused; // large bodies counted by k_iterate_cells (moved to
+                       // largeCount by k_grid_scatter, then cleared)
   // tree
   int32_t splits[kMaxDepth];         // splitting nodes at each depth
   int32_t levelStart[kMaxDepth + 2]; // first node index at each depth

@@ -263,6 +263,51 @@ __global__ __launch_bounds__(kBlock) void k_iterate(
   py[i] = py[i] + nvy * dt;
 }
 
+// The step's iterate with the collide's first pass (k_cells) on the moved
+// bodies: the positions are binned as they are written. Thread 0 resets the
+// collide's counters; the large bodies are counted in largeFused, which no
+// other kernel of this launch touches.
+__global__ __launch_bounds__(kBlock) void k_iterate_cells(
+    uint32_t n, float dt, float *__restrict__ px, float *__restrict__ py,
+    float *__restrict__ vx, float *__restrict__ vy,
+    const float *__restrict__ ax, const float *__restrict__ ay,
+    const float *__restrict__ r, Ctl *__restrict__ ctl, uint32_t mask,
+    uint32_t *__restrict__ bucketOf, uint32_t *__restrict__ bucketRank,
+    uint32_t *__restrict__ bucketCount, int32_t *__restrict__ largeList,
+    float4 *__restrict__ largeBox, int32_t *__restrict__ ufParent) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i == 0) {
+    resetCollideCounters(ctl);
+  }
+  if (i >= n) {
+    return;
+  }
+  // Body::update (Body.h:20-23), as k_iterate.
+  const float nvx = vx[i] + ax[i] * dt;
+  const float nvy = vy[i] + ay[i] * dt;
+  const float nx = px[i] + nvx * dt;
+  const float ny = py[i] + nvy * dt;
+  vx[i] = nvx;
+  vy[i] = nvy;
+  px[i] = nx;
+  py[i] = ny;
+  cellOf(i, nx, ny, r[i], ctl, &ctl->largeFused, mask, bucketOf, bucketRank,
+         bucketCount, largeList, largeBox, ufParent);
+}
+
+// Buckets of the collide's grid hash for n bodies (a power of two).
+uint32_t collideBuckets(uint32_t n) {
+  return static_cast<uint32_t>(
+      std::max<uint64_t>(1024, pow2AtLeast(2ull * n)));
+}
+
+// The step's iterate does the collide's cell pass (k_iterate_cells) when the
+// grid collide follows it.
+bool cellsInIterate(const nbx_sim *s) {
+  return s->params.collisions != 0 &&
+         s->params.collide_mode != NBX_COLLIDE_JACOBI && s->n >= 2;
+}
+
 __global__ void k_zero_counts(Ctl *__restrict__ ctl) {
   ctl->farCount = 0;
   ctl->bodyPairCount = 0;
@@ -281,6 +326,13 @@ int launchIterate(nbx_sim *s, bool resetCollide) {
     return NBX_OK;
   }
   const int c = s->cur;
+  if (resetCollide && cellsInIterate(s)) {
+    LAUNCH(k_iterate_cells, blocksFor(n), kBlock, n, s->params.dt, s->px[c],
+           s->py[c], s->vx[c], s->vy[c], s->ax, s->ay, s->r[c], s->ctl,
+           collideBuckets(n) - 1, s->bucketOf, s->bucketRank, s->bucketCount,
+           s->largeList, s->largeBox, s->ufParent);
+    return NBX_OK;
+  }
   LAUNCH(k_iterate, blocksFor(n), kBlock, n, s->params.dt, s->px[c], s->py[c],
          s->vx[c], s->vy[c], s->ax, s->ay, resetCollide ? s->ctl : nullptr);
   return NBX_OK;
@@ -440,20 +492,23 @@ int launchCollide(nbx_sim *s, bool countersReset) {
   float *px = s->px[c], *py = s->py[c], *vx = s->vx[c], *vy = s->vy[c];
   const float *m = s->m[c], *r = s->r[c];
   const uint32_t nb = blocksFor(n);
-  const uint32_t buckets =
-      static_cast<uint32_t>(std::max<uint64_t>(1024, pow2AtLeast(2ull * n)));
+  const uint32_t buckets = collideBuckets(n);
   const uint32_t mask = buckets - 1;
+  // The step's iterate (k_iterate_cells) has done the cell pass.
+  const bool fused = countersReset && cellsInIterate(s);
   const uint32_t pairGrid = std::min<uint32_t>(blocksFor(s->pairCap), 2048);
 
   if (!countersReset) {
     LAUNCH(k_collide_reset, 1, 1, s->ctl);
   }
-  LAUNCH(k_cells, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
-         s->bucketRank, s->bucketCount, s->largeList, s->largeBox,
-         s->ufParent);
+  if (!fused) {
+    LAUNCH(k_cells, nb, kBlock, n, px, py, r, s->ctl, mask, s->bucketOf,
+           s->bucketRank, s->bucketCount, s->largeList, s->largeBox,
+           s->ufParent);
+  }
   NBX_TRY(launchScan(s, buckets + 1, s->bucketCount, s->bucketStart));
   LAUNCH(k_grid_scatter, nb, kBlock, n, px, py, r, s->ctl, s->bucketOf,
-         s->bucketRank, s->bucketStart, s->grid);
+         s->bucketRank, s->bucketStart, s->grid, fused ? 1u : 0u);
   LAUNCH(k_find_small, blocksFor(n, kFindBlock), kFindBlock, n, px, py, r,
          s->ctl, mask, s->bucketOf, s->bucketStart, s->grid, s->largeList,
          s->largeBox, s->pairs, s->pairCap, s->ufParent);

@@ -885,3 +885,26 @@ def test_jacobi_steps_then_sequential(nbx, oracle):
     o = o.by_id()
     d = np.hypot(got["px"] - o.px, got["py"] - o.py) / np.maximum(np.hypot(o.px, o.py), 1)
     assert np.median(d) < 1e-6 and np.quantile(d, 0.99) < 1e-4
+
+
+def test_step_cell_pass_matches_separate_phases(nbx, states):
+    """A step's iterate also does the collide's cell pass (grid cell, bucket
+    rank, large-body list), counting the large bodies in a word of its own
+    that the scatter moves into place; the separate iterate / collide /
+    build / forces entry points run the plain passes. Both must leave the
+    same bodies, pairs and accelerations, step after step (the large-body
+    word must be cleared for the next step)."""
+    s = states["disc20k_60"]
+    with new_sim(nbx, s) as a, new_sim(nbx, s) as b:
+        for k in range(3):
+            a.step(1)
+            b.iterate()
+            b.collide()
+            b.build()
+            b.forces()
+            ga, gb = a.download(), b.download()
+            assert np.array_equal(ga.id, gb.id), f"body order, step {k + 1}"
+            for f in ("px", "py", "vx", "vy", "ax", "ay"):
+                assert_bits_equal(getattr(ga, f), getattr(gb, f), f"{f}, step {k + 1}")
+            ia, ib = a.collide_info(), b.collide_info()
+            assert ia["pairs"] == ib["pairs"], (ia, ib)
